@@ -1,0 +1,504 @@
+// Device-side building blocks of the MLS-MPM baking substep (gfx950).
+//
+// Everything here is __device__ inline code shared by the fused substep
+// kernels and the per-kernel entry points.  Types follow the reference:
+// its Warp `float` is float here, its wp.float64 is double, and every explicit
+// float()/wp.float64() conversion of the reference is kept so results agree
+// with the reference's arithmetic to rounding.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bk {
+
+// ------------------------------------------------------------------ 3x3 matrices (row-major)
+template <typename T> struct mat3 { T a[9]; };
+
+template <typename T> __device__ __forceinline__ mat3<T> m_load(const T* p) {
+  mat3<T> m;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) m.a[i] = p[i];
+  return m;
+}
+template <typename T> __device__ __forceinline__ void m_store(T* p, const mat3<T>& m) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) p[i] = m.a[i];
+}
+template <typename T> __device__ __forceinline__ mat3<T> m_eye() {
+  mat3<T> m;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) m.a[i] = (i % 4 == 0) ? T(1) : T(0);
+  return m;
+}
+template <typename T> __device__ __forceinline__ mat3<T> m_zero() {
+  mat3<T> m;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) m.a[i] = T(0);
+  return m;
+}
+template <typename T> __device__ __forceinline__ mat3<T> m_mul(const mat3<T>& x, const mat3<T>& y) {
+  mat3<T> r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      r.a[i * 3 + j] = x.a[i * 3 + 0] * y.a[j] + x.a[i * 3 + 1] * y.a[3 + j] + x.a[i * 3 + 2] * y.a[6 + j];
+  return r;
+}
+template <typename T> __device__ __forceinline__ mat3<T> m_mul_bt(const mat3<T>& x, const mat3<T>& y) {  // x * y^T
+  mat3<T> r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      r.a[i * 3 + j] = x.a[i * 3 + 0] * y.a[j * 3 + 0] + x.a[i * 3 + 1] * y.a[j * 3 + 1] + x.a[i * 3 + 2] * y.a[j * 3 + 2];
+  return r;
+}
+template <typename T> __device__ __forceinline__ mat3<T> m_scale(T s, const mat3<T>& m) {
+  mat3<T> r;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) r.a[i] = s * m.a[i];
+  return r;
+}
+template <typename T> __device__ __forceinline__ T m_det(const mat3<T>& m) {
+  const T* a = m.a;
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+template <typename T> __device__ __forceinline__ mat3<T> m_inverse(const mat3<T>& m) {
+  const T* a = m.a;
+  T d = m_det(m);
+  mat3<T> r;
+  r.a[0] = (a[4] * a[8] - a[5] * a[7]) / d;
+  r.a[1] = (a[2] * a[7] - a[1] * a[8]) / d;
+  r.a[2] = (a[1] * a[5] - a[2] * a[4]) / d;
+  r.a[3] = (a[5] * a[6] - a[3] * a[8]) / d;
+  r.a[4] = (a[0] * a[8] - a[2] * a[6]) / d;
+  r.a[5] = (a[2] * a[3] - a[0] * a[5]) / d;
+  r.a[6] = (a[3] * a[7] - a[4] * a[6]) / d;
+  r.a[7] = (a[1] * a[6] - a[0] * a[7]) / d;
+  r.a[8] = (a[0] * a[4] - a[1] * a[3]) / d;
+  return r;
+}
+template <typename T, typename S> __device__ __forceinline__ mat3<T> m_cast(const mat3<S>& m) {
+  mat3<T> r;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) r.a[i] = T(m.a[i]);
+  return r;
+}
+
+// ------------------------------------------------------------------ stencil (physics/transfer.py:4-33)
+__device__ __forceinline__ float quadratic_weight(float l, int i) {
+  if (i == 0) return 0.5f * (1.5f - l) * (1.5f - l);
+  if (i == 1) return 0.75f - (l - 1.0f) * (l - 1.0f);
+  return 0.5f * (l - 0.5f) * (l - 0.5f);
+}
+// All three weights of one axis.
+__device__ __forceinline__ void axis_weights(float l, float w[3]) {
+  w[0] = 0.5f * (1.5f - l) * (1.5f - l);
+  w[1] = 0.75f - (l - 1.0f) * (l - 1.0f);
+  w[2] = 0.5f * (l - 0.5f) * (l - 0.5f);
+}
+// grid_position = (x - origin) / dx with IEEE single-precision subtract and divide.
+__device__ __forceinline__ void grid_position(const float* x, const float o[3], float dx, float g[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) g[a] = __fdiv_rn(__fsub_rn(x[a], o[a]), dx);
+}
+__device__ __forceinline__ void stencil_base(const float g[3], int b[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) b[a] = int(floorf(__fsub_rn(g[a], 0.5f)));
+}
+__device__ __forceinline__ bool valid_stencil(const int b[3], int res) {
+  return b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[0] + 2 < res && b[1] + 2 < res && b[2] + 2 < res;
+}
+__device__ __forceinline__ int64_t flatten_node(int i, int j, int k, int res) { return (int64_t(i) * res + j) * res + k; }
+
+// Tiled (block-major) node numbering: 4x4x4 node blocks, nb blocks per axis.
+__device__ __forceinline__ int64_t tiled_node(int i, int j, int k, int nb) {
+  return ((int64_t((i >> 2) * nb + (j >> 2)) * nb + (k >> 2)) << 6) | ((i & 3) << 4) | ((j & 3) << 2) | (k & 3);
+}
+
+// ------------------------------------------------------------------ SVD (double)
+// Symmetric cyclic Jacobi on A = F^T F (Golub & Van Loan, Matrix Computations,
+// Alg. 8.5.3): V diagonalises A, sigma_i = sqrt(lambda_i) sorted descending,
+// U's first two columns are the normalised F*v_i (Gram-Schmidt), the third is
+// their cross product, and sigma_3 takes the sign of det F.  U and V are
+// rotations and F = U diag(sigma) V^T.
+__device__ __forceinline__ void jacobi_rotate(double A[3][3], double V[3][3], int p, int q) {
+  double apq = A[p][q];
+  if (apq == 0.0) return;
+  double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
+  double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+  double c = rsqrt(1.0 + t * t);
+  double s = t * c;
+  int r = 3 - p - q;
+  double arp = A[r][p], arq = A[r][q];
+  A[p][p] -= t * apq;
+  A[q][q] += t * apq;
+  A[p][q] = A[q][p] = 0.0;
+  A[r][p] = A[p][r] = c * arp - s * arq;
+  A[r][q] = A[q][r] = s * arp + c * arq;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double vp = V[i][p], vq = V[i][q];
+    V[i][p] = c * vp - s * vq;
+    V[i][q] = s * vp + c * vq;
+  }
+}
+
+__device__ inline void svd3(const mat3<double>& F, mat3<double>& U, double sig[3], mat3<double>& Vout) {
+  double A[3][3], V[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      A[i][j] = F.a[0 * 3 + i] * F.a[0 * 3 + j] + F.a[1 * 3 + i] * F.a[1 * 3 + j] + F.a[2 * 3 + i] * F.a[2 * 3 + j];
+      V[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int sweep = 0; sweep < 10; ++sweep) {
+    double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
+    double diag = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
+    if (off <= 1e-34 * diag) break;
+    jacobi_rotate(A, V, 0, 1);
+    jacobi_rotate(A, V, 0, 2);
+    jacobi_rotate(A, V, 1, 2);
+  }
+  double lam[3] = {A[0][0], A[1][1], A[2][2]};
+  int o0 = 0, o1 = 1, o2 = 2;
+  // sort descending (3-element network)
+  if (lam[o0] < lam[o1]) { int t = o0; o0 = o1; o1 = t; }
+  if (lam[o1] < lam[o2]) { int t = o1; o1 = o2; o2 = t; }
+  if (lam[o0] < lam[o1]) { int t = o0; o0 = o1; o1 = t; }
+  int ord[3] = {o0, o1, o2};
+  double Vs[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Vs[i][j] = V[i][ord[j]];
+  double dv = Vs[0][0] * (Vs[1][1] * Vs[2][2] - Vs[1][2] * Vs[2][1]) - Vs[0][1] * (Vs[1][0] * Vs[2][2] - Vs[1][2] * Vs[2][0]) +
+              Vs[0][2] * (Vs[1][0] * Vs[2][1] - Vs[1][1] * Vs[2][0]);
+  if (dv < 0.0) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) Vs[i][2] = -Vs[i][2];
+  }
+  // columns F*v_j
+  double B[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) B[i][j] = F.a[i * 3 + 0] * Vs[0][j] + F.a[i * 3 + 1] * Vs[1][j] + F.a[i * 3 + 2] * Vs[2][j];
+  double n0 = sqrt(B[0][0] * B[0][0] + B[1][0] * B[1][0] + B[2][0] * B[2][0]);
+  double u0[3], u1[3], u2[3];
+  if (n0 > 0.0) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u0[i] = B[i][0] / n0;
+  } else {
+    u0[0] = 1.0; u0[1] = 0.0; u0[2] = 0.0;
+  }
+  double d01 = u0[0] * B[0][1] + u0[1] * B[1][1] + u0[2] * B[2][1];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) u1[i] = B[i][1] - d01 * u0[i];
+  double n1 = sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+  if (n1 > 1e-300) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u1[i] /= n1;
+  } else {
+    double e0 = fabs(u0[0]) < 0.9 ? 1.0 : 0.0, e1 = fabs(u0[0]) < 0.9 ? 0.0 : 1.0;
+    double d = e0 * u0[0] + e1 * u0[1];
+    u1[0] = e0 - d * u0[0]; u1[1] = e1 - d * u0[1]; u1[2] = -d * u0[2];
+    double nn = sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u1[i] /= nn;
+  }
+  u2[0] = u0[1] * u1[2] - u0[2] * u1[1];
+  u2[1] = u0[2] * u1[0] - u0[0] * u1[2];
+  u2[2] = u0[0] * u1[1] - u0[1] * u1[0];
+  sig[0] = n0;
+  sig[1] = u1[0] * B[0][1] + u1[1] * B[1][1] + u1[2] * B[2][1];
+  sig[2] = u2[0] * B[0][2] + u2[1] * B[1][2] + u2[2] * B[2][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    U.a[i * 3 + 0] = u0[i]; U.a[i * 3 + 1] = u1[i]; U.a[i * 3 + 2] = u2[i];
+    Vout.a[i * 3 + 0] = Vs[i][0]; Vout.a[i * 3 + 1] = Vs[i][1]; Vout.a[i * 3 + 2] = Vs[i][2];
+  }
+}
+
+// ------------------------------------------------------------------ Ding material (physics/materials/ding.py)
+// ding.py:56-65
+__device__ __forceinline__ float temperature_factor(float t, const float k[4], const float f[4]) {
+  float r = f[0];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (t >= k[i]) {
+      float frac = fminf((t - k[i]) / (k[i + 1] - k[i]), 1.0f);
+      r = (1.0f - frac) * f[i] + frac * f[i + 1];
+    }
+  return r;
+}
+// ding.py:68-74
+__device__ __forceinline__ float porosity_coefficient(float n) {
+  float c = 0.0f;
+  if (n > 0.0f) {
+    float lg = logf(1.0f / n);
+    c = 9.0f / (4.0f * (lg * lg) * (3.5f * n + 1.0f));
+  }
+  return c;
+}
+// ding.py:77-87 (32 fixed Newton steps on the scalar overstress equation)
+__device__ __forceinline__ float solve_overstress(float q2, float p2, float df, float mf) {
+  float root = 0.0f;
+  if (q2 + p2 > 1.0f) {
+    for (int it = 0; it < 32; ++it) {
+      float dd = 1.0f + df * root;
+      float md = 1.0f + mf * root;
+      float dd2 = dd * dd, md2 = md * md;
+      float res = root + 1.0f - q2 / dd2 - p2 / md2;
+      float der = 1.0f + 2.0f * df * q2 / (dd2 * dd) + 2.0f * mf * p2 / (md2 * md);
+      root = root - res / der;
+    }
+  }
+  return root;
+}
+// porous_return.py:11-16
+__device__ __forceinline__ double pressure_coefficient64(double n) {
+  double c = 0.0;
+  if (n > 0.0) {
+    double lg = log(1.0 / n);
+    c = 9.0 / (4.0 * (lg * lg) * (3.5 * n + 1.0));
+  }
+  return c;
+}
+// porous_return.py:19-31 (<= 48 Newton steps, stop when |r| < 1e-12)
+__device__ __forceinline__ double overstress64(double q, double p, double df, double mf) {
+  double root = 0.0;
+  if (q + p > 1.0) {
+    for (int it = 0; it < 48; ++it) {
+      double u = 1.0 + df * root, v = 1.0 + mf * root;
+      double u2 = u * u, v2 = v * v;
+      double res = root + 1.0 - q / u2 - p / v2;
+      double der = 1.0 + 2.0 * df * q / (u2 * u) + 2.0 * mf * p / (v2 * v);
+      root -= res / der;
+      if (fabs(res) < 1.0e-12) break;
+    }
+  }
+  return root;
+}
+// porous_return.py:34-40
+__device__ __forceinline__ double porosity_prediction(double n, float q, float p, float df, float ms, float ex, double pv, float n0) {
+  double a = pressure_coefficient64(n);
+  double root = overstress64(double(q), a * double(p), double(df), double(ms) * a);
+  double den = 1.0 + double(ms) * a * root;
+  return pv * exp(double(ex) * (1.0 - 1.0 / den)) - 1.0 + double(n0);
+}
+// porous_return.py:43-61 (trial early exit, else 64 bisection steps)
+__device__ __forceinline__ float solve_porosity(float q, float p, float df, float ms, float ex, double pv, float n0) {
+  double trial = pv - 1.0 + double(n0);
+  if (trial > 0.0 && trial < 1.0) {
+    double tc = pressure_coefficient64(trial);
+    if (double(q) + tc * double(p) <= 1.0) return float(trial);
+  }
+  double lo = 0.0, hi = 1.0 - 1.0e-9;
+  for (int it = 0; it < 64; ++it) {
+    double mid = (lo + hi) * 0.5;
+    double pred = porosity_prediction(mid, q, p, df, ms, ex, pv, n0);
+    if (mid < pred) lo = mid; else hi = mid;
+  }
+  return float((lo + hi) * 0.5);
+}
+
+// Result of the Ding return map for one particle.
+struct DingOut {
+  mat3<float> Fe, Fp, tau;
+  float residual, porosity, porosity_residual, T, yield_stress, viscosity;
+  int gelled;
+  int flag0, flag1;  // porosity domain, unresolved coupled porosity
+};
+
+struct DingParams {
+  float mu, lam, yield_stress, n0, dt;
+  float knots[4], yf[4], vf[4];
+  int temperature_enabled, coupled;
+};
+
+// ding.py:103-137 with the SVD of the trial gradient precomputed.
+__device__ __forceinline__ void ding_return(const mat3<float>& trial, const mat3<float>& U, const float s[3], const mat3<float>& V,
+                                            float mu, float lam, float y, float eta, float a, float dt, mat3<float>& Fe,
+                                            mat3<float>& stress, float& residual) {
+  float eps[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
+  float tr = eps[0] + eps[1] + eps[2];
+  float ts[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ts[i] = 2.0f * mu * eps[i] + lam * tr;
+  float mean = (ts[0] + ts[1] + ts[2]) / 3.0f;
+  float dev[3] = {ts[0] - mean, ts[1] - mean, ts[2] - mean};
+  float q2 = 1.5f * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]) / (y * y);
+  float p2 = a * mean * mean / (y * y);
+  float df = 6.0f * mu * dt * y * y / eta;
+  float mf = 2.0f * a / 3.0f * (2.0f * mu + 3.0f * lam) * dt * y * y / eta;
+  float g = solve_overstress(q2, p2, df, mf);
+  float cs[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) cs[i] = dev[i] / (1.0f + df * g) + mean / (1.0f + mf * g);
+  float cm = (cs[0] + cs[1] + cs[2]) / 3.0f;
+  float e[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) e[i] = expf((cs[i] - cm) / (2.0f * mu) + cm / (2.0f * mu + 3.0f * lam));
+  // Fe = U diag(e) V^T ; stress = U diag(cs) U^T
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      Fe.a[i * 3 + j] = U.a[i * 3 + 0] * e[0] * V.a[j * 3 + 0] + U.a[i * 3 + 1] * e[1] * V.a[j * 3 + 1] + U.a[i * 3 + 2] * e[2] * V.a[j * 3 + 2];
+      stress.a[i * 3 + j] =
+          U.a[i * 3 + 0] * cs[0] * U.a[j * 3 + 0] + U.a[i * 3 + 1] * cs[1] * U.a[j * 3 + 1] + U.a[i * 3 + 2] * cs[2] * U.a[j * 3 + 2];
+    }
+  if (a == 0.0f) Fe = m_scale(powf(m_det(trial) / m_det(Fe), 1.0f / 3.0f), Fe);
+  residual = 0.0f;
+  if (q2 + p2 > 1.0f) {
+    float u = 1.0f + df * g, v = 1.0f + mf * g;
+    residual = fabsf(g + 1.0f - q2 / (u * u) - p2 / (v * v));
+  }
+}
+
+// ding.py:140-221 (UpdateDing body for one particle).  `valid` is false when the
+// incoming porosity is outside [0, 1) (flag 0, outputs untouched).
+__device__ inline bool ding_update(const DingParams& P, const mat3<float>& Fe_old, float porosity, int gelled_in, const mat3<float>& F_total,
+                                   const mat3<float>& C_new, float T, float visc_param, DingOut& o) {
+  o.flag0 = 0;
+  o.flag1 = 0;
+  if (!isfinite(porosity) || porosity < 0.0f || porosity >= 1.0f) { o.flag0 = 1; return false; }
+  int gel = gelled_in;
+  float Teff = T;
+  float yv = P.yield_stress;
+  float ev = visc_param;
+  if (P.temperature_enabled) {
+    if (T >= P.knots[3]) gel = 1;
+    if (gel) Teff = P.knots[3];
+    yv = P.yield_stress * temperature_factor(Teff, P.knots, P.yf);
+    ev = visc_param * temperature_factor(Teff, P.knots, P.vf);
+  }
+  mat3<float> trial;
+  {
+    mat3<float> inc = m_scale(P.dt, C_new);
+    inc.a[0] += 1.0f; inc.a[4] += 1.0f; inc.a[8] += 1.0f;
+    trial = m_mul(inc, Fe_old);
+  }
+  mat3<double> Ud, Vd;
+  double sd[3];
+  svd3(m_cast<double>(trial), Ud, sd, Vd);
+  mat3<float> U = m_cast<float>(Ud), V = m_cast<float>(Vd);
+  float s[3] = {float(sd[0]), float(sd[1]), float(sd[2])};
+  float ret_n = porosity;
+  float cres = 0.0f;
+  const float mu = P.mu, lam = P.lam, dt = P.dt;
+  if (P.coupled) {
+    float eps[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
+    float ts[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) ts[i] = 2.0f * mu * eps[i] + lam * (eps[0] + eps[1] + eps[2]);
+    float mean = (ts[0] + ts[1] + ts[2]) / 3.0f;
+    float dev[3] = {ts[0] - mean, ts[1] - mean, ts[2] - mean};
+    float q = 1.5f * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]) / (yv * yv);
+    float pv = mean * mean / (yv * yv);
+    float df = 6.0f * mu * dt * yv * yv / ev;
+    float ms = 2.0f / 3.0f * (2.0f * mu + 3.0f * lam) * dt * yv * yv / ev;
+    float ex = 3.0f * mean / (2.0f * mu + 3.0f * lam);
+    double opv = 1.0 - double(P.n0) + double(porosity);
+    ret_n = solve_porosity(q, pv, df, ms, ex, opv, P.n0);
+    double pred = porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
+    cres = float(fabs(double(ret_n) - pred));
+    if (cres > 1.0e-6f || ret_n <= 0.0f || ret_n >= 1.0f) o.flag1 = 1;
+  }
+  mat3<float> Fe, tau;
+  float resid;
+  ding_return(trial, U, s, V, mu, lam, yv, ev, porosity_coefficient(ret_n), dt, Fe, tau, resid);
+  if (P.coupled) {
+    double target = 1.0 - double(P.n0) + double(ret_n);
+    double vs = m_det(m_cast<double>(F_total)) / (target * m_det(m_cast<double>(Fe)));
+    Fe = m_scale(float(cbrt(vs)), Fe);
+  }
+  mat3<float> Fp = m_mul(m_inverse(Fe), F_total);
+  float nn = m_det(Fp) - 1.0f + P.n0;
+  if (P.coupled) nn = ret_n;
+  if (P.n0 == 0.0f && porosity == 0.0f) nn = 0.0f;
+  if (!isfinite(nn) || nn < 0.0f || nn >= 1.0f) o.flag0 = 1;
+  o.Fe = Fe;
+  o.Fp = Fp;
+  o.tau = tau;
+  o.residual = resid;
+  o.porosity_residual = cres;
+  o.porosity = nn;
+  o.T = T;
+  o.gelled = gel;
+  o.yield_stress = yv;
+  o.viscosity = ev;
+  return true;
+}
+
+// ------------------------------------------------------------------ gas (physics/materials/gas.py)
+constexpr double kGasConstant = 8.31446261815324;
+__device__ __forceinline__ double gas_pressure(double moles, float T, float J, float n) {
+  return moles * kGasConstant * double(T) / (double(J) * double(n));
+}
+
+// Neo-Hookean Kirchhoff stress (physics/materials/elastic.py:4-10).
+__device__ __forceinline__ mat3<float> neo_hookean_kirchhoff(const mat3<float>& F, float mu, float lam, float J) {
+  mat3<float> t = m_mul_bt(F, F);
+  float lj = lam * logf(J);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) t.a[i] = mu * (t.a[i] - ((i % 4 == 0) ? 1.0f : 0.0f)) + ((i % 4 == 0) ? lj : 0.0f);
+  return t;
+}
+
+// ------------------------------------------------------------------ mesh reconstruction (geometry/mesh_tracking.py:13-38)
+// One wavefront per vertex: every lane forms the anchor prediction, lanes
+// stride over the vertex's particle bindings summing weighted deviations from
+// the anchor, and a butterfly reduction combines the lanes.  Call with all 64
+// lanes of the wave active and the same `v`.
+__device__ inline void reconstruct_vertex_wave(int v, int lane, const float* __restrict__ x, const float* __restrict__ F,
+                                               const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
+                                               const int32_t* __restrict__ parts, const float* __restrict__ wts,
+                                               const float* __restrict__ ref_offs, float* __restrict__ out) {
+  int s = offs[v], e = offs[v + 1];
+  if (s == e) {
+    if (lane < 3) out[3 * v + lane] = ref_vertices[3 * v + lane];
+    return;
+  }
+  float anchor[3];
+  {
+    int ap = parts[s];
+    const float* Fa = F + 9 * ap;
+    const float* r = ref_offs + 3 * s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) anchor[a] = x[3 * ap + a] + (Fa[a * 3] * r[0] + Fa[a * 3 + 1] * r[1] + Fa[a * 3 + 2] * r[2]);
+  }
+  float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+  for (int k = s + lane; k < e; k += 64) {
+    int pi = parts[k];
+    const float* Fp = F + 9 * pi;
+    const float* r = ref_offs + 3 * k;
+    float wk = wts[k];
+    d0 += wk * ((x[3 * pi + 0] + (Fp[0] * r[0] + Fp[1] * r[1] + Fp[2] * r[2])) - anchor[0]);
+    d1 += wk * ((x[3 * pi + 1] + (Fp[3] * r[0] + Fp[4] * r[1] + Fp[5] * r[2])) - anchor[1]);
+    d2 += wk * ((x[3 * pi + 2] + (Fp[6] * r[0] + Fp[7] * r[1] + Fp[8] * r[2])) - anchor[2]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    d0 += __shfl_xor(d0, off, 64);
+    d1 += __shfl_xor(d1, off, 64);
+    d2 += __shfl_xor(d2, off, 64);
+  }
+  if (lane == 0) {
+    out[3 * v + 0] = anchor[0] + d0;
+    out[3 * v + 1] = anchor[1] + d1;
+    out[3 * v + 2] = anchor[2] + d2;
+  }
+}
+
+// ------------------------------------------------------------------ wave-level helpers
+__device__ __forceinline__ void flag_max(int* flags, int index, bool set) {
+  // One atomic per wavefront: the lowest set lane reports.
+  unsigned long long m = __ballot(set);
+  if (m != 0ull && __lane_id() == (unsigned)(__ffsll((long long)m) - 1)) atomicMax(flags + index, 1);
+}
+
+}  // namespace bk

@@ -1,0 +1,1115 @@
+// Fused MLS-MPM baking substep for gfx950 (MI355X) and its workspace, cell
+// keys and stable radix sort.
+//
+// One substep (reference physics/mpm.py:163-225) runs as:
+//   clear    zero the grid blocks the previous step touched
+//   keys     cell key of every particle (block-major over 4x4x4 cell blocks)
+//   sort     stable LSD radix sort of (key, particle index), 8-bit digits
+//   tiles    one tile per non-empty particle block; mark the 2x2x2 node blocks
+//            each tile's stencils reach as active
+//   mesh     reconstruct mesh vertices, then boundary-face areas per owner slot
+//   pass A   thermal P2G (7 f64 scalars) accumulated per tile in LDS, flushed
+//            with one global f64 atomic per touched node and scalar
+//   diffuse  6-neighbour FV heat/water fluxes per active node block
+//   pass B1  scalar G2P + gas source + gas and water stress (per particle)
+//   pass B2  mechanical P2G (mass + momentum) per tile in LDS, then flush
+//   grid     momentum -> velocity, gravity, unilateral walls
+//   pass C   G2P + Ding return map + gas refresh + saturations (per particle)
+// The grid lives in a tiled layout (4x4x4 node blocks contiguous) and only
+// active blocks are read, written and cleared.
+#include "bk_device.h"
+#include "../../include/baking_abi.h"
+
+#include <stdio.h>
+#include <string.h>
+#include <string>
+
+using namespace bk;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(const char* what) {
+  g_last_error = what;
+  return -1;
+}
+int check_hip(hipError_t e, const char* where) {
+  if (e == hipSuccess) return 0;
+  g_last_error = std::string(where) + ": " + hipGetErrorString(e);
+  return -2;
+}
+#define BK_HIP(call)                                        \
+  do {                                                      \
+    int _rc = check_hip((call), #call);                     \
+    if (_rc) return _rc;                                    \
+  } while (0)
+#define BK_LAUNCH(name) BK_HIP(hipGetLastError())
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;  // keys per workgroup
+constexpr int kTileThreads = 256;                     // P2G workgroup size
+constexpr int kRegion = 6;                            // 4 cells + 2-node stencil reach
+constexpr int kRegionNodes = kRegion * kRegion * kRegion;
+
+inline int div_up(long long a, long long b) { return int((a + b - 1) / b); }
+inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+}  // namespace
+
+struct bk_workspace {
+  int max_n = 0, res = 0, nb = 0, nblocks = 0, n_vertices = 0, n_slots = 0;
+  uint32_t* keys_raw = nullptr;                 // cell keys in particle order
+  uint32_t* keys_s[2] = {nullptr, nullptr};     // radix scratch pairs
+  int32_t* vals_s[2] = {nullptr, nullptr};
+  uint32_t* sorted_keys = nullptr;              // sorted keys / particle indices
+  int32_t* sorted_vals = nullptr;
+  int32_t* hist = nullptr;
+  int32_t* digit_total = nullptr;
+  int32_t* blk_start = nullptr;
+  int32_t* blk_end = nullptr;
+  int32_t* node_flag = nullptr;
+  int32_t* tiles = nullptr;
+  int32_t* active = nullptr;
+  int32_t* counters = nullptr;  // [0] tiles, [1] active blocks
+  float* tau = nullptr;
+  float* vertices = nullptr;
+  double* slot_exposed = nullptr;
+  double* slot_contact = nullptr;
+  int32_t* flags = nullptr;
+};
+
+// Host-side cache of workspace descriptors (the struct lives in device memory
+// so kernels can take it by pointer; the host keeps a copy per workspace).
+#include <unordered_map>
+#include <mutex>
+static std::mutex g_ws_mutex;
+static std::unordered_map<const bk_workspace*, bk_workspace> g_ws_cache;
+
+// ============================================================================ keys + radix sort
+__device__ __forceinline__ uint32_t tiled_cell_key(const int b[3], int nb) {
+  uint32_t blk = (uint32_t((b[0] >> 2) * nb + (b[1] >> 2)) * uint32_t(nb)) + uint32_t(b[2] >> 2);
+  return (blk << 6) | uint32_t(((b[0] & 3) << 4) | ((b[1] & 3) << 2) | (b[2] & 3));
+}
+
+__global__ void k_cell_keys(const float* __restrict__ x, int n, float ox, float oy, float oz, float dx, int res, int tiled,
+                            uint32_t* __restrict__ keys, int32_t* __restrict__ flags, int32_t* counters) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (counters && p == 0) { counters[0] = 0; counters[1] = 0; }
+  if (p >= n) return;
+  const float o[3] = {ox, oy, oz};
+  float g[3];
+  grid_position(x + 3 * p, o, dx, g);
+  int b[3];
+  stencil_base(g, b);
+  bool ok = valid_stencil(b, res);
+  flag_max(flags, 1, !ok);
+  uint32_t key = 0;
+  if (ok) key = tiled ? tiled_cell_key(b, (res + 3) / 4) : uint32_t(flatten_node(b[0], b[1], b[2], res));
+  keys[p] = key;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                             int32_t* __restrict__ hist, int nwg) {
+  __shared__ int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  int base = blockIdx.x * kSortTile;
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    int e = base + i * kSortThreads + threadIdx.x;
+    if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255], 1);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nwg + blockIdx.x] = h[threadIdx.x];
+}
+
+// Per-digit exclusive scan over workgroups: workgroup d scans hist[d*nwg ..
+// (d+1)*nwg) in 256-wide coalesced chunks and records the digit's total.
+__global__ __launch_bounds__(256) void k_scan_digits(int32_t* __restrict__ hist, int32_t* __restrict__ digit_total, int nwg) {
+  __shared__ int buf[256];
+  const int d = blockIdx.x;
+  int carry = 0;
+  for (int base = 0; base < nwg; base += 256) {
+    int i = base + threadIdx.x;
+    int v = i < nwg ? hist[d * nwg + i] : 0;
+    buf[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+      int t = threadIdx.x >= off ? buf[threadIdx.x - off] : 0;
+      __syncthreads();
+      buf[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < nwg) hist[d * nwg + i] = carry + buf[threadIdx.x] - v;
+    carry += buf[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) digit_total[d] = carry;
+}
+
+// Stable scatter: each wave owns a contiguous 512-key segment of the
+// workgroup's 2048 keys; ranks inside a wave come from 8 ballots (lanes with the
+// same digit), across waves from per-wave digit counts.
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                                                                uint32_t* __restrict__ kout, int32_t* __restrict__ vout, int n,
+                                                                int shift, const int32_t* __restrict__ offsets,
+                                                                const int32_t* __restrict__ digit_total, int nwg) {
+  __shared__ int run[4][256];
+  __shared__ int dbase[256];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int base = blockIdx.x * kSortTile + w * (kSortTile / 4);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) run[q][threadIdx.x] = 0;
+  // exclusive scan of the 256 digit totals (every workgroup recomputes it)
+  int tot = digit_total[threadIdx.x];
+  dbase[threadIdx.x] = tot;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    int t = threadIdx.x >= off ? dbase[threadIdx.x - off] : 0;
+    __syncthreads();
+    dbase[threadIdx.x] += t;
+    __syncthreads();
+  }
+  dbase[threadIdx.x] -= tot;
+  __syncthreads();
+  for (int r = 0; r < kSortItems; ++r) {
+    int e = base + r * 64 + lane;
+    if (e < n) atomicAdd(&run[w][(kin[e] >> shift) & 255], 1);
+  }
+  __syncthreads();
+  {
+    int d = threadIdx.x;
+    int c0 = run[0][d], c1 = run[1][d], c2 = run[2][d];
+    int o = dbase[d] + offsets[d * nwg + blockIdx.x];
+    run[0][d] = o;
+    run[1][d] = o + c0;
+    run[2][d] = o + c0 + c1;
+    run[3][d] = o + c0 + c1 + c2;
+  }
+  __syncthreads();
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < kSortItems; ++r) {
+    int e = base + r * 64 + lane;
+    bool valid = e < n;
+    uint32_t key = valid ? kin[e] : 0u;
+    int d = int((key >> shift) & 255);
+    unsigned long long mask = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      bool bit = (d >> b) & 1;
+      unsigned long long bb = __ballot(valid && bit);
+      mask &= bit ? bb : ~bb;
+    }
+    int pos = 0;
+    if (valid) pos = run[w][d] + __popcll(mask & lt);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    if (valid) {
+      kout[pos] = key;
+      vout[pos] = vin ? vin[e] : e;
+      if ((mask & lt) == 0ull) run[w][d] += __popcll(mask);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// Stable LSD radix sort.  Pass p reads the previous pass's output and writes a
+// scratch pair (A for even p, B for odd p); the last pass writes the caller's
+// output, so neither the input nor the output is ever read and written in one pass.
+static int radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int32_t* vals_in, uint32_t* keys_out,
+                      int32_t* vals_out, int n, int key_bits, hipStream_t st) {
+  if (n <= 0) return 0;
+  int passes = (key_bits + 7) / 8;
+  if (passes < 1) passes = 1;
+  int nwg = div_up(n, kSortTile);
+  const uint32_t* ksrc = keys_in;
+  const int32_t* vsrc = vals_in;
+  for (int p = 0; p < passes; ++p) {
+    bool last = p == passes - 1;
+    uint32_t* kdst = last ? keys_out : ws->keys_s[p & 1];
+    int32_t* vdst = last ? vals_out : ws->vals_s[p & 1];
+    hipLaunchKernelGGL(k_radix_hist, dim3(nwg), dim3(kSortThreads), 0, st, ksrc, n, 8 * p, ws->hist, nwg);
+    hipLaunchKernelGGL(k_scan_digits, dim3(256), dim3(256), 0, st, ws->hist, ws->digit_total, nwg);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nwg), dim3(kSortThreads), 0, st, ksrc, vsrc, kdst, vdst, n, 8 * p, ws->hist,
+                       ws->digit_total, nwg);
+    BK_LAUNCH(radix);
+    ksrc = kdst;
+    vsrc = vdst;
+  }
+  return 0;
+}
+
+static int tiled_key_bits(const bk_workspace* h) {
+  int key_bits = 6;
+  while ((1ll << key_bits) < (long long)h->nblocks * 64) ++key_bits;
+  return key_bits;
+}
+
+// ============================================================================ tiles and active node blocks
+__global__ void k_tile_bounds(const uint32_t* __restrict__ skeys, int n, int32_t* blk_start, int32_t* blk_end, int32_t* tiles,
+                              int32_t* counters) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t blk = skeys[i] >> 6;
+  if (i == 0 || (skeys[i - 1] >> 6) != blk) {
+    blk_start[blk] = i;
+    int slot = atomicAdd(&counters[0], 1);
+    tiles[slot] = int(blk);
+  }
+  if (i == n - 1 || (skeys[i + 1] >> 6) != blk) blk_end[blk] = i + 1;
+}
+
+__global__ void k_activate(const int32_t* __restrict__ tiles, int32_t* counters, int32_t* node_flag, int32_t* active, int nb) {
+  int ntiles = counters[0];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles * 8; t += gridDim.x * blockDim.x) {
+    int blk = tiles[t >> 3];
+    int d = t & 7;
+    int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
+    int cx = bx + (d >> 2), cy = by + ((d >> 1) & 1), cz = bz + (d & 1);
+    if (cx >= nb || cy >= nb || cz >= nb) continue;
+    int nbk = (cx * nb + cy) * nb + cz;
+    if (atomicExch(&node_flag[nbk], 1) == 0) {
+      int slot = atomicAdd(&counters[1], 1);
+      active[slot] = nbk;
+    }
+  }
+}
+
+// Zero every grid array over the previous step's active blocks (64 nodes each).
+__global__ void k_clear_blocks(const int32_t* __restrict__ active, const int32_t* counters, int32_t* node_flag, bk_grid g,
+                               int thermal) {
+  int nact = counters[1];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
+    int blk = active[t >> 6];
+    int64_t node = (int64_t(blk) << 6) | (t & 63);
+    g.m[node] = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { g.p[3 * node + a] = 0.0f; g.v[3 * node + a] = 0.0f; }
+    if (thermal) {
+      g.V[node] = 0.0; g.C[node] = 0.0; g.E[node] = 0.0; g.W[node] = 0.0;
+      g.H[node] = 0.0; g.L[node] = 0.0; g.X[node] = 0.0; g.dT[node] = 0.0; g.dW[node] = 0.0;
+    }
+    if ((t & 63) == 0) node_flag[blk] = 0;
+  }
+}
+
+// ============================================================================ mesh + surface areas
+// geometry/mesh_tracking.py:13-38, one wavefront per vertex (4 vertices per 256-thread workgroup).
+__global__ __launch_bounds__(256) void k_reconstruct(const float* __restrict__ x, const float* __restrict__ F,
+                                                     const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
+                                                     const int32_t* __restrict__ parts, const float* __restrict__ wts,
+                                                     const float* __restrict__ ref_offs, float* __restrict__ out, int nv) {
+  int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (v >= nv) return;
+  reconstruct_vertex_wave(v, threadIdx.x & 63, x, F, ref_vertices, offs, parts, wts, ref_offs, out);
+}
+
+// physics/thermal_water.py:77-92, summed per owner over its faces in face order.
+__device__ __forceinline__ void face_area(const float* verts, const int32_t* f, double floor_cd, double& area, bool& contact) {
+  double P[3][3];
+  for (int k = 0; k < 3; ++k)
+    for (int a = 0; a < 3; ++a) P[k][a] = double(verts[3 * f[k] + a]);
+  double e1[3], e2[3];
+  for (int a = 0; a < 3; ++a) { e1[a] = P[1][a] - P[0][a]; e2[a] = P[2][a] - P[0][a]; }
+  double nx = e1[1] * e2[2] - e1[2] * e2[1], ny = e1[2] * e2[0] - e1[0] * e2[2], nz = e1[0] * e2[1] - e1[1] * e2[0];
+  double len = sqrt(nx * nx + ny * ny + nz * nz);
+  area = 0.5 * len;
+  double cy = (P[0][1] + P[1][1] + P[2][1]) / 3.0;
+  contact = cy <= floor_cd && ny < -0.2 * len;
+}
+
+__global__ void k_slot_areas(const float* __restrict__ verts, const int32_t* __restrict__ faces, const int32_t* __restrict__ soff,
+                             const int32_t* __restrict__ sfaces, double* ex, double* co, int nslots, float floor_h, float cdist) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  double fcd = double(floor_h + cdist);
+  double e = 0.0, c = 0.0;
+  for (int k = soff[s]; k < soff[s + 1]; ++k) {
+    double a;
+    bool isc;
+    face_area(verts, faces + 3 * sfaces[k], fcd, a, isc);
+    if (isc) c += a; else e += a;
+  }
+  ex[s] = e;
+  co[s] = c;
+}
+
+// ============================================================================ per-particle transport terms
+struct Surface {
+  double exposed, contact;
+};
+__device__ __forceinline__ Surface particle_surface(const bk_scene& sc, const bk_workspace* ws, int p) {
+  int slot = sc.particle_slot[p];
+  Surface s{0.0, 0.0};
+  if (slot >= 0) { s.exposed = ws->slot_exposed[slot]; s.contact = ws->slot_contact[slot]; }
+  return s;
+}
+
+// Robin heat, water loss and exchange of one particle (thermal_water.py:108-120).
+struct Robin {
+  double vol, sat, loss, heat, exch;
+};
+__device__ __forceinline__ Robin robin_terms(const bk_params& P, float heat_param, float Jf, float V0, double T, double W,
+                                             const Surface& s) {
+  Robin r;
+  r.vol = double(V0) * double(Jf);
+  r.sat = W / (double(P.rho_w) * r.vol);
+  r.loss = 0.0;
+  if (P.water_enabled) r.loss = double(P.loss_coef) * T * r.sat * s.exposed;
+  r.heat = double(heat_param) * s.exposed * (double(P.T_oven) - T) + double(P.tray_h) * s.contact * (double(P.T_tray) - T) -
+           double(P.latent) * r.loss;
+  r.exch = double(heat_param) * s.exposed + double(P.tray_h) * s.contact;
+  if (P.water_enabled) r.exch += double(P.latent * P.loss_coef) * r.sat * s.exposed;
+  return r;
+}
+
+struct TileGeom {
+  int blk, start, count, bx, by, bz;
+};
+__device__ __forceinline__ TileGeom tile_geom(const bk_workspace* ws, int t) {
+  TileGeom g;
+  g.blk = ws->tiles[t];
+  g.start = ws->blk_start[g.blk];
+  g.count = ws->blk_end[g.blk] - g.start;
+  int nb = ws->nb;
+  g.bz = g.blk % nb;
+  g.by = (g.blk / nb) % nb;
+  g.bx = g.blk / (nb * nb);
+  return g;
+}
+
+// Lane -> sorted particle: lanes of one wave take particles `stride` apart so
+// that, with ~8 particles per cell, neighbouring lanes sit in different cells
+// and their LDS atomics hit different nodes.
+struct TileIter {
+  int stride, sub0;
+};
+
+__device__ __forceinline__ int region_index(int li, int lj, int lk) { return (li * kRegion + lj) * kRegion + lk; }
+
+// ============================================================================ pass A: thermal P2G
+__global__ __launch_bounds__(kTileThreads) void k_pass_a(bk_params P, bk_scene sc, bk_state in, bk_grid g, bk_workspace* ws) {
+  __shared__ double acc[7][kRegionNodes];
+  const int ntiles = ws->counters[0];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+  const float heat_param = P.settling ? 0.0f : sc.heat_param[0];
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    TileGeom tg = tile_geom(ws, t);
+    for (int i = threadIdx.x; i < 7 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
+    __syncthreads();
+    const int S = (tg.count + 63) / 64;
+    for (int sub = w; sub < S; sub += 4) {
+      int j = lane * S + sub;
+      if (j >= tg.count) continue;
+      int p = ws->sorted_vals[tg.start + j];
+      float g3[3];
+      grid_position(in.x + 3 * p, o, P.dx, g3);
+      int b[3];
+      stencil_base(g3, b);
+      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.bx || (b[1] >> 2) != tg.by || (b[2] >> 2) != tg.bz) continue;
+      float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+      float wx[3], wy[3], wz[3];
+      axis_weights(l[0], wx);
+      axis_weights(l[1], wy);
+      axis_weights(l[2], wz);
+      float Jf = m_det(m_load(in.F + 9 * p));
+      double T = in.T[p], W = in.W[p];
+      Robin r = robin_terms(P, heat_param, Jf, sc.V0[p], T, W, particle_surface(sc, ws, p));
+      double cap = sc.capacities[p];
+      double vals[7] = {r.vol, cap, 0.0, W, r.heat, r.loss, r.exch};
+      const int li = b[0] - 4 * tg.bx, lj = b[1] - 4 * tg.by, lk = b[2] - 4 * tg.bz;
+#pragma unroll
+      for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+          for (int oz = 0; oz < 3; ++oz) {
+            double wt = double(wx[ox] * wy[oy] * wz[oz]);
+            int ri = region_index(li + ox, lj + oy, lk + oz);
+            atomicAdd(&acc[0][ri], wt * vals[0]);
+            atomicAdd(&acc[1][ri], wt * cap);
+            atomicAdd(&acc[2][ri], wt * cap * T);
+            atomicAdd(&acc[3][ri], wt * W);
+            atomicAdd(&acc[4][ri], wt * vals[4]);
+            atomicAdd(&acc[5][ri], wt * vals[5]);
+            atomicAdd(&acc[6][ri], wt * vals[6]);
+          }
+    }
+    __syncthreads();
+    for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
+      int li = ri / 36, lj = (ri / 6) % 6, lk = ri % 6;
+      int i = 4 * tg.bx + li, j = 4 * tg.by + lj, k = 4 * tg.bz + lk;
+      if (i >= P.res || j >= P.res || k >= P.res) continue;
+      int64_t node = tiled_node(i, j, k, ws->nb);
+      double* dst[7] = {g.V, g.C, g.E, g.W, g.H, g.L, g.X};
+#pragma unroll
+      for (int a = 0; a < 7; ++a) {
+        double v = acc[a][ri];
+        if (v != 0.0) unsafeAtomicAdd(dst[a] + node, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ============================================================================ diffusion (thermal_water.py:140-190, GridFields fused)
+__global__ __launch_bounds__(256) void k_diffuse(bk_params P, bk_grid g, bk_workspace* ws) {
+  const int nact = ws->counters[1];
+  const int nb = ws->nb;
+  const double rho_w = double(P.rho_w);
+  const double dt = double(P.dt);
+  const double dx3 = double(P.dx * P.dx * P.dx);
+  const double hk = double(P.kappa * P.dx);
+  const double wk = double(P.diffusivity * P.rho_w * P.dx);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
+    int blk = ws->active[t >> 6];
+    int loc = t & 63;
+    int64_t node = (int64_t(blk) << 6) | loc;
+    double Ci = g.C[node];
+    bool heat_bad = false, water_bad = false;
+    if (Ci > 0.0) {
+      int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
+      int c[3] = {4 * bx + (loc >> 4), 4 * by + ((loc >> 2) & 3), 4 * bz + (loc & 3)};
+      double Vi = g.V[node];
+      double Ti = g.E[node] / Ci;
+      double Si = g.W[node] / (rho_w * Vi);
+      double hr = g.H[node], wr = -g.L[node], hd = g.X[node], wd = 0.0;
+#pragma unroll
+      for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+        for (int side = 0; side < 2; ++side) {
+          int o[3] = {c[0], c[1], c[2]};
+          o[ax] += 2 * side - 1;
+          if (o[ax] < 0 || o[ax] >= P.res) continue;
+          int64_t other = tiled_node(o[0], o[1], o[2], nb);
+          double Cj = g.C[other];
+          if (Cj > 0.0) {
+            double Vj = g.V[other];
+            double Tj = g.E[other] / Cj;
+            double fill = 2.0 * Vi * Vj / (Vi + Vj) / dx3;
+            double hc = hk * fill;
+            hr += hc * (Tj - Ti);
+            hd += hc;
+            if (P.water_enabled) {
+              double Sj = g.W[other] / (rho_w * Vj);
+              double mob = wk * fill * 0.5 * (Ti + Tj) * 0.5 * (Si + Sj);
+              wr += mob * (Sj - Si);
+              wd += mob / (rho_w * Vi);
+            }
+          }
+        }
+      heat_bad = dt * hd > Ci;
+      double change = dt * wr;
+      water_bad = g.W[node] + change < 0.0 || dt * wd > 1.0;
+      g.dT[node] = dt * hr / Ci;
+      g.dW[node] = change;
+    }
+    flag_max(ws->flags, BK_FLAG_HEAT, heat_bad);
+    flag_max(ws->flags, BK_FLAG_WATER, water_bad);
+  }
+}
+
+// ============================================================================ pass B1: scalar G2P, gas, stresses
+__global__ __launch_bounds__(256) void k_pass_b1(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_grid g, bk_workspace* ws) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  bool jbad = false, pbad = false, cflbad = false;
+  if (p < P.n) {
+    const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+    mat3<float> F = m_load(in.F + 9 * p);
+    float Jf = m_det(F);
+    mat3<float> tau;
+    float T_mat = 0.0f, dens = 0.0f;
+    double moles = 0.0, sat = 0.0;
+    if (P.transport_enabled) {
+      float g3[3];
+      grid_position(in.x + 3 * p, o, P.dx, g3);
+      int b[3];
+      stencil_base(g3, b);
+      double tc = 0.0, mr = 0.0;
+      if (valid_stencil(b, P.res)) {
+        float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+        float wx[3], wy[3], wz[3];
+        axis_weights(l[0], wx);
+        axis_weights(l[1], wy);
+        axis_weights(l[2], wz);
+#pragma unroll
+        for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+          for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+              double wt = double(wx[ox] * wy[oy] * wz[oz]);
+              int64_t node = tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, ws->nb);
+              tc += wt * g.dT[node];
+              double nw = g.W[node];
+              if (nw > 0.0) mr += wt * g.dW[node] / nw;
+            }
+      }
+      double T_old = in.T[p], W_old = in.W[p];
+      const float heat_param = P.settling ? 0.0f : sc.heat_param[0];
+      Robin r = robin_terms(P, heat_param, Jf, sc.V0[p], T_old, W_old, particle_surface(sc, ws, p));
+      double T_new = T_old + tc;
+      double W_new = W_old * (1.0 + mr);
+      double dry = sc.dry[p];
+      out.T[p] = T_new;
+      T_mat = float(T_new);
+      out.W[p] = W_new;
+      out.losses[p] = in.losses[p] + double(P.dt) * r.loss;
+      out.exchanges[p] = in.exchanges[p] + double(P.dt) * r.heat;
+      out.masses[p] = float(dry + W_new);
+      dens = float((dry + W_new) / double(sc.V0[p]));
+      out.density[p] = dens;
+      out.T_mat[p] = T_mat;
+      sat = W_new / (double(P.rho_w) * double(sc.V0[p]) * double(Jf));
+      if (P.model == 1 && P.gas_enabled) {
+        moles = in.moles[p];
+        if (!P.settling) {
+          double tm = 0.5 * (T_old + T_new);
+          double z = (tm - double(P.gas_peak)) / double(P.gas_width);
+          moles += double(P.dt * P.density) * double(sc.prod_param[0]) * exp(-(z * z));
+        }
+        out.moles[p] = moles;
+      }
+    } else if (P.model == 1) {
+      T_mat = in.T_mat[p];
+      dens = in.density[p];
+      moles = P.gas_enabled ? in.moles[p] : 0.0;
+    }
+    if (P.model == 0) {
+      // physics/materials/elastic.py:13-26
+      if (!isfinite(Jf) || Jf <= 0.0f) { jbad = true; tau = m_zero<float>(); }
+      else tau = neo_hookean_kirchhoff(F, P.mu, P.lam, Jf);
+    } else {
+      tau = m_load(in.tau + 9 * p);
+      if (P.gas_enabled) {
+        // physics/materials/gas.py:16-35
+        float nv = in.porosity[p];
+        if (nv <= 0.0f || nv >= 1.0f || !isfinite(nv) || Jf <= 0.0f) pbad = true;
+        else {
+          float pr = float(gas_pressure(moles, T_mat, Jf, nv));
+          float add = Jf * (P.p_ambient - pr);
+          tau.a[0] += add; tau.a[4] += add; tau.a[8] += add;
+          float bulk = pr * (1.0f + (1.0f - P.n0 + nv) / nv);
+          float wave = sqrtf((P.lam + 2.0f * P.mu + bulk) * Jf / dens);
+          cflbad = !isfinite(pr) || wave * P.dt / P.dx > P.cfl_limit;
+        }
+      }
+    }
+    if (P.transport_enabled) {
+      // physics/thermal_water.py:248-253
+      float k = Jf * (P.water_enabled ? P.water_stress : 0.0f) * (float(sat) - P.s0);
+      tau.a[0] -= k; tau.a[4] -= k; tau.a[8] -= k;
+    }
+    m_store(ws->tau + 9 * p, tau);
+  }
+  flag_max(ws->flags, BK_FLAG_J, jbad);
+  flag_max(ws->flags, BK_FLAG_POROSITY, pbad);
+  flag_max(ws->flags, BK_FLAG_GAS_CFL, cflbad);
+}
+
+// ============================================================================ pass B2: mechanical P2G (physics/transfer.py:36-70)
+__global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_grid g,
+                                                          bk_workspace* ws) {
+  __shared__ float acc[4][kRegionNodes];
+  const int ntiles = ws->counters[0];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+  const float* masses = P.transport_enabled ? out.masses : sc.m0;
+  const float s_coef = 4.0f / (P.dx * P.dx);
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    TileGeom tg = tile_geom(ws, t);
+    for (int i = threadIdx.x; i < 4 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0f;
+    __syncthreads();
+    const int S = (tg.count + 63) / 64;
+    for (int sub = w; sub < S; sub += 4) {
+      int j = lane * S + sub;
+      if (j >= tg.count) continue;
+      int p = ws->sorted_vals[tg.start + j];
+      float g3[3];
+      grid_position(in.x + 3 * p, o, P.dx, g3);
+      int b[3];
+      stencil_base(g3, b);
+      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.bx || (b[1] >> 2) != tg.by || (b[2] >> 2) != tg.bz) continue;
+      float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+      float wx[3], wy[3], wz[3];
+      axis_weights(l[0], wx);
+      axis_weights(l[1], wy);
+      axis_weights(l[2], wz);
+      float mass = masses[p];
+      float sc_tau = P.dt * sc.V0[p] * 4.0f / (P.dx * P.dx);
+      mat3<float> Cm = m_load(in.C + 9 * p);
+      mat3<float> tau = m_load(ws->tau + 9 * p);
+      mat3<float> A;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) A.a[i] = mass * Cm.a[i] - sc_tau * tau.a[i];
+      float mv[3] = {mass * in.v[3 * p + 0], mass * in.v[3 * p + 1], mass * in.v[3 * p + 2]};
+      const int li = b[0] - 4 * tg.bx, lj = b[1] - 4 * tg.by, lk = b[2] - 4 * tg.bz;
+#pragma unroll
+      for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+          for (int oz = 0; oz < 3; ++oz) {
+            float wt = wx[ox] * wy[oy] * wz[oz];
+            float d[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
+            int ri = region_index(li + ox, lj + oy, lk + oz);
+            atomicAdd(&acc[0][ri], wt * mass);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+              atomicAdd(&acc[1 + a][ri], wt * (mv[a] + (A.a[a * 3 + 0] * d[0] + A.a[a * 3 + 1] * d[1] + A.a[a * 3 + 2] * d[2])));
+          }
+    }
+    (void)s_coef;
+    __syncthreads();
+    for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
+      int li = ri / 36, lj = (ri / 6) % 6, lk = ri % 6;
+      int i = 4 * tg.bx + li, j = 4 * tg.by + lj, k = 4 * tg.bz + lk;
+      if (i >= P.res || j >= P.res || k >= P.res) continue;
+      int64_t node = tiled_node(i, j, k, ws->nb);
+      float m = acc[0][ri];
+      if (m != 0.0f) unsafeAtomicAdd(g.m + node, m);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        float v = acc[1 + a][ri];
+        if (v != 0.0f) unsafeAtomicAdd(g.p + 3 * node + a, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ============================================================================ grid update (physics/grid.py:20-38)
+__global__ __launch_bounds__(256) void k_grid_update(bk_params P, bk_grid g, bk_workspace* ws) {
+  const int nact = ws->counters[1];
+  const int nb = ws->nb;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
+    int blk = ws->active[t >> 6];
+    int loc = t & 63;
+    int64_t node = (int64_t(blk) << 6) | loc;
+    float m = g.m[node];
+    if (!(m > 0.0f)) continue;
+    int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
+    int c[3] = {4 * bx + (loc >> 4), 4 * by + ((loc >> 2) & 3), 4 * bz + (loc & 3)};
+    float vel[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      vel[a] = g.p[3 * node + a] / m + P.dt * P.gravity[a];
+      if (c[a] < P.boundary_cells && vel[a] < 0.0f) vel[a] = 0.0f;
+      if (c[a] >= P.res - P.boundary_cells && vel[a] > 0.0f) vel[a] = 0.0f;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g.v[3 * node + a] = vel[a];
+  }
+}
+
+// ============================================================================ pass C: G2P + Ding + gas refresh + saturation
+__global__ __launch_bounds__(256) void k_pass_c(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_grid g, bk_workspace* ws) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  bool stencil_bad = false, f0 = false, f1 = false;
+  if (p < P.n) {
+    const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+    float g3[3];
+    grid_position(in.x + 3 * p, o, P.dx, g3);
+    int b[3];
+    stencil_base(g3, b);
+    if (!valid_stencil(b, P.res)) {
+      stencil_bad = true;
+    } else {
+      float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+      float wx[3], wy[3], wz[3];
+      axis_weights(l[0], wx);
+      axis_weights(l[1], wy);
+      axis_weights(l[2], wz);
+      float nv[3] = {0.f, 0.f, 0.f};
+      mat3<float> Cn = m_zero<float>();
+      const float s4 = 4.0f / (P.dx * P.dx);
+#pragma unroll
+      for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+          for (int oz = 0; oz < 3; ++oz) {
+            float wt = wx[ox] * wy[oy] * wz[oz];
+            int64_t node = tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, ws->nb);
+            float vi[3] = {g.v[3 * node + 0], g.v[3 * node + 1], g.v[3 * node + 2]};
+            float d[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
+            float sw = s4 * wt;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              nv[a] += wt * vi[a];
+#pragma unroll
+              for (int c = 0; c < 3; ++c) Cn.a[a * 3 + c] += sw * (vi[a] * d[c]);
+            }
+          }
+      mat3<float> F_old = m_load(in.F + 9 * p);
+      mat3<float> inc = m_scale(P.dt, Cn);
+      inc.a[0] += 1.0f; inc.a[4] += 1.0f; inc.a[8] += 1.0f;
+      mat3<float> F_new = m_mul(inc, F_old);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        out.v[3 * p + a] = nv[a];
+        out.x[3 * p + a] = in.x[3 * p + a] + P.dt * nv[a];
+      }
+      m_store(out.C + 9 * p, Cn);
+      m_store(out.F + 9 * p, F_new);
+      if (P.model == 1) {
+        DingParams D;
+        D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
+        for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
+        D.temperature_enabled = P.temperature_enabled;
+        D.coupled = P.gas_enabled;
+        float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
+        DingOut r;
+        bool ok = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, sc.visc_param[0], r);
+        f0 = r.flag0;
+        f1 = r.flag1;
+        if (ok) {
+          m_store(out.Fe + 9 * p, r.Fe);
+          m_store(out.Fp + 9 * p, r.Fp);
+          m_store(out.tau + 9 * p, r.tau);
+          out.residual[p] = r.residual;
+          out.porosity_residual[p] = r.porosity_residual;
+          out.porosity[p] = r.porosity;
+          out.T_mat[p] = r.T;
+          out.gelled[p] = r.gelled;
+          out.yield_stress[p] = r.yield_stress;
+          out.viscosity[p] = r.viscosity;
+        }
+        if (!P.transport_enabled) out.density[p] = in.density[p];
+        float Jn = m_det(F_new);
+        if (P.gas_enabled) {
+          double moles = P.transport_enabled ? out.moles[p] : in.moles[p];
+          if (!P.transport_enabled) out.moles[p] = moles;
+          float nn = out.porosity[p];
+          if (nn > 0.0f && nn < 1.0f) out.pressure[p] = gas_pressure(moles, out.T_mat[p], Jn, nn);
+        }
+        if (P.transport_enabled) out.s[p] = out.W[p] / (double(P.rho_w) * double(sc.V0[p]) * double(Jn));
+      }
+    }
+  }
+  flag_max(ws->flags, BK_FLAG_STENCIL, stencil_bad);
+  flag_max(ws->flags, BK_FLAG_POROSITY, f0);
+  flag_max(ws->flags, BK_FLAG_COUPLED, f1);
+}
+
+__global__ void k_untile_f32(const float* __restrict__ tiled, float* __restrict__ dense, int res, int comps) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t total = int64_t(res) * res * res;
+  if (i >= total) return;
+  int k = int(i % res), j = int((i / res) % res), ii = int(i / (int64_t(res) * res));
+  int64_t t = tiled_node(ii, j, k, (res + 3) / 4);
+  for (int c = 0; c < comps; ++c) dense[i * comps + c] = tiled[t * comps + c];
+}
+__global__ void k_untile_f64(const double* __restrict__ tiled, double* __restrict__ dense, int res) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t total = int64_t(res) * res * res;
+  if (i >= total) return;
+  int k = int(i % res), j = int((i / res) % res), ii = int(i / (int64_t(res) * res));
+  dense[i] = tiled[tiled_node(ii, j, k, (res + 3) / 4)];
+}
+
+// ============================================================================ C ABI
+extern "C" {
+
+int bk_abi_version(void) { return BK_ABI_VERSION; }
+const char* bk_last_error(void) { return g_last_error.c_str(); }
+
+int bk_device_info(char* name, int name_len, int* cu_count, int* xcd_count) {
+  int dev = 0;
+  BK_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  BK_HIP(hipGetDeviceProperties(&prop, dev));
+  if (name && name_len > 0) { strncpy(name, prop.gcnArchName, name_len - 1); name[name_len - 1] = 0; }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (xcd_count) *xcd_count = 8;
+  return 0;
+}
+
+bk_workspace* bk_workspace_create(int32_t max_particles, int32_t res, int32_t n_vertices, int32_t n_slots) {
+  if (max_particles < 1 || res < 4) { fail("bk_workspace_create: bad sizes"); return nullptr; }
+  bk_workspace* ws = nullptr;
+  if (hipMalloc(&ws, sizeof(bk_workspace)) != hipSuccess) { fail("hipMalloc workspace"); return nullptr; }
+  bk_workspace h;
+  h.max_n = max_particles;
+  h.res = res;
+  h.nb = (res + 3) / 4;
+  h.nblocks = h.nb * h.nb * h.nb;
+  h.n_vertices = n_vertices;
+  h.n_slots = n_slots;
+  size_t n = size_t(max_particles);
+  int nwg = div_up(max_particles, kSortTile);
+  bool ok = true;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (hipMalloc(p, bytes) != hipSuccess) ok = false;
+    else if (hipMemset(*p, 0, bytes) != hipSuccess) ok = false;
+  };
+  alloc((void**)&h.keys_raw, n * 4);
+  alloc((void**)&h.sorted_keys, n * 4);
+  alloc((void**)&h.sorted_vals, n * 4);
+  for (int i = 0; i < 2; ++i) {
+    alloc((void**)&h.keys_s[i], n * 4);
+    alloc((void**)&h.vals_s[i], n * 4);
+  }
+  alloc((void**)&h.hist, size_t(256) * nwg * 4);
+  alloc((void**)&h.digit_total, 256 * 4);
+  alloc((void**)&h.blk_start, size_t(h.nblocks) * 4);
+  alloc((void**)&h.blk_end, size_t(h.nblocks) * 4);
+  alloc((void**)&h.node_flag, size_t(h.nblocks) * 4);
+  alloc((void**)&h.tiles, size_t(h.nblocks) * 4);
+  alloc((void**)&h.active, size_t(h.nblocks) * 4);
+  alloc((void**)&h.counters, 64);
+  alloc((void**)&h.tau, n * 36);
+  alloc((void**)&h.vertices, size_t(n_vertices) * 12);
+  alloc((void**)&h.slot_exposed, size_t(n_slots) * 8);
+  alloc((void**)&h.slot_contact, size_t(n_slots) * 8);
+  alloc((void**)&h.flags, BK_FLAG_COUNT * 4);
+  if (!ok || hipMemcpy(ws, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
+    fail("bk_workspace_create: allocation failed");
+    return nullptr;
+  }
+  return ws;
+}
+
+static int ws_host(bk_workspace* ws, bk_workspace& h) {
+  BK_HIP(hipMemcpy(&h, ws, sizeof(h), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+void bk_workspace_destroy(bk_workspace* ws) {
+  if (!ws) return;
+  bk_workspace h;
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    g_ws_cache.erase(ws);
+  }
+  if (ws_host(ws, h)) return;
+  void* ptrs[] = {h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1], h.hist, h.digit_total, h.blk_start, h.blk_end, h.node_flag, h.tiles, h.active,
+                  h.counters, h.tau, h.vertices, h.slot_exposed, h.slot_contact, h.flags};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  (void)hipFree(ws);
+}
+
+}  // extern "C"
+
+
+static const bk_workspace* host_view(bk_workspace* ws) {
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  auto it = g_ws_cache.find(ws);
+  if (it != g_ws_cache.end()) return &it->second;
+  bk_workspace h;
+  if (ws_host(ws, h)) return nullptr;
+  return &(g_ws_cache[ws] = h);
+}
+
+// Optional per-stage timing: HIP events recorded around each stage of
+// bk_substep on the launch stream (no extra synchronisation while recording).
+#include <vector>
+enum { kStageClear, kStageKeys, kStageSort, kStageTiles, kStageMesh, kStagePassA, kStageDiffuse, kStagePassB1, kStagePassB2,
+       kStageGrid, kStagePassC, kStageCount };
+static const char* kStageNames[kStageCount] = {"clear", "keys", "sort", "tiles", "mesh", "pass_a", "diffuse", "pass_b1",
+                                               "pass_b2", "grid", "pass_c"};
+struct Profiler {
+  int max_steps = 0, step = 0;
+  std::vector<hipEvent_t> ev;  // [step][stage][start/stop]
+};
+static std::unordered_map<const bk_workspace*, Profiler> g_prof;
+
+struct StageTimer {
+  Profiler* pr;
+  hipStream_t st;
+  int stage;
+  StageTimer(Profiler* p, hipStream_t s, int k) : pr(p), st(s), stage(k) {
+    if (pr) (void)hipEventRecord(pr->ev[(size_t(pr->step) * kStageCount + stage) * 2 + 0], st);
+  }
+  ~StageTimer() {
+    if (pr) (void)hipEventRecord(pr->ev[(size_t(pr->step) * kStageCount + stage) * 2 + 1], st);
+  }
+};
+
+extern "C" {
+
+int32_t* bk_workspace_flags(bk_workspace* ws) {
+  const bk_workspace* h = host_view(ws);
+  return h ? h->flags : nullptr;
+}
+float* bk_workspace_vertices(bk_workspace* ws) {
+  const bk_workspace* h = host_view(ws);
+  return h ? h->vertices : nullptr;
+}
+int bk_workspace_counts(bk_workspace* ws, int32_t* out2, void* stream) {
+  const bk_workspace* h = host_view(ws);
+  if (!h) return -2;
+  BK_HIP(hipMemcpyAsync(out2, h->counters, 8, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  BK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+int bk_workspace_read_flags(bk_workspace* ws, int32_t* host8, void* stream) {
+  const bk_workspace* h = host_view(ws);
+  if (!h || !host8) return fail("bk_workspace_read_flags: bad argument");
+  BK_HIP(hipMemcpyAsync(host8, h->flags, BK_FLAG_COUNT * 4, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  BK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* stream) {
+  const bk_workspace* h = host_view(ws);
+  if (!h || !host8) return fail("bk_workspace_write_flags: bad argument");
+  BK_HIP(hipMemcpyAsync(h->flags, host8, BK_FLAG_COUNT * 4, hipMemcpyHostToDevice, (hipStream_t)stream));
+  BK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  return 0;
+}
+
+int bk_profile_begin(bk_workspace* ws, int32_t max_steps) {
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  Profiler& pr = g_prof[ws];
+  for (hipEvent_t e : pr.ev) (void)hipEventDestroy(e);
+  pr.ev.assign(size_t(max_steps) * kStageCount * 2, nullptr);
+  for (auto& e : pr.ev) BK_HIP(hipEventCreate(&e));
+  pr.max_steps = max_steps;
+  pr.step = 0;
+  return 0;
+}
+
+int bk_profile_end(bk_workspace* ws, float* ms_per_stage, int32_t* steps_recorded) {
+  Profiler* pr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    auto it = g_prof.find(ws);
+    if (it == g_prof.end()) return fail("bk_profile_end: profiling not started");
+    pr = &it->second;
+  }
+  for (int k = 0; k < kStageCount; ++k) ms_per_stage[k] = 0.0f;
+  for (int s = 0; s < pr->step; ++s)
+    for (int k = 0; k < kStageCount; ++k) {
+      hipEvent_t a = pr->ev[(size_t(s) * kStageCount + k) * 2], b = pr->ev[(size_t(s) * kStageCount + k) * 2 + 1];
+      BK_HIP(hipEventSynchronize(b));
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, a, b) == hipSuccess) ms_per_stage[k] += ms;
+    }
+  *steps_recorded = pr->step;
+  for (hipEvent_t e : pr->ev) (void)hipEventDestroy(e);
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    g_prof.erase(ws);
+  }
+  return 0;
+}
+
+const char* bk_profile_stage_name(int32_t k) { return (k >= 0 && k < kStageCount) ? kStageNames[k] : ""; }
+int32_t bk_profile_stage_count(void) { return kStageCount; }
+
+int bk_cell_keys(const float* x, int32_t n, const float* o, float dx, int32_t res, int32_t tiled, uint32_t* keys, int32_t* flags,
+                 void* stream) {
+  if (n < 0 || !o || res < 1) return fail("bk_cell_keys: bad argument");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_cell_keys, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, x, n, o[0], o[1], o[2], dx, res, tiled,
+                     keys, flags, (int32_t*)nullptr);
+  BK_LAUNCH(k_cell_keys);
+  return 0;
+}
+
+int bk_sort_pairs(bk_workspace* ws, const uint32_t* keys_in, const int32_t* values_in, uint32_t* keys_out, int32_t* values_out,
+                  int32_t n, int32_t key_bits, void* stream) {
+  const bk_workspace* h = host_view(ws);
+  if (!h) return -2;
+  if (n < 0 || n > h->max_n || key_bits < 1 || key_bits > 32) return fail("bk_sort_pairs: bad argument");
+  return radix_sort(h, keys_in, values_in, keys_out, values_out, n, key_bits, (hipStream_t)stream);
+}
+
+int bk_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g, bk_workspace* ws,
+               void* stream) {
+  const bk_workspace* h = host_view(ws);
+  if (!h) return -2;
+  if (!P || !sc || !in || !out || !g) return fail("bk_substep: null argument");
+  if (P->n < 1 || P->n > h->max_n || P->res != h->res) return fail("bk_substep: particle count or resolution exceeds workspace");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = P->n;
+  const int nb = h->nb;
+  const int thermal = P->transport_enabled;
+  const int cap_blocks = 4096;
+  Profiler* pr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    auto it = g_prof.find(ws);
+    if (it != g_prof.end() && it->second.step < it->second.max_steps) pr = &it->second;
+  }
+  // 1. clear the previous step's blocks (counters[1] still holds their count)
+  { StageTimer tm(pr, st, kStageClear);
+  hipLaunchKernelGGL(k_clear_blocks, dim3(cap_blocks), dim3(256), 0, st, h->active, h->counters, h->node_flag, *g, thermal);
+  BK_LAUNCH(clear); }
+  // 2. keys (also resets the tile/active counters)
+  { StageTimer tm(pr, st, kStageKeys);
+  hipLaunchKernelGGL(k_cell_keys, dim3(div_up(n, 256)), dim3(256), 0, st, in->x, n, P->origin[0], P->origin[1], P->origin[2], P->dx,
+                     P->res, 1, h->keys_raw, h->flags, h->counters);
+  BK_LAUNCH(keys); }
+  // 3. stable sort by tiled cell key
+  { StageTimer tm(pr, st, kStageSort);
+    int rc = radix_sort(h, h->keys_raw, nullptr, h->sorted_keys, h->sorted_vals, n, tiled_key_bits(h), st);
+    if (rc) return rc;
+  }
+  // 4. tiles and active blocks
+  { StageTimer tm(pr, st, kStageTiles);
+  hipLaunchKernelGGL(k_tile_bounds, dim3(div_up(n, 256)), dim3(256), 0, st, h->sorted_keys, n, h->blk_start, h->blk_end, h->tiles,
+                     h->counters);
+  hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)h->nblocks * 8, 256), 1, 2048)), dim3(256), 0, st, h->tiles, h->counters,
+                     h->node_flag, h->active, nb);
+  BK_LAUNCH(tiles); }
+  // 5. mesh reconstruction and surface areas
+  if (thermal) {
+    { StageTimer tm(pr, st, kStageMesh);
+    if (P->n_vertices > 0)
+      hipLaunchKernelGGL(k_reconstruct, dim3(div_up(P->n_vertices, 4)), dim3(256), 0, st, in->x, in->F, sc->ref_vertices,
+                         sc->bind_offsets, sc->bind_particles, sc->bind_weights, sc->bind_ref_offsets, h->vertices, P->n_vertices);
+    if (P->n_slots > 0) {
+      float floor_h = P->origin[1] + float(P->boundary_cells - 1) * P->dx;
+      hipLaunchKernelGGL(k_slot_areas, dim3(div_up(P->n_slots, 256)), dim3(256), 0, st, h->vertices, sc->faces, sc->slot_face_offsets,
+                         sc->slot_faces, h->slot_exposed, h->slot_contact, P->n_slots, floor_h, P->contact_distance);
+    }
+    BK_LAUNCH(mesh); }
+    // 6. thermal P2G, 7. diffusion
+    { StageTimer tm(pr, st, kStagePassA);
+    hipLaunchKernelGGL(k_pass_a, dim3(cap_blocks), dim3(kTileThreads), 0, st, *P, *sc, *in, *g, ws);
+    BK_LAUNCH(pass_a); }
+    { StageTimer tm(pr, st, kStageDiffuse);
+    hipLaunchKernelGGL(k_diffuse, dim3(cap_blocks), dim3(256), 0, st, *P, *g, ws);
+    BK_LAUNCH(diffuse); }
+  }
+  // 8. scalar G2P, gas and water stress
+  { StageTimer tm(pr, st, kStagePassB1);
+  hipLaunchKernelGGL(k_pass_b1, dim3(div_up(n, 256)), dim3(256), 0, st, *P, *sc, *in, *out, *g, ws);
+  BK_LAUNCH(pass_b1); }
+  // 9. mechanical P2G, 10. grid update, 11. G2P + material
+  { StageTimer tm(pr, st, kStagePassB2);
+  hipLaunchKernelGGL(k_pass_b2, dim3(cap_blocks), dim3(kTileThreads), 0, st, *P, *sc, *in, *out, *g, ws);
+  BK_LAUNCH(pass_b2); }
+  { StageTimer tm(pr, st, kStageGrid);
+  hipLaunchKernelGGL(k_grid_update, dim3(cap_blocks), dim3(256), 0, st, *P, *g, ws);
+  BK_LAUNCH(grid_update); }
+  { StageTimer tm(pr, st, kStagePassC);
+  hipLaunchKernelGGL(k_pass_c, dim3(div_up(n, 256)), dim3(256), 0, st, *P, *sc, *in, *out, *g, ws);
+  BK_LAUNCH(pass_c); }
+  if (pr) ++pr->step;
+  return 0;
+}
+
+int bk_grid_untile(const float* tiled, float* dense, int32_t res, int32_t comps, void* stream) {
+  long long total = (long long)res * res * res;
+  hipLaunchKernelGGL(k_untile_f32, dim3(div_up(total, 256)), dim3(256), 0, (hipStream_t)stream, tiled, dense, res, comps);
+  BK_LAUNCH(untile);
+  return 0;
+}
+int bk_grid_untile_f64(const double* tiled, double* dense, int32_t res, void* stream) {
+  long long total = (long long)res * res * res;
+  hipLaunchKernelGGL(k_untile_f64, dim3(div_up(total, 256)), dim3(256), 0, (hipStream_t)stream, tiled, dense, res);
+  BK_LAUNCH(untile);
+  return 0;
+}
+
+}  // extern "C"

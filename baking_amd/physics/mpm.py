@@ -1,0 +1,460 @@
+"""MLS-MPM baking simulator driver (reference ``physics/mpm.py:17-340``).
+
+Same constructor, methods and attributes as the reference's ``MpmSimulator``;
+the substep itself is one call of the fused gfx950 pipeline ``bk_substep``
+(transport pre-pass, stresses, P2G, grid update, G2P + Ding return map + gas
+refresh + saturations) on preallocated ping-pong state buffers.
+
+Error semantics follow the reference:
+
+* ``Step()`` reads the device flag block once per substep and raises the
+  transport or Ding error before the new state is accepted
+  (``physics/mpm.py:205-207``); J and stencil flags are raised by
+  ``CheckState`` (``physics/mpm.py:243-248``).
+* ``Advance(n)`` runs the n substeps without host synchronisation, then reads
+  the flags once.  If any transport/Ding flag is set it restores the state it
+  started from and replays substep by substep, so the error is raised at the
+  same substep, with the same accepted state, as ``Step()`` would.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from .. import _native
+from ..geometry.mesh_tracking import ParticleMeshBinding
+from ..geometry.mesh_validation import ComputeSignedVolumes
+from .grid import GridState
+from .materials.ding import DingMaterial, DingState
+from .materials.elastic import ElasticMaterial
+from .state import AllocateState, CreateParticleState
+from .thermal_water import ThermalWaterTransport, TransportState
+
+_FLAG_J, _FLAG_STENCIL, _FLAG_POROSITY, _FLAG_COUPLED, _FLAG_GAS, _FLAG_HEAT, _FLAG_WATER = range(7)
+
+
+def _resolve_device(name):
+    if name == "cpu":
+        raise RuntimeError("baking_amd has no CPU substep: use device_name 'auto' or 'cuda[:N]' on a ROCm GPU "
+                           "(the CPU restatement in oracle/ is test infrastructure only)")
+    _native.require_gpu()
+    if name == "auto":
+        return "cuda:0"
+    return name if ":" in name else "cuda:0"
+
+
+class MpmSimulator:
+    """Forward-mode MLS-MPM simulator on one ROCm GPU; history mode is not provided by the fused path."""
+
+    def __init__(self, tet_mesh, particle_samples, simulation_config, material_config, initial_velocity=(0.0, 0.0, 0.0),
+                 retain_history=False, ding_config=None, transport_config=None):
+        simulation_config.Validate(material_config)
+        if retain_history:
+            raise NotImplementedError("retain_history: use baking_amd.adjoint for gradients; the fused substep is forward-only")
+        self.device_name = _resolve_device(simulation_config.device_name)
+        torch.cuda.set_device(torch.device(self.device_name))
+        _native.load()
+        self.tet_mesh = tet_mesh
+        self.particle_samples = particle_samples
+        self.simulation_config = simulation_config
+        self.material = ElasticMaterial(material_config)
+        if material_config.model == "ding2019":
+            from ..config import DingConfig
+            self.material = DingMaterial(material_config, ding_config or DingConfig(), self.device_name, retain_history)
+        if isinstance(self.material, DingMaterial):
+            self.material.simulation_config = simulation_config
+        if (isinstance(self.material, DingMaterial) and self.material.ding_config.gas_enabled
+                and not (transport_config and transport_config.enabled)):
+            raise ValueError("Ding gas simulation requires transport.enabled=true; prescribed temperature schedules were removed")
+        self.initial_velocity = initial_velocity
+        self.retain_history = retain_history
+        self.ValidateInitialDomain()
+        self.mesh_binding = ParticleMeshBinding(tet_mesh, particle_samples, self.device_name)
+        self.transport = None
+        if transport_config is not None and transport_config.enabled:
+            transport_config.Validate(material_config)
+            if transport_config.settling_duration_s > 0:
+                raise NotImplementedError("settling phases are not provided by this build (settling_duration_s must be 0)")
+            self.material.initial_temperature_k = transport_config.initial_temperature_k
+            self.transport = ThermalWaterTransport(self, transport_config)
+        self._workspace = None
+        self.Reset()
+
+    # ------------------------------------------------------------------ properties kept from the reference
+    @property
+    def settling_steps(self):
+        return 0
+
+    @property
+    def IsSettling(self):
+        return False
+
+    @property
+    def baking_steps(self):
+        return self.step_count
+
+    @property
+    def baking_time(self):
+        return self.baking_steps * self.simulation_config.time_step
+
+    def Masses(self, state=None):
+        state = self.current_state if state is None else state
+        return state.transport_state.particle_masses if state.transport_state is not None else self.particle_data.particle_masses
+
+    def ValidateInitialDomain(self):
+        sim = self.simulation_config
+        origin = np.asarray(sim.grid_origin)
+        lower = origin + (sim.boundary_cells - 1) * sim.grid_spacing
+        upper = origin + (sim.grid_resolution - sim.boundary_cells) * sim.grid_spacing
+        points = np.concatenate([self.tet_mesh.vertex_positions, self.particle_samples.particle_positions])
+        if np.any(points < lower) or np.any(points > upper):
+            raise ValueError(f"model must initially fit inside {lower} .. {upper}; "
+                             "adjust [mesh] mesh_scale/mesh_translation or [simulation] grid settings in the config file")
+
+    # ------------------------------------------------------------------ buffers
+    def _new_state(self):
+        """A full set of particle buffers (x, v, F, C + Ding + transport) for the ping-pong pair."""
+        state = AllocateState(self.particle_data.particle_count, self.device_name)
+        if isinstance(self.material, DingMaterial):
+            self.material.InitializeState(state)
+        if self.transport is not None:
+            self.transport.InitializeState(state)
+        return state
+
+    def Reset(self):
+        if isinstance(self.material, DingMaterial):
+            self.material.Reset()
+        self.current_state, self.particle_data = CreateParticleState(self.particle_samples, self.device_name,
+                                                                     self.initial_velocity, self.retain_history)
+        if isinstance(self.material, DingMaterial):
+            self.material.InitializeState(self.current_state)
+        if self.transport is not None:
+            self.transport.InitializeState(self.current_state)
+        self.next_state = self._new_state()
+        self.grid_state = GridState(self.simulation_config, self.device_name, self.retain_history)
+        n = self.particle_data.particle_count
+        self.kirchhoff_stresses = torch.zeros(n, 3, 3, dtype=torch.float32, device=self.device_name)
+        self.vertex_positions = torch.as_tensor(self.tet_mesh.vertex_positions.astype(np.float32), device=self.device_name)
+        self.error_flags = torch.zeros(2, dtype=torch.int32, device=self.device_name)
+        self.step_count = 0
+        self._baking_start_step = 0
+        self.settling_reason = "disabled"
+        self.settling_diagnostics = dict(max_speed_m_s=0.0, max_strain_rate_per_s=0.0, max_plastic_rate_per_s=0.0)
+        self.mesh_step_count = 0
+        self._build_native()
+        if getattr(self, "result_recorder", None) is not None:
+            self.result_recorder.Reset()
+
+    def _build_native(self):
+        lib = _native.load()
+        if self._workspace is not None:
+            lib.bk_workspace_destroy(self._workspace)
+            self._workspace = None
+        sim = self.simulation_config
+        n = self.particle_data.particle_count
+        dev = self.device_name
+        nv = len(self.tet_mesh.vertex_positions) if self.transport is not None else 0
+        nslots = int(len(self.transport.slots["slot_owner"])) if self.transport is not None else 0
+        ws = lib.bk_workspace_create(n, sim.grid_resolution, nv, nslots)
+        if not ws:
+            _native.check(-2, "bk_workspace_create")
+        self._workspace = ctypes.c_void_p(ws)
+        flags_ptr = lib.bk_workspace_flags(self._workspace)
+        self._ws_flags = flags_ptr
+        self._flags_host = torch.zeros(8, dtype=torch.int32).pin_memory()
+        tiled = self.grid_state.tiled_count
+        if self.transport is not None:
+            self.thermal_grid = {k: torch.zeros(tiled, dtype=torch.float64, device=dev)
+                                 for k in ("V", "C", "E", "W", "H", "L", "X", "dT", "dW")}
+        else:
+            self.thermal_grid = {}
+        # static scene
+        keep = {}
+        keep["V0"] = self.particle_data.reference_volumes
+        keep["m0"] = self.particle_data.particle_masses
+        if isinstance(self.material, DingMaterial):
+            keep["visc_param"] = self.material.viscosity_parameter
+            keep["prod_param"] = self.material.production_parameter
+        if self.transport is not None:
+            t = self.transport
+            keep.update(capacities=t.capacities, dry=t.dry_masses, heat_param=t.heat_transfer_parameter,
+                        ref_vertices=self.mesh_binding.reference_vertices, bind_offsets=self.mesh_binding.binding_offsets,
+                        bind_particles=self.mesh_binding.binding_particles, bind_weights=self.mesh_binding.binding_weights,
+                        bind_ref_offsets=self.mesh_binding.reference_offsets, faces=t.faces,
+                        slot_face_offsets=t.slots["slot_face_offsets"], slot_faces=t.slots["slot_faces"],
+                        slot_owner=t.slots["slot_owner"], particle_slot=t.slots["particle_slot"])
+        self._scene_tensors = keep
+        scene = _native.BkScene()
+        for name in _native.SCENE_FIELDS:
+            setattr(scene, name, keep[name].data_ptr() if name in keep else None)
+        self._scene = scene
+        grid = _native.BkGrid()
+        grid.m = self.grid_state.tiled_masses.data_ptr()
+        grid.p = self.grid_state.tiled_momenta.data_ptr()
+        grid.v = self.grid_state.tiled_velocities.data_ptr()
+        for k, v in self.thermal_grid.items():
+            setattr(grid, k, v.data_ptr())
+        self._grid = grid
+        self._params = self._make_params(settling=False)
+
+    def __del__(self):
+        try:
+            if self._workspace is not None:
+                _native.load().bk_workspace_destroy(self._workspace)
+        except Exception:
+            pass
+
+    def _make_params(self, settling):
+        sim, mat = self.simulation_config, self.material.material_config
+        P = _native.BkParams()
+        P.origin = (ctypes.c_float * 3)(*sim.grid_origin)
+        P.dx = sim.grid_spacing
+        P.res = sim.grid_resolution
+        P.boundary_cells = sim.boundary_cells
+        P.gravity = (ctypes.c_float * 3)(*sim.gravity)
+        P.dt = sim.time_step
+        P.cfl_limit = sim.cfl_limit
+        P.mu, P.lam, P.density = mat.shear_modulus, mat.lame_lambda, mat.density
+        P.model = 1 if isinstance(self.material, DingMaterial) else 0
+        if P.model == 1:
+            c = self.material.ding_config
+            P.yield_stress, P.n0 = c.yield_stress, c.porosity
+            P.temperature_enabled, P.gas_enabled = int(c.temperature_enabled), int(c.gas_enabled)
+            P.knots = (ctypes.c_float * 4)(*c.temperature_knots_k)
+            P.yield_factors = (ctypes.c_float * 4)(*c.yield_factors)
+            P.viscosity_factors = (ctypes.c_float * 4)(*c.viscosity_factors)
+            P.p_ambient, P.gas_peak, P.gas_width = c.gas_ambient_pressure_pa, c.gas_reaction_temperature_k, c.gas_reaction_width_k
+        if self.transport is not None:
+            t = self.transport.config
+            P.transport_enabled, P.water_enabled, P.settling = 1, int(t.water_enabled), int(settling)
+            P.T_oven, P.T_tray = t.oven_temperature_k, t.tray_temperature_k
+            P.tray_h = 0.0 if settling else t.tray_heat_transfer
+            P.loss_coef = 0.0 if settling else t.surface_water_loss
+            P.rho_w, P.latent = t.water_density, t.latent_heat
+            P.kappa = 0.0 if settling else t.conductivity
+            P.diffusivity = 0.0 if settling else t.water_diffusivity
+            P.s0, P.water_stress, P.contact_distance = t.initial_saturation, t.water_stress, t.contact_distance
+            P.n_vertices = len(self.tet_mesh.vertex_positions)
+            P.n_faces = len(self.transport.faces)
+            P.n_slots = len(self.transport.slots["slot_owner"])
+        P.n = self.particle_data.particle_count
+        return P
+
+    @staticmethod
+    def _state_struct(state):
+        s = _native.BkState()
+        s.x = state.particle_positions.data_ptr()
+        s.v = state.particle_velocities.data_ptr()
+        s.F = state.deformation_gradients.data_ptr()
+        s.C = state.affine_velocities.data_ptr()
+        m = state.material_state
+        if m is not None:
+            for field, name in (("Fe", "elastic_gradients"), ("Fp", "plastic_gradients"), ("tau", "kirchhoff_stresses"),
+                                ("residual", "return_residuals"), ("porosity", "porosities"), ("T_mat", "temperatures"),
+                                ("gelled", "gelled"), ("yield_stress", "yield_stresses"), ("viscosity", "viscosities"),
+                                ("porosity_residual", "porosity_residuals"), ("density", "reference_densities"),
+                                ("moles", "gas_moles"), ("pressure", "gas_pressures")):
+                t = getattr(m, name)
+                if not t.is_contiguous():
+                    raise ValueError(f"material_state.{name} must be contiguous")
+                setattr(s, field, t.data_ptr())
+        t = state.transport_state
+        if t is not None:
+            for field, name in (("T", "temperatures"), ("W", "water_masses"), ("s", "saturations"), ("losses", "water_losses"),
+                                ("exchanges", "heat_exchanges"), ("masses", "particle_masses")):
+                setattr(s, field, getattr(t, name).data_ptr())
+        return s
+
+    # ------------------------------------------------------------------ substep
+    def _launch_substep(self, stream=None):
+        lib = _native.load()
+        s_in = self._state_struct(self.current_state)
+        s_out = self._state_struct(self.next_state)
+        rc = lib.bk_substep(ctypes.byref(self._params), ctypes.byref(self._scene), ctypes.byref(s_in), ctypes.byref(s_out),
+                            ctypes.byref(self._grid), self._workspace, _native.stream_ptr(stream))
+        _native.check(rc, "bk_substep")
+
+    def _accept(self):
+        self.current_state, self.next_state = self.next_state, self.current_state
+        self.step_count += 1
+
+    def _read_flags(self):
+        """Blocking read of the 8-int device flag block (waits for the substeps already enqueued)."""
+        lib = _native.load()
+        _native.check(lib.bk_workspace_read_flags(self._workspace, ctypes.c_void_p(self._flags_host.data_ptr()),
+                                                  _native.stream_ptr()), "bk_workspace_read_flags")
+        return self._flags_host.tolist()
+
+    def _write_flags(self, values):
+        lib = _native.load()
+        arr = (ctypes.c_int32 * 8)(*values)
+        _native.check(lib.bk_workspace_write_flags(self._workspace, ctypes.cast(arr, ctypes.c_void_p), _native.stream_ptr()),
+                      "bk_workspace_write_flags")
+
+    def _raise_for(self, flags):
+        if flags[_FLAG_HEAT] or flags[_FLAG_WATER]:
+            raise RuntimeError('Heat/water explicit stability bound exceeded; reduce time_step; '
+                               'no temperature or water clipping is applied')
+        if isinstance(self.material, DingMaterial):
+            self.material.RaiseFor([flags[_FLAG_POROSITY], flags[_FLAG_COUPLED], flags[_FLAG_GAS]])
+
+    def _sync_api_flags(self, flags):
+        self.error_flags.copy_(torch.tensor(flags[:2], dtype=torch.int32))
+        if isinstance(self.material, DingMaterial):
+            self.material.error_flags.copy_(torch.tensor(flags[2:5], dtype=torch.int32))
+        if self.transport is not None:
+            self.transport.error_flags.copy_(torch.tensor(flags[5:7], dtype=torch.int32))
+
+    def Step(self):
+        self._launch_substep()
+        flags = self._read_flags()
+        self._sync_api_flags(flags)
+        self._raise_for(flags)
+        self._accept()
+        if getattr(self, "result_recorder", None) is not None:
+            self.result_recorder.Record()
+
+    def Advance(self, step_count, check=True):
+        """Run ``step_count`` substeps with one flag read at the end, then ``CheckState``."""
+        if step_count < 1:
+            raise ValueError("step_count must be positive")
+        if getattr(self, "result_recorder", None) is not None:
+            for _ in range(step_count):
+                self.Step()
+            return self.CheckState() if check else None
+        flags_before = self._read_flags()
+        start_state, start_next, start_steps = self.current_state, self.next_state, self.step_count
+        saved = [t.clone() for t in _state_tensors(start_state)]
+        for _ in range(step_count):
+            self._launch_substep()
+            self._accept()
+        flags = self._read_flags()
+        if any(flags[i] and not flags_before[i] for i in range(_FLAG_POROSITY, _FLAG_WATER + 1)):
+            # Replay from the starting state, checking after every substep, so the
+            # error surfaces at the same substep with the same accepted state.
+            self.current_state, self.next_state, self.step_count = start_state, start_next, start_steps
+            for dst, src in zip(_state_tensors(start_state), saved):
+                dst.copy_(src)
+            self._write_flags(flags_before)
+            for _ in range(step_count):
+                self.Step()
+        del saved
+        self._sync_api_flags(self._read_flags())
+        return self.CheckState() if check else None
+
+    def UpdateMesh(self, force_update=False):
+        if force_update or self.mesh_step_count != self.step_count:
+            self.mesh_binding.Reconstruct(self.current_state, self.vertex_positions)
+            self.mesh_step_count = self.step_count
+
+    def CheckState(self):
+        """Host-side health checks and run statistics (physics/mpm.py:243-308)."""
+        flags = self._read_flags()
+        self._sync_api_flags(flags)
+        if flags[_FLAG_J]:
+            raise RuntimeError("nonpositive/nonfinite J; reduce dt or inspect loading and sampling")
+        if flags[_FLAG_STENCIL]:
+            raise RuntimeError("particle stencil left the grid; enlarge the domain or reduce dt")
+        s = self.current_state
+        x = s.particle_positions.cpu().numpy()
+        v = s.particle_velocities.cpu().numpy()
+        F = s.deformation_gradients.cpu().numpy()
+        C = s.affine_velocities.cpu().numpy()
+        if not all(np.isfinite(a).all() for a in (x, v, F, C)):
+            raise RuntimeError("nonfinite particle state; simulation stopped")
+        J = np.linalg.det(F)
+        if J.min() <= 0.0:
+            raise RuntimeError("inverted particle deformation; reduce dt or loading")
+        sim = self.simulation_config
+        g = (x - np.array(sim.grid_origin)) / sim.grid_spacing
+        if np.any(g < 0.5) or np.any(g >= sim.grid_resolution - 1.5):
+            raise RuntimeError("particle outside valid grid stencil")
+        masses = self.Masses().cpu().numpy().astype(np.float64)
+        vmax = float(np.linalg.norm(v, axis=1).max())
+        wave = self.material.material_config.wave_speed
+        if isinstance(self.material, DingMaterial) and self.material.ding_config.gas_enabled:
+            wave = max(wave, self.material.GasWaveSpeed(s))
+        if self.transport is not None and self.transport.config.water_enabled:
+            water = (self.transport.config.water_stress * s.transport_state.saturations.cpu().numpy() * J
+                     / s.material_state.reference_densities.cpu().numpy())
+            wave = float(np.sqrt(wave * wave + water.max()))
+        cfl = (vmax + wave) * sim.time_step / sim.grid_spacing
+        if cfl > sim.cfl_limit:
+            raise RuntimeError(f"estimated CFL={cfl:.3f} > {sim.cfl_limit}; reduce dt")
+        if isinstance(self.material, DingMaterial):
+            self.material.RaiseFor(flags[2:5])
+            m = s.material_state
+            if not torch.isfinite(m.elastic_gradients).all() or not torch.isfinite(m.plastic_gradients).all():
+                raise RuntimeError("nonfinite Ding internal state")
+            if float(m.return_residuals.max()) > 1e-4:
+                raise RuntimeError("Ding local return mapping did not converge")
+        self.UpdateMesh()
+        verts = self.vertex_positions.cpu().numpy()
+        if not np.isfinite(verts).all():
+            raise RuntimeError("nonfinite reconstructed mesh vertices")
+        return {
+            "step_count": self.step_count,
+            "simulation_time": self.step_count * sim.time_step,
+            "particle_count": self.particle_data.particle_count,
+            "particle_volume_m3": float(np.dot(self.particle_samples.reference_volumes, J)),
+            "total_mass": float(masses.sum()),
+            "grid_mass": float(self.grid_state.tiled_masses.double().sum().cpu()),
+            "minimum_j": float(J.min()),
+            "maximum_j": float(J.max()),
+            "maximum_speed": vmax,
+            "kinetic_energy": float(0.5 * np.sum(masses * np.sum(v.astype(np.float64) ** 2, axis=1))),
+            "estimated_cfl": cfl,
+            "unmapped_mesh_vertices": self.mesh_binding.unmapped_vertex_count,
+            "inverted_mesh_tets": int(np.count_nonzero(ComputeSignedVolumes(verts.astype(np.float64),
+                                                                             self.tet_mesh.tet_indices) <= 0.0)),
+        }
+
+    def CaptureSnapshot(self):
+        """Host copy of the full state with the reference's NPZ keys (physics/mpm.py:310-334)."""
+        self.UpdateMesh()
+        s = self.current_state
+        data = dict(
+            particle_positions=s.particle_positions.cpu().numpy().copy(),
+            particle_velocities=s.particle_velocities.cpu().numpy().copy(),
+            deformation_gradients=s.deformation_gradients.cpu().numpy().copy(),
+            affine_velocities=s.affine_velocities.cpu().numpy().copy(),
+            particle_masses=self.Masses().cpu().numpy().copy(),
+            reference_volumes=self.particle_data.reference_volumes.cpu().numpy().copy(),
+            vertices=self.vertex_positions.cpu().numpy().copy(), tetrahedra=self.tet_mesh.tet_indices.copy(),
+            reference_vertices=self.tet_mesh.vertex_positions.copy(),
+            source_tets=self.particle_samples.source_tets.copy(),
+            simulation_time=self.step_count * self.simulation_config.time_step,
+            baking_time_s=self.baking_time, settling_steps=self.settling_steps)
+        if isinstance(self.material, DingMaterial):
+            for name in ("elastic_gradients", "plastic_gradients", "kirchhoff_stresses", "return_residuals", "porosities",
+                         "temperatures", "gelled", "yield_stresses", "viscosities", "gas_moles", "gas_pressures",
+                         "porosity_residuals", "reference_densities"):
+                data["ding_" + name] = getattr(s.material_state, name).cpu().numpy().copy()
+        if s.transport_state is not None:
+            from dataclasses import fields
+            for f in fields(s.transport_state):
+                data["transport_" + f.name] = getattr(s.transport_state, f.name).cpu().numpy().copy()
+        return data
+
+    def SaveSnapshot(self, output_path):
+        self.UpdateMesh()
+        output_path = Path(output_path)
+        output_path.parent.mkdir(parents=True, exist_ok=True)
+        np.savez_compressed(output_path, **self.CaptureSnapshot())
+
+
+def _state_tensors(state):
+    out = [state.particle_positions, state.particle_velocities, state.deformation_gradients, state.affine_velocities]
+    if state.material_state is not None:
+        from dataclasses import fields
+        out += [getattr(state.material_state, f.name) for f in fields(state.material_state)]
+    if state.transport_state is not None:
+        from dataclasses import fields
+        out += [getattr(state.transport_state, f.name) for f in fields(state.transport_state)]
+    return out
+
+

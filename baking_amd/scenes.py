@@ -1,0 +1,227 @@
+"""Synthetic dough scenes: one jittered-seeded dough plus the tet mesh its transport needs.
+
+Workloads (all SI units):
+
+* ``sphere_small``   -- sphere r = 0.1 at (0.5, 0.5, 0.3), 64^3 grid (dx = 1/64), 8 particles per cell
+  (2x2x2 jittered, seed 0) inside the sphere (~8.8k particles), dt = 1e-4, oven 453.15 K.
+* ``loaf_256``       -- box 0.30 x 0.15 x 0.12 on a 256^3 grid (dx = 1/256), 8 particles per cell (~0.72M),
+  moisture 0.4 + 0.05 * Gaussian random field (correlation length 0.02, seed 1), porosity U[0.03, 0.07],
+  oven 473.15 K, dt = 5e-5.
+
+Parameters the workload text leaves open are taken from the reference's
+example scene (``examples/oven_baking.txt``) and recorded in ``SceneSpec``.
+Seeding: x = dx (c + (s + 1/2 + u) / 2), s in {0,1}^3, u ~ U(-1/2, 1/2)^3 drawn
+cell by cell in row-major cell order from ``default_rng(seed)``; particles
+outside the body are dropped.  Every particle is assigned to the tet that
+contains it (or the nearest one for particles just outside the faceted
+surface), and tets left empty take the nearest particle from a tet that has
+several, so mesh reconstruction has a binding for every tet.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field, replace
+
+import numpy as np
+
+from .config import DingConfig, MaterialConfig, SimulationConfig, TransportConfig
+from .geometry.mesh_loader import CreateSphereMesh, VoxelTetMesh
+from .geometry.mesh_validation import ValidateTetMesh
+from .geometry.particle_sampling import ParticleSamples
+from .physics.materials.gas import gas_constant
+
+
+@dataclass
+class SceneSpec:
+    name: str
+    tet_mesh: object
+    samples: ParticleSamples
+    simulation: SimulationConfig
+    material: MaterialConfig
+    ding: DingConfig
+    transport: TransportConfig
+    steps: int
+    initial_velocity: tuple = (0.0, 0.0, 0.0)
+    # per-particle initial fields assigned after construction (None = config defaults)
+    water_masses: np.ndarray | None = None
+    porosities: np.ndarray | None = None
+    gas_moles: np.ndarray | None = None
+    notes: dict = field(default_factory=dict)
+
+
+def jittered_cells(inside, lo, hi, dx, seed):
+    """8 jittered particles per cell (2x2x2) for cells overlapping [lo, hi]; keep those with inside(x)."""
+    c0 = np.floor(np.asarray(lo) / dx).astype(int)
+    c1 = np.ceil(np.asarray(hi) / dx).astype(int)
+    shape = c1 - c0
+    cells = np.stack(np.meshgrid(*[np.arange(a, b) for a, b in zip(c0, c1)], indexing="ij"), axis=-1).reshape(-1, 3)
+    sub = np.stack(np.meshgrid([0, 1], [0, 1], [0, 1], indexing="ij"), axis=-1).reshape(-1, 3)
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(-0.5, 0.5, size=(len(cells), 8, 3))
+    pos = dx * (cells[:, None, :] + (sub[None, :, :] + 0.5 + u) / 2.0)
+    pos = pos.reshape(-1, 3)
+    keep = inside(pos)
+    del shape
+    return pos[keep]
+
+
+def assign_tets(tet_mesh, positions, k=12):
+    """Containing tet for every point (nearest by barycentric margin when outside), then fill empty tets."""
+    from scipy.spatial import cKDTree
+    verts = tet_mesh.vertex_positions
+    tets = tet_mesh.tet_indices
+    corners = verts[tets]
+    origin = corners[:, 0]
+    edges = np.transpose(corners[:, 1:] - origin[:, None, :], (0, 2, 1))  # (T,3,3) columns = edges
+    inv = np.linalg.inv(edges)
+    centroids = corners.mean(axis=1)
+    tree = cKDTree(centroids)
+    k = min(k, len(tets))
+    _, cand = tree.query(positions, k=k)
+    cand = cand.reshape(len(positions), k)
+    rel = positions[:, None, :] - origin[cand]
+    lam = np.einsum("nkij,nkj->nki", inv[cand], rel)
+    bary = np.concatenate([1.0 - lam.sum(axis=2, keepdims=True), lam], axis=2)
+    margin = bary.min(axis=2)
+    best = cand[np.arange(len(positions)), np.argmax(margin, axis=1)]
+    source = best.astype(np.int64)
+    counts = np.bincount(source, minlength=len(tets))
+    empty = np.flatnonzero(counts == 0)
+    if len(empty):
+        ptree = cKDTree(positions)
+        for t in empty:
+            _, near = ptree.query(centroids[t], k=min(64, len(positions)))
+            for p in np.atleast_1d(near):
+                if counts[source[p]] >= 2:
+                    counts[source[p]] -= 1
+                    source[p] = t
+                    counts[t] += 1
+                    break
+            else:
+                raise ValueError("could not give every tet a particle; refine seeding or coarsen the mesh")
+    return source
+
+
+def make_samples(tet_mesh, positions, dx, density):
+    """Particles of volume dx^3 / 8 and mass density * volume, tagged with their source tet."""
+    n = len(positions)
+    volumes = np.full(n, dx ** 3 / 8.0)
+    source = assign_tets(tet_mesh, positions)
+    return ParticleSamples(positions, volumes, density * volumes, source)
+
+
+def _oven_material(young=1.25e5, poisson=0.485):
+    return MaterialConfig(density=1150.0, young_modulus=young, poisson_ratio=poisson, model="ding2019")
+
+
+def _oven_ding(porosity, p0):
+    return DingConfig(yield_stress=810.82, viscosity=1.5e6, porosity=porosity, temperature_enabled=True,
+                      temperature_knots_k=(278.15, 300.15, 328.15, 353.15), yield_factors=(1.5, 1.0, 0.5, 3.0),
+                      viscosity_factors=(1.5, 1.0, 0.5, 5.0), gas_enabled=True, gas_initial_pressure_pa=p0,
+                      gas_ambient_pressure_pa=p0, gas_production_rate=2.04e-5, gas_reaction_temperature_k=318.15,
+                      gas_reaction_width_k=15.42)
+
+
+def _oven_transport(T0, T_oven, s0=0.4):
+    return TransportConfig(enabled=True, water_enabled=True, initial_temperature_k=T0, oven_temperature_k=T_oven,
+                           conductivity=0.36, specific_heat=2830.0, heat_transfer=35.0, tray_temperature_k=T_oven,
+                           tray_heat_transfer=100.0, contact_distance=0.001, initial_saturation=s0, water_density=1000.0,
+                           water_diffusivity=1.5e-10, surface_water_loss=5.0e-7, water_stress=10000.0, latent_heat=2.26e6)
+
+
+def sphere_small(seed=0, mesh_subdivisions=6):
+    """The small plain scene: sphere r = 0.1 at (0.5, 0.5, 0.3), 64^3 grid, ~8.8k particles."""
+    res, dx = 64, 1.0 / 64.0
+    centre, radius = np.array([0.5, 0.5, 0.3]), 0.1
+    sphere = CreateSphereMesh(radius, mesh_subdivisions)
+    mesh = ValidateTetMesh(sphere.vertex_positions + centre, sphere.tet_indices)
+    pos = jittered_cells(lambda p: np.linalg.norm(p - centre, axis=1) <= radius, centre - radius, centre + radius, dx, seed)
+    material = _oven_material()
+    samples = make_samples(mesh, pos, dx, material.density)
+    sim = SimulationConfig(grid_resolution=res, grid_spacing=dx, grid_origin=(0.0, 0.0, 0.0), time_step=1.0e-4,
+                           gravity=(0.0, -9.8, 0.0), boundary_cells=3, device_name="auto")
+    T0 = 298.15
+    return SceneSpec("sphere_small", mesh, samples, sim, material, _oven_ding(0.05, 101325.0),
+                     _oven_transport(T0, 453.15), steps=1000,
+                     notes=dict(seed=seed, mesh=f"CreateSphereMesh(0.1, {mesh_subdivisions}) + (0.5, 0.5, 0.3)",
+                                parameters="examples/oven_baking.txt material/ding/transport; n0 = 0.05, T0 = 298.15 K"))
+
+
+def gaussian_random_field(shape, spacing, correlation_length, rng):
+    """Unit-variance periodic Gaussian random field: white noise filtered by exp(-k^2 l^2 / 4) in Fourier space."""
+    noise = rng.standard_normal(shape)
+    spec = np.fft.rfftn(noise)
+    ks = [np.fft.fftfreq(n, d=spacing) * 2 * np.pi for n in shape[:-1]] + [np.fft.rfftfreq(shape[-1], d=spacing) * 2 * np.pi]
+    k2 = sum(np.meshgrid(*[k ** 2 for k in ks], indexing="ij"))
+    field = np.fft.irfftn(spec * np.exp(-k2 * correlation_length ** 2 / 4.0), s=shape)
+    field -= field.mean()
+    return field / field.std()
+
+
+def trilinear(field, origin, spacing, points):
+    g = (points - origin) / spacing
+    i0 = np.clip(np.floor(g).astype(int), 0, np.array(field.shape) - 2)
+    f = g - i0
+    out = np.zeros(len(points))
+    for dx_ in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                w = (f[:, 0] if dx_ else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1]) * (f[:, 2] if dz else 1 - f[:, 2])
+                out += w * field[i0[:, 0] + dx_, i0[:, 1] + dy, i0[:, 2] + dz]
+    return out
+
+
+def loaf_256(seed=0, field_seed=1, voxel=0.015, p0=1000.0):
+    """The scaled plain scene: 0.30 x 0.15 x 0.12 loaf on a 256^3 grid, ~0.72M particles.
+
+    The pore-gas pressure p0 = ambient = 1000 Pa keeps the gas-acoustic CFL
+    below 0.4 at dx = 1/256, dt = 5e-5 (atmospheric pore gas would trip it on
+    the first substep); Poisson's ratio 0.45 keeps the elastic CFL below 0.4.
+    """
+    res, dx = 256, 1.0 / 256.0
+    size = np.array([0.30, 0.15, 0.12])
+    floor = (3 - 1) * dx
+    lo = np.array([0.5 - size[0] / 2, floor + 1.0e-4, 0.5 - size[2] / 2])
+    hi = lo + size
+    pos = jittered_cells(lambda p: np.all((p >= lo) & (p <= hi), axis=1), lo, hi, dx, seed)
+    shape = np.round(size / voxel).astype(int)
+    verts, tets = VoxelTetMesh(shape, voxel, lo)
+    verts = lo + (verts - lo) * (size / (shape * voxel))
+    mesh = ValidateTetMesh(verts, tets)
+    material = _oven_material(poisson=0.45)
+    samples = make_samples(mesh, pos, dx, material.density)
+    rng = np.random.default_rng(field_seed)
+    margin = 0.06
+    g_lo = lo - margin
+    g_shape = tuple(int(v) for v in np.ceil((size + 2 * margin) / dx) + 2)
+    grf = gaussian_random_field(g_shape, dx, 0.02, rng)
+    s = 0.4 + 0.05 * trilinear(grf, g_lo, dx, pos)
+    porosity = rng.uniform(0.03, 0.07, size=len(pos))
+    T0 = 298.15
+    water = 1000.0 * s * samples.reference_volumes
+    moles = p0 * porosity / (gas_constant * T0)
+    sim = SimulationConfig(grid_resolution=res, grid_spacing=dx, grid_origin=(0.0, 0.0, 0.0), time_step=5.0e-5,
+                           gravity=(0.0, -9.8, 0.0), boundary_cells=3, device_name="auto")
+    return SceneSpec("loaf_256", mesh, samples, sim, material, _oven_ding(0.05, p0), _oven_transport(T0, 473.15), steps=2000,
+                     water_masses=water, porosities=porosity.astype(np.float32), gas_moles=moles,
+                     notes=dict(seed=seed, field_seed=field_seed, p0=p0, poisson=0.45, voxel=voxel))
+
+
+def scaled(spec: SceneSpec, **sim_changes):
+    return replace(spec, simulation=replace(spec.simulation, **sim_changes))
+
+
+def build_simulator(spec: SceneSpec):
+    """``MpmSimulator`` for a scene, with its per-particle initial fields applied."""
+    import torch
+    from .physics.mpm import MpmSimulator
+    sim = MpmSimulator(spec.tet_mesh, spec.samples, spec.simulation, spec.material, spec.initial_velocity,
+                       ding_config=spec.ding, transport_config=spec.transport)
+    state = sim.current_state
+    dev = sim.device_name
+    if spec.water_masses is not None:
+        state.transport_state.water_masses.copy_(torch.as_tensor(spec.water_masses, dtype=torch.float64, device=dev))
+    if spec.porosities is not None:
+        state.material_state.porosities.copy_(torch.as_tensor(spec.porosities, dtype=torch.float32, device=dev))
+    if spec.gas_moles is not None:
+        state.material_state.gas_moles.copy_(torch.as_tensor(spec.gas_moles, dtype=torch.float64, device=dev))
+    return sim

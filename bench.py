@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-substeps/s of the fused MLS-MPM baking substep on one MI355X.
+
+Workload (default ``loaf_256``): the 0.30 x 0.15 x 0.12 loaf on a 256^3 grid,
+~0.72M particles, Ding + gas + heat/water transport, FP32 state with the
+reference's FP64 transport, synthetic seeded inputs.  One "step" is one
+substep of the whole particle set.
+
+Prints ONE JSON line with the throughput, a roofline entry for the dominant
+kernel (HIP events on the launch stream, algorithmic bytes at the
+reference's precisions) and a CPU baseline (the serial C++ oracle on a
+bounded sample of the same workload, on this host).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+# Algorithmic bytes of each stage at the reference's precisions: (per particle, per active stencil node).
+# Particle terms count each array the stage must read once and write once; node terms the grid arrays it
+# must read or write once per touched node.  Sort/tiles/clear move no algorithmic bytes (pure overhead).
+STAGE_BYTES = {
+    # x, F, V0, capacity, T, W (f64), exposed+contact (f64) -> 7 f64 scatters per node
+    "pass_a": (12 + 36 + 4 + 8 + 8 + 8 + 16, 56),
+    # reads V, C, E, W, heat, loss, exchange (f64) ; writes dT, dW (f64)
+    "diffuse": (0, 56 + 16),
+    # reads x, F, V0, dry, T, W, losses, exchanges, moles, tau_solid, porosity; writes T, W, losses, exchanges,
+    # masses, density, T_mat, moles; gathers dT, dW, W_node
+    "pass_b1": (12 + 36 + 4 + 8 + 32 + 8 + 36 + 4 + 32 + 4 + 4 + 4 + 8, 24),
+    # reads x, v, C, tau_total, mass, V0; scatters m, p
+    "pass_b2": (12 + 12 + 36 + 36 + 4 + 4, 16),
+    # reads m, p ; writes v
+    "grid": (0, 16 + 12),
+    # reads x, F, Fe_old, porosity, gelled, T_mat, moles, W, V0 ; writes x, v, C, F, Fe, Fp, tau, residual,
+    # porosity, porosity residual, T_mat, gelled, yield, viscosity, pressure, saturation; gathers v
+    "pass_c": (12 + 36 + 36 + 4 + 4 + 4 + 8 + 8 + 4 + 96 + 108 + 8 + 16 + 8 + 8, 12),
+}
+SUBSTEP_BYTES = (552, 208)  # whole substep: particle state read + written once, grid work per active node
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="loaf_256", choices=["loaf_256", "sphere_small"])
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--cpu-baseline-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def build_scene(name):
+    from baking_amd import scenes
+    return scenes.loaf_256() if name == "loaf_256" else scenes.sphere_small()
+
+
+def cpu_baseline(spec, steps):
+    """The serial C++ oracle (CPU restatement of the reference kernels) timed on this host."""
+    from tests.harness import oracle_from_spec
+    sim = oracle_from_spec(spec)
+    sim.Step()  # first substep outside the timing (allocations, page faults)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.Step()
+    elapsed = time.perf_counter() - t0
+    n = sim.n
+    return {"value": n * steps / elapsed, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} substeps (after 1 untimed) of the full {spec.name} scene ({n} particles, "
+                      f"{spec.simulation.grid_resolution}^3 dense grid) from rest, serial oracle (liboracle.so, g++ -O2)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every configuration here fits one MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from baking_amd import _native
+    from baking_amd.scenes import build_simulator
+
+    spec = build_scene(args.workload)
+    sim = build_simulator(spec)
+    n = sim.particle_data.particle_count
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        sim._launch_substep()
+        sim._accept()
+    torch.cuda.synchronize()
+    flags_before = sim._read_flags()
+
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        sim._launch_substep()
+        sim._accept()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(stop)
+    flags = sim._read_flags()
+
+    # per-stage timing with events on the launch stream
+    lib = _native.load()
+    _native.check(lib.bk_profile_begin(sim._workspace, args.profile_steps), "bk_profile_begin")
+    for _ in range(args.profile_steps):
+        sim._launch_substep()
+        sim._accept()
+    nst = lib.bk_profile_stage_count()
+    import ctypes
+    ms = (ctypes.c_float * nst)()
+    rec = ctypes.c_int32(0)
+    _native.check(lib.bk_profile_end(sim._workspace, ctypes.cast(ms, ctypes.c_void_p), ctypes.cast(ctypes.pointer(rec), ctypes.c_void_p)),
+                  "bk_profile_end")
+    stages = {lib.bk_profile_stage_name(k).decode(): ms[k] / max(rec.value, 1) for k in range(nst)}
+    active_nodes = int((sim.grid_state.tiled_masses > 0).sum().item())
+    dominant = max(stages, key=lambda k: stages[k])
+    per_p, per_node = STAGE_BYTES.get(dominant, (0, 0))
+    dom_bytes = per_p * n + per_node * active_nodes
+    dom_ms = stages[dominant]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    substep_ms = gpu_ms / args.steps
+    substep_bytes = SUBSTEP_BYTES[0] * n + SUBSTEP_BYTES[1] * active_nodes
+    yield_fraction = float((sim.current_state.material_state.return_residuals > 0).float().mean().item()) \
+        if sim.current_state.material_state is not None else 0.0
+
+    value = n * args.steps / (gpu_ms * 1e-3)
+    result = {
+        "metric": "particle-substeps/s on 1 MI355X, plain vs surrogate+batched at <=1e-3 rel err",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": substep_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": spec.name, "method": "plain", "particles": n, "grid": spec.simulation.grid_resolution,
+                   "dt": spec.simulation.time_step, "active_stencil_nodes": active_nodes,
+                   "yield_fraction": yield_fraction, "wall_s": wall, "flags": flags, "flags_before_timing": flags_before},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dom_bytes,
+                     "avg_ms": dom_ms, "stages_ms": stages,
+                     "substep_achieved": substep_bytes / (substep_ms * 1e-3) / 1e9,
+                     "substep_frac": substep_bytes / (substep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(spec, args.cpu_baseline_steps)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

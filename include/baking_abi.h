@@ -1,0 +1,228 @@
+/*
+ * baking_abi.h -- C ABI of libbaking_hip.so, the MI355X (gfx950) MLS-MPM
+ * dough-baking substep.
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream
+ * (hipStream_t passed as void*; NULL = the null stream), enqueues its work
+ * and returns without synchronising.  Return codes: 0 on success, a negative
+ * value on a launch or argument error (-1 bad argument, -2 HIP error; call
+ * bk_last_error() for text).  Physical failure conditions are reported the
+ * reference's way, through device int flag arrays with the reference's
+ * indices; the caller reads them where the reference raises.
+ *
+ * Layouts follow the reference's Warp arrays: vec3 = float[3], mat33 =
+ * row-major float[9] per element, wp.float64 = double, int = int32.  The
+ * per-kernel entries take the reference's dense node arrays (node id
+ * (i*res + j)*res + k).  The fused substep keeps its grid "tiled": 4x4x4
+ * node blocks stored contiguously, node id ((bi*nb + bj)*nb + bk)*64 +
+ * (i&3)*16 + (j&3)*4 + (k&3), nb = ceil(res/4).
+ *
+ * Reference = bbbpimasheep/baking (Python + Warp 1.18); each entry names the
+ * Warp kernel / function it replaces (file:line).
+ */
+#ifndef BAKING_ABI_H
+#define BAKING_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BK_ABI_VERSION 1
+
+/* Flag block of the fused substep (one int32[8] array):
+ *   [0] J nonpositive / nonfinite        (reference MpmSimulator.error_flags[0], physics/mpm.py:245)
+ *   [1] particle stencil left the grid   (MpmSimulator.error_flags[1], physics/transfer.py:50-52)
+ *   [2] Ding porosity domain             (DingMaterial.error_flags[0], ding.py:156-158,210-211)
+ *   [3] coupled porosity unresolved      (DingMaterial.error_flags[1], ding.py:193-194)
+ *   [4] gas acoustic CFL                 (DingMaterial.error_flags[2], gas.py:34-35)
+ *   [5] heat explicit bound              (ThermalWaterTransport.error_flags[0], thermal_water.py:184-185)
+ *   [6] water explicit bound             (ThermalWaterTransport.error_flags[1], thermal_water.py:186-188)
+ *   [7] reserved                                                                  */
+enum { BK_FLAG_J = 0, BK_FLAG_STENCIL = 1, BK_FLAG_POROSITY = 2, BK_FLAG_COUPLED = 3, BK_FLAG_GAS_CFL = 4,
+       BK_FLAG_HEAT = 5, BK_FLAG_WATER = 6, BK_FLAG_COUNT = 8 };
+
+/* Scalar parameters of one substep (config.py MaterialConfig/SimulationConfig/DingConfig/TransportConfig). */
+typedef struct bk_params {
+  float origin[3];
+  float dx;
+  int32_t res;
+  int32_t boundary_cells;
+  float gravity[3];
+  float dt;
+  float cfl_limit;
+  int32_t model;              /* 0 = neo_hookean, 1 = ding2019 */
+  float mu, lam, density;     /* shear modulus, Lame lambda, material density */
+  /* Ding (config.py:109-145) */
+  float yield_stress, n0;
+  int32_t temperature_enabled, gas_enabled;
+  float knots[4], yield_factors[4], viscosity_factors[4];
+  float p_ambient, gas_peak, gas_width;
+  /* transport (config.py:148-196) */
+  int32_t transport_enabled, water_enabled, settling;
+  float T_oven, T_tray, tray_h, loss_coef, rho_w, latent, kappa, diffusivity, s0, water_stress, contact_distance;
+  /* sizes */
+  int32_t n, n_vertices, n_faces, n_slots;
+} bk_params;
+
+/* Per-particle state of one step (physics/state.py ParticleState, ding.py DingState, thermal_water.py TransportState). */
+typedef struct bk_state {
+  float *x, *v, *F, *C;                                    /* N x 3, N x 3, N x 9, N x 9 */
+  float *Fe, *Fp, *tau, *residual, *porosity, *T_mat;      /* DingState (f32) */
+  int32_t* gelled;
+  float *yield_stress, *viscosity, *porosity_residual, *density;
+  double *moles, *pressure;                                /* gas (f64) */
+  double *T, *W, *s, *losses, *exchanges;                  /* TransportState (f64) */
+  float* masses;                                           /* TransportState.particle_masses */
+} bk_state;
+
+/* Static scene data (set once per simulator). */
+typedef struct bk_scene {
+  const float *V0, *m0;                      /* reference volumes, initial masses */
+  const double *capacities, *dry;            /* thermal_water.py:272-274 */
+  const float* visc_param;                   /* DingMaterial.viscosity_parameter [1] */
+  const float* prod_param;                   /* DingMaterial.production_parameter [1] */
+  const float* heat_param;                   /* ThermalWaterTransport.heat_transfer_parameter [1] */
+  /* mesh reconstruction CSR (geometry/mesh_tracking.py:41-75) */
+  const float* ref_vertices;                 /* V x 3 */
+  const int32_t *bind_offsets, *bind_particles;
+  const float *bind_weights, *bind_ref_offsets;
+  /* boundary faces (thermal_water.py:51-74) grouped by owner slot */
+  const int32_t* faces;                      /* F x 3 */
+  const int32_t *slot_face_offsets, *slot_faces, *slot_owner, *particle_slot;
+} bk_scene;
+
+/* Tiled grid of the fused substep (GridState + transport workspace). */
+typedef struct bk_grid {
+  float *m, *p, *v;                          /* node mass, momentum (x3), velocity (x3) */
+  double *V, *C, *E, *W, *H, *L, *X;         /* thermal P2G: volume, capacity, energy, water, heat, loss, exchange */
+  double *dT, *dW;                           /* diffusion increments */
+} bk_grid;
+
+typedef struct bk_workspace bk_workspace;
+
+/* ---------------------------------------------------------------- library */
+int bk_abi_version(void);
+const char* bk_last_error(void);
+int bk_device_info(char* name, int name_len, int* cu_count, int* xcd_count);
+
+/* Workspace: the only library-owned device memory (sort scratch, tile and
+ * active-block lists, per-step stress and surface buffers, flags). */
+bk_workspace* bk_workspace_create(int32_t max_particles, int32_t res, int32_t n_vertices, int32_t n_slots);
+void bk_workspace_destroy(bk_workspace* ws);
+int32_t* bk_workspace_flags(bk_workspace* ws);          /* device int32[BK_FLAG_COUNT] */
+float* bk_workspace_vertices(bk_workspace* ws);         /* device float[V*3]: last reconstructed mesh */
+int bk_workspace_counts(bk_workspace* ws, int32_t* out2, void* stream); /* copies {tiles, active blocks}; synchronises */
+/* Blocking copy of the flag block to host int32[8] after the work already on `stream`; and the reverse. */
+int bk_workspace_read_flags(bk_workspace* ws, int32_t* host8, void* stream);
+int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* stream);
+
+/* ---------------------------------------------------------------- sort (build-only; no reference counterpart)
+ * Cell key of every particle: flatten_node(stencil_base((x-o)/dx)) of
+ * physics/transfer.py:16-33 (tiled=0), or the same cell numbered block-major
+ * over 4x4x4 cell blocks (tiled=1).  Off-grid particles get key 0 and set
+ * flags[1]. */
+int bk_cell_keys(const float* x, int32_t n, const float* origin3_host, float dx, int32_t res, int32_t tiled, uint32_t* keys,
+                 int32_t* flags, void* stream);
+/* Stable LSD radix sort of (key, value) pairs on the low key_bits bits; values
+ * = NULL means 0..n-1.  Outputs go to keys_out/values_out. */
+int bk_sort_pairs(bk_workspace* ws, const uint32_t* keys_in, const int32_t* values_in, uint32_t* keys_out, int32_t* values_out,
+                  int32_t n, int32_t key_bits, void* stream);
+
+/* ---------------------------------------------------------------- fused substep
+ * One MpmSimulator.Step (physics/mpm.py:163-225) in forward mode: transport
+ * pre-pass, gas and water stress, P2G, grid update, G2P, Ding return map, gas
+ * refresh and saturations, from state `in` to state `out`.  The tiled grid
+ * keeps this step's values afterwards (inactive blocks are zero). */
+int bk_substep(const bk_params* params, const bk_scene* scene, const bk_state* in, bk_state* out, bk_grid* grid,
+               bk_workspace* ws, void* stream);
+/* Per-stage timing of bk_substep with HIP events on its stream: begin() arms
+ * recording for the next max_steps substeps; end() waits for them and returns
+ * the summed milliseconds per stage (bk_profile_stage_count() entries, named
+ * by bk_profile_stage_name). */
+int bk_profile_begin(bk_workspace* ws, int32_t max_steps);
+int bk_profile_end(bk_workspace* ws, float* ms_per_stage, int32_t* steps_recorded);
+int32_t bk_profile_stage_count(void);
+const char* bk_profile_stage_name(int32_t stage);
+/* Convert the tiled grid's mass/velocity/momentum to the reference's dense layout. */
+int bk_grid_untile(const float* tiled, float* dense, int32_t res, int32_t components, void* stream);
+int bk_grid_untile_f64(const double* tiled, double* dense, int32_t res, void* stream);
+
+/* ---------------------------------------------------------------- per-kernel entries (reference layouts)
+ * Same arguments, order and semantics as the Warp kernel named; n = launch dim. */
+/* physics/transfer.py:36-70 */
+int bk_ParticleToGrid(const float* x, const float* v, const float* C, const float* tau, const float* masses, const float* V0,
+                      float* node_masses, float* node_momenta, const float* origin3_host, float dx, int32_t res, float dt,
+                      int32_t* flags, int32_t n, void* stream);
+/* physics/grid.py:20-38 */
+int bk_UpdateGrid(const float* node_masses, const float* node_momenta, float* node_velocities, int32_t res, int32_t boundary_cells,
+                  const float* gravity3_host, float dt, int64_t node_count, void* stream);
+/* physics/transfer.py:73-109 */
+int bk_GridToParticle(const float* x_in, const float* F_in, const float* node_velocities, float* x_out, float* v_out, float* C_out,
+                      float* F_out, const float* origin3_host, float dx, int32_t res, float dt, int32_t* flags, int32_t n,
+                      void* stream);
+/* physics/materials/elastic.py:13-26 */
+int bk_ComputeElasticStress(const float* F, float* tau, float mu, float lam, int32_t* flags, int32_t n, void* stream);
+/* physics/materials/ding.py:140-221; ding flags = int32[3] */
+int bk_UpdateDing(const float* old_Fe, const float* old_porosity, const int32_t* old_gelled, const float* F_total,
+                  const float* C_affine, float* Fe, float* Fp, float* tau, float* residuals, float* porosity, float* temperatures,
+                  int32_t* gelled, float* yield_stresses, float* viscosities, const float* viscosity_parameter, int32_t* flags,
+                  float* porosity_residuals, int32_t coupled, float mu, float lam, float yield_stress, float n0,
+                  int32_t temperature_enabled, const float* knots4_host, const float* yield4_host, const float* visc4_host,
+                  float dt, const float* local_temperatures, int32_t n, void* stream);
+/* physics/materials/gas.py:16-35 */
+int bk_CombineGasStress(const float* solid, const float* F, const float* porosity, const double* moles, const float* T,
+                        float* total, int32_t* flags, float p_ambient, float n0, const float* density, float elastic_modulus,
+                        float dt, float dx, float cfl, int32_t n, void* stream);
+/* physics/materials/gas.py:38-45 */
+int bk_RefreshGas(const double* old_moles, double* new_moles, const float* F, const float* porosity, double* pressures,
+                  const float* T, int32_t n, void* stream);
+/* physics/thermal_water.py:77-92 */
+int bk_SurfaceAreas(const float* vertices, const int32_t* faces, const int32_t* owners, double* exposed, double* contact,
+                    float floor_height, float contact_distance, int32_t n_faces, void* stream);
+/* physics/thermal_water.py:95-137 */
+int bk_ScalarToGrid(const float* x, const float* F, const float* V0, const double* capacities, const double* T, const double* W,
+                    const double* exposed, const double* contact, const float* heat_parameter, float T_oven, float T_tray,
+                    float tray_h, float loss_coef, float rho_w, float latent, int32_t water_enabled, double* nV, double* nC,
+                    double* nE, double* nW, double* nH, double* nL, double* nX, double* particle_heat, double* particle_loss,
+                    const float* origin3_host, float dx, int32_t res, int32_t n, void* stream);
+/* physics/thermal_water.py:140-147 */
+int bk_GridFields(const double* V, const double* C, const double* E, const double* W, double* T, double* S, float rho_w,
+                  int64_t node_count, void* stream);
+/* physics/thermal_water.py:150-190; errors = int32[2] */
+int bk_DiffuseGrid(const double* V, const double* C, const double* T, const double* S, const double* W, const double* heat,
+                   const double* loss, const double* exchange, double* dT, double* dW, int32_t* errors, float conductivity,
+                   float diffusivity, float rho_w, int32_t water_enabled, float dx, int32_t res, float dt, int64_t node_count,
+                   void* stream);
+/* physics/thermal_water.py:193-229 */
+int bk_ScalarToParticles(const float* x, const double* T_old, const double* W_old, const double* L_old, const double* X_old,
+                         const double* dry, const float* V0, const float* F, const double* node_water, const double* dT,
+                         const double* dW, const double* particle_heat, const double* particle_loss, double* T_new, float* T_mat,
+                         double* W_new, double* L_new, double* X_new, float* m_new, float* densities, double* s_new, float rho_w,
+                         const float* origin3_host, float dx, int32_t res, float dt, int32_t n, void* stream);
+/* physics/thermal_water.py:232-245 */
+int bk_LocalGas(const double* old_moles, const double* T_old, const double* T_new, double* new_moles, double* pressures,
+                const float* F, const float* porosity, const float* parameter, float density, float peak, float width, float dt,
+                int32_t produce, int32_t n, void* stream);
+/* physics/thermal_water.py:248-253 */
+int bk_WaterStress(const float* F, const double* saturations, const float* old_stress, float* new_stress, float s0,
+                   float stiffness, int32_t n, void* stream);
+/* physics/thermal_water.py:256-260 */
+int bk_Saturations(const double* W, const float* V0, const float* F, double* saturations, float rho_w, int32_t n, void* stream);
+/* geometry/mesh_tracking.py:13-38 */
+int bk_ReconstructMeshVertices(const float* x, const float* F, const float* ref_vertices, const int32_t* offsets,
+                               const int32_t* particles, const float* weights, const float* ref_offsets, float* out,
+                               int32_t n_vertices, void* stream);
+/* Scalar functions evaluated on device for arrays of inputs:
+ * physics/materials/gas.py:11-13, ding.py:68-74, porous_return.py:43-61 and 34-40.
+ * bk_solve_porosity: args6 = (q, p, dev_factor, mean_scale, exponent, initial_porosity) per element. */
+int bk_gas_pressure(const double* moles, const float* T, const float* J, const float* porosity, double* out, int32_t n, void* stream);
+int bk_porosity_coefficient(const float* porosity, float* out, int32_t n, void* stream);
+int bk_solve_porosity(const float* args6, const double* plastic_volume, float* out_porosity, double* out_prediction, int32_t n,
+                      void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BAKING_ABI_H */
