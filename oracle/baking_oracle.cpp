@@ -298,6 +298,13 @@ template <typename R> void svd_of_trial(const M3<R>& trial, M3<R>& U, R s[3], M3
 
 }  // namespace
 
+static const double kGasConstant = 8.31446261815324;  // physics/materials/gas.py:8
+
+template <typename R> static inline double gas_pressure(double moles, R T, R J, R n) {
+  // physics/materials/gas.py:11-13
+  return moles * kGasConstant * double(T) / (double(J) * double(n));
+}
+
 // ============================================================================ exported kernels
 template <typename R>
 static void ParticleToGrid(int n, const R* x, const R* v, const R* C, const R* tau, const R* m, const R* V0, R* node_m,
@@ -457,13 +464,6 @@ static void UpdateDing(int n, const R* old_Fe, const R* old_n, const int* old_ge
     yields[p] = yv;
     viscs[p] = ev;
   }
-}
-
-static const double kGasConstant = 8.31446261815324;  // physics/materials/gas.py:8
-
-template <typename R> static inline double gas_pressure(double moles, R T, R J, R n) {
-  // physics/materials/gas.py:11-13
-  return moles * kGasConstant * double(T) / (double(J) * double(n));
 }
 
 template <typename R>
@@ -723,6 +723,7 @@ double oracle_porosity_prediction(double n, float q, float p, float df, float ms
   return porosity_prediction<float>(n, q, p, df, ms, ex, pv, n0);
 }
 float oracle_temperature_factor(float t, const float* knots, const float* factors) { return temperature_factor<float>(t, knots, factors); }
+double oracle_gas_pressure(double moles, float T, float J, float n) { return gas_pressure<float>(moles, T, J, n); }
 
 void oracle_GridFields(int64_t count, const double* V, const double* C, const double* E, const double* W, double* T, double* S,
                        double rho_w) {

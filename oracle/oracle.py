@@ -71,6 +71,8 @@ def lib():
         _lib.oracle_solve_porosity.argtypes = [ctypes.c_float] * 5 + [ctypes.c_double, ctypes.c_float]
         _lib.oracle_porosity_prediction.restype = ctypes.c_double
         _lib.oracle_porosity_prediction.argtypes = [ctypes.c_double] + [ctypes.c_float] * 5 + [ctypes.c_double, ctypes.c_float]
+        _lib.oracle_gas_pressure.restype = ctypes.c_double
+        _lib.oracle_gas_pressure.argtypes = [ctypes.c_double, ctypes.c_float, ctypes.c_float, ctypes.c_float]
         _lib.oracle_temperature_factor.restype = ctypes.c_float
         _lib.oracle_temperature_factor.argtypes = [ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
     return _lib
@@ -95,19 +97,24 @@ class Kernels:
         self._lib = lib()
 
     def _call(self, name, *args):
-        fn = getattr(self._lib, f"oracle_{name}_{self.precision}")
+        symbol = f"oracle_{name}_{self.precision}"
+        fn = getattr(self._lib, symbol)
+        types = _signatures()[symbol]
+        if len(types) != len(args):
+            raise TypeError(f"{symbol} takes {len(types)} arguments, got {len(args)}")
         conv = []
-        for a in args:
-            if isinstance(a, np.ndarray) or a is None:
+        for t, a in zip(types, args):
+            if t is ctypes.c_void_p:
+                if not (isinstance(a, np.ndarray) or a is None):
+                    raise TypeError(f"{symbol}: expected an array, got {type(a).__name__}")
                 conv.append(_p(a))
-            elif isinstance(a, _I64):
-                conv.append(ctypes.c_int64(a.v))
-            elif isinstance(a, (int, np.integer)) and not isinstance(a, bool):
-                conv.append(ctypes.c_int(int(a)))
-            elif isinstance(a, bool):
+            elif t is ctypes.c_int64:
+                conv.append(ctypes.c_int64(a.v if isinstance(a, _I64) else int(a)))
+            elif t is ctypes.c_int:
                 conv.append(ctypes.c_int(int(a)))
             else:
-                conv.append(self._c(float(a)))
+                conv.append(t(float(a)))
+        fn.argtypes = types
         fn.restype = None
         fn(*conv)
 
@@ -118,6 +125,40 @@ class Kernels:
         if name.startswith("_"):
             raise AttributeError(name)
         return lambda *args: self._call(name, *args)
+
+
+_SIGS = None
+
+
+def _signatures():
+    """ctypes argtypes of every exported kernel, parsed from the ORACLE_BOTH declarations of the C++ source."""
+    global _SIGS
+    if _SIGS is not None:
+        return _SIGS
+    import re
+    text = (_HERE / "baking_oracle.cpp").read_text()
+    sigs = {}
+    for m in re.finditer(r"ORACLE_BOTH\((\w+),\s*\(([^)]*)\),\s*\(([^)]*)\),", text):
+        name = m.group(1)
+        for suffix, params in (("f32", m.group(2)), ("f64", m.group(3))):
+            types = []
+            for p in params.split(","):
+                p = p.strip()
+                if "*" in p:
+                    types.append(ctypes.c_void_p)
+                elif p.startswith("int64_t"):
+                    types.append(ctypes.c_int64)
+                elif p.startswith("int"):
+                    types.append(ctypes.c_int)
+                elif p.startswith("float"):
+                    types.append(ctypes.c_float)
+                elif p.startswith("double"):
+                    types.append(ctypes.c_double)
+                else:
+                    raise ValueError(f"unparsed parameter '{p}' of {name}")
+            sigs[f"oracle_{name}_{suffix}"] = types
+    _SIGS = sigs
+    return sigs
 
 
 class _I64:
@@ -144,6 +185,10 @@ def solve_porosity(q, p, df, ms, ex, pv, n0):
 
 def porosity_prediction(n, q, p, df, ms, ex, pv, n0):
     return float(lib().oracle_porosity_prediction(n, q, p, df, ms, ex, pv, n0))
+
+
+def gas_pressure(moles, T, J, n):
+    return float(lib().oracle_gas_pressure(moles, T, J, n))
 
 
 def temperature_factor(t, knots, factors):
@@ -341,9 +386,10 @@ class OracleSimulator:
         self.k.ReconstructMeshVertices(len(out), state.x, state.F, self.reference_vertices, b["offsets"], b["particles"],
                                        b["weights"], b["reference_offsets"], out)
 
-    def transport_advance(self, state, settling=False):
+    def transport_advance(self, state, settling=False, dt=None):
         """thermal_water.py:296-355; returns the prepared state and the workspace."""
         k, t, n, G = self.k, self.tr, self.n, self.node_count
+        dt = self.dt if dt is None else dt
         sim = self.sim
         R = self.R
         heat_param = np.zeros(1, R) if settling else self.heat_param
@@ -367,12 +413,12 @@ class OracleSimulator:
                                 ctypes.c_double(float(R(t.water_density))))
         k.DiffuseGrid(i64(G), w["V"], w["C"], w["T"], w["S"], w["W"], w["H"], w["L"], w["X"], w["dT"], w["dW"], self.transport_flags,
                       0.0 if settling else t.conductivity, 0.0 if settling else t.water_diffusivity, t.water_density,
-                      int(t.water_enabled), self.dx, self.res, self.dt)
+                      int(t.water_enabled), self.dx, self.res, dt)
         if self.transport_flags[0] or self.transport_flags[1]:
             raise RuntimeError('Heat/water explicit stability bound exceeded; reduce time_step; no temperature or water clipping is applied')
         k.ScalarToParticles(n, state.x, old["T"], old["W"], old["losses"], old["exchanges"], self.dry, self.V0, state.F, w["W"],
                             w["dT"], w["dW"], p_heat, p_loss, out["T"], T_mat, out["W"], out["losses"], out["exchanges"],
-                            out["masses"], rho_ref, out["s"], t.water_density, self.origin, self.dx, self.res, self.dt)
+                            out["masses"], rho_ref, out["s"], t.water_density, self.origin, self.dx, self.res, dt)
         ding = dict(state.ding)
         ding["T"] = T_mat
         ding["density"] = rho_ref
@@ -382,7 +428,7 @@ class OracleSimulator:
             ding["pressure"] = np.zeros(n)
             k.LocalGas(n, state.ding["moles"], old["T"], out["T"], ding["moles"], ding["pressure"], state.F,
                        state.ding["porosity"], self.prod_param, self.mat.density, c.gas_reaction_temperature_k,
-                       c.gas_reaction_width_k, self.dt, int(not settling))
+                       c.gas_reaction_width_k, dt, int(not settling))
         prepared = OracleState(state.x, state.v, state.F, state.C, ding, out)
         work = dict(w, exposed=exposed, contact=contact, vertices=verts, p_heat=p_heat, p_loss=p_loss)
         return prepared, work

@@ -89,3 +89,48 @@ def field_errors(a, b, names=None, floors=None):
         scale = max(np.abs(y).max(initial=0.0), floors.get(k, 0.0), 1e-30)
         errs[k] = float(np.abs(x - y).max(initial=0.0) / scale)
     return errs
+
+
+class OraclePoint:
+    """One-particle Ding state advanced by the oracle's UpdateDing (+ RefreshGas when gas is on).
+
+    ``step(F_total, dt)`` mirrors the reference's single-point driver
+    (tests/test_ding.py:23-29): the new total gradient is F_total and
+    C = (F_total F_old^-1 - I) / dt.
+    """
+
+    def __init__(self, material, ding, initial_temperature_k=293.15, precision="f32"):
+        self.k = O.Kernels(precision)
+        R = self.k.R
+        self.R, self.mat, self.ding = R, material, ding
+        T0 = initial_temperature_k
+        gel = int(ding.temperature_enabled and T0 >= ding.temperature_knots_k[3])
+        yf = float(np.interp(T0, ding.temperature_knots_k, ding.yield_factors)) if ding.temperature_enabled else 1.0
+        vf = float(np.interp(T0, ding.temperature_knots_k, ding.viscosity_factors)) if ding.temperature_enabled else 1.0
+        self.visc = np.array([ding.viscosity], R)
+        self.F = np.eye(3, dtype=R)[None].copy()
+        self.s = dict(Fe=np.eye(3, dtype=R)[None].copy(), Fp=np.eye(3, dtype=R)[None].copy(), tau=np.zeros((1, 3, 3), R),
+                      residual=np.zeros(1, R), porosity=np.full(1, ding.porosity, R), T=np.full(1, T0, R),
+                      gelled=np.full(1, gel, np.int32), yield_stress=np.full(1, ding.yield_stress * yf, R),
+                      viscosity=np.full(1, float(R(ding.viscosity)) * vf, R), porosity_residual=np.zeros(1, R),
+                      moles=np.full(1, ding.gas_initial_pressure_pa * ding.porosity / (O.GAS_CONSTANT * T0)
+                                    if ding.gas_enabled else 0.0),
+                      pressure=np.full(1, ding.gas_initial_pressure_pa if ding.gas_enabled else 0.0))
+        self.flags = np.zeros(3, np.int32)
+
+    def step(self, F_total, dt=0.01):
+        R = self.R
+        inc = np.asarray(F_total, np.float64) @ np.linalg.inv(self.F[0].astype(np.float64))
+        C = ((inc - np.eye(3)) / dt).astype(R)[None].copy()
+        Fn = np.asarray(F_total).astype(R)[None].copy()
+        o, d = self.s, self.ding
+        n = {k: v.copy() for k, v in o.items()}
+        self.k.UpdateDing(1, o["Fe"], o["porosity"], o["gelled"], Fn, C, n["Fe"], n["Fp"], n["tau"], n["residual"], n["porosity"],
+                          n["T"], n["gelled"], n["yield_stress"], n["viscosity"], self.visc, self.flags, n["porosity_residual"],
+                          int(d.gas_enabled), self.mat.shear_modulus, self.mat.lame_lambda, d.yield_stress, d.porosity,
+                          int(d.temperature_enabled), np.asarray(d.temperature_knots_k, R), np.asarray(d.yield_factors, R),
+                          np.asarray(d.viscosity_factors, R), dt, o["T"])
+        if d.gas_enabled:
+            self.k.RefreshGas(1, o["moles"], n["moles"], Fn, n["porosity"], n["pressure"], n["T"])
+        self.s, self.F = n, Fn
+        return n
