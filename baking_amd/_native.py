@@ -77,6 +77,10 @@ SIGNATURES = {
     "bk_cell_keys": ([P, I, P, F, I, I, P, P, P], c_int),
     "bk_sort_pairs": ([P, P, P, P, P, I, I, P], c_int),
     "bk_substep": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P, P], c_int),
+    "bk_substep_graph_create": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P],
+                                c_void_p),
+    "bk_graph_launch": ([P, P], c_int),
+    "bk_graph_destroy": ([P], None),
     "bk_profile_begin": ([P, I], c_int),
     "bk_profile_end": ([P, P, P], c_int),
     "bk_profile_stage_count": ([], c_int32),

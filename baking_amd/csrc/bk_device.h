@@ -359,13 +359,20 @@ __device__ __forceinline__ void ding_return(const mat3<float>& trial, const mat3
   }
 }
 
-// ding.py:140-221 (UpdateDing body for one particle).  `valid` is false when the
-// incoming porosity is outside [0, 1) (flag 0, outputs untouched).
-__device__ inline bool ding_update(const DingParams& P, const mat3<float>& Fe_old, float porosity, int gelled_in, const mat3<float>& F_total,
-                                   const mat3<float>& C_new, float T, float visc_param, DingOut& o) {
+// Outcome of ding_update.
+enum DingStatus { kDingInvalid = 0, kDingDone = 1, kDingDeferred = 2 };
+
+// ding.py:140-221 (UpdateDing body for one particle).  Returns kDingInvalid when
+// the incoming porosity is outside [0, 1) (flag 0, outputs untouched).  With
+// `defer` set, a particle whose coupled porosity needs the bisection solve
+// (porous_return.py:50-61) returns kDingDeferred without outputs, so a second
+// pass can run the expensive solves densely; recomputing from the same inputs
+// gives the same result.
+__device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old, float porosity, int gelled_in, const mat3<float>& F_total,
+                                  const mat3<float>& C_new, float T, float visc_param, DingOut& o, bool defer = false) {
   o.flag0 = 0;
   o.flag1 = 0;
-  if (!isfinite(porosity) || porosity < 0.0f || porosity >= 1.0f) { o.flag0 = 1; return false; }
+  if (!isfinite(porosity) || porosity < 0.0f || porosity >= 1.0f) { o.flag0 = 1; return kDingInvalid; }
   int gel = gelled_in;
   float Teff = T;
   float yv = P.yield_stress;
@@ -403,6 +410,11 @@ __device__ inline bool ding_update(const DingParams& P, const mat3<float>& Fe_ol
     float ms = 2.0f / 3.0f * (2.0f * mu + 3.0f * lam) * dt * yv * yv / ev;
     float ex = 3.0f * mean / (2.0f * mu + 3.0f * lam);
     double opv = 1.0 - double(P.n0) + double(porosity);
+    if (defer) {
+      double trial_n = opv - 1.0 + double(P.n0);
+      bool inside = trial_n > 0.0 && trial_n < 1.0 && double(q) + pressure_coefficient64(trial_n) * double(pv) <= 1.0;
+      if (!inside) return kDingDeferred;
+    }
     ret_n = solve_porosity(q, pv, df, ms, ex, opv, P.n0);
     double pred = porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
     cres = float(fabs(double(ret_n) - pred));
@@ -431,7 +443,7 @@ __device__ inline bool ding_update(const DingParams& P, const mat3<float>& Fe_ol
   o.gelled = gel;
   o.yield_stress = yv;
   o.viscosity = ev;
-  return true;
+  return kDingDone;
 }
 
 // ------------------------------------------------------------------ gas (physics/materials/gas.py)

@@ -114,10 +114,10 @@ __global__ void k_UpdateDing(const float* oFe, const float* on, const int* og, c
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   DingOut r;
-  bool ok = ding_update(D, m_load(oFe + 9 * p), on[p], og[p], m_load(Ft + 9 * p), m_load(Ca + 9 * p), lT[p], vp[0], r);
+  int st = ding_update(D, m_load(oFe + 9 * p), on[p], og[p], m_load(Ft + 9 * p), m_load(Ca + 9 * p), lT[p], vp[0], r);
   if (r.flag0) atomicMax(flags + 0, 1);
   if (r.flag1) atomicMax(flags + 1, 1);
-  if (!ok) return;
+  if (st != kDingDone) return;
   m_store(nFe + 9 * p, r.Fe);
   m_store(nFp + 9 * p, r.Fp);
   m_store(nt + 9 * p, r.tau);

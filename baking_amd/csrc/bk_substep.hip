@@ -72,7 +72,8 @@ struct bk_workspace {
   int32_t* node_flag = nullptr;
   int32_t* tiles = nullptr;
   int32_t* active = nullptr;
-  int32_t* counters = nullptr;  // [0] tiles, [1] active blocks
+  int32_t* counters = nullptr;  // [0] tiles, [1] active blocks, [2] deferred return maps
+  int32_t* queue = nullptr;     // particles whose coupled porosity needs the bisection solve
   float* tau = nullptr;
   float* vertices = nullptr;
   double* slot_exposed = nullptr;
@@ -96,7 +97,7 @@ __device__ __forceinline__ uint32_t tiled_cell_key(const int b[3], int nb) {
 __global__ void k_cell_keys(const float* __restrict__ x, int n, float ox, float oy, float oz, float dx, int res, int tiled,
                             uint32_t* __restrict__ keys, int32_t* __restrict__ flags, int32_t* counters) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (counters && p == 0) { counters[0] = 0; counters[1] = 0; }
+  if (counters && p == 0) { counters[0] = 0; counters[1] = 0; counters[2] = 0; }
   if (p >= n) return;
   const float o[3] = {ox, oy, oz};
   float g[3];
@@ -615,7 +616,9 @@ __global__ __launch_bounds__(256) void k_pass_b1(bk_params P, bk_scene sc, bk_st
 // ============================================================================ pass B2: mechanical P2G (physics/transfer.py:36-70)
 __global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_grid g,
                                                           bk_workspace* ws) {
-  __shared__ float acc[4][kRegionNodes];
+  // f64 LDS accumulators: ds_add_f64 sustains far higher throughput than ds_add_f32 on gfx950
+  // for this scatter, and each node's sum is rounded to float once, at the flush.
+  __shared__ double acc[4][kRegionNodes];
   const int ntiles = ws->counters[0];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
@@ -623,7 +626,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene 
   const float s_coef = 4.0f / (P.dx * P.dx);
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     TileGeom tg = tile_geom(ws, t);
-    for (int i = threadIdx.x; i < 4 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0f;
+    for (int i = threadIdx.x; i < 4 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
     __syncthreads();
     const int S = (tg.count + 63) / 64;
     for (int sub = w; sub < S; sub += 4) {
@@ -658,10 +661,10 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene 
             float wt = wx[ox] * wy[oy] * wz[oz];
             float d[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
             int ri = region_index(li + ox, lj + oy, lk + oz);
-            atomicAdd(&acc[0][ri], wt * mass);
+            atomicAdd(&acc[0][ri], double(wt * mass));
 #pragma unroll
             for (int a = 0; a < 3; ++a)
-              atomicAdd(&acc[1 + a][ri], wt * (mv[a] + (A.a[a * 3 + 0] * d[0] + A.a[a * 3 + 1] * d[1] + A.a[a * 3 + 2] * d[2])));
+              atomicAdd(&acc[1 + a][ri], double(wt * (mv[a] + (A.a[a * 3 + 0] * d[0] + A.a[a * 3 + 1] * d[1] + A.a[a * 3 + 2] * d[2]))));
           }
     }
     (void)s_coef;
@@ -671,11 +674,11 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene 
       int i = 4 * tg.bx + li, j = 4 * tg.by + lj, k = 4 * tg.bz + lk;
       if (i >= P.res || j >= P.res || k >= P.res) continue;
       int64_t node = tiled_node(i, j, k, ws->nb);
-      float m = acc[0][ri];
+      float m = float(acc[0][ri]);
       if (m != 0.0f) unsafeAtomicAdd(g.m + node, m);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        float v = acc[1 + a][ri];
+        float v = float(acc[1 + a][ri]);
         if (v != 0.0f) unsafeAtomicAdd(g.p + 3 * node + a, v);
       }
     }
@@ -705,6 +708,61 @@ __global__ __launch_bounds__(256) void k_grid_update(bk_params P, bk_grid g, bk_
 #pragma unroll
     for (int a = 0; a < 3; ++a) g.v[3 * node + a] = vel[a];
   }
+}
+
+// Ding return map, gas refresh and saturation of particle p after its G2P
+// (ding.py:303-327, gas.py:38-45, thermal_water.py:256-260).
+__device__ inline int material_update(const bk_params& P, const bk_scene& sc, const bk_state& in, const bk_state& out, int p,
+                                      const mat3<float>& F_new, const mat3<float>& Cn, bool defer, bool& f0, bool& f1) {
+  DingParams D;
+  D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
+  for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
+  D.temperature_enabled = P.temperature_enabled;
+  D.coupled = P.gas_enabled;
+  float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
+  DingOut r;
+  int status = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, sc.visc_param[0], r, defer);
+  f0 = r.flag0;
+  f1 = r.flag1;
+  if (status == kDingDeferred) return status;
+  if (status == kDingDone) {
+    m_store(out.Fe + 9 * p, r.Fe);
+    m_store(out.Fp + 9 * p, r.Fp);
+    m_store(out.tau + 9 * p, r.tau);
+    out.residual[p] = r.residual;
+    out.porosity_residual[p] = r.porosity_residual;
+    out.porosity[p] = r.porosity;
+    out.T_mat[p] = r.T;
+    out.gelled[p] = r.gelled;
+    out.yield_stress[p] = r.yield_stress;
+    out.viscosity[p] = r.viscosity;
+  }
+  if (!P.transport_enabled) out.density[p] = in.density[p];
+  float Jn = m_det(F_new);
+  if (P.gas_enabled) {
+    double moles = P.transport_enabled ? out.moles[p] : in.moles[p];
+    if (!P.transport_enabled) out.moles[p] = moles;
+    float nn = out.porosity[p];
+    if (nn > 0.0f && nn < 1.0f) out.pressure[p] = gas_pressure(moles, out.T_mat[p], Jn, nn);
+  }
+  if (P.transport_enabled) out.s[p] = out.W[p] / (double(P.rho_w) * double(sc.V0[p]) * double(Jn));
+  return status;
+}
+
+// Second half of pass C: the deferred coupled-porosity return maps, 64 per
+// wavefront with no idle lanes waiting on a neighbour's bisection.
+__global__ __launch_bounds__(256) void k_pass_c_deferred(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_workspace* ws) {
+  const int nq = ws->counters[2];
+  bool f0 = false, f1 = false;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
+    int p = ws->queue[i];
+    bool a = false, b = false;
+    material_update(P, sc, in, out, p, m_load(out.F + 9 * p), m_load(out.C + 9 * p), false, a, b);
+    f0 |= a;
+    f1 |= b;
+  }
+  flag_max(ws->flags, BK_FLAG_POROSITY, f0);
+  flag_max(ws->flags, BK_FLAG_COUPLED, f1);
 }
 
 // ============================================================================ pass C: G2P + Ding + gas refresh + saturation
@@ -758,37 +816,8 @@ __global__ __launch_bounds__(256) void k_pass_c(bk_params P, bk_scene sc, bk_sta
       m_store(out.C + 9 * p, Cn);
       m_store(out.F + 9 * p, F_new);
       if (P.model == 1) {
-        DingParams D;
-        D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
-        for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
-        D.temperature_enabled = P.temperature_enabled;
-        D.coupled = P.gas_enabled;
-        float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
-        DingOut r;
-        bool ok = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, sc.visc_param[0], r);
-        f0 = r.flag0;
-        f1 = r.flag1;
-        if (ok) {
-          m_store(out.Fe + 9 * p, r.Fe);
-          m_store(out.Fp + 9 * p, r.Fp);
-          m_store(out.tau + 9 * p, r.tau);
-          out.residual[p] = r.residual;
-          out.porosity_residual[p] = r.porosity_residual;
-          out.porosity[p] = r.porosity;
-          out.T_mat[p] = r.T;
-          out.gelled[p] = r.gelled;
-          out.yield_stress[p] = r.yield_stress;
-          out.viscosity[p] = r.viscosity;
-        }
-        if (!P.transport_enabled) out.density[p] = in.density[p];
-        float Jn = m_det(F_new);
-        if (P.gas_enabled) {
-          double moles = P.transport_enabled ? out.moles[p] : in.moles[p];
-          if (!P.transport_enabled) out.moles[p] = moles;
-          float nn = out.porosity[p];
-          if (nn > 0.0f && nn < 1.0f) out.pressure[p] = gas_pressure(moles, out.T_mat[p], Jn, nn);
-        }
-        if (P.transport_enabled) out.s[p] = out.W[p] / (double(P.rho_w) * double(sc.V0[p]) * double(Jn));
+        int status = material_update(P, sc, in, out, p, F_new, Cn, true, f0, f1);
+        if (status == kDingDeferred) ws->queue[atomicAdd(&ws->counters[2], 1)] = p;
       }
     }
   }
@@ -865,6 +894,7 @@ bk_workspace* bk_workspace_create(int32_t max_particles, int32_t res, int32_t n_
   alloc((void**)&h.active, size_t(h.nblocks) * 4);
   alloc((void**)&h.counters, 64);
   alloc((void**)&h.tau, n * 36);
+  alloc((void**)&h.queue, n * 4);
   alloc((void**)&h.vertices, size_t(n_vertices) * 12);
   alloc((void**)&h.slot_exposed, size_t(n_slots) * 8);
   alloc((void**)&h.slot_contact, size_t(n_slots) * 8);
@@ -890,7 +920,7 @@ void bk_workspace_destroy(bk_workspace* ws) {
   }
   if (ws_host(ws, h)) return;
   void* ptrs[] = {h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1], h.hist, h.digit_total, h.blk_start, h.blk_end, h.node_flag, h.tiles, h.active,
-                  h.counters, h.tau, h.vertices, h.slot_exposed, h.slot_contact, h.flags};
+                  h.counters, h.queue, h.tau, h.vertices, h.slot_exposed, h.slot_contact, h.flags};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   (void)hipFree(ws);
@@ -1023,8 +1053,57 @@ int bk_sort_pairs(bk_workspace* ws, const uint32_t* keys_in, const int32_t* valu
   return radix_sort(h, keys_in, values_in, keys_out, values_out, n, key_bits, (hipStream_t)stream);
 }
 
+static int substep_impl(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g, bk_workspace* ws,
+                        void* stream, bool allow_profile);
+
 int bk_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g, bk_workspace* ws,
                void* stream) {
+  return substep_impl(P, sc, in, out, g, ws, stream, true);
+}
+
+struct bk_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+bk_graph* bk_substep_graph_create(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g,
+                                  bk_workspace* ws) {
+  if (!host_view(ws)) return nullptr;  // fills the host cache before capture (no device calls while capturing)
+  hipStream_t cap = nullptr;
+  if (check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return nullptr;
+  bk_graph* gr = new bk_graph();
+  int rc = check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (!rc) {
+    int rc_sub = substep_impl(P, sc, in, out, g, ws, cap, false);
+    int rc_end = check_hip(hipStreamEndCapture(cap, &gr->graph), "hipStreamEndCapture");
+    rc = rc_sub ? rc_sub : rc_end;
+  }
+  if (!rc) rc = check_hip(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipStreamDestroy(cap);
+  if (rc) {
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    delete gr;
+    return nullptr;
+  }
+  return gr;
+}
+
+int bk_graph_launch(bk_graph* gr, void* stream) {
+  if (!gr) return fail("bk_graph_launch: null graph");
+  BK_HIP(hipGraphLaunch(gr->exec, (hipStream_t)stream));
+  return 0;
+}
+
+void bk_graph_destroy(bk_graph* gr) {
+  if (!gr) return;
+  if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+  if (gr->graph) (void)hipGraphDestroy(gr->graph);
+  delete gr;
+}
+
+static int substep_impl(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g, bk_workspace* ws,
+                        void* stream, bool allow_profile) {
   const bk_workspace* h = host_view(ws);
   if (!h) return -2;
   if (!P || !sc || !in || !out || !g) return fail("bk_substep: null argument");
@@ -1035,7 +1114,7 @@ int bk_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_st
   const int thermal = P->transport_enabled;
   const int cap_blocks = 4096;
   Profiler* pr = nullptr;
-  {
+  if (allow_profile) {
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     auto it = g_prof.find(ws);
     if (it != g_prof.end() && it->second.step < it->second.max_steps) pr = &it->second;
@@ -1094,7 +1173,11 @@ int bk_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_st
   BK_LAUNCH(grid_update); }
   { StageTimer tm(pr, st, kStagePassC);
   hipLaunchKernelGGL(k_pass_c, dim3(div_up(n, 256)), dim3(256), 0, st, *P, *sc, *in, *out, *g, ws);
-  BK_LAUNCH(pass_c); }
+  BK_LAUNCH(pass_c);
+  if (P->model == 1 && P->gas_enabled) {
+    hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, *P, *sc, *in, *out, ws);
+    BK_LAUNCH(pass_c_deferred);
+  } }
   if (pr) ++pr->step;
   return 0;
 }

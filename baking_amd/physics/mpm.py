@@ -151,6 +151,7 @@ class MpmSimulator:
 
     def _build_native(self):
         lib = _native.load()
+        self._destroy_graphs()
         if self._workspace is not None:
             lib.bk_workspace_destroy(self._workspace)
             self._workspace = None
@@ -203,6 +204,7 @@ class MpmSimulator:
 
     def __del__(self):
         try:
+            self._destroy_graphs()
             if self._workspace is not None:
                 _native.load().bk_workspace_destroy(self._workspace)
         except Exception:
@@ -270,10 +272,33 @@ class MpmSimulator:
         return s
 
     # ------------------------------------------------------------------ substep
-    def _launch_substep(self, stream=None):
+    use_graphs = True
+
+    def _destroy_graphs(self):
+        lib = _native.load()
+        for handle in getattr(self, "_graphs", {}).values():
+            lib.bk_graph_destroy(handle)
+        self._graphs = {}
+
+    def _launch_substep(self, stream=None, use_graph=None):
+        """Enqueue one substep; replays a captured HIP graph for this ping-pong direction when possible."""
         lib = _native.load()
         s_in = self._state_struct(self.current_state)
         s_out = self._state_struct(self.next_state)
+        use_graph = self.use_graphs if use_graph is None else use_graph
+        if use_graph:
+            key = bytes(s_in) + bytes(s_out) + bytes(self._params)
+            graph = self._graphs.get(key)
+            if graph is None:
+                if len(self._graphs) >= 4:
+                    self._destroy_graphs()
+                handle = lib.bk_substep_graph_create(ctypes.byref(self._params), ctypes.byref(self._scene), ctypes.byref(s_in),
+                                                     ctypes.byref(s_out), ctypes.byref(self._grid), self._workspace)
+                if not handle:
+                    _native.check(-2, "bk_substep_graph_create")
+                graph = self._graphs[key] = ctypes.c_void_p(handle)
+            _native.check(lib.bk_graph_launch(graph, _native.stream_ptr(stream)), "bk_graph_launch")
+            return
         rc = lib.bk_substep(ctypes.byref(self._params), ctypes.byref(self._scene), ctypes.byref(s_in), ctypes.byref(s_out),
                             ctypes.byref(self._grid), self._workspace, _native.stream_ptr(stream))
         _native.check(rc, "bk_substep")

@@ -45,6 +45,9 @@ class SceneSpec:
     water_masses: np.ndarray | None = None
     porosities: np.ndarray | None = None
     gas_moles: np.ndarray | None = None
+    velocities: np.ndarray | None = None
+    affine_velocities: np.ndarray | None = None
+    temperatures: np.ndarray | None = None
     notes: dict = field(default_factory=dict)
 
 
@@ -224,4 +227,55 @@ def build_simulator(spec: SceneSpec):
         state.material_state.porosities.copy_(torch.as_tensor(spec.porosities, dtype=torch.float32, device=dev))
     if spec.gas_moles is not None:
         state.material_state.gas_moles.copy_(torch.as_tensor(spec.gas_moles, dtype=torch.float64, device=dev))
+    if spec.velocities is not None:
+        state.particle_velocities.copy_(torch.as_tensor(spec.velocities, dtype=torch.float32, device=dev))
+    if spec.affine_velocities is not None:
+        state.affine_velocities.copy_(torch.as_tensor(spec.affine_velocities, dtype=torch.float32, device=dev))
+    if spec.temperatures is not None:
+        state.transport_state.temperatures.copy_(torch.as_tensor(spec.temperatures, dtype=torch.float64, device=dev))
     return sim
+
+
+def strain_rate_field(spec: SceneSpec, rates, centre=(0.5, 0.6, 0.5)):
+    """Give a scene the affine velocity v = L (x - centre), C = L (reference tests/test_ding.py:124-127)."""
+    L = np.asarray(rates, dtype=np.float32)
+    x = spec.samples.particle_positions.astype(np.float32)
+    return replace(spec, velocities=(x - np.asarray(centre, np.float32)) @ L.T,
+                   affine_velocities=np.tile(L, (len(x), 1, 1)))
+
+
+def demo_stretch(subdivisions=2, yield_stress=100.0, viscosity=1e8):
+    """Ding dough cube in a diag(1.5, -0.75, 0.2) strain-rate flow (reference tests/test_ding.py:117-137)."""
+    from .geometry.mesh_loader import CreateDemoMesh
+    from .geometry.particle_sampling import SampleParticles
+    mesh = CreateDemoMesh(subdivisions)
+    samples = SampleParticles(mesh, 1000.0, 4)
+    spec = SceneSpec("demo_stretch", mesh, samples, SimulationConfig(device_name="auto"), MaterialConfig(model="ding2019"),
+                     DingConfig(yield_stress=yield_stress, viscosity=viscosity), TransportConfig(), steps=600)
+    return strain_rate_field(spec, np.diag([1.5, -0.75, 0.2]))
+
+
+def demo_gas(rate=0.005, dt=0.0005, subdivisions=1):
+    """Gas-producing porous dough at 333.15 K (reference tests/test_gas.py:325-333)."""
+    from .geometry.mesh_loader import CreateDemoMesh
+    from .geometry.particle_sampling import SampleParticles
+    mesh = CreateDemoMesh(subdivisions)
+    ding = DingConfig(gas_enabled=True, porosity=0.2, yield_stress=20., viscosity=1e5, gas_initial_pressure_pa=1000.,
+                      gas_ambient_pressure_pa=1000., gas_production_rate=rate)
+    transport = TransportConfig(enabled=True, water_enabled=False, initial_temperature_k=333.15, conductivity=0.,
+                                heat_transfer=0., tray_heat_transfer=0.)
+    sim = SimulationConfig(device_name="auto", time_step=dt, gravity=(0.0, 0.0, 0.0))
+    return SceneSpec("demo_gas", mesh, SampleParticles(mesh, 1000., 4), sim,
+                     MaterialConfig(model="ding2019", young_modulus=10000.), ding, transport, steps=100)
+
+
+def demo_heated(viscosity=1e8, yield_stress=100.0, dt=1e-4):
+    """Temperature-dependent porous dough with transport in a strain-rate flow (reference tests/test_ding_thermal.py:212-225)."""
+    from .geometry.mesh_loader import CreateDemoMesh
+    from .geometry.particle_sampling import SampleParticles
+    mesh = CreateDemoMesh(1)
+    sim = SimulationConfig(device_name="auto", time_step=dt, gravity=(0.0, 0.0, 0.0))
+    spec = SceneSpec("demo_heated", mesh, SampleParticles(mesh, 1000., 4), sim, MaterialConfig(model="ding2019"),
+                     DingConfig(temperature_enabled=True, porosity=.2, yield_stress=yield_stress, viscosity=viscosity),
+                     TransportConfig(enabled=True, water_enabled=False, initial_temperature_k=300.), steps=100)
+    return strain_rate_field(spec, np.diag([1.5, -.75, .2]))

@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-baseline-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of replaying a HIP graph")
     return ap.parse_args()
 
 
@@ -92,6 +93,7 @@ def main():
 
     spec = build_scene(args.workload)
     sim = build_simulator(spec)
+    sim.use_graphs = not args.no_graph
     n = sim.particle_data.particle_count
     stream = torch.cuda.current_stream()
 
@@ -118,7 +120,7 @@ def main():
     lib = _native.load()
     _native.check(lib.bk_profile_begin(sim._workspace, args.profile_steps), "bk_profile_begin")
     for _ in range(args.profile_steps):
-        sim._launch_substep()
+        sim._launch_substep(use_graph=False)
         sim._accept()
     nst = lib.bk_profile_stage_count()
     import ctypes
@@ -152,7 +154,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": spec.name, "method": "plain", "particles": n, "grid": spec.simulation.grid_resolution,
+        "config": {"workload": spec.name, "method": "plain", "hip_graph": sim.use_graphs, "particles": n, "grid": spec.simulation.grid_resolution,
                    "dt": spec.simulation.time_step, "active_stencil_nodes": active_nodes,
                    "yield_fraction": yield_fraction, "wall_s": wall, "flags": flags, "flags_before_timing": flags_before},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -137,6 +137,15 @@ int bk_sort_pairs(bk_workspace* ws, const uint32_t* keys_in, const int32_t* valu
  * keeps this step's values afterwards (inactive blocks are zero). */
 int bk_substep(const bk_params* params, const bk_scene* scene, const bk_state* in, bk_state* out, bk_grid* grid,
                bk_workspace* ws, void* stream);
+/* The same substep captured once into a HIP graph (on a library-owned
+ * capture stream) and replayed with one launch: the state/grid pointers and
+ * parameters are frozen at capture, so keep one graph per ping-pong direction
+ * and recapture when a buffer or parameter changes. */
+typedef struct bk_graph bk_graph;
+bk_graph* bk_substep_graph_create(const bk_params* params, const bk_scene* scene, const bk_state* in, bk_state* out,
+                                  bk_grid* grid, bk_workspace* ws);
+int bk_graph_launch(bk_graph* graph, void* stream);
+void bk_graph_destroy(bk_graph* graph);
 /* Per-stage timing of bk_substep with HIP events on its stream: begin() arms
  * recording for the next max_steps substeps; end() waits for them and returns
  * the summed milliseconds per stage (bk_profile_stage_count() entries, named

@@ -27,6 +27,12 @@ def oracle_from_spec(spec, precision="f32"):
         st.ding["porosity"][...] = spec.porosities
     if spec.gas_moles is not None:
         st.ding["moles"][...] = spec.gas_moles
+    if spec.velocities is not None:
+        st.v[...] = spec.velocities
+    if spec.affine_velocities is not None:
+        st.C[...] = spec.affine_velocities
+    if spec.temperatures is not None:
+        st.transport["T"][...] = spec.temperatures
     return sim
 
 
@@ -134,3 +140,35 @@ class OraclePoint:
             self.k.RefreshGas(1, o["moles"], n["moles"], Fn, n["porosity"], n["pressure"], n["T"])
         self.s, self.F = n, Fn
         return n
+
+
+def three_way(spec, steps, rtol=1e-5, factor=4.0, check_every=1, log=print):
+    """Run the GPU path, the FP32 oracle and the FP64 oracle side by side.
+
+    After each checked substep every field must satisfy
+        err(gpu, oracle32) <= max(rtol, factor * err(oracle32, oracle64)),
+    i.e. the GPU agrees with the reference algorithm to 1e-5 (scale-relative),
+    or, where the field is rounding-dominated (stress of a nearly undeformed
+    body), to within a few times the reference's own FP32 rounding error.
+    Returns the last errors (gpu vs oracle32, oracle32 vs oracle64).
+    """
+    gpu = gpu_from_spec(spec)
+    o32 = oracle_from_spec(spec, "f32")
+    o64 = oracle_from_spec(spec, "f64")
+    e_gpu = e_ref = None
+    for step in range(1, steps + 1):
+        gpu.Step()
+        o32.Step()
+        o64.Step()
+        if step % check_every and step != steps:
+            continue
+        r32, r64 = oracle_fields(o32), oracle_fields(o64)
+        floors = field_scales(r32, o32.dx)
+        e_gpu = field_errors(gpu_fields(gpu), r32, floors=floors)
+        e_ref = field_errors(r32, r64, floors=floors)
+        bad = {k: (e_gpu[k], e_ref[k]) for k in e_gpu if e_gpu[k] > max(rtol, factor * e_ref[k])}
+        if log:
+            log(f"{spec.name} step {step}: worst gpu/o32 {max(e_gpu.values()):.2e}; "
+                + ", ".join(f"{k} {e_gpu[k]:.1e}/{e_ref[k]:.1e}" for k in sorted(e_gpu) if e_gpu[k] > 1e-7))
+        assert not bad, f"{spec.name} step {step}: fields outside tolerance (gpu/o32, o32/o64): {bad}"
+    return e_gpu, e_ref

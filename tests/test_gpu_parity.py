@@ -7,7 +7,7 @@ and numpy's stable argsort; per-substep particle and grid fields agree within
 import numpy as np
 import pytest
 
-from tests.harness import O, field_errors, field_scales, gpu_fields, gpu_from_spec, oracle_fields, oracle_from_spec
+from tests.harness import O, three_way
 
 pytestmark = pytest.mark.gpu
 
@@ -73,24 +73,16 @@ def test_cell_keys_bit_exact(tiled):
     np.testing.assert_array_equal(perm, O.sort_permutation(ref))
 
 
-def _compare(gpu, orc, rtol, label):
-    ref = oracle_fields(orc)
-    errs = field_errors(gpu_fields(gpu), ref, floors=field_scales(ref, orc.dx))
-    worst = max(errs.items(), key=lambda kv: kv[1])
-    print(label, {k: f"{v:.2e}" for k, v in errs.items()})
-    return errs, worst
-
-
 def test_sphere_substep_parity():
-    spec = scenes.sphere_small()
-    gpu = gpu_from_spec(spec)
-    orc = oracle_from_spec(spec)
-    for step in range(3):
-        gpu.Step()
-        orc.Step()
-        errs, worst = _compare(gpu, orc, RTOL, f"step {step + 1}")
-        bad = {k: v for k, v in errs.items() if v > RTOL}
-        assert not bad, f"fields above {RTOL} after step {step + 1}: {bad}"
+    """The small plain scene (Ding + gas + heat/water transport): every field, 3 then 20 substeps."""
+    three_way(scenes.sphere_small(), 20, rtol=RTOL, check_every=1)
+
+
+@pytest.mark.parametrize("builder,steps", [("demo_stretch", 200), ("demo_gas", 40), ("demo_heated", 40)])
+def test_loaded_dough_parity(builder, steps):
+    """Plastic flow, gas-driven porous expansion and heated porous flow: the return map under load."""
+    spec = getattr(scenes, builder)()
+    three_way(spec, steps, rtol=RTOL, check_every=10)
 
 
 def test_free_fall_symplectic_euler():
