@@ -30,6 +30,7 @@ class BkParams(ctypes.Structure):
         ("latent", c_float), ("kappa", c_float), ("diffusivity", c_float), ("s0", c_float), ("water_stress", c_float),
         ("contact_distance", c_float),
         ("n", c_int32), ("n_vertices", c_int32), ("n_faces", c_int32), ("n_slots", c_int32),
+        ("return_solver", c_int32),
     ]
 
 
@@ -105,7 +106,7 @@ SIGNATURES = {
     "bk_ReconstructMeshVertices": ([P, P, P, P, P, P, P, P, I, P], c_int),
     "bk_gas_pressure": ([P, P, P, P, P, I, P], c_int),
     "bk_porosity_coefficient": ([P, P, I, P], c_int),
-    "bk_solve_porosity": ([P, P, P, P, I, P], c_int),
+    "bk_solve_porosity": ([P, P, P, P, I, I, P], c_int),
 }
 
 _lib = None

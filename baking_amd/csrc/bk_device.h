@@ -305,6 +305,87 @@ __device__ __forceinline__ float solve_porosity(float q, float p, float df, floa
   return float((lo + hi) * 0.5);
 }
 
+// Overstress Newton of porous_return.py:19-31 started from `guess` when the
+// residual there is not positive (left of the root on this increasing,
+// concave residual Newton's iterates stay left and increase monotonically),
+// else from 0 as the reference does.
+__device__ __forceinline__ double overstress64_from(double q, double p, double df, double mf, double guess) {
+  double root = 0.0;
+  if (q + p > 1.0) {
+    if (guess > 0.0) {
+      double u = 1.0 + df * guess, v = 1.0 + mf * guess;
+      if (guess + 1.0 - q / (u * u) - p / (v * v) <= 0.0) root = guess;
+    }
+    for (int it = 0; it < 48; ++it) {
+      double u = 1.0 + df * root, v = 1.0 + mf * root;
+      double u2 = u * u, v2 = v * v;
+      double res = root + 1.0 - q / u2 - p / v2;
+      double der = 1.0 + 2.0 * df * q / (u2 * u) + 2.0 * mf * p / (v2 * v);
+      root -= res / der;
+      if (fabs(res) < 1.0e-12) break;
+    }
+  }
+  return root;
+}
+
+// f(n) = n - porosity_prediction(n) and f'(n), with the inner overstress
+// solve warm-started from `gamma` (updated).  f' follows the implicit
+// derivative of the converged scalar equations (porous_return.py:113-137).
+__device__ __forceinline__ double porosity_gap(double n, float qf, float pf, float dff, float msf, float exf, double pv, float n0,
+                                               double& gamma, double& slope) {
+  const double q = qf, p = pf, d = dff, h = msf, ex = exf;
+  double a = pressure_coefficient64(n);
+  gamma = overstress64_from(q, a * p, d, h * a, gamma);
+  double u = 1.0 + d * gamma, v = 1.0 + h * a * gamma;
+  double grown = pv * exp(ex * (1.0 - 1.0 / v));
+  double ra = 0.0;
+  if (q + a * p > 1.0) {
+    double rd = 1.0 + 2.0 * d * q / (u * u * u) + 2.0 * h * a * a * p / (v * v * v);
+    ra = p * (1.0 - h * a * gamma) / (rd * v * v * v);
+  }
+  double ad = n > 0.0 ? a * (2.0 / (n * log(1.0 / n)) - 3.5 / (3.5 * n + 1.0)) : 0.0;
+  slope = 1.0 - grown * ex / (v * v) * h * ad * (gamma + a * ra);
+  return n - (grown - 1.0 + double(n0));
+}
+
+// The end-state porosity equation of solve_porosity (n = prediction(n) on
+// [0, 1 - 1e-9]) solved by safeguarded Newton (Numerical Recipes' rtsafe):
+// start at the old porosity, keep the bisection's bracket invariant
+// f(lo) < 0 <= f(hi), take the Newton step when it stays inside the bracket
+// and at least halves the previous step, else bisect; stop when the step or
+// the bracket is a few ulps.  The root agrees with the reference's 64-step
+// bisection to double rounding, hence to the float32 result it returns.
+// Results at the endpoints (the reference's flagged no-solution cases) are
+// recomputed with the literal bisection.
+__device__ inline float solve_porosity_bracketed(float q, float p, float df, float ms, float ex, double pv, float n0) {
+  double trial = pv - 1.0 + double(n0);
+  if (trial > 0.0 && trial < 1.0) {
+    double tc = pressure_coefficient64(trial);
+    if (double(q) + tc * double(p) <= 1.0) return float(trial);
+  }
+  double lo = 0.0, hi = 1.0 - 1.0e-9;
+  double x = (trial > 0.0 && trial < 1.0) ? trial : 0.5 * (lo + hi);
+  double gamma = 0.0, slope = 1.0, last = hi - lo;
+  bool done = false;
+  for (int it = 0; it < 100 && !done; ++it) {
+    double f = porosity_gap(x, q, p, df, ms, ex, pv, n0, gamma, slope);
+    if (f < 0.0) lo = x; else hi = x;
+    if (hi - lo <= 4.0e-16 * hi) break;
+    double step = slope > 0.0 ? f / slope : 2.0;
+    double xn = x - step;
+    if (!(xn > lo && xn < hi) || fabs(step) > 0.5 * last) {
+      xn = 0.5 * (lo + hi);
+      last = hi - lo;
+    } else {
+      last = fabs(step);
+    }
+    done = fabs(xn - x) <= 4.0e-16 * fabs(x);
+    x = xn;
+  }
+  if (!(x > 1.0e-300) || x >= 1.0 - 2.0e-9) return solve_porosity(q, p, df, ms, ex, pv, n0);
+  return float(x);
+}
+
 // Result of the Ding return map for one particle.
 struct DingOut {
   mat3<float> Fe, Fp, tau;
@@ -317,6 +398,7 @@ struct DingParams {
   float mu, lam, yield_stress, n0, dt;
   float knots[4], yf[4], vf[4];
   int temperature_enabled, coupled;
+  int bracketed;  // 1: solve_porosity_bracketed, 0: the reference's 64-step bisection
 };
 
 // ding.py:103-137 with the SVD of the trial gradient precomputed.
@@ -415,7 +497,7 @@ __device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old
       bool inside = trial_n > 0.0 && trial_n < 1.0 && double(q) + pressure_coefficient64(trial_n) * double(pv) <= 1.0;
       if (!inside) return kDingDeferred;
     }
-    ret_n = solve_porosity(q, pv, df, ms, ex, opv, P.n0);
+    ret_n = P.bracketed ? solve_porosity_bracketed(q, pv, df, ms, ex, opv, P.n0) : solve_porosity(q, pv, df, ms, ex, opv, P.n0);
     double pred = porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
     cres = float(fabs(double(ret_n) - pred));
     if (cres > 1.0e-6f || ret_n <= 0.0f || ret_n >= 1.0f) o.flag1 = 1;
@@ -507,6 +589,23 @@ __device__ inline void reconstruct_vertex_wave(int v, int lane, const float* __r
 }
 
 // ------------------------------------------------------------------ wave-level helpers
+// Append `value` to a device queue for every lane with `take` set: one atomic
+// per wavefront reserves the slots, lanes fill them in lane order.  Call with
+// the whole wavefront converged.
+__device__ __forceinline__ void wave_append(int32_t* counter, int32_t* queue, bool take, int32_t value) {
+  unsigned long long m = __ballot(take);
+  if (m == 0ull) return;
+  const unsigned lane = __lane_id();
+  const unsigned leader = unsigned(__ffsll((long long)m) - 1);
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counter, __popcll(m));
+  base = __shfl(base, int(leader), 64);
+  if (take) {
+    unsigned long long below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+    queue[base + __popcll(below)] = value;
+  }
+}
+
 __device__ __forceinline__ void flag_max(int* flags, int index, bool set) {
   // One atomic per wavefront: the lowest set lane reports.
   unsigned long long m = __ballot(set);

@@ -342,11 +342,12 @@ __global__ void k_porosity_coefficient(const float* por, float* out, int n) {
   if (p < n) out[p] = porosity_coefficient(por[p]);
 }
 // args6 = (q, p, dev_factor, mean_scale, exponent, initial_porosity) per element.
-__global__ void k_solve_porosity(const float* a, const double* pv, float* out, double* pred, int n) {
+__global__ void k_solve_porosity(const float* a, const double* pv, float* out, double* pred, int solver, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* q = a + 6 * i;
-  float r = solve_porosity(q[0], q[1], q[2], q[3], q[4], pv[i], q[5]);
+  float r = solver ? solve_porosity_bracketed(q[0], q[1], q[2], q[3], q[4], pv[i], q[5])
+                   : solve_porosity(q[0], q[1], q[2], q[3], q[4], pv[i], q[5]);
   out[i] = r;
   pred[i] = porosity_prediction(double(r), q[0], q[1], q[2], q[3], q[4], pv[i], q[5]);
 }
@@ -388,6 +389,7 @@ int bk_UpdateDing(const float* oFe, const float* on, const int32_t* og, const fl
   for (int i = 0; i < 4; ++i) { D.knots[i] = kn[i]; D.yf[i] = yf[i]; D.vf[i] = vf[i]; }
   D.temperature_enabled = ton;
   D.coupled = coupled;
+  D.bracketed = 0;
   hipLaunchKernelGGL(k_UpdateDing, GRID(n), oFe, on, og, Ft, Ca, nFe, nFp, nt, res, nn, temps, ng, ys, vs, vp, flags, pr, D, lT, n);
   return launch_status();
 }
@@ -472,9 +474,9 @@ int bk_porosity_coefficient(const float* por, float* out, int32_t n, void* strea
   hipLaunchKernelGGL(k_porosity_coefficient, GRID(n), por, out, n);
   return launch_status();
 }
-int bk_solve_porosity(const float* args6, const double* pv, float* out, double* pred, int32_t n, void* stream) {
+int bk_solve_porosity(const float* args6, const double* pv, float* out, double* pred, int32_t solver, int32_t n, void* stream) {
   if (n <= 0) return 0;
-  hipLaunchKernelGGL(k_solve_porosity, GRID(n), args6, pv, out, pred, n);
+  hipLaunchKernelGGL(k_solve_porosity, GRID(n), args6, pv, out, pred, solver, n);
   return launch_status();
 }
 

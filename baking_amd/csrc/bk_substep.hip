@@ -719,6 +719,7 @@ __device__ inline int material_update(const bk_params& P, const bk_scene& sc, co
   for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
   D.temperature_enabled = P.temperature_enabled;
   D.coupled = P.gas_enabled;
+  D.bracketed = P.return_solver;
   float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
   DingOut r;
   int status = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, sc.visc_param[0], r, defer);
@@ -768,7 +769,7 @@ __global__ __launch_bounds__(256) void k_pass_c_deferred(bk_params P, bk_scene s
 // ============================================================================ pass C: G2P + Ding + gas refresh + saturation
 __global__ __launch_bounds__(256) void k_pass_c(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_grid g, bk_workspace* ws) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  bool stencil_bad = false, f0 = false, f1 = false;
+  bool stencil_bad = false, f0 = false, f1 = false, deferred = false;
   if (p < P.n) {
     const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
     float g3[3];
@@ -816,11 +817,11 @@ __global__ __launch_bounds__(256) void k_pass_c(bk_params P, bk_scene sc, bk_sta
       m_store(out.C + 9 * p, Cn);
       m_store(out.F + 9 * p, F_new);
       if (P.model == 1) {
-        int status = material_update(P, sc, in, out, p, F_new, Cn, true, f0, f1);
-        if (status == kDingDeferred) ws->queue[atomicAdd(&ws->counters[2], 1)] = p;
+        deferred = material_update(P, sc, in, out, p, F_new, Cn, true, f0, f1) == kDingDeferred;
       }
     }
   }
+  wave_append(&ws->counters[2], ws->queue, deferred, p);
   flag_max(ws->flags, BK_FLAG_STENCIL, stencil_bad);
   flag_max(ws->flags, BK_FLAG_POROSITY, f0);
   flag_max(ws->flags, BK_FLAG_COUPLED, f1);

@@ -244,6 +244,7 @@ class MpmSimulator:
             P.n_faces = len(self.transport.faces)
             P.n_slots = len(self.transport.slots["slot_owner"])
         P.n = self.particle_data.particle_count
+        P.return_solver = {"bisection": 0, "bracketed": 1}[self.return_solver]
         return P
 
     @staticmethod
@@ -273,6 +274,16 @@ class MpmSimulator:
 
     # ------------------------------------------------------------------ substep
     use_graphs = True
+    # Coupled-porosity solve of the fused substep: "bracketed" (default) solves the
+    # reference's end-state porosity equation to a few ulps in ~10-20 residual
+    # evaluations; "bisection" runs the reference's 64 fixed bisection steps.
+    return_solver = "bracketed"
+
+    def SetReturnSolver(self, name):
+        if name not in ("bisection", "bracketed"):
+            raise ValueError("return solver must be 'bisection' or 'bracketed'")
+        self.return_solver = name
+        self._params = self._make_params(settling=False)
 
     def _destroy_graphs(self):
         lib = _native.load()

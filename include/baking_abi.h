@@ -64,6 +64,9 @@ typedef struct bk_params {
   float T_oven, T_tray, tray_h, loss_coef, rho_w, latent, kappa, diffusivity, s0, water_stress, contact_distance;
   /* sizes */
   int32_t n, n_vertices, n_faces, n_slots;
+  /* coupled-porosity solve: 0 = the reference's 64-step bisection (porous_return.py:43-61),
+   * 1 = a bracketed Illinois/bisection solve of the same equation to a few ulps */
+  int32_t return_solver;
 } bk_params;
 
 /* Per-particle state of one step (physics/state.py ParticleState, ding.py DingState, thermal_water.py TransportState). */
@@ -225,11 +228,12 @@ int bk_ReconstructMeshVertices(const float* x, const float* F, const float* ref_
                                int32_t n_vertices, void* stream);
 /* Scalar functions evaluated on device for arrays of inputs:
  * physics/materials/gas.py:11-13, ding.py:68-74, porous_return.py:43-61 and 34-40.
- * bk_solve_porosity: args6 = (q, p, dev_factor, mean_scale, exponent, initial_porosity) per element. */
+ * bk_solve_porosity: args6 = (q, p, dev_factor, mean_scale, exponent, initial_porosity) per element;
+ * solver 0 = the reference's bisection, 1 = the bracketed Newton solve of bk_params.return_solver. */
 int bk_gas_pressure(const double* moles, const float* T, const float* J, const float* porosity, double* out, int32_t n, void* stream);
 int bk_porosity_coefficient(const float* porosity, float* out, int32_t n, void* stream);
-int bk_solve_porosity(const float* args6, const double* plastic_volume, float* out_porosity, double* out_prediction, int32_t n,
-                      void* stream);
+int bk_solve_porosity(const float* args6, const double* plastic_volume, float* out_porosity, double* out_prediction,
+                      int32_t solver, int32_t n, void* stream);
 
 #ifdef __cplusplus
 }

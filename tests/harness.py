@@ -142,7 +142,7 @@ class OraclePoint:
         return n
 
 
-def three_way(spec, steps, rtol=1e-5, factor=4.0, check_every=1, log=print):
+def three_way(spec, steps, rtol=1e-5, factor=4.0, check_every=1, log=print, solver=None):
     """Run the GPU path, the FP32 oracle and the FP64 oracle side by side.
 
     After each checked substep every field must satisfy
@@ -153,6 +153,8 @@ def three_way(spec, steps, rtol=1e-5, factor=4.0, check_every=1, log=print):
     Returns the last errors (gpu vs oracle32, oracle32 vs oracle64).
     """
     gpu = gpu_from_spec(spec)
+    if solver is not None:
+        gpu.SetReturnSolver(solver)
     o32 = oracle_from_spec(spec, "f32")
     o64 = oracle_from_spec(spec, "f64")
     e_gpu = e_ref = None

@@ -78,11 +78,49 @@ def test_sphere_substep_parity():
     three_way(scenes.sphere_small(), 20, rtol=RTOL, check_every=1)
 
 
+@pytest.mark.parametrize("solver", ["bracketed", "bisection"])
 @pytest.mark.parametrize("builder,steps", [("demo_stretch", 200), ("demo_gas", 40), ("demo_heated", 40)])
-def test_loaded_dough_parity(builder, steps):
-    """Plastic flow, gas-driven porous expansion and heated porous flow: the return map under load."""
+def test_loaded_dough_parity(builder, steps, solver):
+    """Plastic flow, gas-driven porous expansion and heated porous flow: the return map under load,
+    with both coupled-porosity solvers against the oracle's literal bisection."""
     spec = getattr(scenes, builder)()
-    three_way(spec, steps, rtol=RTOL, check_every=10)
+    three_way(spec, steps, rtol=RTOL, check_every=10, solver=solver)
+
+
+@pytest.mark.parametrize("solver", ["bisection", "bracketed"])
+def test_solve_porosity_matches_oracle(solver):
+    """Coupled porosity on random and realistic inputs, plus the near-closure known answer
+    (reference tests/test_gas.py:208-222): float32 results equal the oracle's bisection."""
+    from baking_amd.physics.materials.porous_return import solve_porosity
+    rng = np.random.default_rng(11)
+    m = 4000
+    por = rng.uniform(0.02, 0.08, m)
+    cols = [rng.uniform(0.5, 20, m), rng.uniform(0, 30, m), 10 ** rng.uniform(-4, 1, m), 10 ** rng.uniform(-4, 1, m),
+            rng.uniform(-0.1, 0.05, m)]
+    real = [rng.uniform(1.0, 1.5, m), rng.uniform(0, 0.05, m), 10 ** rng.uniform(-1, 1, m), 10 ** rng.uniform(-3, 0, m),
+            rng.uniform(-0.01, 0.003, m)]
+    q, p, d, h, ex = [np.concatenate([a, b]).astype(np.float32) for a, b in zip(cols, real)]
+    n0 = np.float32(0.05)
+    pv = 1.0 - float(n0) + np.concatenate([por, por]).astype(np.float32).astype(np.float64)
+    q = np.append(q, np.float32(4.115)); p = np.append(p, np.float32(29.464)); d = np.append(d, np.float32(0.001052))
+    h = np.append(h, np.float32(0.003390)); ex = np.append(ex, np.float32(-0.0230067))
+    pv = np.append(pv, 1.0 - float(np.float32(0.2)) + float(np.float32(3.725290298461914e-7)))
+    n0s = np.full(len(q), n0, np.float32)
+    n0s[-1] = np.float32(0.2)
+    got, pred = solve_porosity(torch.tensor(q), torch.tensor(p), torch.tensor(d), torch.tensor(h), torch.tensor(ex),
+                               torch.tensor(pv), torch.tensor(n0s), solver=solver)
+    got = got.cpu().numpy()
+    ref = np.array([O.solve_porosity(float(a), float(b), float(c), float(e), float(f), float(g), float(k))
+                    for a, b, c, e, f, g, k in zip(q, p, d, h, ex, pv, n0s)], dtype=np.float32)
+    assert 0 < got[-1] < 3.725290298461914e-7
+    np.testing.assert_allclose(got[-1], pred.cpu().numpy()[-1], atol=1e-12)
+    body = slice(0, len(q) - 1)
+    # fp64 log/exp of the device and of the host libm may differ in the last bit, which can flip a
+    # bisection predicate evaluated within ~1e-16 of the root: allow a couple of 1-ulp float differences.
+    mismatch = np.flatnonzero(got[body] != ref[body])
+    print(solver, "float mismatches vs the oracle bisection:", len(mismatch), "of", len(ref) - 1)
+    assert len(mismatch) <= 2, f"{len(mismatch)} float mismatches vs the reference bisection"
+    np.testing.assert_allclose(got[body], ref[body], rtol=1e-6)
 
 
 def test_free_fall_symplectic_euler():
