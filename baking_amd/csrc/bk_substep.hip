@@ -943,9 +943,9 @@ static const bk_workspace* host_view(bk_workspace* ws) {
 // bk_substep on the launch stream (no extra synchronisation while recording).
 #include <vector>
 enum { kStageClear, kStageKeys, kStageSort, kStageTiles, kStageMesh, kStagePassA, kStageDiffuse, kStagePassB1, kStagePassB2,
-       kStageGrid, kStagePassC, kStageCount };
+       kStageGrid, kStagePassC, kStagePassCDeferred, kStageCount };
 static const char* kStageNames[kStageCount] = {"clear", "keys", "sort", "tiles", "mesh", "pass_a", "diffuse", "pass_b1",
-                                               "pass_b2", "grid", "pass_c"};
+                                               "pass_b2", "grid", "pass_c", "pass_c_deferred"};
 struct Profiler {
   int max_steps = 0, step = 0;
   std::vector<hipEvent_t> ev;  // [step][stage][start/stop]
@@ -977,7 +977,7 @@ float* bk_workspace_vertices(bk_workspace* ws) {
 int bk_workspace_counts(bk_workspace* ws, int32_t* out2, void* stream) {
   const bk_workspace* h = host_view(ws);
   if (!h) return -2;
-  BK_HIP(hipMemcpyAsync(out2, h->counters, 8, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  BK_HIP(hipMemcpyAsync(out2, h->counters, 12, hipMemcpyDeviceToHost, (hipStream_t)stream));
   BK_HIP(hipStreamSynchronize((hipStream_t)stream));
   return 0;
 }
@@ -1174,11 +1174,12 @@ static int substep_impl(const bk_params* P, const bk_scene* sc, const bk_state* 
   BK_LAUNCH(grid_update); }
   { StageTimer tm(pr, st, kStagePassC);
   hipLaunchKernelGGL(k_pass_c, dim3(div_up(n, 256)), dim3(256), 0, st, *P, *sc, *in, *out, *g, ws);
-  BK_LAUNCH(pass_c);
+  BK_LAUNCH(pass_c); }
   if (P->model == 1 && P->gas_enabled) {
+    StageTimer tm(pr, st, kStagePassCDeferred);
     hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, *P, *sc, *in, *out, ws);
     BK_LAUNCH(pass_c_deferred);
-  } }
+  }
   if (pr) ++pr->step;
   return 0;
 }

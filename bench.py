@@ -45,6 +45,10 @@ STAGE_BYTES = {
     # porosity, porosity residual, T_mat, gelled, yield, viscosity, pressure, saturation; gathers v
     "pass_c": (12 + 36 + 36 + 4 + 4 + 4 + 8 + 8 + 4 + 96 + 108 + 8 + 16 + 8 + 8, 12),
 }
+# Deferred coupled-porosity return maps, per queued particle: reads Fe_old, porosity, gelled, F, C, T_mat,
+# moles, W, V0; writes Fe, Fp, tau, residual, porosity residual, porosity, T_mat, gelled, yield, viscosity,
+# pressure, saturation.
+DEFERRED_BYTES = 36 + 4 + 4 + 36 + 36 + 4 + 8 + 8 + 4 + 108 + 8 + 4 + 4 + 4 + 4 + 4 + 8 + 8
 SUBSTEP_BYTES = (552, 208)  # whole substep: particle state read + written once, grid work per active node
 
 
@@ -58,6 +62,8 @@ def parse():
     ap.add_argument("--cpu-baseline-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of replaying a HIP graph")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a 1 GiB device-to-device copy 5 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -81,8 +87,36 @@ def cpu_baseline(spec, steps):
                       f"{spec.simulation.grid_resolution}^3 dense grid) from rest, serial oracle (liboracle.so, g++ -O2)"}
 
 
+def measured_traffic(workload, kernel, particles, active_nodes, deferred):
+    """HBM bytes of one launch of `kernel`, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
+    (per-unit bytes, calibrated against a plain copy) scaled to this launch's units; None if not profiled."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as stream:
+        per_kernel = json.load(stream)["per_kernel"]
+    entry = per_kernel.get(kernel)
+    if entry is None:
+        return None
+    units = {"particle": particles, "queued particle": deferred, "active stencil node": active_nodes}[entry["unit"]]
+    return entry["hbm_bytes_per_unit"] * units
+
+
+def copy_probe():
+    import torch
+    a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": a.numel() * 4, "repeats": 5}))
+
+
 def main():
     args = parse()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every configuration here fits one MI355X)",
               file=sys.stderr)
@@ -129,12 +163,20 @@ def main():
     _native.check(lib.bk_profile_end(sim._workspace, ctypes.cast(ms, ctypes.c_void_p), ctypes.cast(ctypes.pointer(rec), ctypes.c_void_p)),
                   "bk_profile_end")
     stages = {lib.bk_profile_stage_name(k).decode(): ms[k] / max(rec.value, 1) for k in range(nst)}
+    counts = (ctypes.c_int32 * 3)()
+    _native.check(lib.bk_workspace_counts(sim._workspace, ctypes.cast(counts, ctypes.c_void_p), _native.stream_ptr()),
+                  "bk_workspace_counts")
+    deferred = int(counts[2])
     active_nodes = int((sim.grid_state.tiled_masses > 0).sum().item())
     dominant = max(stages, key=lambda k: stages[k])
-    per_p, per_node = STAGE_BYTES.get(dominant, (0, 0))
-    dom_bytes = per_p * n + per_node * active_nodes
+    if dominant == "pass_c_deferred":
+        dom_bytes = DEFERRED_BYTES * deferred
+    else:
+        per_p, per_node = STAGE_BYTES.get(dominant, (0, 0))
+        dom_bytes = per_p * n + per_node * active_nodes
     dom_ms = stages[dominant]
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    traffic = measured_traffic(spec.name, dominant, n, active_nodes, deferred)
     substep_ms = gpu_ms / args.steps
     substep_bytes = SUBSTEP_BYTES[0] * n + SUBSTEP_BYTES[1] * active_nodes
     yield_fraction = float((sim.current_state.material_state.return_residuals > 0).float().mean().item()) \
@@ -156,9 +198,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": spec.name, "method": "plain", "hip_graph": sim.use_graphs, "particles": n, "grid": spec.simulation.grid_resolution,
                    "dt": spec.simulation.time_step, "active_stencil_nodes": active_nodes,
-                   "yield_fraction": yield_fraction, "wall_s": wall, "flags": flags, "flags_before_timing": flags_before},
+                   "yield_fraction": yield_fraction, "deferred_return_maps": deferred,
+                   "tiles": int(counts[0]), "active_node_blocks": int(counts[1]), "wall_s": wall, "flags": flags, "flags_before_timing": flags_before},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dom_bytes,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": dom_bytes,
                      "avg_ms": dom_ms, "stages_ms": stages,
                      "substep_achieved": substep_bytes / (substep_ms * 1e-3) / 1e9,
                      "substep_frac": substep_bytes / (substep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},

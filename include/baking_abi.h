@@ -116,7 +116,8 @@ bk_workspace* bk_workspace_create(int32_t max_particles, int32_t res, int32_t n_
 void bk_workspace_destroy(bk_workspace* ws);
 int32_t* bk_workspace_flags(bk_workspace* ws);          /* device int32[BK_FLAG_COUNT] */
 float* bk_workspace_vertices(bk_workspace* ws);         /* device float[V*3]: last reconstructed mesh */
-int bk_workspace_counts(bk_workspace* ws, int32_t* out2, void* stream); /* copies {tiles, active blocks}; synchronises */
+/* Copies {tiles, active node blocks, deferred return maps} of the last substep; synchronises. */
+int bk_workspace_counts(bk_workspace* ws, int32_t* out3, void* stream);
 /* Blocking copy of the flag block to host int32[8] after the work already on `stream`; and the reverse. */
 int bk_workspace_read_flags(bk_workspace* ws, int32_t* host8, void* stream);
 int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* stream);
