@@ -44,11 +44,19 @@ class BkState(ctypes.Structure):
 
 SCENE_FIELDS = ["V0", "m0", "capacities", "dry", "visc_param", "prod_param", "heat_param", "ref_vertices", "bind_offsets",
                 "bind_particles", "bind_weights", "bind_ref_offsets", "faces", "slot_face_offsets", "slot_faces", "slot_owner",
-                "particle_slot"]
+                "particle_slot", "dough_of"]
 
 
 class BkScene(ctypes.Structure):
     _fields_ = [(name, c_void_p) for name in SCENE_FIELDS]
+
+
+class BkDough(ctypes.Structure):
+    _fields_ = [("params", BkParams), ("particle_offset", c_int32), ("block_offset", c_int32)]
+
+
+class BkTuning(ctypes.Structure):
+    _fields_ = [("tile_chunk", c_int32), ("particle_block", c_int32), ("grid_cap", c_int32)]
 
 
 GRID_FIELDS = ["m", "p", "v", "V", "C", "E", "W", "H", "L", "X", "dT", "dW"]
@@ -69,18 +77,26 @@ SIGNATURES = {
     "bk_last_error": ([], c_char_p),
     "bk_device_info": ([c_char_p, c_int, POINTER(c_int), POINTER(c_int)], c_int),
     "bk_workspace_create": ([I, I, I, I], c_void_p),
+    "bk_workspace_create_batch": ([I, I, I, I, I], c_void_p),
     "bk_workspace_destroy": ([P], None),
+    "bk_workspace_set_doughs": ([P, POINTER(BkDough), I], c_int),
+    "bk_workspace_set_tuning": ([P, POINTER(BkTuning)], c_int),
+    "bk_workspace_get_tuning": ([P, POINTER(BkTuning)], c_int),
     "bk_workspace_flags": ([P], c_void_p),
     "bk_workspace_vertices": ([P], c_void_p),
     "bk_workspace_counts": ([P, P, P], c_int),
     "bk_workspace_read_flags": ([P, P, P], c_int),
     "bk_workspace_write_flags": ([P, P, P], c_int),
+    "bk_workspace_read_flags_all": ([P, P, I, P], c_int),
+    "bk_workspace_write_flags_all": ([P, P, I, P], c_int),
     "bk_cell_keys": ([P, I, P, F, I, I, P, P, P], c_int),
     "bk_sort_pairs": ([P, P, P, P, P, I, I, P], c_int),
     "bk_substep": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P, P], c_int),
     "bk_substep_graph_create": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P],
                                 c_void_p),
     "bk_graph_launch": ([P, P], c_int),
+    "bk_batch_substep": ([P, POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P], c_int),
+    "bk_batch_graph_create": ([P, POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid)], c_void_p),
     "bk_graph_destroy": ([P], None),
     "bk_profile_begin": ([P, I], c_int),
     "bk_profile_end": ([P, P, P], c_int),

@@ -1,28 +1,43 @@
-// Fused MLS-MPM baking substep for gfx950 (MI355X) and its workspace, cell
-// keys and stable radix sort.
+// Fused MLS-MPM baking substep for gfx950 (MI355X): workspace, cell keys,
+// stable radix sort, and the substep of a batch of independent doughs.
+//
+// A batch is a table of doughs (a single simulator is a batch of one).  Their
+// particles, tiled grids, meshes and boundary faces are concatenated: dough d
+// owns particles [particle_offset, +n) and node blocks [block_offset, +nb^3)
+// of the batch grid (4x4x4 node blocks of 64 contiguous nodes); mesh, face and
+// slot indices in the scene are global.  Every stage is one launch for the
+// whole batch.  Kernels take the table as a read-only argument and look up
+// their dough per tile, node block or particle with a wave-uniform index, so a
+// dough's parameters are read with scalar loads.
 //
 // One substep (reference physics/mpm.py:163-225) runs as:
-//   clear    zero the grid blocks the previous step touched
-//   keys     cell key of every particle (block-major over 4x4x4 cell blocks)
+//   clear    zero the node blocks the previous substep touched
+//   keys     batch cell key of every particle: (batch block << 6) | cell in block
 //   sort     stable LSD radix sort of (key, particle index), 8-bit digits
-//   tiles    one tile per non-empty particle block; mark the 2x2x2 node blocks
-//            each tile's stencils reach as active
+//   tiles    bounds of each non-empty particle block in the sorted order;
+//            activate the 2x2x2 node blocks each reaches; cut blocks into P2G
+//            tiles of at most `tile_chunk` particles
 //   mesh     reconstruct mesh vertices, then boundary-face areas per owner slot
 //   pass A   thermal P2G (7 f64 scalars) accumulated per tile in LDS, flushed
 //            with one global f64 atomic per touched node and scalar
 //   diffuse  6-neighbour FV heat/water fluxes per active node block
 //   pass B1  scalar G2P + gas source + gas and water stress (per particle)
-//   pass B2  mechanical P2G (mass + momentum) per tile in LDS, then flush
+//   pass B2  mechanical P2G (mass + momentum) per tile in f64 LDS, then flush
 //   grid     momentum -> velocity, gravity, unilateral walls
-//   pass C   G2P + Ding return map + gas refresh + saturations (per particle)
-// The grid lives in a tiled layout (4x4x4 node blocks contiguous) and only
-// active blocks are read, written and cleared.
+//   pass C   G2P + Ding return map + gas refresh + saturations (per particle);
+//            particles whose coupled porosity needs the full solve are queued
+//   pass C'  the queued return maps, densely
 #include "bk_device.h"
 #include "../../include/baking_abi.h"
 
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
+
+#include <mutex>
 #include <string>
+#include <unordered_map>
+#include <vector>
 
 using namespace bk;
 
@@ -52,6 +67,7 @@ constexpr int kSortTile = kSortThreads * kSortItems;  // keys per workgroup
 constexpr int kTileThreads = 256;                     // P2G workgroup size
 constexpr int kRegion = 6;                            // 4 cells + 2-node stencil reach
 constexpr int kRegionNodes = kRegion * kRegion * kRegion;
+constexpr int kParticleThreads = 256;                 // launch bound of the per-particle kernels
 
 inline int div_up(long long a, long long b) { return int((a + b - 1) / b); }
 inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
@@ -59,7 +75,8 @@ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v
 }  // namespace
 
 struct bk_workspace {
-  int max_n = 0, res = 0, nb = 0, nblocks = 0, n_vertices = 0, n_slots = 0;
+  int max_n = 0, total_blocks = 0, max_doughs = 0, n_vertices = 0, n_slots = 0;
+  bk_dough* doughs = nullptr;                   // device copy of the dough table
   uint32_t* keys_raw = nullptr;                 // cell keys in particle order
   uint32_t* keys_s[2] = {nullptr, nullptr};     // radix scratch pairs
   int32_t* vals_s[2] = {nullptr, nullptr};
@@ -67,26 +84,76 @@ struct bk_workspace {
   int32_t* sorted_vals = nullptr;
   int32_t* hist = nullptr;
   int32_t* digit_total = nullptr;
-  int32_t* blk_start = nullptr;
+  int32_t* blk_start = nullptr;                 // per batch block: first sorted particle
   int32_t* blk_end = nullptr;
-  int32_t* node_flag = nullptr;
-  int32_t* tiles = nullptr;
-  int32_t* active = nullptr;
-  int32_t* counters = nullptr;  // [0] tiles, [1] active blocks, [2] deferred return maps
-  int32_t* queue = nullptr;     // particles whose coupled porosity needs the bisection solve
+  int32_t* node_flag = nullptr;                 // per batch block: active this substep
+  int32_t* blocks = nullptr;                    // non-empty particle blocks
+  int4* tiles = nullptr;                        // P2G tiles: (block, dough, first sorted particle, count)
+  int32_t* active = nullptr;                    // active node blocks
+  int32_t* counters = nullptr;                  // [0] tiles, [1] active blocks, [2] deferred, [3] particle blocks
+  int32_t* queue = nullptr;                     // particles whose coupled porosity needs the full solve
   float* tau = nullptr;
   float* vertices = nullptr;
   double* slot_exposed = nullptr;
   double* slot_contact = nullptr;
-  int32_t* flags = nullptr;
+  int32_t* flags = nullptr;                     // BK_FLAG_COUNT per dough
 };
 
-// Host-side cache of workspace descriptors (the struct lives in device memory
-// so kernels can take it by pointer; the host keeps a copy per workspace).
-#include <unordered_map>
-#include <mutex>
+// Host side of a workspace: a copy of the device descriptor, the uploaded
+// dough table with its totals, and the launch tuning.
+struct HostWs {
+  bk_workspace dev;
+  std::vector<bk_dough> doughs;
+  int n = 0, n_vertices = 0, n_slots = 0, key_bits = 6;
+  bool thermal = false, gas = false;
+  bk_tuning tuning{4096, 256, 4096};
+};
 static std::mutex g_ws_mutex;
-static std::unordered_map<const bk_workspace*, bk_workspace> g_ws_cache;
+static std::unordered_map<const bk_workspace*, HostWs> g_ws_cache;
+
+// ============================================================================ dough lookup
+// Run body(d) for the wavefront's active lanes.  Doughs are contiguous in
+// particle order, so nearly every wavefront holds one dough: then body gets it
+// wave-uniform (an SGPR) and reads the dough's table entry with scalar loads.
+// A wavefront straddling two doughs runs body with the per-lane dough (vector
+// loads of the entry).  The test goes through a ballot: a branch on (d == u)
+// would let the compiler substitute the per-lane d for u inside body.
+template <class Body> __device__ __forceinline__ void per_dough(int d, Body&& body) {
+  const int u = __builtin_amdgcn_readfirstlane(d);
+  if (__ballot(d != u) == 0ull) body(u);
+  else body(d);
+}
+
+__device__ __forceinline__ int dough_of_particle(const int32_t* dough_of, int p) { return dough_of ? dough_of[p] : 0; }
+
+// Dough owning batch block b: binary search over the table's block offsets.
+__device__ __forceinline__ int dough_of_block(const bk_dough* __restrict__ D, int nd, int b) {
+  int lo = 0, hi = nd - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (D[mid].block_offset <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Position of batch block b inside its dough's grid.
+struct BlockGeom {
+  int d, nb, bx, by, bz;
+  int64_t node_base;  // first node of the dough's grid in the batch grid
+};
+__device__ __forceinline__ BlockGeom block_geom(const bk_dough* __restrict__ D, int d, int b) {
+  BlockGeom g;
+  g.d = d;
+  g.nb = (D[d].params.res + 3) / 4;
+  int lb = b - D[d].block_offset;
+  g.bz = lb % g.nb;
+  g.by = (lb / g.nb) % g.nb;
+  g.bx = lb / (g.nb * g.nb);
+  g.node_base = int64_t(D[d].block_offset) << 6;
+  return g;
+}
+
+__device__ __forceinline__ int32_t* dough_flags(const bk_workspace* ws, int d) { return ws->flags + d * BK_FLAG_COUNT; }
 
 // ============================================================================ keys + radix sort
 __device__ __forceinline__ uint32_t tiled_cell_key(const int b[3], int nb) {
@@ -94,10 +161,10 @@ __device__ __forceinline__ uint32_t tiled_cell_key(const int b[3], int nb) {
   return (blk << 6) | uint32_t(((b[0] & 3) << 4) | ((b[1] & 3) << 2) | (b[2] & 3));
 }
 
+// Stand-alone cell keys (bk_cell_keys): row-major flatten_node, or tiled over one grid.
 __global__ void k_cell_keys(const float* __restrict__ x, int n, float ox, float oy, float oz, float dx, int res, int tiled,
-                            uint32_t* __restrict__ keys, int32_t* __restrict__ flags, int32_t* counters) {
+                            uint32_t* __restrict__ keys, int32_t* __restrict__ flags) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (counters && p == 0) { counters[0] = 0; counters[1] = 0; counters[2] = 0; }
   if (p >= n) return;
   const float o[3] = {ox, oy, oz};
   float g[3];
@@ -109,6 +176,28 @@ __global__ void k_cell_keys(const float* __restrict__ x, int n, float ox, float 
   uint32_t key = 0;
   if (ok) key = tiled ? tiled_cell_key(b, (res + 3) / 4) : uint32_t(flatten_node(b[0], b[1], b[2], res));
   keys[p] = key;
+}
+
+// Batch keys: the dough's first block plus its tiled key, so one sort orders
+// every dough.  Thread 0 also resets the per-substep counters.  Off-grid
+// particles take their dough's first cell (the P2G tiles skip them) and raise
+// the stencil flag.
+__global__ void k_batch_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of,
+                             const float* __restrict__ x, int n, uint32_t* __restrict__ keys, bk_workspace* ws) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p == 0) { ws->counters[0] = 0; ws->counters[1] = 0; ws->counters[2] = 0; ws->counters[3] = 0; }
+  if (p >= n) return;
+  per_dough(dough_of_particle(dough_of, p), [&](int d) {
+    const bk_params& P = doughs[d].params;
+    const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+    float g[3];
+    grid_position(x + 3 * p, o, P.dx, g);
+    int b[3];
+    stencil_base(g, b);
+    bool ok = valid_stencil(b, P.res);
+    flag_max(dough_flags(ws, d), BK_FLAG_STENCIL, !ok);
+    keys[p] = (uint32_t(doughs[d].block_offset) << 6) + (ok ? tiled_cell_key(b, (P.res + 3) / 4) : 0u);
+  });
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
@@ -244,48 +333,52 @@ static int radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int
   return 0;
 }
 
-static int tiled_key_bits(const bk_workspace* h) {
-  int key_bits = 6;
-  while ((1ll << key_bits) < (long long)h->nblocks * 64) ++key_bits;
-  return key_bits;
-}
-
-// ============================================================================ tiles and active node blocks
-__global__ void k_tile_bounds(const uint32_t* __restrict__ skeys, int n, int32_t* blk_start, int32_t* blk_end, int32_t* tiles,
-                              int32_t* counters) {
+// ============================================================================ blocks, tiles and active node blocks
+// Bounds of every non-empty particle block in the sorted order, and the list of those blocks.
+__global__ void k_block_bounds(const uint32_t* __restrict__ skeys, int n, bk_workspace* ws) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t blk = skeys[i] >> 6;
-  if (i == 0 || (skeys[i - 1] >> 6) != blk) {
-    blk_start[blk] = i;
-    int slot = atomicAdd(&counters[0], 1);
-    tiles[slot] = int(blk);
+  bool start = false;
+  uint32_t blk = 0;
+  if (i < n) {
+    blk = skeys[i] >> 6;
+    start = i == 0 || (skeys[i - 1] >> 6) != blk;
+    if (start) ws->blk_start[blk] = i;
+    if (i == n - 1 || (skeys[i + 1] >> 6) != blk) ws->blk_end[blk] = i + 1;
   }
-  if (i == n - 1 || (skeys[i + 1] >> 6) != blk) blk_end[blk] = i + 1;
+  wave_append(&ws->counters[3], ws->blocks, start, int32_t(blk));
 }
 
-__global__ void k_activate(const int32_t* __restrict__ tiles, int32_t* counters, int32_t* node_flag, int32_t* active, int nb) {
-  int ntiles = counters[0];
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles * 8; t += gridDim.x * blockDim.x) {
-    int blk = tiles[t >> 3];
-    int d = t & 7;
-    int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
-    int cx = bx + (d >> 2), cy = by + ((d >> 1) & 1), cz = bz + (d & 1);
-    if (cx >= nb || cy >= nb || cz >= nb) continue;
-    int nbk = (cx * nb + cy) * nb + cz;
-    if (atomicExch(&node_flag[nbk], 1) == 0) {
-      int slot = atomicAdd(&counters[1], 1);
-      active[slot] = nbk;
+// Eight lanes per non-empty particle block: each activates one of the 2x2x2
+// node blocks the block's stencils reach; lane 0 also cuts the block into
+// tiles of at most `chunk` particles.
+__global__ void k_activate(const bk_dough* __restrict__ doughs, int nd, bk_workspace* ws, int chunk) {
+  const int nblk = ws->counters[3];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nblk * 8; t += gridDim.x * blockDim.x) {
+    const int b = ws->blocks[t >> 3];
+    const int corner = t & 7;
+    const BlockGeom g = block_geom(doughs, dough_of_block(doughs, nd, b), b);
+    if (corner == 0) {
+      const int start = ws->blk_start[b], end = ws->blk_end[b];
+      for (int s = start; s < end; s += chunk) {
+        int slot = atomicAdd(&ws->counters[0], 1);
+        ws->tiles[slot] = make_int4(b, g.d, s, min(chunk, end - s));
+      }
+    }
+    int cx = g.bx + (corner >> 2), cy = g.by + ((corner >> 1) & 1), cz = g.bz + (corner & 1);
+    if (cx >= g.nb || cy >= g.nb || cz >= g.nb) continue;
+    int nbk = doughs[g.d].block_offset + (cx * g.nb + cy) * g.nb + cz;
+    if (atomicExch(&ws->node_flag[nbk], 1) == 0) {
+      int slot = atomicAdd(&ws->counters[1], 1);
+      ws->active[slot] = nbk;
     }
   }
 }
 
-// Zero every grid array over the previous step's active blocks (64 nodes each).
-__global__ void k_clear_blocks(const int32_t* __restrict__ active, const int32_t* counters, int32_t* node_flag, bk_grid g,
-                               int thermal) {
-  int nact = counters[1];
+// Zero every grid array over the previous substep's active blocks (64 nodes each).
+__global__ void k_clear_blocks(bk_workspace* ws, bk_grid g, int thermal) {
+  int nact = ws->counters[1];
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
-    int blk = active[t >> 6];
+    int blk = ws->active[t >> 6];
     int64_t node = (int64_t(blk) << 6) | (t & 63);
     g.m[node] = 0.0f;
 #pragma unroll
@@ -294,7 +387,7 @@ __global__ void k_clear_blocks(const int32_t* __restrict__ active, const int32_t
       g.V[node] = 0.0; g.C[node] = 0.0; g.E[node] = 0.0; g.W[node] = 0.0;
       g.H[node] = 0.0; g.L[node] = 0.0; g.X[node] = 0.0; g.dT[node] = 0.0; g.dW[node] = 0.0;
     }
-    if ((t & 63) == 0) node_flag[blk] = 0;
+    if ((t & 63) == 0) ws->node_flag[blk] = 0;
   }
 }
 
@@ -323,20 +416,24 @@ __device__ __forceinline__ void face_area(const float* verts, const int32_t* f, 
   contact = cy <= floor_cd && ny < -0.2 * len;
 }
 
-__global__ void k_slot_areas(const float* __restrict__ verts, const int32_t* __restrict__ faces, const int32_t* __restrict__ soff,
-                             const int32_t* __restrict__ sfaces, double* ex, double* co, int nslots, float floor_h, float cdist) {
+__global__ void k_slot_areas(const bk_dough* __restrict__ doughs, bk_scene sc, const float* __restrict__ verts, bk_workspace* ws,
+                             int nslots) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nslots) return;
-  double fcd = double(floor_h + cdist);
-  double e = 0.0, c = 0.0;
-  for (int k = soff[s]; k < soff[s + 1]; ++k) {
-    double a;
-    bool isc;
-    face_area(verts, faces + 3 * sfaces[k], fcd, a, isc);
-    if (isc) c += a; else e += a;
-  }
-  ex[s] = e;
-  co[s] = c;
+  per_dough(dough_of_particle(sc.dough_of, sc.slot_owner[s]), [&](int d) {
+    const bk_params& P = doughs[d].params;
+    const float floor_h = P.origin[1] + float(P.boundary_cells - 1) * P.dx;
+    double fcd = double(floor_h + P.contact_distance);
+    double e = 0.0, c = 0.0;
+    for (int k = sc.slot_face_offsets[s]; k < sc.slot_face_offsets[s + 1]; ++k) {
+      double a;
+      bool isc;
+      face_area(verts, sc.faces + 3 * sc.slot_faces[k], fcd, a, isc);
+      if (isc) c += a; else e += a;
+    }
+    ws->slot_exposed[s] = e;
+    ws->slot_contact[s] = c;
+  });
 }
 
 // ============================================================================ per-particle transport terms
@@ -368,39 +465,41 @@ __device__ __forceinline__ Robin robin_terms(const bk_params& P, float heat_para
   return r;
 }
 
-struct TileGeom {
-  int blk, start, count, bx, by, bz;
+// One P2G tile: up to `chunk` sorted particles of one particle block.  Every
+// field is wave-uniform.
+struct Tile {
+  int d, start, count;
+  BlockGeom g;
 };
-__device__ __forceinline__ TileGeom tile_geom(const bk_workspace* ws, int t) {
-  TileGeom g;
-  g.blk = ws->tiles[t];
-  g.start = ws->blk_start[g.blk];
-  g.count = ws->blk_end[g.blk] - g.start;
-  int nb = ws->nb;
-  g.bz = g.blk % nb;
-  g.by = (g.blk / nb) % nb;
-  g.bx = g.blk / (nb * nb);
-  return g;
+__device__ __forceinline__ Tile tile_at(const bk_dough* __restrict__ doughs, const bk_workspace* ws, int t) {
+  const int4 e = ws->tiles[t];
+  Tile T;
+  const int blk = __builtin_amdgcn_readfirstlane(e.x);
+  T.d = __builtin_amdgcn_readfirstlane(e.y);
+  T.start = __builtin_amdgcn_readfirstlane(e.z);
+  T.count = __builtin_amdgcn_readfirstlane(e.w);
+  T.g = block_geom(doughs, T.d, blk);
+  return T;
 }
-
-// Lane -> sorted particle: lanes of one wave take particles `stride` apart so
-// that, with ~8 particles per cell, neighbouring lanes sit in different cells
-// and their LDS atomics hit different nodes.
-struct TileIter {
-  int stride, sub0;
-};
 
 __device__ __forceinline__ int region_index(int li, int lj, int lk) { return (li * kRegion + lj) * kRegion + lk; }
 
+// In both P2G passes, lanes of one wave take a tile's particles S apart
+// (S = ceil(count / 64)) so that, with ~8 particles per cell, neighbouring
+// lanes sit in different cells and their LDS atomics hit different nodes.
+
 // ============================================================================ pass A: thermal P2G
-__global__ __launch_bounds__(kTileThreads) void k_pass_a(bk_params P, bk_scene sc, bk_state in, bk_grid g, bk_workspace* ws) {
+__global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_grid g,
+                                                         bk_workspace* ws) {
   __shared__ double acc[7][kRegionNodes];
   const int ntiles = ws->counters[0];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
-  const float heat_param = P.settling ? 0.0f : sc.heat_param[0];
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    TileGeom tg = tile_geom(ws, t);
+    const Tile tg = tile_at(doughs, ws, t);
+    const bk_params& P = doughs[tg.d].params;
+    if (!P.transport_enabled) continue;  // a dough without transport in a batch with some (uniform per tile)
+    const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+    const float heat_param = P.settling ? 0.0f : sc.heat_param[tg.d];
     for (int i = threadIdx.x; i < 7 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
     __syncthreads();
     const int S = (tg.count + 63) / 64;
@@ -412,7 +511,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(bk_params P, bk_scene s
       grid_position(in.x + 3 * p, o, P.dx, g3);
       int b[3];
       stencil_base(g3, b);
-      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.bx || (b[1] >> 2) != tg.by || (b[2] >> 2) != tg.bz) continue;
+      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.g.bx || (b[1] >> 2) != tg.g.by || (b[2] >> 2) != tg.g.bz) continue;
       float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
       float wx[3], wy[3], wz[3];
       axis_weights(l[0], wx);
@@ -422,8 +521,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(bk_params P, bk_scene s
       double T = in.T[p], W = in.W[p];
       Robin r = robin_terms(P, heat_param, Jf, sc.V0[p], T, W, particle_surface(sc, ws, p));
       double cap = sc.capacities[p];
-      double vals[7] = {r.vol, cap, 0.0, W, r.heat, r.loss, r.exch};
-      const int li = b[0] - 4 * tg.bx, lj = b[1] - 4 * tg.by, lk = b[2] - 4 * tg.bz;
+      const int li = b[0] - 4 * tg.g.bx, lj = b[1] - 4 * tg.g.by, lk = b[2] - 4 * tg.g.bz;
 #pragma unroll
       for (int ox = 0; ox < 3; ++ox)
 #pragma unroll
@@ -432,21 +530,21 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(bk_params P, bk_scene s
           for (int oz = 0; oz < 3; ++oz) {
             double wt = double(wx[ox] * wy[oy] * wz[oz]);
             int ri = region_index(li + ox, lj + oy, lk + oz);
-            atomicAdd(&acc[0][ri], wt * vals[0]);
+            atomicAdd(&acc[0][ri], wt * r.vol);
             atomicAdd(&acc[1][ri], wt * cap);
             atomicAdd(&acc[2][ri], wt * cap * T);
             atomicAdd(&acc[3][ri], wt * W);
-            atomicAdd(&acc[4][ri], wt * vals[4]);
-            atomicAdd(&acc[5][ri], wt * vals[5]);
-            atomicAdd(&acc[6][ri], wt * vals[6]);
+            atomicAdd(&acc[4][ri], wt * r.heat);
+            atomicAdd(&acc[5][ri], wt * r.loss);
+            atomicAdd(&acc[6][ri], wt * r.exch);
           }
     }
     __syncthreads();
     for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
       int li = ri / 36, lj = (ri / 6) % 6, lk = ri % 6;
-      int i = 4 * tg.bx + li, j = 4 * tg.by + lj, k = 4 * tg.bz + lk;
+      int i = 4 * tg.g.bx + li, j = 4 * tg.g.by + lj, k = 4 * tg.g.bz + lk;
       if (i >= P.res || j >= P.res || k >= P.res) continue;
-      int64_t node = tiled_node(i, j, k, ws->nb);
+      int64_t node = tg.g.node_base + tiled_node(i, j, k, tg.g.nb);
       double* dst[7] = {g.V, g.C, g.E, g.W, g.H, g.L, g.X};
 #pragma unroll
       for (int a = 0; a < 7; ++a) {
@@ -459,23 +557,25 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(bk_params P, bk_scene s
 }
 
 // ============================================================================ diffusion (thermal_water.py:140-190, GridFields fused)
-__global__ __launch_bounds__(256) void k_diffuse(bk_params P, bk_grid g, bk_workspace* ws) {
+// 64 consecutive threads own one active node block, so the block (and its
+// dough) is wave-uniform.
+__global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ doughs, int nd, bk_grid g, bk_workspace* ws) {
   const int nact = ws->counters[1];
-  const int nb = ws->nb;
-  const double rho_w = double(P.rho_w);
-  const double dt = double(P.dt);
-  const double dx3 = double(P.dx * P.dx * P.dx);
-  const double hk = double(P.kappa * P.dx);
-  const double wk = double(P.diffusivity * P.rho_w * P.dx);
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
-    int blk = ws->active[t >> 6];
-    int loc = t & 63;
-    int64_t node = (int64_t(blk) << 6) | loc;
+    const int blk = __builtin_amdgcn_readfirstlane(ws->active[t >> 6]);
+    const BlockGeom bg = block_geom(doughs, dough_of_block(doughs, nd, blk), blk);
+    const bk_params& P = doughs[bg.d].params;
+    const int loc = t & 63;
+    const int64_t node = (int64_t(blk) << 6) | loc;
     double Ci = g.C[node];
     bool heat_bad = false, water_bad = false;
     if (Ci > 0.0) {
-      int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
-      int c[3] = {4 * bx + (loc >> 4), 4 * by + ((loc >> 2) & 3), 4 * bz + (loc & 3)};
+      const double rho_w = double(P.rho_w);
+      const double dt = double(P.dt);
+      const double dx3 = double(P.dx * P.dx * P.dx);
+      const double hk = double(P.kappa * P.dx);
+      const double wk = double(P.diffusivity * P.rho_w * P.dx);
+      int c[3] = {4 * bg.bx + (loc >> 4), 4 * bg.by + ((loc >> 2) & 3), 4 * bg.bz + (loc & 3)};
       double Vi = g.V[node];
       double Ti = g.E[node] / Ci;
       double Si = g.W[node] / (rho_w * Vi);
@@ -487,7 +587,7 @@ __global__ __launch_bounds__(256) void k_diffuse(bk_params P, bk_grid g, bk_work
           int o[3] = {c[0], c[1], c[2]};
           o[ax] += 2 * side - 1;
           if (o[ax] < 0 || o[ax] >= P.res) continue;
-          int64_t other = tiled_node(o[0], o[1], o[2], nb);
+          int64_t other = bg.node_base + tiled_node(o[0], o[1], o[2], bg.nb);
           double Cj = g.C[other];
           if (Cj > 0.0) {
             double Vj = g.V[other];
@@ -510,17 +610,22 @@ __global__ __launch_bounds__(256) void k_diffuse(bk_params P, bk_grid g, bk_work
       g.dT[node] = dt * hr / Ci;
       g.dW[node] = change;
     }
-    flag_max(ws->flags, BK_FLAG_HEAT, heat_bad);
-    flag_max(ws->flags, BK_FLAG_WATER, water_bad);
+    flag_max(dough_flags(ws, bg.d), BK_FLAG_HEAT, heat_bad);
+    flag_max(dough_flags(ws, bg.d), BK_FLAG_WATER, water_bad);
   }
 }
 
 // ============================================================================ pass B1: scalar G2P, gas, stresses
-__global__ __launch_bounds__(256) void k_pass_b1(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_grid g, bk_workspace* ws) {
+__global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in,
+                                                              bk_state out, bk_grid g, bk_workspace* ws, int n) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  bool jbad = false, pbad = false, cflbad = false;
-  if (p < P.n) {
+  if (p >= n) return;
+  per_dough(dough_of_particle(sc.dough_of, p), [&](int d) {
+    const bk_params& P = doughs[d].params;
     const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+    const int nb = (P.res + 3) / 4;
+    const int64_t node_base = int64_t(doughs[d].block_offset) << 6;
+    bool jbad = false, pbad = false, cflbad = false;
     mat3<float> F = m_load(in.F + 9 * p);
     float Jf = m_det(F);
     mat3<float> tau;
@@ -545,14 +650,14 @@ __global__ __launch_bounds__(256) void k_pass_b1(bk_params P, bk_scene sc, bk_st
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
               double wt = double(wx[ox] * wy[oy] * wz[oz]);
-              int64_t node = tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, ws->nb);
+              int64_t node = node_base + tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, nb);
               tc += wt * g.dT[node];
               double nw = g.W[node];
               if (nw > 0.0) mr += wt * g.dW[node] / nw;
             }
       }
       double T_old = in.T[p], W_old = in.W[p];
-      const float heat_param = P.settling ? 0.0f : sc.heat_param[0];
+      const float heat_param = P.settling ? 0.0f : sc.heat_param[d];
       Robin r = robin_terms(P, heat_param, Jf, sc.V0[p], T_old, W_old, particle_surface(sc, ws, p));
       double T_new = T_old + tc;
       double W_new = W_old * (1.0 + mr);
@@ -572,7 +677,7 @@ __global__ __launch_bounds__(256) void k_pass_b1(bk_params P, bk_scene sc, bk_st
         if (!P.settling) {
           double tm = 0.5 * (T_old + T_new);
           double z = (tm - double(P.gas_peak)) / double(P.gas_width);
-          moles += double(P.dt * P.density) * double(sc.prod_param[0]) * exp(-(z * z));
+          moles += double(P.dt * P.density) * double(sc.prod_param[d]) * exp(-(z * z));
         }
         out.moles[p] = moles;
       }
@@ -607,25 +712,26 @@ __global__ __launch_bounds__(256) void k_pass_b1(bk_params P, bk_scene sc, bk_st
       tau.a[0] -= k; tau.a[4] -= k; tau.a[8] -= k;
     }
     m_store(ws->tau + 9 * p, tau);
-  }
-  flag_max(ws->flags, BK_FLAG_J, jbad);
-  flag_max(ws->flags, BK_FLAG_POROSITY, pbad);
-  flag_max(ws->flags, BK_FLAG_GAS_CFL, cflbad);
+    int32_t* fl = dough_flags(ws, d);
+    flag_max(fl, BK_FLAG_J, jbad);
+    flag_max(fl, BK_FLAG_POROSITY, pbad);
+    flag_max(fl, BK_FLAG_GAS_CFL, cflbad);
+  });
 }
 
 // ============================================================================ pass B2: mechanical P2G (physics/transfer.py:36-70)
-__global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_grid g,
-                                                          bk_workspace* ws) {
+__global__ __launch_bounds__(kTileThreads) void k_pass_b2(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_state out,
+                                                          bk_grid g, bk_workspace* ws) {
   // f64 LDS accumulators: ds_add_f64 sustains far higher throughput than ds_add_f32 on gfx950
   // for this scatter, and each node's sum is rounded to float once, at the flush.
   __shared__ double acc[4][kRegionNodes];
   const int ntiles = ws->counters[0];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
-  const float* masses = P.transport_enabled ? out.masses : sc.m0;
-  const float s_coef = 4.0f / (P.dx * P.dx);
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    TileGeom tg = tile_geom(ws, t);
+    const Tile tg = tile_at(doughs, ws, t);
+    const bk_params& P = doughs[tg.d].params;
+    const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+    const float* masses = P.transport_enabled ? out.masses : sc.m0;
     for (int i = threadIdx.x; i < 4 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
     __syncthreads();
     const int S = (tg.count + 63) / 64;
@@ -637,7 +743,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene 
       grid_position(in.x + 3 * p, o, P.dx, g3);
       int b[3];
       stencil_base(g3, b);
-      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.bx || (b[1] >> 2) != tg.by || (b[2] >> 2) != tg.bz) continue;
+      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.g.bx || (b[1] >> 2) != tg.g.by || (b[2] >> 2) != tg.g.bz) continue;
       float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
       float wx[3], wy[3], wz[3];
       axis_weights(l[0], wx);
@@ -651,7 +757,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene 
 #pragma unroll
       for (int i = 0; i < 9; ++i) A.a[i] = mass * Cm.a[i] - sc_tau * tau.a[i];
       float mv[3] = {mass * in.v[3 * p + 0], mass * in.v[3 * p + 1], mass * in.v[3 * p + 2]};
-      const int li = b[0] - 4 * tg.bx, lj = b[1] - 4 * tg.by, lk = b[2] - 4 * tg.bz;
+      const int li = b[0] - 4 * tg.g.bx, lj = b[1] - 4 * tg.g.by, lk = b[2] - 4 * tg.g.bz;
 #pragma unroll
       for (int ox = 0; ox < 3; ++ox)
 #pragma unroll
@@ -659,21 +765,21 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene 
 #pragma unroll
           for (int oz = 0; oz < 3; ++oz) {
             float wt = wx[ox] * wy[oy] * wz[oz];
-            float d[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
+            float dd[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
             int ri = region_index(li + ox, lj + oy, lk + oz);
             atomicAdd(&acc[0][ri], double(wt * mass));
 #pragma unroll
             for (int a = 0; a < 3; ++a)
-              atomicAdd(&acc[1 + a][ri], double(wt * (mv[a] + (A.a[a * 3 + 0] * d[0] + A.a[a * 3 + 1] * d[1] + A.a[a * 3 + 2] * d[2]))));
+              atomicAdd(&acc[1 + a][ri],
+                        double(wt * (mv[a] + (A.a[a * 3 + 0] * dd[0] + A.a[a * 3 + 1] * dd[1] + A.a[a * 3 + 2] * dd[2]))));
           }
     }
-    (void)s_coef;
     __syncthreads();
     for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
       int li = ri / 36, lj = (ri / 6) % 6, lk = ri % 6;
-      int i = 4 * tg.bx + li, j = 4 * tg.by + lj, k = 4 * tg.bz + lk;
+      int i = 4 * tg.g.bx + li, j = 4 * tg.g.by + lj, k = 4 * tg.g.bz + lk;
       if (i >= P.res || j >= P.res || k >= P.res) continue;
-      int64_t node = tiled_node(i, j, k, ws->nb);
+      int64_t node = tg.g.node_base + tiled_node(i, j, k, tg.g.nb);
       float m = float(acc[0][ri]);
       if (m != 0.0f) unsafeAtomicAdd(g.m + node, m);
 #pragma unroll
@@ -687,17 +793,17 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(bk_params P, bk_scene 
 }
 
 // ============================================================================ grid update (physics/grid.py:20-38)
-__global__ __launch_bounds__(256) void k_grid_update(bk_params P, bk_grid g, bk_workspace* ws) {
+__global__ __launch_bounds__(256) void k_grid_update(const bk_dough* __restrict__ doughs, int nd, bk_grid g, bk_workspace* ws) {
   const int nact = ws->counters[1];
-  const int nb = ws->nb;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
-    int blk = ws->active[t >> 6];
-    int loc = t & 63;
-    int64_t node = (int64_t(blk) << 6) | loc;
+    const int blk = __builtin_amdgcn_readfirstlane(ws->active[t >> 6]);
+    const int loc = t & 63;
+    const int64_t node = (int64_t(blk) << 6) | loc;
     float m = g.m[node];
     if (!(m > 0.0f)) continue;
-    int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
-    int c[3] = {4 * bx + (loc >> 4), 4 * by + ((loc >> 2) & 3), 4 * bz + (loc & 3)};
+    const BlockGeom bg = block_geom(doughs, dough_of_block(doughs, nd, blk), blk);
+    const bk_params& P = doughs[bg.d].params;
+    int c[3] = {4 * bg.bx + (loc >> 4), 4 * bg.by + ((loc >> 2) & 3), 4 * bg.bz + (loc & 3)};
     float vel[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -710,10 +816,12 @@ __global__ __launch_bounds__(256) void k_grid_update(bk_params P, bk_grid g, bk_
   }
 }
 
-// Ding return map, gas refresh and saturation of particle p after its G2P
+// ============================================================================ material update after G2P
+// Ding return map, gas refresh and saturation of particle p
 // (ding.py:303-327, gas.py:38-45, thermal_water.py:256-260).
-__device__ inline int material_update(const bk_params& P, const bk_scene& sc, const bk_state& in, const bk_state& out, int p,
-                                      const mat3<float>& F_new, const mat3<float>& Cn, bool defer, bool& f0, bool& f1) {
+__device__ inline int material_update(const bk_params& P, float visc_param, const bk_scene& sc, const bk_state& in,
+                                      const bk_state& out, int p, const mat3<float>& F_new, const mat3<float>& Cn, bool defer,
+                                      bool& f0, bool& f1) {
   DingParams D;
   D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
   for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
@@ -722,7 +830,7 @@ __device__ inline int material_update(const bk_params& P, const bk_scene& sc, co
   D.bracketed = P.return_solver;
   float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
   DingOut r;
-  int status = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, sc.visc_param[0], r, defer);
+  int status = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, visc_param, r, defer);
   f0 = r.flag0;
   f1 = r.flag1;
   if (status == kDingDeferred) return status;
@@ -751,80 +859,88 @@ __device__ inline int material_update(const bk_params& P, const bk_scene& sc, co
 }
 
 // Second half of pass C: the deferred coupled-porosity return maps, 64 per
-// wavefront with no idle lanes waiting on a neighbour's bisection.
-__global__ __launch_bounds__(256) void k_pass_c_deferred(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_workspace* ws) {
+// wavefront with no idle lanes waiting on a neighbour's solve.
+__global__ __launch_bounds__(256) void k_pass_c_deferred(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_state out,
+                                                         bk_workspace* ws) {
   const int nq = ws->counters[2];
-  bool f0 = false, f1 = false;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
-    int p = ws->queue[i];
-    bool a = false, b = false;
-    material_update(P, sc, in, out, p, m_load(out.F + 9 * p), m_load(out.C + 9 * p), false, a, b);
-    f0 |= a;
-    f1 |= b;
+    const int p = ws->queue[i];
+    per_dough(dough_of_particle(sc.dough_of, p), [&](int d) {
+      bool f0 = false, f1 = false;
+      material_update(doughs[d].params, sc.visc_param[d], sc, in, out, p, m_load(out.F + 9 * p), m_load(out.C + 9 * p), false,
+                      f0, f1);
+      flag_max(dough_flags(ws, d), BK_FLAG_POROSITY, f0);
+      flag_max(dough_flags(ws, d), BK_FLAG_COUPLED, f1);
+    });
   }
-  flag_max(ws->flags, BK_FLAG_POROSITY, f0);
-  flag_max(ws->flags, BK_FLAG_COUPLED, f1);
 }
 
 // ============================================================================ pass C: G2P + Ding + gas refresh + saturation
-__global__ __launch_bounds__(256) void k_pass_c(bk_params P, bk_scene sc, bk_state in, bk_state out, bk_grid g, bk_workspace* ws) {
+__global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in,
+                                                             bk_state out, bk_grid g, bk_workspace* ws, int n) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  bool stencil_bad = false, f0 = false, f1 = false, deferred = false;
-  if (p < P.n) {
-    const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
-    float g3[3];
-    grid_position(in.x + 3 * p, o, P.dx, g3);
-    int b[3];
-    stencil_base(g3, b);
-    if (!valid_stencil(b, P.res)) {
-      stencil_bad = true;
-    } else {
-      float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
-      float wx[3], wy[3], wz[3];
-      axis_weights(l[0], wx);
-      axis_weights(l[1], wy);
-      axis_weights(l[2], wz);
-      float nv[3] = {0.f, 0.f, 0.f};
-      mat3<float> Cn = m_zero<float>();
-      const float s4 = 4.0f / (P.dx * P.dx);
+  bool deferred = false;
+  if (p < n) {
+    per_dough(dough_of_particle(sc.dough_of, p), [&](int d) {
+      const bk_params& P = doughs[d].params;
+      const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+      const int nb = (P.res + 3) / 4;
+      const int64_t node_base = int64_t(doughs[d].block_offset) << 6;
+      bool stencil_bad = false, f0 = false, f1 = false;
+      float g3[3];
+      grid_position(in.x + 3 * p, o, P.dx, g3);
+      int b[3];
+      stencil_base(g3, b);
+      if (!valid_stencil(b, P.res)) {
+        stencil_bad = true;
+      } else {
+        float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+        float wx[3], wy[3], wz[3];
+        axis_weights(l[0], wx);
+        axis_weights(l[1], wy);
+        axis_weights(l[2], wz);
+        float nv[3] = {0.f, 0.f, 0.f};
+        mat3<float> Cn = m_zero<float>();
+        const float s4 = 4.0f / (P.dx * P.dx);
 #pragma unroll
-      for (int ox = 0; ox < 3; ++ox)
+        for (int ox = 0; ox < 3; ++ox)
 #pragma unroll
-        for (int oy = 0; oy < 3; ++oy)
+          for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
-          for (int oz = 0; oz < 3; ++oz) {
-            float wt = wx[ox] * wy[oy] * wz[oz];
-            int64_t node = tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, ws->nb);
-            float vi[3] = {g.v[3 * node + 0], g.v[3 * node + 1], g.v[3 * node + 2]};
-            float d[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
-            float sw = s4 * wt;
+            for (int oz = 0; oz < 3; ++oz) {
+              float wt = wx[ox] * wy[oy] * wz[oz];
+              int64_t node = node_base + tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, nb);
+              float vi[3] = {g.v[3 * node + 0], g.v[3 * node + 1], g.v[3 * node + 2]};
+              float dd[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
+              float sw = s4 * wt;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              nv[a] += wt * vi[a];
+              for (int a = 0; a < 3; ++a) {
+                nv[a] += wt * vi[a];
 #pragma unroll
-              for (int c = 0; c < 3; ++c) Cn.a[a * 3 + c] += sw * (vi[a] * d[c]);
+                for (int c = 0; c < 3; ++c) Cn.a[a * 3 + c] += sw * (vi[a] * dd[c]);
+              }
             }
-          }
-      mat3<float> F_old = m_load(in.F + 9 * p);
-      mat3<float> inc = m_scale(P.dt, Cn);
-      inc.a[0] += 1.0f; inc.a[4] += 1.0f; inc.a[8] += 1.0f;
-      mat3<float> F_new = m_mul(inc, F_old);
+        mat3<float> F_old = m_load(in.F + 9 * p);
+        mat3<float> inc = m_scale(P.dt, Cn);
+        inc.a[0] += 1.0f; inc.a[4] += 1.0f; inc.a[8] += 1.0f;
+        mat3<float> F_new = m_mul(inc, F_old);
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        out.v[3 * p + a] = nv[a];
-        out.x[3 * p + a] = in.x[3 * p + a] + P.dt * nv[a];
+        for (int a = 0; a < 3; ++a) {
+          out.v[3 * p + a] = nv[a];
+          out.x[3 * p + a] = in.x[3 * p + a] + P.dt * nv[a];
+        }
+        m_store(out.C + 9 * p, Cn);
+        m_store(out.F + 9 * p, F_new);
+        if (P.model == 1)
+          deferred = material_update(P, sc.visc_param[d], sc, in, out, p, F_new, Cn, true, f0, f1) == kDingDeferred;
       }
-      m_store(out.C + 9 * p, Cn);
-      m_store(out.F + 9 * p, F_new);
-      if (P.model == 1) {
-        deferred = material_update(P, sc, in, out, p, F_new, Cn, true, f0, f1) == kDingDeferred;
-      }
-    }
+      int32_t* fl = dough_flags(ws, d);
+      flag_max(fl, BK_FLAG_STENCIL, stencil_bad);
+      flag_max(fl, BK_FLAG_POROSITY, f0);
+      flag_max(fl, BK_FLAG_COUPLED, f1);
+    });
   }
   wave_append(&ws->counters[2], ws->queue, deferred, p);
-  flag_max(ws->flags, BK_FLAG_STENCIL, stencil_bad);
-  flag_max(ws->flags, BK_FLAG_POROSITY, f0);
-  flag_max(ws->flags, BK_FLAG_COUPLED, f1);
 }
 
 __global__ void k_untile_f32(const float* __restrict__ tiled, float* __restrict__ dense, int res, int comps) {
@@ -843,105 +959,24 @@ __global__ void k_untile_f64(const double* __restrict__ tiled, double* __restric
   dense[i] = tiled[tiled_node(ii, j, k, (res + 3) / 4)];
 }
 
-// ============================================================================ C ABI
-extern "C" {
-
-int bk_abi_version(void) { return BK_ABI_VERSION; }
-const char* bk_last_error(void) { return g_last_error.c_str(); }
-
-int bk_device_info(char* name, int name_len, int* cu_count, int* xcd_count) {
-  int dev = 0;
-  BK_HIP(hipGetDevice(&dev));
-  hipDeviceProp_t prop;
-  BK_HIP(hipGetDeviceProperties(&prop, dev));
-  if (name && name_len > 0) { strncpy(name, prop.gcnArchName, name_len - 1); name[name_len - 1] = 0; }
-  if (cu_count) *cu_count = prop.multiProcessorCount;
-  if (xcd_count) *xcd_count = 8;
-  return 0;
-}
-
-bk_workspace* bk_workspace_create(int32_t max_particles, int32_t res, int32_t n_vertices, int32_t n_slots) {
-  if (max_particles < 1 || res < 4) { fail("bk_workspace_create: bad sizes"); return nullptr; }
-  bk_workspace* ws = nullptr;
-  if (hipMalloc(&ws, sizeof(bk_workspace)) != hipSuccess) { fail("hipMalloc workspace"); return nullptr; }
-  bk_workspace h;
-  h.max_n = max_particles;
-  h.res = res;
-  h.nb = (res + 3) / 4;
-  h.nblocks = h.nb * h.nb * h.nb;
-  h.n_vertices = n_vertices;
-  h.n_slots = n_slots;
-  size_t n = size_t(max_particles);
-  int nwg = div_up(max_particles, kSortTile);
-  bool ok = true;
-  auto alloc = [&](void** p, size_t bytes) {
-    if (bytes == 0) bytes = 256;
-    if (hipMalloc(p, bytes) != hipSuccess) ok = false;
-    else if (hipMemset(*p, 0, bytes) != hipSuccess) ok = false;
-  };
-  alloc((void**)&h.keys_raw, n * 4);
-  alloc((void**)&h.sorted_keys, n * 4);
-  alloc((void**)&h.sorted_vals, n * 4);
-  for (int i = 0; i < 2; ++i) {
-    alloc((void**)&h.keys_s[i], n * 4);
-    alloc((void**)&h.vals_s[i], n * 4);
-  }
-  alloc((void**)&h.hist, size_t(256) * nwg * 4);
-  alloc((void**)&h.digit_total, 256 * 4);
-  alloc((void**)&h.blk_start, size_t(h.nblocks) * 4);
-  alloc((void**)&h.blk_end, size_t(h.nblocks) * 4);
-  alloc((void**)&h.node_flag, size_t(h.nblocks) * 4);
-  alloc((void**)&h.tiles, size_t(h.nblocks) * 4);
-  alloc((void**)&h.active, size_t(h.nblocks) * 4);
-  alloc((void**)&h.counters, 64);
-  alloc((void**)&h.tau, n * 36);
-  alloc((void**)&h.queue, n * 4);
-  alloc((void**)&h.vertices, size_t(n_vertices) * 12);
-  alloc((void**)&h.slot_exposed, size_t(n_slots) * 8);
-  alloc((void**)&h.slot_contact, size_t(n_slots) * 8);
-  alloc((void**)&h.flags, BK_FLAG_COUNT * 4);
-  if (!ok || hipMemcpy(ws, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
-    fail("bk_workspace_create: allocation failed");
-    return nullptr;
-  }
-  return ws;
-}
-
+// ============================================================================ host side
 static int ws_host(bk_workspace* ws, bk_workspace& h) {
   BK_HIP(hipMemcpy(&h, ws, sizeof(h), hipMemcpyDeviceToHost));
   return 0;
 }
 
-void bk_workspace_destroy(bk_workspace* ws) {
-  if (!ws) return;
-  bk_workspace h;
-  {
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    g_ws_cache.erase(ws);
-  }
-  if (ws_host(ws, h)) return;
-  void* ptrs[] = {h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1], h.hist, h.digit_total, h.blk_start, h.blk_end, h.node_flag, h.tiles, h.active,
-                  h.counters, h.queue, h.tau, h.vertices, h.slot_exposed, h.slot_contact, h.flags};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
-  (void)hipFree(ws);
-}
-
-}  // extern "C"
-
-
-static const bk_workspace* host_view(bk_workspace* ws) {
+static HostWs* host_view(bk_workspace* ws) {
+  if (!ws) return nullptr;
   std::lock_guard<std::mutex> lock(g_ws_mutex);
   auto it = g_ws_cache.find(ws);
   if (it != g_ws_cache.end()) return &it->second;
-  bk_workspace h;
-  if (ws_host(ws, h)) return nullptr;
+  HostWs h;
+  if (ws_host(ws, h.dev)) return nullptr;
   return &(g_ws_cache[ws] = h);
 }
 
-// Optional per-stage timing: HIP events recorded around each stage of
-// bk_substep on the launch stream (no extra synchronisation while recording).
-#include <vector>
+// Optional per-stage timing: HIP events recorded around each stage of the
+// substep on the launch stream (no extra synchronisation while recording).
 enum { kStageClear, kStageKeys, kStageSort, kStageTiles, kStageMesh, kStagePassA, kStageDiffuse, kStagePassB1, kStagePassB2,
        kStageGrid, kStagePassC, kStagePassCDeferred, kStageCount };
 static const char* kStageNames[kStageCount] = {"clear", "keys", "sort", "tiles", "mesh", "pass_a", "diffuse", "pass_b1",
@@ -964,40 +999,349 @@ struct StageTimer {
   }
 };
 
+// One substep of the uploaded batch, enqueued on `st`.
+static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g,
+                         hipStream_t st, bool allow_profile) {
+  if (hw->doughs.empty()) return fail("bk_batch_substep: no dough table uploaded");
+  if (!sc || !in || !out || !g) return fail("bk_batch_substep: null argument");
+  if (!sc->dough_of && hw->doughs.size() > 1) return fail("bk_batch_substep: a batch of several doughs needs scene.dough_of");
+  const bk_workspace* h = &hw->dev;
+  const bk_dough* D = h->doughs;
+  const int nd = int(hw->doughs.size());
+  const int n = hw->n;
+  const bk_tuning& tu = hw->tuning;
+  const int cap = tu.grid_cap;
+  const int pblock = tu.particle_block;
+  Profiler* pr = nullptr;
+  if (allow_profile) {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    auto it = g_prof.find(ws);
+    if (it != g_prof.end() && it->second.step < it->second.max_steps) pr = &it->second;
+  }
+  {  // 1. clear the previous substep's blocks (counters[1] still holds their count)
+    StageTimer tm(pr, st, kStageClear);
+    hipLaunchKernelGGL(k_clear_blocks, dim3(cap), dim3(256), 0, st, ws, *g, int(hw->thermal));
+    BK_LAUNCH(clear);
+  }
+  {  // 2. keys (also resets the per-substep counters)
+    StageTimer tm(pr, st, kStageKeys);
+    hipLaunchKernelGGL(k_batch_keys, dim3(div_up(n, 256)), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, ws);
+    BK_LAUNCH(keys);
+  }
+  {  // 3. stable sort by batch cell key
+    StageTimer tm(pr, st, kStageSort);
+    int rc = radix_sort(h, h->keys_raw, nullptr, h->sorted_keys, h->sorted_vals, n, hw->key_bits, st);
+    if (rc) return rc;
+  }
+  {  // 4. block bounds, tiles, active node blocks
+    StageTimer tm(pr, st, kStageTiles);
+    hipLaunchKernelGGL(k_block_bounds, dim3(div_up(n, 256)), dim3(256), 0, st, h->sorted_keys, n, ws);
+    hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)n * 8, 256), 1, 2048)), dim3(256), 0, st, D, nd, ws, tu.tile_chunk);
+    BK_LAUNCH(tiles);
+  }
+  if (hw->thermal) {
+    {  // 5. mesh reconstruction and surface areas
+      StageTimer tm(pr, st, kStageMesh);
+      if (hw->n_vertices > 0)
+        hipLaunchKernelGGL(k_reconstruct, dim3(div_up(hw->n_vertices, 4)), dim3(256), 0, st, in->x, in->F, sc->ref_vertices,
+                           sc->bind_offsets, sc->bind_particles, sc->bind_weights, sc->bind_ref_offsets, h->vertices,
+                           hw->n_vertices);
+      if (hw->n_slots > 0)
+        hipLaunchKernelGGL(k_slot_areas, dim3(div_up(hw->n_slots, 256)), dim3(256), 0, st, D, *sc, h->vertices, ws, hw->n_slots);
+      BK_LAUNCH(mesh);
+    }
+    {  // 6. thermal P2G
+      StageTimer tm(pr, st, kStagePassA);
+      hipLaunchKernelGGL(k_pass_a, dim3(cap), dim3(kTileThreads), 0, st, D, *sc, *in, *g, ws);
+      BK_LAUNCH(pass_a);
+    }
+    {  // 7. diffusion
+      StageTimer tm(pr, st, kStageDiffuse);
+      hipLaunchKernelGGL(k_diffuse, dim3(cap), dim3(256), 0, st, D, nd, *g, ws);
+      BK_LAUNCH(diffuse);
+    }
+  }
+  {  // 8. scalar G2P, gas and water stress
+    StageTimer tm(pr, st, kStagePassB1);
+    hipLaunchKernelGGL(k_pass_b1, dim3(div_up(n, pblock)), dim3(pblock), 0, st, D, *sc, *in, *out, *g, ws, n);
+    BK_LAUNCH(pass_b1);
+  }
+  {  // 9. mechanical P2G
+    StageTimer tm(pr, st, kStagePassB2);
+    hipLaunchKernelGGL(k_pass_b2, dim3(cap), dim3(kTileThreads), 0, st, D, *sc, *in, *out, *g, ws);
+    BK_LAUNCH(pass_b2);
+  }
+  {  // 10. grid update
+    StageTimer tm(pr, st, kStageGrid);
+    hipLaunchKernelGGL(k_grid_update, dim3(cap), dim3(256), 0, st, D, nd, *g, ws);
+    BK_LAUNCH(grid_update);
+  }
+  {  // 11. G2P + material
+    StageTimer tm(pr, st, kStagePassC);
+    hipLaunchKernelGGL(k_pass_c, dim3(div_up(n, pblock)), dim3(pblock), 0, st, D, *sc, *in, *out, *g, ws, n);
+    BK_LAUNCH(pass_c);
+  }
+  if (hw->gas) {  // 12. deferred coupled-porosity return maps
+    StageTimer tm(pr, st, kStagePassCDeferred);
+    hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, D, *sc, *in, *out, ws);
+    BK_LAUNCH(pass_c_deferred);
+  }
+  if (pr) ++pr->step;
+  return 0;
+}
+
+static bool same_layout(const std::vector<bk_dough>& a, const bk_dough* b, int nd) {
+  if (int(a.size()) != nd) return false;
+  for (int d = 0; d < nd; ++d)
+    if (a[d].block_offset != b[d].block_offset || a[d].params.res != b[d].params.res) return false;
+  return true;
+}
+
+static bool same_table(const HostWs* hw, const bk_dough* table, int nd) {
+  return int(hw->doughs.size()) == nd && memcmp(hw->doughs.data(), table, sizeof(bk_dough) * nd) == 0;
+}
+
+// Validate and upload a dough table.  Waits for the device first, since
+// kernels in flight may read the table.  A table whose grid layout differs from
+// the last one also forgets the last substep's active blocks: the first
+// substep after it needs a zeroed grid.
+static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, int nd) {
+  const bk_workspace& h = hw->dev;
+  if (!table || nd < 1 || nd > h.max_doughs) return fail("bk_workspace_set_doughs: dough count exceeds the workspace");
+  if (same_table(hw, table, nd)) return 0;
+  long long n = 0, blocks = 0, nv = 0, ns = 0;
+  bool thermal = false, gas = false;
+  for (int d = 0; d < nd; ++d) {
+    const bk_dough& D = table[d];
+    if (D.params.res < 4 || D.params.n < 1 || !(D.params.dx > 0.0f))
+      return fail("bk_workspace_set_doughs: every dough needs res >= 4, n >= 1 and dx > 0");
+    if (D.particle_offset != n || D.block_offset != blocks)
+      return fail("bk_workspace_set_doughs: doughs must be packed in order (particle_offset, block_offset)");
+    long long nb = (D.params.res + 3) / 4;
+    n += D.params.n;
+    blocks += nb * nb * nb;
+    nv += D.params.n_vertices;
+    ns += D.params.n_slots;
+    thermal |= D.params.transport_enabled != 0;
+    gas |= D.params.model == 1 && D.params.gas_enabled != 0;
+  }
+  if (n > h.max_n || blocks > h.total_blocks || nv > h.n_vertices || ns > h.n_slots)
+    return fail("bk_workspace_set_doughs: batch exceeds the workspace (particles, blocks, vertices or slots)");
+  const bool relayout = !same_layout(hw->doughs, table, nd);
+  BK_HIP(hipDeviceSynchronize());
+  BK_HIP(hipMemcpy(h.doughs, table, sizeof(bk_dough) * nd, hipMemcpyHostToDevice));
+  if (relayout) {
+    BK_HIP(hipMemset(h.node_flag, 0, size_t(h.total_blocks) * 4));
+    BK_HIP(hipMemset(h.counters, 0, 64));
+  }
+  hw->doughs.assign(table, table + nd);
+  hw->n = int(n);
+  hw->n_vertices = int(nv);
+  hw->n_slots = int(ns);
+  hw->thermal = thermal;
+  hw->gas = gas;
+  int bits = 6;
+  while ((1ll << bits) < blocks * 64) ++bits;
+  hw->key_bits = bits;
+  return 0;
+}
+
+struct bk_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bk_workspace* ws = nullptr;
+  std::vector<bk_dough> table;  // the table the graph was captured with
+};
+
+static bk_graph* capture(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g) {
+  hipStream_t cap = nullptr;
+  if (check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return nullptr;
+  bk_graph* gr = new bk_graph();
+  gr->ws = ws;
+  gr->table = hw->doughs;
+  int rc = check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (!rc) {
+    int rc_sub = batch_substep(ws, hw, sc, in, out, g, cap, false);
+    int rc_end = check_hip(hipStreamEndCapture(cap, &gr->graph), "hipStreamEndCapture");
+    rc = rc_sub ? rc_sub : rc_end;
+  }
+  if (!rc) rc = check_hip(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  (void)hipStreamDestroy(cap);
+  if (rc) {
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    delete gr;
+    return nullptr;
+  }
+  return gr;
+}
+
+static bk_dough single_dough(const bk_params* P) {
+  bk_dough d;
+  memset(&d, 0, sizeof(d));
+  d.params = *P;
+  return d;
+}
+
+// ============================================================================ C ABI
 extern "C" {
 
+int bk_abi_version(void) { return BK_ABI_VERSION; }
+const char* bk_last_error(void) { return g_last_error.c_str(); }
+
+int bk_device_info(char* name, int name_len, int* cu_count, int* xcd_count) {
+  int dev = 0;
+  BK_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  BK_HIP(hipGetDeviceProperties(&prop, dev));
+  if (name && name_len > 0) { strncpy(name, prop.gcnArchName, name_len - 1); name[name_len - 1] = 0; }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (xcd_count) *xcd_count = 8;
+  return 0;
+}
+
+bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blocks, int32_t max_doughs, int32_t n_vertices,
+                                        int32_t n_slots) {
+  if (max_particles < 1 || total_blocks < 1 || max_doughs < 1 || n_vertices < 0 || n_slots < 0 ||
+      (long long)total_blocks * 64 > (1ll << 32)) {
+    fail("bk_workspace_create: bad sizes");
+    return nullptr;
+  }
+  bk_workspace* ws = nullptr;
+  if (hipMalloc(&ws, sizeof(bk_workspace)) != hipSuccess) { fail("hipMalloc workspace"); return nullptr; }
+  bk_workspace h;
+  h.max_n = max_particles;
+  h.total_blocks = total_blocks;
+  h.max_doughs = max_doughs;
+  h.n_vertices = n_vertices;
+  h.n_slots = n_slots;
+  size_t n = size_t(max_particles);
+  size_t nbk = size_t(total_blocks);
+  int nwg = div_up(max_particles, kSortTile);
+  bool ok = true;
+  auto alloc = [&](void** p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (hipMalloc(p, bytes) != hipSuccess) ok = false;
+    else if (hipMemset(*p, 0, bytes) != hipSuccess) ok = false;
+  };
+  alloc((void**)&h.doughs, sizeof(bk_dough) * size_t(max_doughs));
+  alloc((void**)&h.keys_raw, n * 4);
+  alloc((void**)&h.sorted_keys, n * 4);
+  alloc((void**)&h.sorted_vals, n * 4);
+  for (int i = 0; i < 2; ++i) {
+    alloc((void**)&h.keys_s[i], n * 4);
+    alloc((void**)&h.vals_s[i], n * 4);
+  }
+  alloc((void**)&h.hist, size_t(256) * nwg * 4);
+  alloc((void**)&h.digit_total, 256 * 4);
+  alloc((void**)&h.blk_start, nbk * 4);
+  alloc((void**)&h.blk_end, nbk * 4);
+  alloc((void**)&h.node_flag, nbk * 4);
+  alloc((void**)&h.blocks, nbk * 4);
+  alloc((void**)&h.tiles, (n + nbk) * sizeof(int4));  // a tile per chunk: at most one per particle plus one per block
+  alloc((void**)&h.active, nbk * 4);
+  alloc((void**)&h.counters, 64);
+  alloc((void**)&h.tau, n * 36);
+  alloc((void**)&h.queue, n * 4);
+  alloc((void**)&h.vertices, size_t(n_vertices) * 12);
+  alloc((void**)&h.slot_exposed, size_t(n_slots) * 8);
+  alloc((void**)&h.slot_contact, size_t(n_slots) * 8);
+  alloc((void**)&h.flags, size_t(BK_FLAG_COUNT) * max_doughs * 4);
+  if (!ok || hipMemcpy(ws, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
+    fail("bk_workspace_create: allocation failed");
+    return nullptr;
+  }
+  return ws;
+}
+
+bk_workspace* bk_workspace_create(int32_t max_particles, int32_t res, int32_t n_vertices, int32_t n_slots) {
+  if (res < 4) { fail("bk_workspace_create: bad sizes"); return nullptr; }
+  int nb = (res + 3) / 4;
+  return bk_workspace_create_batch(max_particles, nb * nb * nb, 1, n_vertices, n_slots);
+}
+
+void bk_workspace_destroy(bk_workspace* ws) {
+  if (!ws) return;
+  bk_workspace h;
+  {
+    std::lock_guard<std::mutex> lock(g_ws_mutex);
+    g_ws_cache.erase(ws);
+    auto it = g_prof.find(ws);
+    if (it != g_prof.end()) {
+      for (hipEvent_t e : it->second.ev) (void)hipEventDestroy(e);
+      g_prof.erase(it);
+    }
+  }
+  if (ws_host(ws, h)) return;
+  void* ptrs[] = {h.doughs, h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1],
+                  h.hist, h.digit_total, h.blk_start, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
+                  h.queue, h.tau, h.vertices, h.slot_exposed, h.slot_contact, h.flags};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  (void)hipFree(ws);
+}
+
+int bk_workspace_set_doughs(bk_workspace* ws, const bk_dough* table, int32_t n_doughs) {
+  HostWs* hw = host_view(ws);
+  if (!hw) return fail("bk_workspace_set_doughs: bad workspace");
+  return upload_doughs(ws, hw, table, n_doughs);
+}
+
+int bk_workspace_set_tuning(bk_workspace* ws, const bk_tuning* tuning) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !tuning) return fail("bk_workspace_set_tuning: bad argument");
+  const bk_tuning t = *tuning;
+  if (t.tile_chunk < 1 || t.grid_cap < 1 || t.grid_cap > 65536 || t.particle_block < 64 || t.particle_block > kParticleThreads ||
+      t.particle_block % 64)
+    return fail("bk_workspace_set_tuning: need tile_chunk >= 1, grid_cap in [1, 65536], particle_block in {64, 128, 192, 256}");
+  hw->tuning = t;
+  return 0;
+}
+
+int bk_workspace_get_tuning(bk_workspace* ws, bk_tuning* tuning) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !tuning) return fail("bk_workspace_get_tuning: bad argument");
+  *tuning = hw->tuning;
+  return 0;
+}
+
 int32_t* bk_workspace_flags(bk_workspace* ws) {
-  const bk_workspace* h = host_view(ws);
-  return h ? h->flags : nullptr;
+  HostWs* hw = host_view(ws);
+  return hw ? hw->dev.flags : nullptr;
 }
 float* bk_workspace_vertices(bk_workspace* ws) {
-  const bk_workspace* h = host_view(ws);
-  return h ? h->vertices : nullptr;
+  HostWs* hw = host_view(ws);
+  return hw ? hw->dev.vertices : nullptr;
 }
-int bk_workspace_counts(bk_workspace* ws, int32_t* out2, void* stream) {
-  const bk_workspace* h = host_view(ws);
-  if (!h) return -2;
-  BK_HIP(hipMemcpyAsync(out2, h->counters, 12, hipMemcpyDeviceToHost, (hipStream_t)stream));
+int bk_workspace_counts(bk_workspace* ws, int32_t* out3, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !out3) return fail("bk_workspace_counts: bad argument");
+  BK_HIP(hipMemcpyAsync(out3, hw->dev.counters, 12, hipMemcpyDeviceToHost, (hipStream_t)stream));
   BK_HIP(hipStreamSynchronize((hipStream_t)stream));
   return 0;
 }
 
-int bk_workspace_read_flags(bk_workspace* ws, int32_t* host8, void* stream) {
-  const bk_workspace* h = host_view(ws);
-  if (!h || !host8) return fail("bk_workspace_read_flags: bad argument");
-  BK_HIP(hipMemcpyAsync(host8, h->flags, BK_FLAG_COUNT * 4, hipMemcpyDeviceToHost, (hipStream_t)stream));
+int bk_workspace_read_flags_all(bk_workspace* ws, int32_t* host, int32_t n_doughs, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !host || n_doughs < 1 || n_doughs > hw->dev.max_doughs) return fail("bk_workspace_read_flags: bad argument");
+  BK_HIP(hipMemcpyAsync(host, hw->dev.flags, size_t(BK_FLAG_COUNT) * n_doughs * 4, hipMemcpyDeviceToHost, (hipStream_t)stream));
   BK_HIP(hipStreamSynchronize((hipStream_t)stream));
   return 0;
 }
-int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* stream) {
-  const bk_workspace* h = host_view(ws);
-  if (!h || !host8) return fail("bk_workspace_write_flags: bad argument");
-  BK_HIP(hipMemcpyAsync(h->flags, host8, BK_FLAG_COUNT * 4, hipMemcpyHostToDevice, (hipStream_t)stream));
+int bk_workspace_write_flags_all(bk_workspace* ws, const int32_t* host, int32_t n_doughs, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !host || n_doughs < 1 || n_doughs > hw->dev.max_doughs) return fail("bk_workspace_write_flags: bad argument");
+  BK_HIP(hipMemcpyAsync(hw->dev.flags, host, size_t(BK_FLAG_COUNT) * n_doughs * 4, hipMemcpyHostToDevice, (hipStream_t)stream));
   BK_HIP(hipStreamSynchronize((hipStream_t)stream));
   return 0;
+}
+int bk_workspace_read_flags(bk_workspace* ws, int32_t* host8, void* stream) { return bk_workspace_read_flags_all(ws, host8, 1, stream); }
+int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* stream) {
+  return bk_workspace_write_flags_all(ws, host8, 1, stream);
 }
 
 int bk_profile_begin(bk_workspace* ws, int32_t max_steps) {
+  if (max_steps < 1) return fail("bk_profile_begin: max_steps must be positive");
   std::lock_guard<std::mutex> lock(g_ws_mutex);
   Profiler& pr = g_prof[ws];
   for (hipEvent_t e : pr.ev) (void)hipEventDestroy(e);
@@ -1041,57 +1385,56 @@ int bk_cell_keys(const float* x, int32_t n, const float* o, float dx, int32_t re
   if (n < 0 || !o || res < 1) return fail("bk_cell_keys: bad argument");
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_cell_keys, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, x, n, o[0], o[1], o[2], dx, res, tiled,
-                     keys, flags, (int32_t*)nullptr);
+                     keys, flags);
   BK_LAUNCH(k_cell_keys);
   return 0;
 }
 
 int bk_sort_pairs(bk_workspace* ws, const uint32_t* keys_in, const int32_t* values_in, uint32_t* keys_out, int32_t* values_out,
                   int32_t n, int32_t key_bits, void* stream) {
-  const bk_workspace* h = host_view(ws);
-  if (!h) return -2;
-  if (n < 0 || n > h->max_n || key_bits < 1 || key_bits > 32) return fail("bk_sort_pairs: bad argument");
-  return radix_sort(h, keys_in, values_in, keys_out, values_out, n, key_bits, (hipStream_t)stream);
+  HostWs* hw = host_view(ws);
+  if (!hw) return -2;
+  if (n < 0 || n > hw->dev.max_n || key_bits < 1 || key_bits > 32) return fail("bk_sort_pairs: bad argument");
+  return radix_sort(&hw->dev, keys_in, values_in, keys_out, values_out, n, key_bits, (hipStream_t)stream);
 }
 
-static int substep_impl(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g, bk_workspace* ws,
-                        void* stream, bool allow_profile);
+int bk_batch_substep(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out, bk_grid* grid, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw) return fail("bk_batch_substep: bad workspace");
+  return batch_substep(ws, hw, scene, in, out, grid, (hipStream_t)stream, true);
+}
+
+bk_graph* bk_batch_graph_create(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out, bk_grid* grid) {
+  HostWs* hw = host_view(ws);
+  if (!hw) { fail("bk_batch_graph_create: bad workspace"); return nullptr; }
+  return capture(ws, hw, scene, in, out, grid);
+}
 
 int bk_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g, bk_workspace* ws,
                void* stream) {
-  return substep_impl(P, sc, in, out, g, ws, stream, true);
+  HostWs* hw = host_view(ws);
+  if (!hw || !P) return fail("bk_substep: bad argument");
+  bk_dough d = single_dough(P);
+  int rc = upload_doughs(ws, hw, &d, 1);
+  if (rc) return rc;
+  return batch_substep(ws, hw, sc, in, out, g, (hipStream_t)stream, true);
 }
-
-struct bk_graph {
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-};
 
 bk_graph* bk_substep_graph_create(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g,
                                   bk_workspace* ws) {
-  if (!host_view(ws)) return nullptr;  // fills the host cache before capture (no device calls while capturing)
-  hipStream_t cap = nullptr;
-  if (check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return nullptr;
-  bk_graph* gr = new bk_graph();
-  int rc = check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-  if (!rc) {
-    int rc_sub = substep_impl(P, sc, in, out, g, ws, cap, false);
-    int rc_end = check_hip(hipStreamEndCapture(cap, &gr->graph), "hipStreamEndCapture");
-    rc = rc_sub ? rc_sub : rc_end;
-  }
-  if (!rc) rc = check_hip(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-  (void)hipStreamDestroy(cap);
-  if (rc) {
-    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
-    if (gr->graph) (void)hipGraphDestroy(gr->graph);
-    delete gr;
-    return nullptr;
-  }
-  return gr;
+  HostWs* hw = host_view(ws);
+  if (!hw || !P) { fail("bk_substep_graph_create: bad argument"); return nullptr; }
+  bk_dough d = single_dough(P);
+  if (upload_doughs(ws, hw, &d, 1)) return nullptr;
+  return capture(ws, hw, sc, in, out, g);
 }
 
 int bk_graph_launch(bk_graph* gr, void* stream) {
   if (!gr) return fail("bk_graph_launch: null graph");
+  HostWs* hw = host_view(gr->ws);
+  if (!hw) return fail("bk_graph_launch: the graph's workspace is gone");
+  int rc = upload_doughs(gr->ws, hw, gr->table.data(), int(gr->table.size()));  // no-op unless another table was uploaded since
+  if (rc) return rc;
   BK_HIP(hipGraphLaunch(gr->exec, (hipStream_t)stream));
   return 0;
 }
@@ -1101,87 +1444,6 @@ void bk_graph_destroy(bk_graph* gr) {
   if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
   if (gr->graph) (void)hipGraphDestroy(gr->graph);
   delete gr;
-}
-
-static int substep_impl(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g, bk_workspace* ws,
-                        void* stream, bool allow_profile) {
-  const bk_workspace* h = host_view(ws);
-  if (!h) return -2;
-  if (!P || !sc || !in || !out || !g) return fail("bk_substep: null argument");
-  if (P->n < 1 || P->n > h->max_n || P->res != h->res) return fail("bk_substep: particle count or resolution exceeds workspace");
-  hipStream_t st = (hipStream_t)stream;
-  const int n = P->n;
-  const int nb = h->nb;
-  const int thermal = P->transport_enabled;
-  const int cap_blocks = 4096;
-  Profiler* pr = nullptr;
-  if (allow_profile) {
-    std::lock_guard<std::mutex> lock(g_ws_mutex);
-    auto it = g_prof.find(ws);
-    if (it != g_prof.end() && it->second.step < it->second.max_steps) pr = &it->second;
-  }
-  // 1. clear the previous step's blocks (counters[1] still holds their count)
-  { StageTimer tm(pr, st, kStageClear);
-  hipLaunchKernelGGL(k_clear_blocks, dim3(cap_blocks), dim3(256), 0, st, h->active, h->counters, h->node_flag, *g, thermal);
-  BK_LAUNCH(clear); }
-  // 2. keys (also resets the tile/active counters)
-  { StageTimer tm(pr, st, kStageKeys);
-  hipLaunchKernelGGL(k_cell_keys, dim3(div_up(n, 256)), dim3(256), 0, st, in->x, n, P->origin[0], P->origin[1], P->origin[2], P->dx,
-                     P->res, 1, h->keys_raw, h->flags, h->counters);
-  BK_LAUNCH(keys); }
-  // 3. stable sort by tiled cell key
-  { StageTimer tm(pr, st, kStageSort);
-    int rc = radix_sort(h, h->keys_raw, nullptr, h->sorted_keys, h->sorted_vals, n, tiled_key_bits(h), st);
-    if (rc) return rc;
-  }
-  // 4. tiles and active blocks
-  { StageTimer tm(pr, st, kStageTiles);
-  hipLaunchKernelGGL(k_tile_bounds, dim3(div_up(n, 256)), dim3(256), 0, st, h->sorted_keys, n, h->blk_start, h->blk_end, h->tiles,
-                     h->counters);
-  hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)h->nblocks * 8, 256), 1, 2048)), dim3(256), 0, st, h->tiles, h->counters,
-                     h->node_flag, h->active, nb);
-  BK_LAUNCH(tiles); }
-  // 5. mesh reconstruction and surface areas
-  if (thermal) {
-    { StageTimer tm(pr, st, kStageMesh);
-    if (P->n_vertices > 0)
-      hipLaunchKernelGGL(k_reconstruct, dim3(div_up(P->n_vertices, 4)), dim3(256), 0, st, in->x, in->F, sc->ref_vertices,
-                         sc->bind_offsets, sc->bind_particles, sc->bind_weights, sc->bind_ref_offsets, h->vertices, P->n_vertices);
-    if (P->n_slots > 0) {
-      float floor_h = P->origin[1] + float(P->boundary_cells - 1) * P->dx;
-      hipLaunchKernelGGL(k_slot_areas, dim3(div_up(P->n_slots, 256)), dim3(256), 0, st, h->vertices, sc->faces, sc->slot_face_offsets,
-                         sc->slot_faces, h->slot_exposed, h->slot_contact, P->n_slots, floor_h, P->contact_distance);
-    }
-    BK_LAUNCH(mesh); }
-    // 6. thermal P2G, 7. diffusion
-    { StageTimer tm(pr, st, kStagePassA);
-    hipLaunchKernelGGL(k_pass_a, dim3(cap_blocks), dim3(kTileThreads), 0, st, *P, *sc, *in, *g, ws);
-    BK_LAUNCH(pass_a); }
-    { StageTimer tm(pr, st, kStageDiffuse);
-    hipLaunchKernelGGL(k_diffuse, dim3(cap_blocks), dim3(256), 0, st, *P, *g, ws);
-    BK_LAUNCH(diffuse); }
-  }
-  // 8. scalar G2P, gas and water stress
-  { StageTimer tm(pr, st, kStagePassB1);
-  hipLaunchKernelGGL(k_pass_b1, dim3(div_up(n, 256)), dim3(256), 0, st, *P, *sc, *in, *out, *g, ws);
-  BK_LAUNCH(pass_b1); }
-  // 9. mechanical P2G, 10. grid update, 11. G2P + material
-  { StageTimer tm(pr, st, kStagePassB2);
-  hipLaunchKernelGGL(k_pass_b2, dim3(cap_blocks), dim3(kTileThreads), 0, st, *P, *sc, *in, *out, *g, ws);
-  BK_LAUNCH(pass_b2); }
-  { StageTimer tm(pr, st, kStageGrid);
-  hipLaunchKernelGGL(k_grid_update, dim3(cap_blocks), dim3(256), 0, st, *P, *g, ws);
-  BK_LAUNCH(grid_update); }
-  { StageTimer tm(pr, st, kStagePassC);
-  hipLaunchKernelGGL(k_pass_c, dim3(div_up(n, 256)), dim3(256), 0, st, *P, *sc, *in, *out, *g, ws);
-  BK_LAUNCH(pass_c); }
-  if (P->model == 1 && P->gas_enabled) {
-    StageTimer tm(pr, st, kStagePassCDeferred);
-    hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, *P, *sc, *in, *out, ws);
-    BK_LAUNCH(pass_c_deferred);
-  }
-  if (pr) ++pr->step;
-  return 0;
 }
 
 int bk_grid_untile(const float* tiled, float* dense, int32_t res, int32_t comps, void* stream) {

@@ -193,6 +193,10 @@ class MpmSimulator:
         for name in _native.SCENE_FIELDS:
             setattr(scene, name, keep[name].data_ptr() if name in keep else None)
         self._scene = scene
+        self._refresh_grid_struct()
+        self._params = self._make_params(settling=False)
+
+    def _refresh_grid_struct(self):
         grid = _native.BkGrid()
         grid.m = self.grid_state.tiled_masses.data_ptr()
         grid.p = self.grid_state.tiled_momenta.data_ptr()
@@ -200,7 +204,6 @@ class MpmSimulator:
         for k, v in self.thermal_grid.items():
             setattr(grid, k, v.data_ptr())
         self._grid = grid
-        self._params = self._make_params(settling=False)
 
     def __del__(self):
         try:
@@ -250,26 +253,8 @@ class MpmSimulator:
     @staticmethod
     def _state_struct(state):
         s = _native.BkState()
-        s.x = state.particle_positions.data_ptr()
-        s.v = state.particle_velocities.data_ptr()
-        s.F = state.deformation_gradients.data_ptr()
-        s.C = state.affine_velocities.data_ptr()
-        m = state.material_state
-        if m is not None:
-            for field, name in (("Fe", "elastic_gradients"), ("Fp", "plastic_gradients"), ("tau", "kirchhoff_stresses"),
-                                ("residual", "return_residuals"), ("porosity", "porosities"), ("T_mat", "temperatures"),
-                                ("gelled", "gelled"), ("yield_stress", "yield_stresses"), ("viscosity", "viscosities"),
-                                ("porosity_residual", "porosity_residuals"), ("density", "reference_densities"),
-                                ("moles", "gas_moles"), ("pressure", "gas_pressures")):
-                t = getattr(m, name)
-                if not t.is_contiguous():
-                    raise ValueError(f"material_state.{name} must be contiguous")
-                setattr(s, field, t.data_ptr())
-        t = state.transport_state
-        if t is not None:
-            for field, name in (("T", "temperatures"), ("W", "water_masses"), ("s", "saturations"), ("losses", "water_losses"),
-                                ("exchanges", "heat_exchanges"), ("masses", "particle_masses")):
-                setattr(s, field, getattr(t, name).data_ptr())
+        for field, t in state_field_tensors(state).items():
+            setattr(s, field, t.data_ptr())
         return s
 
     # ------------------------------------------------------------------ substep
@@ -481,6 +466,35 @@ class MpmSimulator:
         output_path = Path(output_path)
         output_path.parent.mkdir(parents=True, exist_ok=True)
         np.savez_compressed(output_path, **self.CaptureSnapshot())
+
+
+_MATERIAL_FIELDS = (("Fe", "elastic_gradients"), ("Fp", "plastic_gradients"), ("tau", "kirchhoff_stresses"),
+                    ("residual", "return_residuals"), ("porosity", "porosities"), ("T_mat", "temperatures"),
+                    ("gelled", "gelled"), ("yield_stress", "yield_stresses"), ("viscosity", "viscosities"),
+                    ("porosity_residual", "porosity_residuals"), ("density", "reference_densities"),
+                    ("moles", "gas_moles"), ("pressure", "gas_pressures"))
+_TRANSPORT_FIELDS = (("T", "temperatures"), ("W", "water_masses"), ("s", "saturations"), ("losses", "water_losses"),
+                     ("exchanges", "heat_exchanges"), ("masses", "particle_masses"))
+
+
+def state_field_slots(state):
+    """Where each ``bk_state`` field of a state lives: {field: (object, attribute name)}."""
+    out = dict(x=(state, "particle_positions"), v=(state, "particle_velocities"), F=(state, "deformation_gradients"),
+               C=(state, "affine_velocities"))
+    if state.material_state is not None:
+        out.update((field, (state.material_state, name)) for field, name in _MATERIAL_FIELDS)
+    if state.transport_state is not None:
+        out.update((field, (state.transport_state, name)) for field, name in _TRANSPORT_FIELDS)
+    return out
+
+
+def state_field_tensors(state):
+    """The particle tensors of a state keyed by their ``bk_state`` field names (reference layouts, contiguous)."""
+    out = {field: getattr(obj, name) for field, (obj, name) in state_field_slots(state).items()}
+    for field, t in out.items():
+        if not t.is_contiguous():
+            raise ValueError(f"state field {field} must be contiguous")
+    return out
 
 
 def _state_tensors(state):

@@ -7,6 +7,8 @@ Workloads (all SI units):
 * ``loaf_256``       -- box 0.30 x 0.15 x 0.12 on a 256^3 grid (dx = 1/256), 8 particles per cell (~0.72M),
   moisture 0.4 + 0.05 * Gaussian random field (correlation length 0.02, seed 1), porosity U[0.03, 0.07],
   oven 473.15 K, dt = 5e-5.
+* ``batched_doughs`` -- 48 independent spheres, boxes and ellipsoids (half-extent 0.05-0.15, seed 2) on their own
+  64^3 or 128^3 grids, 3k-60k particles each (~1.0M total), physics as ``loaf_256``, dt = 5e-5.
 
 Parameters the workload text leaves open are taken from the reference's
 example scene (``examples/oven_baking.txt``) and recorded in ``SceneSpec``.
@@ -207,6 +209,83 @@ def loaf_256(seed=0, field_seed=1, voxel=0.015, p0=1000.0):
     return SceneSpec("loaf_256", mesh, samples, sim, material, _oven_ding(0.05, p0), _oven_transport(T0, 473.15), steps=2000,
                      water_masses=water, porosities=porosity.astype(np.float32), gas_moles=moles,
                      notes=dict(seed=seed, field_seed=field_seed, p0=p0, poisson=0.45, voxel=voxel))
+
+
+def dough_shapes(count=48, seed=2, half_extent=(0.05, 0.15), max_particles=60000):
+    """Shapes, half-extents and grids of a batch of uneven doughs, drawn from ``default_rng(seed)``.
+
+    Each dough is a sphere, box or ellipsoid (uniform choice) with half-extent
+    h ~ U[half_extent]: radius h, or half-edges / semi-axes h * U[0.6, 1.0] per
+    axis.  It gets a 128^3 grid when 8 particles per cell of it stay within
+    ``max_particles``, else a 64^3 grid.  Returns a list of dicts.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        shape = ("sphere", "box", "ellipsoid")[int(rng.integers(3))]
+        h = float(rng.uniform(*half_extent))
+        axes = np.full(3, h) if shape == "sphere" else h * rng.uniform(0.6, 1.0, 3)
+        volume = 8.0 * np.prod(axes) if shape == "box" else 4.0 / 3.0 * np.pi * np.prod(axes)
+        res = 128 if 8.0 * volume * 128 ** 3 <= max_particles else 64
+        out.append(dict(shape=shape, half_extent=h, axes=axes, res=res, expected_particles=int(8.0 * volume * res ** 3)))
+    return out
+
+
+def dough_scene(shape, axes, res, seed, name="dough", p0=1000.0, porosity_range=(0.03, 0.07)):
+    """One dough resting on the tray of a unit-cube oven: sphere, box or ellipsoid with semi-axes ``axes``.
+
+    Physics as ``loaf_256`` (Poisson's ratio 0.45, pore gas at ambient p0 =
+    1000 Pa, oven 473.15 K, dt = 5e-5), moisture at the configured initial
+    saturation, porosity U[porosity_range] per particle; particles jittered 8
+    per cell from ``default_rng(seed)``, porosities from the same generator's
+    next draws.
+    """
+    dx = 1.0 / res
+    axes = np.asarray(axes, dtype=np.float64)
+    floor = (3 - 1) * dx
+    centre = np.array([0.5, floor + 1.0e-4 + axes[1], 0.5])
+    lo, hi = centre - axes, centre + axes
+    # mesh resolution: at most about one tet per 8 expected particles, so every tet gets one
+    if shape == "box":
+        inside = lambda p: np.all(np.abs(p - centre) <= axes, axis=1)  # noqa: E731
+        size = 2.0 * axes
+        expected = 8.0 * np.prod(size) / dx ** 3
+        per_axis = int(np.clip(np.floor((expected / 48.0) ** (1.0 / 3.0)), 1, 6))
+        counts = np.maximum(1, np.round(size / (size.max() / per_axis))).astype(int)
+        verts, tets = VoxelTetMesh(counts, 1.0, (0.0, 0.0, 0.0))
+        verts = lo + verts * (size / counts)
+    else:
+        inside = lambda p: np.sum(((p - centre) / axes) ** 2, axis=1) <= 1.0  # noqa: E731
+        expected = 8.0 * (4.0 / 3.0) * np.pi * np.prod(axes) / dx ** 3
+        tets_by_subdivision = {2: 48, 3: 162, 4: 384, 5: 750}
+        subdivision = max([k for k, t in tets_by_subdivision.items() if 8 * t <= expected] or [2])
+        unit = CreateSphereMesh(1.0, subdivision)
+        verts, tets = unit.vertex_positions * axes + centre, unit.tet_indices
+    mesh = ValidateTetMesh(verts, tets)
+    rng = np.random.default_rng(seed)
+    pos = jittered_cells(inside, lo, hi, dx, rng)
+    material = _oven_material(poisson=0.45)
+    samples = make_samples(mesh, pos, dx, material.density)
+    porosity = rng.uniform(*porosity_range, size=len(pos))
+    T0 = 298.15
+    sim = SimulationConfig(grid_resolution=res, grid_spacing=dx, grid_origin=(0.0, 0.0, 0.0), time_step=5.0e-5,
+                           gravity=(0.0, -9.8, 0.0), boundary_cells=3, device_name="auto")
+    return SceneSpec(name, mesh, samples, sim, material, _oven_ding(0.05, p0), _oven_transport(T0, 473.15), steps=1000,
+                     porosities=porosity.astype(np.float32), gas_moles=p0 * porosity / (gas_constant * T0),
+                     notes=dict(shape=shape, axes=axes.tolist(), res=res, seed=seed, p0=p0))
+
+
+def batched_doughs(count=48, seed=2, half_extent=(0.05, 0.15), max_particles=60000):
+    """The batched workload: ``count`` independent doughs of uneven size (1k-60k particles each).
+
+    Shapes and grids from ``dough_shapes(count, seed)``; dough i is seeded
+    with ``default_rng([seed, i])``.  With the defaults: 48 doughs, ~1.0M
+    particles, 64^3 and 128^3 grids, 1000 substeps.
+    """
+    specs = []
+    for i, d in enumerate(dough_shapes(count, seed, half_extent, max_particles)):
+        specs.append(dough_scene(d["shape"], d["axes"], d["res"], seed=[seed, i], name=f"dough{i:02d}_{d['shape']}"))
+    return specs
 
 
 def scaled(spec: SceneSpec, **sim_changes):

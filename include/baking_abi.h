@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define BK_ABI_VERSION 1
+#define BK_ABI_VERSION 2
 
 /* Flag block of the fused substep (one int32[8] array):
  *   [0] J nonpositive / nonfinite        (reference MpmSimulator.error_flags[0], physics/mpm.py:245)
@@ -65,9 +65,30 @@ typedef struct bk_params {
   /* sizes */
   int32_t n, n_vertices, n_faces, n_slots;
   /* coupled-porosity solve: 0 = the reference's 64-step bisection (porous_return.py:43-61),
-   * 1 = a bracketed Illinois/bisection solve of the same equation to a few ulps */
+   * 1 = a safeguarded Newton solve of the same equation inside the bisection's bracket */
   int32_t return_solver;
 } bk_params;
+
+/* One dough of a batch (no reference counterpart: the reference runs one
+ * dough per MpmSimulator).  Doughs are packed in order: dough d owns particles
+ * [particle_offset, particle_offset + params.n) and node blocks
+ * [block_offset, block_offset + ceil(params.res/4)^3) of the batch grid, with
+ * particle_offset and block_offset the running sums over the doughs before it. */
+typedef struct bk_dough {
+  bk_params params;
+  int32_t particle_offset;
+  int32_t block_offset;
+} bk_dough;
+
+/* Launch tuning of a workspace (no reference counterpart):
+ *   tile_chunk      most particles per P2G tile (a particle block holding more is split)
+ *   particle_block  workgroup size of the per-particle passes (64, 128, 192 or 256)
+ *   grid_cap        workgroups of the tile and node-block passes (grid-stride)      */
+typedef struct bk_tuning {
+  int32_t tile_chunk;
+  int32_t particle_block;
+  int32_t grid_cap;
+} bk_tuning;
 
 /* Per-particle state of one step (physics/state.py ParticleState, ding.py DingState, thermal_water.py TransportState). */
 typedef struct bk_state {
@@ -94,9 +115,14 @@ typedef struct bk_scene {
   /* boundary faces (thermal_water.py:51-74) grouped by owner slot */
   const int32_t* faces;                      /* F x 3 */
   const int32_t *slot_face_offsets, *slot_faces, *slot_owner, *particle_slot;
+  /* batch only: dough of every particle (NULL for a batch of one).  In a
+   * batch, the per-dough parameters above (visc/prod/heat_param) hold one value
+   * per dough and every mesh, face, slot and particle index is global. */
+  const int32_t* dough_of;
 } bk_scene;
 
-/* Tiled grid of the fused substep (GridState + transport workspace). */
+/* Tiled grid of the fused substep (GridState + transport workspace); a
+ * batch grid holds every dough's blocks, 64 nodes per block. */
 typedef struct bk_grid {
   float *m, *p, *v;                          /* node mass, momentum (x3), velocity (x3) */
   double *V, *C, *E, *W, *H, *L, *X;         /* thermal P2G: volume, capacity, energy, water, heat, loss, exchange */
@@ -113,14 +139,27 @@ int bk_device_info(char* name, int name_len, int* cu_count, int* xcd_count);
 /* Workspace: the only library-owned device memory (sort scratch, tile and
  * active-block lists, per-step stress and surface buffers, flags). */
 bk_workspace* bk_workspace_create(int32_t max_particles, int32_t res, int32_t n_vertices, int32_t n_slots);
+/* Workspace for batches of up to max_doughs doughs whose grids total at most
+ * total_blocks node blocks (no reference counterpart). */
+bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blocks, int32_t max_doughs, int32_t n_vertices,
+                                        int32_t n_slots);
 void bk_workspace_destroy(bk_workspace* ws);
-int32_t* bk_workspace_flags(bk_workspace* ws);          /* device int32[BK_FLAG_COUNT] */
+/* Upload the dough table used by bk_batch_substep (synchronises the device
+ * first).  A table with another grid layout than the last one starts from a
+ * zeroed batch grid. */
+int bk_workspace_set_doughs(bk_workspace* ws, const bk_dough* table, int32_t n_doughs);
+int bk_workspace_set_tuning(bk_workspace* ws, const bk_tuning* tuning);
+int bk_workspace_get_tuning(bk_workspace* ws, bk_tuning* tuning);
+int32_t* bk_workspace_flags(bk_workspace* ws);          /* device int32[BK_FLAG_COUNT * max_doughs], dough-major */
 float* bk_workspace_vertices(bk_workspace* ws);         /* device float[V*3]: last reconstructed mesh */
 /* Copies {tiles, active node blocks, deferred return maps} of the last substep; synchronises. */
 int bk_workspace_counts(bk_workspace* ws, int32_t* out3, void* stream);
 /* Blocking copy of the flag block to host int32[8] after the work already on `stream`; and the reverse. */
 int bk_workspace_read_flags(bk_workspace* ws, int32_t* host8, void* stream);
 int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* stream);
+/* The same for the first n_doughs flag blocks (int32[8 * n_doughs], dough-major). */
+int bk_workspace_read_flags_all(bk_workspace* ws, int32_t* host, int32_t n_doughs, void* stream);
+int bk_workspace_write_flags_all(bk_workspace* ws, const int32_t* host, int32_t n_doughs, void* stream);
 
 /* ---------------------------------------------------------------- sort (build-only; no reference counterpart)
  * Cell key of every particle: flatten_node(stencil_base((x-o)/dx)) of
@@ -148,8 +187,15 @@ int bk_substep(const bk_params* params, const bk_scene* scene, const bk_state* i
 typedef struct bk_graph bk_graph;
 bk_graph* bk_substep_graph_create(const bk_params* params, const bk_scene* scene, const bk_state* in, bk_state* out,
                                   bk_grid* grid, bk_workspace* ws);
+/* A graph keeps the dough table it was captured with and re-uploads it at
+ * launch if another table was set on its workspace since. */
 int bk_graph_launch(bk_graph* graph, void* stream);
 void bk_graph_destroy(bk_graph* graph);
+/* One substep of every dough of the uploaded table, each as MpmSimulator.Step
+ * (physics/mpm.py:163-225) of that dough alone, in one launch per stage; flags
+ * are raised in the dough's own block.  bk_substep is this with a table of one. */
+int bk_batch_substep(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out, bk_grid* grid, void* stream);
+bk_graph* bk_batch_graph_create(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out, bk_grid* grid);
 /* Per-stage timing of bk_substep with HIP events on its stream: begin() arms
  * recording for the next max_steps substeps; end() waits for them and returns
  * the summed milliseconds per stage (bk_profile_stage_count() entries, named

@@ -40,4 +40,28 @@ def test_ctypes_binding_covers_the_header():
         pytest.skip("libbaking_hip.so not built")
     lib = _native.load()  # loads on a machine without a GPU: no device call happens
     assert set(_native.SIGNATURES) == set(declared_functions())
-    assert lib.bk_abi_version() == 1
+    assert lib.bk_abi_version() == 2
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """sizeof and field offsets of every ABI struct agree between the header (gcc) and the ctypes mirror."""
+    import ctypes
+    from baking_amd import _native
+    structs = {"bk_params": _native.BkParams, "bk_dough": _native.BkDough, "bk_tuning": _native.BkTuning,
+               "bk_state": _native.BkState, "bk_scene": _native.BkScene, "bk_grid": _native.BkGrid}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, cls in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    got = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    for line in filter(None, got):
+        cname, field, value = line.split()
+        cls = structs[cname]
+        want = ctypes.sizeof(cls) if field == "size" else getattr(cls, field).offset
+        assert int(value) == want, (cname, field, int(value), want)
