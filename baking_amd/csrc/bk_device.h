@@ -606,6 +606,46 @@ __device__ __forceinline__ void wave_append(int32_t* counter, int32_t* queue, bo
   }
 }
 
+// Reserve `count` consecutive slots of a device list for every thread of a
+// workgroup with one atomic per workgroup: an inclusive scan of the counts
+// within each wavefront (shuffles), then across the wavefronts in LDS.
+// Returns the thread's first slot.  Every thread of the workgroup must call
+// it (blockDim.x a multiple of 64, at most 1024); `lds` holds 17 ints.
+// Appends to one counter from many wavefronts serialise at that address, so
+// batching them per workgroup keeps the counter off the critical path.
+__device__ __forceinline__ int block_reserve(int32_t* counter, int count, int* lds) {
+  const int lane = int(__lane_id());
+  const int w = int(threadIdx.x) >> 6;
+  const int nw = int(blockDim.x) >> 6;
+  int incl = count;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    int y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) lds[w] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int i = 0; i < nw; ++i) {
+      int c = lds[i];
+      lds[i] = run;
+      run += c;
+    }
+    lds[16] = run > 0 ? atomicAdd(counter, run) : 0;
+  }
+  __syncthreads();
+  const int first = lds[16] + lds[w] + incl - count;
+  __syncthreads();  // lds may be reused right after
+  return first;
+}
+
+// block_reserve for one optional item per thread.
+__device__ __forceinline__ void block_append(int32_t* counter, int32_t* list, bool take, int32_t value, int* lds) {
+  const int slot = block_reserve(counter, take ? 1 : 0, lds);
+  if (take) list[slot] = value;
+}
+
 __device__ __forceinline__ void flag_max(int* flags, int index, bool set) {
   // One atomic per wavefront: the lowest set lane reports.
   unsigned long long m = __ballot(set);

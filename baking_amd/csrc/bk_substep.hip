@@ -334,43 +334,70 @@ static int radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int
 }
 
 // ============================================================================ blocks, tiles and active node blocks
-// Bounds of every non-empty particle block in the sorted order, and the list of those blocks.
-__global__ void k_block_bounds(const uint32_t* __restrict__ skeys, int n, bk_workspace* ws) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool start = false;
-  uint32_t blk = 0;
-  if (i < n) {
-    blk = skeys[i] >> 6;
-    start = i == 0 || (skeys[i - 1] >> 6) != blk;
-    if (start) ws->blk_start[blk] = i;
-    if (i == n - 1 || (skeys[i + 1] >> 6) != blk) ws->blk_end[blk] = i + 1;
+// Bounds of every non-empty particle block in the sorted order, and the list
+// of those blocks.  Each thread covers kBoundsItems keys (coalesced), so one
+// atomic per workgroup appends its block starts.
+constexpr int kBoundsItems = 8;
+__global__ __launch_bounds__(256) void k_block_bounds(const uint32_t* __restrict__ skeys, int n, bk_workspace* ws) {
+  __shared__ int lds[17];
+  const int base = blockIdx.x * (256 * kBoundsItems) + int(threadIdx.x);
+  int count = 0;
+#pragma unroll
+  for (int k = 0; k < kBoundsItems; ++k) {
+    const int i = base + k * 256;
+    if (i < n) {
+      const uint32_t blk = skeys[i] >> 6;
+      if (i == 0 || (skeys[i - 1] >> 6) != blk) {
+        ws->blk_start[blk] = i;
+        ++count;
+      }
+      if (i == n - 1 || (skeys[i + 1] >> 6) != blk) ws->blk_end[blk] = i + 1;
+    }
   }
-  wave_append(&ws->counters[3], ws->blocks, start, int32_t(blk));
+  int slot = block_reserve(&ws->counters[3], count, lds);
+#pragma unroll
+  for (int k = 0; k < kBoundsItems; ++k) {
+    const int i = base + k * 256;
+    if (i < n) {
+      const uint32_t blk = skeys[i] >> 6;
+      if (i == 0 || (skeys[i - 1] >> 6) != blk) ws->blocks[slot++] = int32_t(blk);
+    }
+  }
 }
 
-// Eight lanes per non-empty particle block: each activates one of the 2x2x2
-// node blocks the block's stencils reach; lane 0 also cuts the block into
-// tiles of at most `chunk` particles.
-__global__ void k_activate(const bk_dough* __restrict__ doughs, int nd, bk_workspace* ws, int chunk) {
-  const int nblk = ws->counters[3];
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nblk * 8; t += gridDim.x * blockDim.x) {
-    const int b = ws->blocks[t >> 3];
-    const int corner = t & 7;
-    const BlockGeom g = block_geom(doughs, dough_of_block(doughs, nd, b), b);
-    if (corner == 0) {
-      const int start = ws->blk_start[b], end = ws->blk_end[b];
-      for (int s = start; s < end; s += chunk) {
-        int slot = atomicAdd(&ws->counters[0], 1);
-        ws->tiles[slot] = make_int4(b, g.d, s, min(chunk, end - s));
+// Eight threads per non-empty particle block: each activates one of the 2x2x2
+// node blocks the block's stencils reach; the first also cuts the block into
+// tiles of at most `chunk` particles.  The loop's trip count is uniform per
+// workgroup, so every thread reaches the per-workgroup appends.
+__global__ __launch_bounds__(256) void k_activate(const bk_dough* __restrict__ doughs, int nd, bk_workspace* ws, int chunk) {
+  __shared__ int lds[17];
+  const int total = ws->counters[3] * 8;
+  for (int base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+    const int t = base + int(threadIdx.x);
+    int b = 0, d = 0, start = 0, end = 0, ntile = 0, nbk = 0;
+    bool fresh = false;
+    if (t < total) {
+      b = ws->blocks[t >> 3];
+      const int corner = t & 7;
+      d = dough_of_block(doughs, nd, b);
+      const BlockGeom g = block_geom(doughs, d, b);
+      if (corner == 0) {
+        start = ws->blk_start[b];
+        end = ws->blk_end[b];
+        ntile = (end - start + chunk - 1) / chunk;
+      }
+      const int cx = g.bx + (corner >> 2), cy = g.by + ((corner >> 1) & 1), cz = g.bz + (corner & 1);
+      if (cx < g.nb && cy < g.nb && cz < g.nb) {
+        nbk = doughs[d].block_offset + (cx * g.nb + cy) * g.nb + cz;
+        fresh = atomicExch(&ws->node_flag[nbk], 1) == 0;
       }
     }
-    int cx = g.bx + (corner >> 2), cy = g.by + ((corner >> 1) & 1), cz = g.bz + (corner & 1);
-    if (cx >= g.nb || cy >= g.nb || cz >= g.nb) continue;
-    int nbk = doughs[g.d].block_offset + (cx * g.nb + cy) * g.nb + cz;
-    if (atomicExch(&ws->node_flag[nbk], 1) == 0) {
-      int slot = atomicAdd(&ws->counters[1], 1);
-      ws->active[slot] = nbk;
+    const int slot = block_reserve(&ws->counters[0], ntile, lds);
+    for (int k = 0; k < ntile; ++k) {
+      const int s = start + k * chunk;
+      ws->tiles[slot + k] = make_int4(b, d, s, min(chunk, end - s));
     }
+    block_append(&ws->counters[1], ws->active, fresh, nbk, lds);
   }
 }
 
@@ -878,6 +905,7 @@ __global__ __launch_bounds__(256) void k_pass_c_deferred(const bk_dough* __restr
 // ============================================================================ pass C: G2P + Ding + gas refresh + saturation
 __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in,
                                                              bk_state out, bk_grid g, bk_workspace* ws, int n) {
+  __shared__ int lds[17];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   bool deferred = false;
   if (p < n) {
@@ -940,7 +968,7 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __r
       flag_max(fl, BK_FLAG_COUPLED, f1);
     });
   }
-  wave_append(&ws->counters[2], ws->queue, deferred, p);
+  block_append(&ws->counters[2], ws->queue, deferred, p, lds);
 }
 
 __global__ void k_untile_f32(const float* __restrict__ tiled, float* __restrict__ dense, int res, int comps) {
@@ -1035,8 +1063,8 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
   }
   {  // 4. block bounds, tiles, active node blocks
     StageTimer tm(pr, st, kStageTiles);
-    hipLaunchKernelGGL(k_block_bounds, dim3(div_up(n, 256)), dim3(256), 0, st, h->sorted_keys, n, ws);
-    hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)n * 8, 256), 1, 2048)), dim3(256), 0, st, D, nd, ws, tu.tile_chunk);
+    hipLaunchKernelGGL(k_block_bounds, dim3(div_up(n, 256 * kBoundsItems)), dim3(256), 0, st, h->sorted_keys, n, ws);
+    hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)n * 8, 256), 1, 1024)), dim3(256), 0, st, D, nd, ws, tu.tile_chunk);
     BK_LAUNCH(tiles);
   }
   if (hw->thermal) {

@@ -290,6 +290,28 @@ class BatchSimulator:
         s._write_flags(self.ReadFlags()[dough].tolist())
         return s.CheckState()
 
+    # ------------------------------------------------------------------ autotuner hooks
+    def _launch_substeps(self, count, stream=None):
+        for _ in range(count):
+            self._launch(stream)
+
+    def _save_for_tuning(self):
+        flags = self.ReadFlags()
+        return self._cur, flags, [t.clone() for side in self._states for t in side.values()]
+
+    def _restore_after_tuning(self, saved):
+        cur, flags, clones = saved
+        torch.cuda.synchronize()
+        for dst, src in zip([t for side in self._states for t in side.values()], clones):
+            dst.copy_(src)
+        for t in self.grid.values():
+            t.zero_()
+        self._cur = cur
+        arr = (ctypes.c_int32 * flags.size)(*flags.reshape(-1).tolist())
+        _native.check(_native.load().bk_workspace_write_flags_all(self._workspace, ctypes.cast(arr, ctypes.c_void_p),
+                                                                  self.dough_count, _native.stream_ptr()),
+                      "bk_workspace_write_flags_all")
+
     # ------------------------------------------------------------------ views
     def dough_slice(self, dough):
         return slice(int(self.particle_offsets[dough]), int(self.particle_offsets[dough + 1]))

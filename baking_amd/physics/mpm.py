@@ -270,6 +270,43 @@ class MpmSimulator:
         self.return_solver = name
         self._params = self._make_params(settling=False)
 
+    def SetTuning(self, tile_chunk=None, particle_block=None, grid_cap=None):
+        """Launch shapes of the fused substep (``bk_tuning``); captured graphs are dropped."""
+        lib = _native.load()
+        t = self.Tuning()
+        for name, value in (("tile_chunk", tile_chunk), ("particle_block", particle_block), ("grid_cap", grid_cap)):
+            if value is not None:
+                t[name] = int(value)
+        struct = _native.BkTuning(t["tile_chunk"], t["particle_block"], t["grid_cap"])
+        _native.check(lib.bk_workspace_set_tuning(self._workspace, ctypes.byref(struct)), "bk_workspace_set_tuning")
+        self._destroy_graphs()
+
+    def Tuning(self):
+        struct = _native.BkTuning()
+        _native.check(_native.load().bk_workspace_get_tuning(self._workspace, ctypes.byref(struct)), "bk_workspace_get_tuning")
+        return dict(tile_chunk=struct.tile_chunk, particle_block=struct.particle_block, grid_cap=struct.grid_cap)
+
+    def _launch_substeps(self, count, stream=None):
+        for _ in range(count):
+            self._launch_substep(stream)
+            self._accept()
+
+    def _save_for_tuning(self):
+        flags = self._read_flags()
+        tensors = _state_tensors(self.current_state) + _state_tensors(self.next_state)
+        return self.current_state, self.next_state, self.step_count, flags, [t.clone() for t in tensors]
+
+    def _restore_after_tuning(self, saved):
+        current, nxt, steps, flags, clones = saved
+        torch.cuda.synchronize()
+        self.current_state, self.next_state, self.step_count = current, nxt, steps
+        for dst, src in zip(_state_tensors(current) + _state_tensors(nxt), clones):
+            dst.copy_(src)
+        self.grid_state.Clear()
+        for t in self.thermal_grid.values():
+            t.zero_()
+        self._write_flags(flags)
+
     def _destroy_graphs(self):
         lib = _native.load()
         for handle in getattr(self, "_graphs", {}).values():

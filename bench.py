@@ -1,10 +1,17 @@
 #!/usr/bin/env python3
 """Benchmark: particle-substeps/s of the fused MLS-MPM baking substep on one MI355X.
 
-Workload (default ``loaf_256``): the 0.30 x 0.15 x 0.12 loaf on a 256^3 grid,
-~0.72M particles, Ding + gas + heat/water transport, FP32 state with the
-reference's FP64 transport, synthetic seeded inputs.  One "step" is one
-substep of the whole particle set.
+Workloads (synthetic seeded inputs, Ding + gas + heat/water transport, FP32
+state with the reference's FP64 transport; one "step" is one substep of every
+particle):
+
+* ``loaf_256`` (default): the 0.30 x 0.15 x 0.12 loaf on a 256^3 grid, ~0.72M
+  particles, one simulator.
+* ``batched_48``: 48 independent doughs of 3k-60k particles (~1.0M) on their
+  own 64^3 / 128^3 grids, advanced as one batch (one launch per stage over
+  every dough, one HIP graph per substep, autotuned launch shapes).  Also
+  times the plain comparator: the same doughs one at a time through the
+  single-dough substep, launched kernel by kernel without a graph.
 
 Prints ONE JSON line with the throughput, a roofline entry for the dominant
 kernel (HIP events on the launch stream, algorithmic bytes at the
@@ -57,7 +64,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="loaf_256", choices=["loaf_256", "sphere_small"])
+    ap.add_argument("--workload", default="loaf_256", choices=["loaf_256", "sphere_small", "batched_48"])
+    ap.add_argument("--plain-steps", type=int, default=20, help="batched_48: substeps per dough of the plain comparator")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-baseline-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -102,6 +111,157 @@ def measured_traffic(workload, kernel, particles, active_nodes, deferred):
     return entry["hbm_bytes_per_unit"] * units
 
 
+def stage_profile(lib_launch, workspace, steps):
+    """Per-stage milliseconds per substep (HIP events around each stage on the launch stream) and the
+    workspace counters {tiles, active node blocks, deferred return maps} of the last substep."""
+    import ctypes
+    from baking_amd import _native
+    lib = _native.load()
+    _native.check(lib.bk_profile_begin(workspace, steps), "bk_profile_begin")
+    for _ in range(steps):
+        lib_launch()
+    nst = lib.bk_profile_stage_count()
+    ms = (ctypes.c_float * nst)()
+    rec = ctypes.c_int32(0)
+    _native.check(lib.bk_profile_end(workspace, ctypes.cast(ms, ctypes.c_void_p), ctypes.cast(ctypes.pointer(rec), ctypes.c_void_p)),
+                  "bk_profile_end")
+    stages = {lib.bk_profile_stage_name(k).decode(): ms[k] / max(rec.value, 1) for k in range(nst)}
+    counts = (ctypes.c_int32 * 3)()
+    _native.check(lib.bk_workspace_counts(workspace, ctypes.cast(counts, ctypes.c_void_p), _native.stream_ptr()),
+                  "bk_workspace_counts")
+    return stages, [int(c) for c in counts]
+
+
+def roofline(stages, n, active_nodes, deferred, substep_ms, workload):
+    """Roofline entry of the dominant stage: algorithmic bytes per launch over its event-timed duration."""
+    dominant = max(stages, key=lambda k: stages[k])
+    if dominant == "pass_c_deferred":
+        dom_bytes = DEFERRED_BYTES * deferred
+    else:
+        per_p, per_node = STAGE_BYTES.get(dominant, (0, 0))
+        dom_bytes = per_p * n + per_node * active_nodes
+    dom_ms = stages[dominant]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
+    substep_bytes = SUBSTEP_BYTES[0] * n + SUBSTEP_BYTES[1] * active_nodes
+    substep_gbs = substep_bytes / (substep_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload, dominant, n, active_nodes, deferred),
+            "algorithmic_bytes": dom_bytes, "avg_ms": dom_ms, "stages_ms": stages,
+            "substep_achieved": substep_gbs, "substep_frac": substep_gbs / HBM_PEAK_GBS}
+    return roof, dominant, dom_bytes, dom_ms
+
+
+def run_batched(args):
+    """The batched workload: plain one-at-a-time comparator, then the batch (autotuned, graph replay)."""
+    import torch
+    from baking_amd import scenes
+    from baking_amd.autotune import autotune
+    from baking_amd.physics.batch import BatchSimulator
+
+    t_build = time.perf_counter()
+    specs = scenes.batched_doughs()
+    sims = [scenes.build_simulator(s) for s in specs]
+    build_s = time.perf_counter() - t_build
+    counts = [s.particle_data.particle_count for s in sims]
+    n = int(sum(counts))
+    stream = torch.cuda.current_stream()
+
+    # plain comparator: each dough alone, single-dough substep launched kernel by kernel
+    plain_ms = 0.0
+    for s in sims:
+        s.use_graphs = False
+        for _ in range(2):
+            s._launch_substep()
+            s._accept()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(args.plain_steps):
+            s._launch_substep()
+            s._accept()
+        stop.record(stream)
+        stop.synchronize()
+        plain_ms += start.elapsed_time(stop)
+    plain_rate = n * args.plain_steps / (plain_ms * 1e-3)
+
+    batch = BatchSimulator(sims)
+    tuned = None if args.no_autotune else autotune(batch, log=None)
+    for _ in range(args.warmup):
+        batch._launch()
+    torch.cuda.synchronize()
+    flags_before = batch.ReadFlags()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for _ in range(args.steps):
+        batch._launch()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(stop)
+    flags = batch.ReadFlags()
+    batch.step_count += args.warmup + args.steps
+    batch._sync_doughs(args.warmup + args.steps)
+
+    stages, wcounts = stage_profile(lib_launch=lambda: batch._launch(use_graph=False), workspace=batch._workspace,
+                                    steps=args.profile_steps)
+    active_nodes = int((batch.grid["m"] > 0).sum().item())
+    substep_ms = gpu_ms / args.steps
+    roof, _, _, _ = roofline(stages, n, active_nodes, wcounts[2], substep_ms, "batched_48")
+    residuals = batch._states[batch._cur].get("residual")
+    yield_fraction = float((residuals > 0).float().mean().item()) if residuals is not None else 0.0
+    value = n * args.steps / (gpu_ms * 1e-3)
+    result = {
+        "metric": "particle-substeps/s on 1 MI355X, plain vs surrogate+batched at <=1e-3 rel err",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": substep_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched_48", "method": "batched (exact return map)", "hip_graph": True, "doughs": len(sims),
+                   "particles": n, "particles_per_dough": [min(counts), max(counts)],
+                   "grids": sorted(set(s.simulation_config.grid_resolution for s in sims)), "dt": 5e-5,
+                   "active_stencil_nodes": active_nodes, "yield_fraction": yield_fraction, "deferred_return_maps": wcounts[2],
+                   "tiles": wcounts[0], "active_node_blocks": wcounts[1], "tuning": batch.Tuning(),
+                   "autotune": None if tuned is None else {"start": tuned["start"], "best_ms": tuned["ms"],
+                                                           "trials": len(tuned["trials"])},
+                   "wall_s": wall, "build_s": build_s, "flags_max": int(flags.max()), "flags_before_max": int(flags_before.max())},
+        "plain": {"value": plain_rate, "unit": "particle-substeps/s", "steps_per_dough": args.plain_steps,
+                  "method": "each dough alone, single-dough substep launched kernel by kernel (no graph), exact return map",
+                  "ms_total": plain_ms},
+        "speedup_vs_plain": value / plain_rate,
+        "roofline": roof,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline_batched(specs, args.cpu_baseline_steps)
+    print(json.dumps(result))
+
+
+def cpu_baseline_batched(specs, steps, every=4):
+    """The serial C++ oracle on every `every`-th dough of the batch (a bounded sample of the workload)."""
+    from tests.harness import oracle_from_spec
+    total_p, total_s = 0, 0.0
+    for spec in specs[::every]:
+        sim = oracle_from_spec(spec)
+        sim.Step()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.Step()
+        total_s += time.perf_counter() - t0
+        total_p += sim.n
+        del sim
+    return {"value": total_p * steps / total_s, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} substeps (after 1 untimed) of every {every}th dough of the batch ({len(specs[::every])} doughs, "
+                      f"{total_p} particles, each on its own dense grid), serial oracle (liboracle.so, g++ -O2)"}
+
+
 def copy_probe():
     import torch
     a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
@@ -125,11 +285,18 @@ def main():
     from baking_amd import _native
     from baking_amd.scenes import build_simulator
 
+    if args.workload == "batched_48":
+        return run_batched(args)
+    lib = _native.load()
     spec = build_scene(args.workload)
     sim = build_simulator(spec)
     sim.use_graphs = not args.no_graph
     n = sim.particle_data.particle_count
     stream = torch.cuda.current_stream()
+    tuned = None
+    if not args.no_autotune and sim.use_graphs:
+        from baking_amd.autotune import autotune
+        tuned = autotune(sim)
 
     for _ in range(args.warmup):
         sim._launch_substep()
@@ -150,35 +317,12 @@ def main():
     gpu_ms = start.elapsed_time(stop)
     flags = sim._read_flags()
 
-    # per-stage timing with events on the launch stream
-    lib = _native.load()
-    _native.check(lib.bk_profile_begin(sim._workspace, args.profile_steps), "bk_profile_begin")
-    for _ in range(args.profile_steps):
-        sim._launch_substep(use_graph=False)
-        sim._accept()
-    nst = lib.bk_profile_stage_count()
-    import ctypes
-    ms = (ctypes.c_float * nst)()
-    rec = ctypes.c_int32(0)
-    _native.check(lib.bk_profile_end(sim._workspace, ctypes.cast(ms, ctypes.c_void_p), ctypes.cast(ctypes.pointer(rec), ctypes.c_void_p)),
-                  "bk_profile_end")
-    stages = {lib.bk_profile_stage_name(k).decode(): ms[k] / max(rec.value, 1) for k in range(nst)}
-    counts = (ctypes.c_int32 * 3)()
-    _native.check(lib.bk_workspace_counts(sim._workspace, ctypes.cast(counts, ctypes.c_void_p), _native.stream_ptr()),
-                  "bk_workspace_counts")
+    stages, counts = stage_profile(lib_launch=lambda: (sim._launch_substep(use_graph=False), sim._accept()),
+                                   workspace=sim._workspace, steps=args.profile_steps)
     deferred = int(counts[2])
     active_nodes = int((sim.grid_state.tiled_masses > 0).sum().item())
-    dominant = max(stages, key=lambda k: stages[k])
-    if dominant == "pass_c_deferred":
-        dom_bytes = DEFERRED_BYTES * deferred
-    else:
-        per_p, per_node = STAGE_BYTES.get(dominant, (0, 0))
-        dom_bytes = per_p * n + per_node * active_nodes
-    dom_ms = stages[dominant]
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
-    traffic = measured_traffic(spec.name, dominant, n, active_nodes, deferred)
+    roof, dominant, dom_bytes, dom_ms = roofline(stages, n, active_nodes, deferred, gpu_ms / args.steps, spec.name)
     substep_ms = gpu_ms / args.steps
-    substep_bytes = SUBSTEP_BYTES[0] * n + SUBSTEP_BYTES[1] * active_nodes
     yield_fraction = float((sim.current_state.material_state.return_residuals > 0).float().mean().item()) \
         if sim.current_state.material_state is not None else 0.0
 
@@ -199,12 +343,11 @@ def main():
         "config": {"workload": spec.name, "method": "plain", "hip_graph": sim.use_graphs, "particles": n, "grid": spec.simulation.grid_resolution,
                    "dt": spec.simulation.time_step, "active_stencil_nodes": active_nodes,
                    "yield_fraction": yield_fraction, "deferred_return_maps": deferred,
-                   "tiles": int(counts[0]), "active_node_blocks": int(counts[1]), "wall_s": wall, "flags": flags, "flags_before_timing": flags_before},
-        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": dom_bytes,
-                     "avg_ms": dom_ms, "stages_ms": stages,
-                     "substep_achieved": substep_bytes / (substep_ms * 1e-3) / 1e9,
-                     "substep_frac": substep_bytes / (substep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                   "tiles": int(counts[0]), "active_node_blocks": int(counts[1]), "tuning": sim.Tuning(),
+                   "autotune": None if tuned is None else {"start": tuned["start"], "best_ms": tuned["ms"],
+                                                           "trials": len(tuned["trials"])},
+                   "wall_s": wall, "flags": flags, "flags_before_timing": flags_before},
+        "roofline": roof,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
