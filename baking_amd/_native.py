@@ -30,7 +30,7 @@ class BkParams(ctypes.Structure):
         ("latent", c_float), ("kappa", c_float), ("diffusivity", c_float), ("s0", c_float), ("water_stress", c_float),
         ("contact_distance", c_float),
         ("n", c_int32), ("n_vertices", c_int32), ("n_faces", c_int32), ("n_slots", c_int32),
-        ("return_solver", c_int32),
+        ("return_solver", c_int32), ("surrogate", c_int32),
     ]
 
 
@@ -123,6 +123,16 @@ SIGNATURES = {
     "bk_gas_pressure": ([P, P, P, P, P, I, P], c_int),
     "bk_porosity_coefficient": ([P, P, I, P], c_int),
     "bk_solve_porosity": ([P, P, P, P, I, I, P], c_int),
+    "bk_mlp_theta_count": ([], c_int32),
+    "bk_mlp_blob_bytes": ([], c_int32),
+    "bk_mlp_grad_chunk": ([], c_int32),
+    "bk_mlp_infer": ([P, P, P, I, P], c_int),
+    "bk_mlp_forward": ([P, P, P, I, P, P], c_int),
+    "bk_mlp_gradient": ([P, P, P, I, P, P, P, P, P, P, P, P], c_int),
+    "bk_adam_step": ([P, P, P, P, I, F, F, F, F, I, P], c_int),
+    "bk_workspace_set_surrogate": ([P, P], c_int),
+    "bk_workspace_set_samples": ([P, P, I], c_int),
+    "bk_workspace_stats": ([P, P, I, P], c_int),
 }
 
 _lib = None

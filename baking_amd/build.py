@@ -23,8 +23,17 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-t
           "-I", str(HERE.parent / "include")]
 
 
-def _compile(src: Path) -> Path:
+def _stale(src: Path, obj: Path) -> bool:
+    if not obj.exists():
+        return True
+    deps = [src, *CSRC.glob("*.h"), *(HERE.parent / "include").glob("*.h")]
+    return obj.stat().st_mtime < max(d.stat().st_mtime for d in deps)
+
+
+def _compile(src: Path, force: bool = True) -> Path:
     obj = BUILD / (src.stem + ".o")
+    if not force and not _stale(src, obj):
+        return obj
     cmd = [HIPCC, *COMMON, "-c", str(src), "-o", str(obj)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
@@ -34,11 +43,12 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build(verbose: bool = True) -> Path:
+def build(verbose: bool = True, force: bool = True) -> Path:
+    """Compile every source (or, with force=False, those older than a source or header) and link."""
     BUILD.mkdir(exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
     with ThreadPoolExecutor(max_workers=min(8, len(sources))) as pool:
-        objects = list(pool.map(_compile, sources))
+        objects = list(pool.map(lambda src: _compile(src, force), sources))
     tmp = LIB.with_name(LIB.name + ".tmp")
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *map(str, objects), "-o", str(tmp)]
     subprocess.run(cmd, check=True, capture_output=True, text=True)

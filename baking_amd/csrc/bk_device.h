@@ -253,7 +253,11 @@ __device__ __forceinline__ float solve_overstress(float q2, float p2, float df, 
       float dd2 = dd * dd, md2 = md * md;
       float res = root + 1.0f - q2 / dd2 - p2 / md2;
       float der = 1.0f + 2.0f * df * q2 / (dd2 * dd) + 2.0f * mf * p2 / (md2 * md);
-      root = root - res / der;
+      float next = root - res / der;
+      // a fixed point of the iteration stays fixed, so stopping there returns
+      // exactly what the remaining fixed steps would
+      if (next == root) break;
+      root = next;
     }
   }
   return root;
@@ -357,7 +361,8 @@ __device__ __forceinline__ double porosity_gap(double n, float qf, float pf, flo
 // bisection to double rounding, hence to the float32 result it returns.
 // Results at the endpoints (the reference's flagged no-solution cases) are
 // recomputed with the literal bisection.
-__device__ inline float solve_porosity_bracketed(float q, float p, float df, float ms, float ex, double pv, float n0) {
+__device__ inline float solve_porosity_bracketed(float q, float p, float df, float ms, float ex, double pv, float n0,
+                                                int* evals = nullptr) {
   double trial = pv - 1.0 + double(n0);
   if (trial > 0.0 && trial < 1.0) {
     double tc = pressure_coefficient64(trial);
@@ -369,6 +374,7 @@ __device__ inline float solve_porosity_bracketed(float q, float p, float df, flo
   bool done = false;
   for (int it = 0; it < 100 && !done; ++it) {
     double f = porosity_gap(x, q, p, df, ms, ex, pv, n0, gamma, slope);
+    if (evals) ++*evals;
     if (f < 0.0) lo = x; else hi = x;
     if (hi - lo <= 4.0e-16 * hi) break;
     double step = slope > 0.0 ? f / slope : 2.0;
@@ -382,8 +388,38 @@ __device__ inline float solve_porosity_bracketed(float q, float p, float df, flo
     done = fabs(xn - x) <= 4.0e-16 * fabs(x);
     x = xn;
   }
-  if (!(x > 1.0e-300) || x >= 1.0 - 2.0e-9) return solve_porosity(q, p, df, ms, ex, pv, n0);
+  if (!(x > 1.0e-300) || x >= 1.0 - 2.0e-9) {
+    if (evals) *evals += 64;
+    return solve_porosity(q, p, df, ms, ex, pv, n0);
+  }
   return float(x);
+}
+
+// The same root from a predicted start (n_start, with gamma_start for the
+// inner overstress solve): plain Newton on f(n) = n - prediction(n), at most
+// kGuidedEvals evaluations, every iterate inside (0, 1 - 1e-9).  Converged
+// when a Newton step is below 1e-10: the next iterate is then within
+// ~f''/f' * 1e-20 of the root, so float(root) is the float the bracketed
+// solve returns.  Returns false when not converged (the caller then runs the
+// bracketed solve).
+constexpr int kGuidedEvals = 4;
+__device__ inline bool solve_porosity_guided(float q, float p, float df, float ms, float ex, double pv, float n0, double n_start,
+                                             double gamma_start, float& root, int& evals) {
+  double x = n_start, gamma = gamma_start > 0.0 ? gamma_start : 0.0, slope = 1.0;
+  for (int it = 0; it < kGuidedEvals; ++it) {
+    if (!(x > 0.0 && x < 1.0 - 1.0e-9)) return false;
+    double f = porosity_gap(x, q, p, df, ms, ex, pv, n0, gamma, slope);
+    ++evals;
+    if (!(slope > 0.0) || !isfinite(f)) return false;
+    double step = f / slope;
+    x -= step;
+    if (fabs(step) <= 1.0e-10) {
+      if (!(x > 0.0 && x < 1.0 - 2.0e-9)) return false;
+      root = float(x);
+      return true;
+    }
+  }
+  return false;
 }
 
 // Result of the Ding return map for one particle.
@@ -404,7 +440,7 @@ struct DingParams {
 // ding.py:103-137 with the SVD of the trial gradient precomputed.
 __device__ __forceinline__ void ding_return(const mat3<float>& trial, const mat3<float>& U, const float s[3], const mat3<float>& V,
                                             float mu, float lam, float y, float eta, float a, float dt, mat3<float>& Fe,
-                                            mat3<float>& stress, float& residual) {
+                                            mat3<float>& stress, float& residual, float* eps_ret = nullptr) {
   float eps[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
   float tr = eps[0] + eps[1] + eps[2];
   float ts[3];
@@ -423,7 +459,11 @@ __device__ __forceinline__ void ding_return(const mat3<float>& trial, const mat3
   float cm = (cs[0] + cs[1] + cs[2]) / 3.0f;
   float e[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) e[i] = expf((cs[i] - cm) / (2.0f * mu) + cm / (2.0f * mu + 3.0f * lam));
+  for (int i = 0; i < 3; ++i) {
+    const float le = (cs[i] - cm) / (2.0f * mu) + cm / (2.0f * mu + 3.0f * lam);
+    if (eps_ret) eps_ret[i] = le;
+    e[i] = expf(le);
+  }
   // Fe = U diag(e) V^T ; stress = U diag(cs) U^T
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -444,6 +484,22 @@ __device__ __forceinline__ void ding_return(const mat3<float>& trial, const mat3
 // Outcome of ding_update.
 enum DingStatus { kDingInvalid = 0, kDingDone = 1, kDingDeferred = 2 };
 
+// Optional extras of ding_update for the coupled (gas) return map.
+struct DingAux {
+  // in: solve the coupled porosity from a predicted start first (see solve_porosity_guided)
+  bool guided = false;
+  double n_start = 0.0, gamma_start = 0.0;
+  float dev_target = 0.0f;  // predicted |deviatoric returned Kirchhoff stress| (0: start the overstress at 0)
+  // out
+  bool accepted = false;  // the guided solve converged
+  int evals = 0;          // evaluations of the porosity equation
+  float eps[3];           // principal trial log-strains, descending
+  int order[3];           // eps[k] is the principal value order[k] of the SVD
+  float eps_ret[3];       // returned log-strains (before the volume fit), in the order of eps
+  float T;                // temperature the yield and viscosity factors were taken at
+  float dev_norm;         // |deviatoric trial Kirchhoff stress|
+};
+
 // ding.py:140-221 (UpdateDing body for one particle).  Returns kDingInvalid when
 // the incoming porosity is outside [0, 1) (flag 0, outputs untouched).  With
 // `defer` set, a particle whose coupled porosity needs the bisection solve
@@ -451,7 +507,8 @@ enum DingStatus { kDingInvalid = 0, kDingDone = 1, kDingDeferred = 2 };
 // pass can run the expensive solves densely; recomputing from the same inputs
 // gives the same result.
 __device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old, float porosity, int gelled_in, const mat3<float>& F_total,
-                                  const mat3<float>& C_new, float T, float visc_param, DingOut& o, bool defer = false) {
+                                  const mat3<float>& C_new, float T, float visc_param, DingOut& o, bool defer = false,
+                                  DingAux* aux = nullptr) {
   o.flag0 = 0;
   o.flag1 = 0;
   if (!isfinite(porosity) || porosity < 0.0f || porosity >= 1.0f) { o.flag0 = 1; return kDingInvalid; }
@@ -492,19 +549,56 @@ __device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old
     float ms = 2.0f / 3.0f * (2.0f * mu + 3.0f * lam) * dt * yv * yv / ev;
     float ex = 3.0f * mean / (2.0f * mu + 3.0f * lam);
     double opv = 1.0 - double(P.n0) + double(porosity);
+    if (aux) {
+      // descending order by a 3-element network on (value, index) pairs, kept in registers
+      float e0 = eps[0], e1 = eps[1], e2 = eps[2];
+      int o0 = 0, o1 = 1, o2 = 2;
+      if (e0 < e1) { float t = e0; e0 = e1; e1 = t; int u = o0; o0 = o1; o1 = u; }
+      if (e1 < e2) { float t = e1; e1 = e2; e2 = t; int u = o1; o1 = o2; o2 = u; }
+      if (e0 < e1) { float t = e0; e0 = e1; e1 = t; int u = o0; o0 = o1; o1 = u; }
+      aux->order[0] = o0; aux->order[1] = o1; aux->order[2] = o2;
+      aux->eps[0] = e0; aux->eps[1] = e1; aux->eps[2] = e2;
+      aux->T = Teff;
+      aux->dev_norm = sqrtf(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+    }
+    if (aux && aux->guided && aux->dev_target > 0.0f && df > 0.0f) {
+      const double g = (double(aux->dev_norm) / double(aux->dev_target) - 1.0) / double(df);
+      aux->gamma_start = g > 0.0 ? g : 0.0;
+    }
     if (defer) {
       double trial_n = opv - 1.0 + double(P.n0);
       bool inside = trial_n > 0.0 && trial_n < 1.0 && double(q) + pressure_coefficient64(trial_n) * double(pv) <= 1.0;
       if (!inside) return kDingDeferred;
     }
-    ret_n = P.bracketed ? solve_porosity_bracketed(q, pv, df, ms, ex, opv, P.n0) : solve_porosity(q, pv, df, ms, ex, opv, P.n0);
+    bool solved = false;
+    if (aux && aux->guided) {
+      solved = solve_porosity_guided(q, pv, df, ms, ex, opv, P.n0, aux->n_start, aux->gamma_start, ret_n, aux->evals);
+      aux->accepted = solved;
+    }
+    if (!solved) {
+      if (P.bracketed) {
+        ret_n = solve_porosity_bracketed(q, pv, df, ms, ex, opv, P.n0, aux ? &aux->evals : nullptr);
+      } else {
+        ret_n = solve_porosity(q, pv, df, ms, ex, opv, P.n0);
+        if (aux) aux->evals += 64;
+      }
+    }
     double pred = porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
+    if (aux) ++aux->evals;
     cres = float(fabs(double(ret_n) - pred));
     if (cres > 1.0e-6f || ret_n <= 0.0f || ret_n >= 1.0f) o.flag1 = 1;
   }
   mat3<float> Fe, tau;
   float resid;
-  ding_return(trial, U, s, V, mu, lam, yv, ev, porosity_coefficient(ret_n), dt, Fe, tau, resid);
+  float eps_ret[3];
+  ding_return(trial, U, s, V, mu, lam, yv, ev, porosity_coefficient(ret_n), dt, Fe, tau, resid, eps_ret);
+  if (aux && P.coupled) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int o = aux->order[k];
+      aux->eps_ret[k] = o == 0 ? eps_ret[0] : (o == 1 ? eps_ret[1] : eps_ret[2]);
+    }
+  }
   if (P.coupled) {
     double target = 1.0 - double(P.n0) + double(ret_n);
     double vs = m_det(m_cast<double>(F_total)) / (target * m_det(m_cast<double>(Fe)));

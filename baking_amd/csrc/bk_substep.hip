@@ -26,8 +26,11 @@
 //   grid     momentum -> velocity, gravity, unilateral walls
 //   pass C   G2P + Ding return map + gas refresh + saturations (per particle);
 //            particles whose coupled porosity needs the full solve are queued
-//   pass C'  the queued return maps, densely
+//   surrogate (optional) the return-map network on the queued particles' features
+//   pass C'  the queued return maps, densely; with the surrogate, each coupled
+//            porosity solve starts from the network's prediction
 #include "bk_device.h"
+#include "bk_surrogate.h"
 #include "../../include/baking_abi.h"
 
 #include <stddef.h>
@@ -74,6 +77,9 @@ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v
 
 }  // namespace
 
+// The library's one error text (bk_last_error), also set by the other sources.
+void bk_internal_set_error(const char* what) { g_last_error = what; }
+
 struct bk_workspace {
   int max_n = 0, total_blocks = 0, max_doughs = 0, n_vertices = 0, n_slots = 0;
   bk_dough* doughs = nullptr;                   // device copy of the dough table
@@ -97,7 +103,15 @@ struct bk_workspace {
   double* slot_exposed = nullptr;
   double* slot_contact = nullptr;
   int32_t* flags = nullptr;                     // BK_FLAG_COUNT per dough
+  float* features = nullptr;                    // surrogate inputs of the queued particles, kMlpRow per queue slot
+  float* predictions = nullptr;                 // surrogate outputs, kMlpOut per queue slot
+  const uint8_t* mlp = nullptr;                 // surrogate inference blob (caller-owned), or null
+  float* samples = nullptr;                     // emitted (features, exact outputs) rows of kSampleRow floats, or null
+  int32_t sample_cap = 0;
+  unsigned long long* stats = nullptr;          // [0] porosity evaluations [1] guided solves accepted [2] guided solves
+                                                // rejected [3] queued return maps; summed over substeps
 };
+constexpr int kSampleRow = 12;  // 7 features, 4 exact outputs, 1 unused
 
 // Host side of a workspace: a copy of the device descriptor, the uploaded
 // dough table with its totals, and the launch tuning.
@@ -105,7 +119,7 @@ struct HostWs {
   bk_workspace dev;
   std::vector<bk_dough> doughs;
   int n = 0, n_vertices = 0, n_slots = 0, key_bits = 6;
-  bool thermal = false, gas = false;
+  bool thermal = false, gas = false, surrogate = false;
   bk_tuning tuning{4096, 256, 4096};
 };
 static std::mutex g_ws_mutex;
@@ -848,7 +862,7 @@ __global__ __launch_bounds__(256) void k_grid_update(const bk_dough* __restrict_
 // (ding.py:303-327, gas.py:38-45, thermal_water.py:256-260).
 __device__ inline int material_update(const bk_params& P, float visc_param, const bk_scene& sc, const bk_state& in,
                                       const bk_state& out, int p, const mat3<float>& F_new, const mat3<float>& Cn, bool defer,
-                                      bool& f0, bool& f1) {
+                                      bool& f0, bool& f1, DingAux* aux = nullptr) {
   DingParams D;
   D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
   for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
@@ -857,7 +871,7 @@ __device__ inline int material_update(const bk_params& P, float visc_param, cons
   D.bracketed = P.return_solver;
   float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
   DingOut r;
-  int status = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, visc_param, r, defer);
+  int status = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, visc_param, r, defer, aux);
   f0 = r.flag0;
   f1 = r.flag1;
   if (status == kDingDeferred) return status;
@@ -885,21 +899,107 @@ __device__ inline int material_update(const bk_params& P, float visc_param, cons
   return status;
 }
 
+// Surrogate inputs of particle p from its Ding extras (see bk_surrogate.h).
+__device__ __forceinline__ void surrogate_features(const bk_params& P, const bk_state& in, int p, const DingAux& aux, float f[kMlpRow]) {
+  f[0] = aux.eps[0];
+  f[1] = aux.eps[1];
+  f[2] = aux.eps[2];
+  f[3] = aux.T;
+  f[4] = P.transport_enabled ? float(in.s[p]) : 0.0f;
+  f[5] = in.porosity[p];
+  const mat3<float> Fp = m_load(in.Fp + 9 * p);
+  float acc = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const float e = Fp.a[i] - ((i % 4 == 0) ? 1.0f : 0.0f);
+    acc += e * e;
+  }
+  f[6] = sqrtf(acc);
+  f[7] = 0.0f;
+}
+
+// Surrogate network over the queued particles (between the two halves of pass C).
+__global__ __launch_bounds__(256) void k_surrogate(bk_workspace* ws) {
+  __shared__ __align__(16) uint8_t lds[kMlpLdsBytes];
+  mlp_rows(ws->mlp, ws->features, ws->predictions, ws->counters[2], lds);
+}
+
 // Second half of pass C: the deferred coupled-porosity return maps, 64 per
-// wavefront with no idle lanes waiting on a neighbour's solve.
+// wavefront with no idle lanes waiting on a neighbour's solve.  The loop's
+// trip count is uniform per workgroup (sample appends are per workgroup).
 __global__ __launch_bounds__(256) void k_pass_c_deferred(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_state out,
                                                          bk_workspace* ws) {
+  __shared__ int lds[17];
+  __shared__ unsigned long long totals[4];
+  if (threadIdx.x < 4) totals[threadIdx.x] = 0ull;
   const int nq = ws->counters[2];
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nq; i += gridDim.x * blockDim.x) {
-    const int p = ws->queue[i];
-    per_dough(dough_of_particle(sc.dough_of, p), [&](int d) {
-      bool f0 = false, f1 = false;
-      material_update(doughs[d].params, sc.visc_param[d], sc, in, out, p, m_load(out.F + 9 * p), m_load(out.C + 9 * p), false,
-                      f0, f1);
-      flag_max(dough_flags(ws, d), BK_FLAG_POROSITY, f0);
-      flag_max(dough_flags(ws, d), BK_FLAG_COUPLED, f1);
-    });
+  const bool emit_on = ws->samples != nullptr;
+  const bool have_mlp = ws->mlp != nullptr;
+  unsigned evals = 0, accepted = 0, rejected = 0, handled = 0;
+  for (int base = blockIdx.x * blockDim.x; base < nq; base += gridDim.x * blockDim.x) {
+    const int i = base + int(threadIdx.x);
+    bool emit = false;
+    float row[kSampleRow];
+    if (i < nq) {
+      const int p = ws->queue[i];
+      ++handled;
+      // per-lane dough: this kernel is bound by its fp64 solves, not by reading the table
+      const int d = dough_of_particle(sc.dough_of, p);
+      {
+        const bk_params& P = doughs[d].params;
+        bool f0 = false, f1 = false;
+        DingAux aux;
+        if (P.surrogate && have_mlp) {
+          // start from the network's porosity; the overstress start maps the trial
+          // deviator onto the predicted one (|dev tau_ret| = |dev tau_trial| / (1 + df gamma))
+          const float* pr = ws->predictions + size_t(i) * kMlpOut;
+          aux.guided = true;
+          aux.n_start = double(in.porosity[p]) + double(pr[3]);
+          const float m = (pr[0] + pr[1] + pr[2]) / 3.0f;
+          aux.dev_target = 2.0f * P.mu *
+                           sqrtf((pr[0] - m) * (pr[0] - m) + (pr[1] - m) * (pr[1] - m) + (pr[2] - m) * (pr[2] - m));
+        }
+        const int status =
+            material_update(P, sc.visc_param[d], sc, in, out, p, m_load(out.F + 9 * p), m_load(out.C + 9 * p), false, f0, f1, &aux);
+        flag_max(dough_flags(ws, d), BK_FLAG_POROSITY, f0);
+        flag_max(dough_flags(ws, d), BK_FLAG_COUPLED, f1);
+        evals += unsigned(aux.evals);
+        if (aux.guided) {
+          if (aux.accepted) ++accepted; else ++rejected;
+        }
+        if (emit_on && P.gas_enabled && status == kDingDone) {
+          const float* f = ws->features + size_t(i) * kMlpRow;
+#pragma unroll
+          for (int k = 0; k < kMlpIn; ++k) row[k] = f[k];
+          row[7] = aux.eps_ret[0];
+          row[8] = aux.eps_ret[1];
+          row[9] = aux.eps_ret[2];
+          row[10] = out.porosity[p] - in.porosity[p];
+          row[11] = 0.0f;
+          emit = true;
+        }
+      }
+    }
+    if (emit_on) {
+      const int slot = block_reserve(&ws->counters[4], emit ? 1 : 0, lds);
+      if (emit && slot < ws->sample_cap) {
+#pragma unroll
+        for (int k = 0; k < kSampleRow; ++k) ws->samples[size_t(slot) * kSampleRow + k] = row[k];
+      }
+    }
   }
+  // statistics: wavefront sums, then one LDS atomic per wavefront and one global atomic per workgroup
+  unsigned v[4] = {evals, accepted, rejected, handled};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  }
+  __syncthreads();
+  if (__lane_id() == 0)
+    for (int k = 0; k < 4; ++k) atomicAdd(&totals[k], (unsigned long long)v[k]);
+  __syncthreads();
+  if (threadIdx.x < 4 && totals[threadIdx.x]) atomicAdd(&ws->stats[threadIdx.x], totals[threadIdx.x]);
 }
 
 // ============================================================================ pass C: G2P + Ding + gas refresh + saturation
@@ -908,6 +1008,7 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __r
   __shared__ int lds[17];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   bool deferred = false;
+  float feat[kMlpRow];
   if (p < n) {
     per_dough(dough_of_particle(sc.dough_of, p), [&](int d) {
       const bk_params& P = doughs[d].params;
@@ -959,8 +1060,12 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __r
         }
         m_store(out.C + 9 * p, Cn);
         m_store(out.F + 9 * p, F_new);
-        if (P.model == 1)
-          deferred = material_update(P, sc.visc_param[d], sc, in, out, p, F_new, Cn, true, f0, f1) == kDingDeferred;
+        if (P.model == 1) {
+          DingAux aux;
+          deferred = material_update(P, sc.visc_param[d], sc, in, out, p, F_new, Cn, true, f0, f1, P.gas_enabled ? &aux : nullptr) ==
+                     kDingDeferred;
+          if (deferred) surrogate_features(P, in, p, aux, feat);
+        }
       }
       int32_t* fl = dough_flags(ws, d);
       flag_max(fl, BK_FLAG_STENCIL, stencil_bad);
@@ -968,7 +1073,12 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __r
       flag_max(fl, BK_FLAG_COUPLED, f1);
     });
   }
-  block_append(&ws->counters[2], ws->queue, deferred, p, lds);
+  const int slot = block_reserve(&ws->counters[2], deferred ? 1 : 0, lds);
+  if (deferred) {
+    ws->queue[slot] = p;
+#pragma unroll
+    for (int k = 0; k < kMlpRow; ++k) ws->features[size_t(slot) * kMlpRow + k] = feat[k];
+  }
 }
 
 __global__ void k_untile_f32(const float* __restrict__ tiled, float* __restrict__ dense, int res, int comps) {
@@ -1006,9 +1116,9 @@ static HostWs* host_view(bk_workspace* ws) {
 // Optional per-stage timing: HIP events recorded around each stage of the
 // substep on the launch stream (no extra synchronisation while recording).
 enum { kStageClear, kStageKeys, kStageSort, kStageTiles, kStageMesh, kStagePassA, kStageDiffuse, kStagePassB1, kStagePassB2,
-       kStageGrid, kStagePassC, kStagePassCDeferred, kStageCount };
+       kStageGrid, kStagePassC, kStageSurrogate, kStagePassCDeferred, kStageCount };
 static const char* kStageNames[kStageCount] = {"clear", "keys", "sort", "tiles", "mesh", "pass_a", "diffuse", "pass_b1",
-                                               "pass_b2", "grid", "pass_c", "pass_c_deferred"};
+                                               "pass_b2", "grid", "pass_c", "surrogate", "pass_c_deferred"};
 struct Profiler {
   int max_steps = 0, step = 0;
   std::vector<hipEvent_t> ev;  // [step][stage][start/stop]
@@ -1109,6 +1219,11 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     hipLaunchKernelGGL(k_pass_c, dim3(div_up(n, pblock)), dim3(pblock), 0, st, D, *sc, *in, *out, *g, ws, n);
     BK_LAUNCH(pass_c);
   }
+  if (hw->gas && hw->surrogate && h->mlp) {  // 11b. return-map network on the queued particles
+    StageTimer tm(pr, st, kStageSurrogate);
+    hipLaunchKernelGGL(k_surrogate, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, ws);
+    BK_LAUNCH(surrogate);
+  }
   if (hw->gas) {  // 12. deferred coupled-porosity return maps
     StageTimer tm(pr, st, kStagePassCDeferred);
     hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, D, *sc, *in, *out, ws);
@@ -1138,7 +1253,7 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
   if (!table || nd < 1 || nd > h.max_doughs) return fail("bk_workspace_set_doughs: dough count exceeds the workspace");
   if (same_table(hw, table, nd)) return 0;
   long long n = 0, blocks = 0, nv = 0, ns = 0;
-  bool thermal = false, gas = false;
+  bool thermal = false, gas = false, surrogate = false;
   for (int d = 0; d < nd; ++d) {
     const bk_dough& D = table[d];
     if (D.params.res < 4 || D.params.n < 1 || !(D.params.dx > 0.0f))
@@ -1152,6 +1267,7 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
     ns += D.params.n_slots;
     thermal |= D.params.transport_enabled != 0;
     gas |= D.params.model == 1 && D.params.gas_enabled != 0;
+    surrogate |= D.params.surrogate != 0;
   }
   if (n > h.max_n || blocks > h.total_blocks || nv > h.n_vertices || ns > h.n_slots)
     return fail("bk_workspace_set_doughs: batch exceeds the workspace (particles, blocks, vertices or slots)");
@@ -1168,6 +1284,7 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
   hw->n_slots = int(ns);
   hw->thermal = thermal;
   hw->gas = gas;
+  hw->surrogate = surrogate;
   int bits = 6;
   while ((1ll << bits) < blocks * 64) ++bits;
   hw->key_bits = bits;
@@ -1275,6 +1392,9 @@ bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blo
   alloc((void**)&h.slot_exposed, size_t(n_slots) * 8);
   alloc((void**)&h.slot_contact, size_t(n_slots) * 8);
   alloc((void**)&h.flags, size_t(BK_FLAG_COUNT) * max_doughs * 4);
+  alloc((void**)&h.features, n * kMlpRow * 4);
+  alloc((void**)&h.predictions, n * kMlpOut * 4);
+  alloc((void**)&h.stats, 4 * sizeof(unsigned long long));
   if (!ok || hipMemcpy(ws, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
     fail("bk_workspace_create: allocation failed");
     return nullptr;
@@ -1303,7 +1423,7 @@ void bk_workspace_destroy(bk_workspace* ws) {
   if (ws_host(ws, h)) return;
   void* ptrs[] = {h.doughs, h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1],
                   h.hist, h.digit_total, h.blk_start, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
-                  h.queue, h.tau, h.vertices, h.slot_exposed, h.slot_contact, h.flags};
+                  h.queue, h.tau, h.vertices, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions, h.stats};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   (void)hipFree(ws);
@@ -1330,6 +1450,53 @@ int bk_workspace_get_tuning(bk_workspace* ws, bk_tuning* tuning) {
   HostWs* hw = host_view(ws);
   if (!hw || !tuning) return fail("bk_workspace_get_tuning: bad argument");
   *tuning = hw->tuning;
+  return 0;
+}
+
+// Device pointers stored in the device descriptor: written with a blocking copy after the device is idle.
+static int set_device_field(bk_workspace* ws, size_t offset, const void* value, size_t bytes) {
+  BK_HIP(hipDeviceSynchronize());
+  BK_HIP(hipMemcpy((char*)ws + offset, value, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+int bk_workspace_set_surrogate(bk_workspace* ws, const void* blob) {
+  HostWs* hw = host_view(ws);
+  if (!hw) return fail("bk_workspace_set_surrogate: bad workspace");
+  const uint8_t* b = (const uint8_t*)blob;
+  int rc = set_device_field(ws, offsetof(bk_workspace, mlp), &b, sizeof(b));
+  if (rc) return rc;
+  hw->dev.mlp = b;
+  return 0;
+}
+
+int bk_workspace_set_samples(bk_workspace* ws, float* buffer, int32_t capacity) {
+  HostWs* hw = host_view(ws);
+  if (!hw || capacity < 0 || (buffer && capacity < 1)) return fail("bk_workspace_set_samples: bad argument");
+  int rc = set_device_field(ws, offsetof(bk_workspace, samples), &buffer, sizeof(buffer));
+  if (!rc) rc = set_device_field(ws, offsetof(bk_workspace, sample_cap), &capacity, sizeof(capacity));
+  if (rc) return rc;
+  BK_HIP(hipMemset(hw->dev.counters + 4, 0, 4));
+  hw->dev.samples = buffer;
+  hw->dev.sample_cap = capacity;
+  return 0;
+}
+
+int bk_workspace_stats(bk_workspace* ws, int64_t* out, int32_t reset, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !out) return fail("bk_workspace_stats: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long s[4];
+  int32_t samples = 0;
+  BK_HIP(hipMemcpyAsync(s, hw->dev.stats, sizeof(s), hipMemcpyDeviceToHost, st));
+  BK_HIP(hipMemcpyAsync(&samples, hw->dev.counters + 4, 4, hipMemcpyDeviceToHost, st));
+  BK_HIP(hipStreamSynchronize(st));
+  for (int k = 0; k < 4; ++k) out[k] = int64_t(s[k]);
+  out[4] = samples;
+  if (reset) {
+    BK_HIP(hipMemsetAsync(hw->dev.stats, 0, sizeof(s), st));
+    BK_HIP(hipStreamSynchronize(st));
+  }
   return 0;
 }
 

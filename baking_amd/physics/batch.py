@@ -186,6 +186,15 @@ class BatchSimulator:
             setattr(scene, name, keep[name].data_ptr() if name in keep else None)
         self._scene = scene
 
+    def SetSurrogate(self, using):
+        """Per-dough switch of the guided coupled-porosity solve (see baking_amd.surrogate.attach)."""
+        lib = _native.load()
+        for d, on in enumerate(using):
+            self.doughs[d]._surrogate_on = bool(on)
+            self._table[d].params.surrogate = int(bool(on))
+        _native.check(lib.bk_workspace_set_doughs(self._workspace, self._table, self.dough_count), "bk_workspace_set_doughs")
+        self._destroy_graphs()
+
     # ------------------------------------------------------------------ tuning
     def SetTuning(self, tile_chunk=None, particle_block=None, grid_cap=None):
         """Launch shapes of the batch substep (see ``bk_tuning``); captured graphs are dropped."""

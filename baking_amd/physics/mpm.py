@@ -248,6 +248,7 @@ class MpmSimulator:
             P.n_slots = len(self.transport.slots["slot_owner"])
         P.n = self.particle_data.particle_count
         P.return_solver = {"bisection": 0, "bracketed": 1}[self.return_solver]
+        P.surrogate = int(self._surrogate_on)
         return P
 
     @staticmethod
@@ -263,6 +264,13 @@ class MpmSimulator:
     # reference's end-state porosity equation to a few ulps in ~10-20 residual
     # evaluations; "bisection" runs the reference's 64 fixed bisection steps.
     return_solver = "bracketed"
+
+    # Start coupled-porosity solves from the return-map network (baking_amd.surrogate.attach sets it)
+    _surrogate_on = False
+
+    def SetSurrogate(self, on):
+        self._surrogate_on = bool(on)
+        self._params = self._make_params(settling=False)
 
     def SetReturnSolver(self, name):
         if name not in ("bisection", "bracketed"):

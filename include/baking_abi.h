@@ -67,6 +67,9 @@ typedef struct bk_params {
   /* coupled-porosity solve: 0 = the reference's 64-step bisection (porous_return.py:43-61),
    * 1 = a safeguarded Newton solve of the same equation inside the bisection's bracket */
   int32_t return_solver;
+  /* 1: start each coupled-porosity solve from the return-map network's prediction (needs a network set
+   * with bk_workspace_set_surrogate; no reference counterpart), 0: from the old porosity */
+  int32_t surrogate;
 } bk_params;
 
 /* One dough of a batch (no reference counterpart: the reference runs one
@@ -160,6 +163,42 @@ int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* strea
 /* The same for the first n_doughs flag blocks (int32[8 * n_doughs], dough-major). */
 int bk_workspace_read_flags_all(bk_workspace* ws, int32_t* host, int32_t n_doughs, void* stream);
 int bk_workspace_write_flags_all(bk_workspace* ws, const int32_t* host, int32_t n_doughs, void* stream);
+
+/* ---------------------------------------------------------------- return-map surrogate (no reference counterpart)
+ * A 7 -> 4 x 64 SiLU -> 4 network standing in for the start of the coupled (gas)
+ * Ding return map (physics/materials/ding.py:168-221 with porous_return.py:43-61).
+ * Rows: 7 input floats + 1 unused per particle (principal trial log-strains in
+ * descending order, temperature of the yield/viscosity factors, saturation,
+ * porosity before the substep, |Fp - I|_F).  Outputs: 4 floats (returned
+ * log-strains in the same order, porosity increment).  Parameters: FP32 vector
+ * of bk_mlp_theta_count() floats, W1[64][8] b1 W2[64][64] b2 W3 b3 W4 b4 W5[4][64]
+ * b5[4].  The inference blob (bk_mlp_blob_bytes() bytes: normalisation, biases,
+ * BF16 weights in matrix-core operand order) is built by the host; its layout
+ * is given in baking_amd/csrc/bk_surrogate.h. */
+int32_t bk_mlp_theta_count(void);
+int32_t bk_mlp_blob_bytes(void);
+int32_t bk_mlp_grad_chunk(void);
+/* BF16 matrix-core inference of n rows (rows: n x 8 floats, out: n x 4 floats). */
+int bk_mlp_infer(const void* blob, const float* rows, float* out, int32_t n, void* stream);
+/* FP32 forward of normalised rows (n x 8) to normalised outputs (n x 4); z: scratch of 4 * n * 64 floats. */
+int bk_mlp_forward(const float* theta, const float* rows, float* out, int32_t n, float* z, void* stream);
+/* Gradient of the mean squared error between the FP32 forward of normalised rows and normalised targets
+ * (n x 4) with respect to theta; loss += the error (double on device).  Scratch: z and d of 4 * n * 64
+ * floats, y and d_out of n * 4, partial of ceil(n / bk_mlp_grad_chunk()) * bk_mlp_theta_count(). */
+int bk_mlp_gradient(const float* theta, const float* rows, const float* targets, int32_t n, float* z, float* y, float* d,
+                    float* d_out, float* partial, float* grad, double* loss, void* stream);
+/* One Adam update (bias-corrected, step >= 1) of count parameters. */
+int bk_adam_step(float* theta, const float* grad, float* m, float* v, int32_t count, float lr, float beta1, float beta2, float eps,
+                 int32_t step, void* stream);
+/* Use the network (blob in device memory, kept alive by the caller; NULL: none) in the substeps of
+ * doughs with params.surrogate = 1.  Synchronises the device. */
+int bk_workspace_set_surrogate(bk_workspace* ws, const void* blob);
+/* Emit one row of 12 floats (7 inputs, the 4 exact outputs, 1 unused) per coupled return map of the
+ * following substeps into buffer, up to capacity rows (NULL: stop); resets the emitted count. */
+int bk_workspace_set_samples(bk_workspace* ws, float* buffer, int32_t capacity);
+/* Counters summed over the substeps since the last reset: {porosity-equation evaluations, guided solves
+ * accepted, guided solves handed to the exact solve, queued return maps, rows emitted}; synchronises. */
+int bk_workspace_stats(bk_workspace* ws, int64_t* out5, int32_t reset, void* stream);
 
 /* ---------------------------------------------------------------- sort (build-only; no reference counterpart)
  * Cell key of every particle: flatten_node(stencil_base((x-o)/dx)) of

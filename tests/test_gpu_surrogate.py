@@ -1,0 +1,152 @@
+"""GPU checks of the return-map surrogate: training kernels, matrix-core inference, and the guided return map.
+
+* ``bk_mlp_gradient`` (FP32 forward/backward/weight-gradient kernels) against
+  the FP64 oracle gradient; ``bk_mlp_forward`` against the oracle forward;
+* ``bk_mlp_infer`` (BF16 MFMA, FP32 accumulation) against the oracle's BF16
+  restatement, for row counts that are not multiples of a wavefront;
+* training lowers the held-out error;
+* a yielding gas dough run with a network trained on its own emitted rows
+  gives the exact run's fields (the guided solve converges to the same root)
+  with fewer evaluations of the porosity equation.
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp as M
+from tests.harness import field_errors, field_scales, gpu_fields, oracle_fields, oracle_from_spec
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("needs a ROCm GPU", allow_module_level=True)
+
+from baking_amd import _native, scenes, surrogate  # noqa: E402
+
+
+def _theta(seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    th = rng.normal(0.0, scale, M.COUNT).astype(np.float32)
+    M.unpack(th)["W1"][:, 7] = 0.0
+    return th
+
+
+def test_training_gradient_matches_oracle():
+    rng = np.random.default_rng(11)
+    n = 3000
+    th = _theta(12)
+    x = rng.normal(size=(n, 8)).astype(np.float32)
+    x[:, 7] = 0.0
+    y = rng.normal(size=(n, 4)).astype(np.float32)
+    lib = _native.load()
+    dev = "cuda"
+    T = lambda a: torch.as_tensor(a, device=dev).contiguous()  # noqa: E731
+    thd, xd, yd = T(th), T(x), T(y)
+    chunk = lib.bk_mlp_grad_chunk()
+    z, d = torch.empty(4 * n * 64, device=dev), torch.empty(4 * n * 64, device=dev)
+    yo, do = torch.empty(n * 4, device=dev), torch.empty(n * 4, device=dev)
+    partial = torch.empty(((n + chunk - 1) // chunk) * M.COUNT, device=dev)
+    grad = torch.empty(M.COUNT, device=dev)
+    loss = torch.zeros(1, dtype=torch.float64, device=dev)
+    P = _native.ptr
+    _native.check(lib.bk_mlp_gradient(P(thd), P(xd), P(yd), n, P(z), P(yo), P(d), P(do), P(partial), P(grad), P(loss),
+                                      _native.stream_ptr()), "bk_mlp_gradient")
+    g_ref = M.gradient(th.astype(np.float64), x, y)
+    g = grad.cpu().numpy().astype(np.float64)
+    err = np.abs(g - g_ref).max() / np.abs(g_ref).max()
+    print("gradient max error / max |g|:", err)
+    assert err < 1e-4
+    assert abs(loss.item() - M.loss(th.astype(np.float64), x, y)) <= 1e-5 * M.loss(th.astype(np.float64), x, y)
+    out_ref, _ = M.forward(th.astype(np.float64), x)
+    np.testing.assert_allclose(yo.cpu().numpy().reshape(n, 4), out_ref, rtol=1e-4, atol=1e-5)
+
+
+@pytest.mark.parametrize("n", [0, 5, 64, 1000, 70001])
+def test_matrix_core_inference_matches_bf16_oracle(n):
+    rng = np.random.default_rng(n + 1)
+    th = _theta(21, 0.25)
+    model = surrogate.ReturnMapSurrogate(th, rng.normal(size=8).astype(np.float32), rng.uniform(0.5, 2, 8).astype(np.float32),
+                                         rng.normal(size=4).astype(np.float32), rng.uniform(0.5, 2, 4).astype(np.float32))
+    rows = rng.normal(size=(n, 8)).astype(np.float32) * 2
+    rows[:, 7] = 0.0
+    got = model.predict(torch.as_tensor(rows, device="cuda")).cpu().numpy() if n else np.zeros((0, 4), np.float32)
+    if n == 0:
+        return
+    want = M.infer_bf16(th, rows, model.in_shift, model.in_scale, model.out_shift, model.out_scale)
+    norm = lambda v: (v - model.out_shift) / model.out_scale  # noqa: E731
+    diff = np.abs(norm(got) - norm(want))
+    print(f"n={n}: max |gpu - bf16 oracle| (normalised) {diff.max():.2e}, mean {diff.mean():.2e}")
+    assert diff.max() <= 1e-2 * (1.0 + np.abs(norm(want)).max())
+    assert diff.mean() <= 1e-3
+    # and against the unquantised FP64 network: the BF16 error itself
+    x = (rows[:, :7] - model.in_shift[:7]) * model.in_scale[:7]
+    exact, _ = M.forward(th.astype(np.float64), np.pad(x, ((0, 0), (0, 1))))
+    assert np.abs(norm(got) - exact).mean() <= 5e-2 * (1.0 + np.abs(exact).mean())
+
+
+def test_training_lowers_held_out_error():
+    """Fit smooth functions of the inputs; the held-out error must fall far below the untrained network's."""
+    rng = np.random.default_rng(31)
+    m = 60000
+    x = rng.normal(size=(m, 7)).astype(np.float32)
+    y = np.stack([np.sin(x[:, 0]) + 0.5 * x[:, 1], x[:, 1] * x[:, 2], np.tanh(x[:, 3] + x[:, 4]),
+                  0.5 * x[:, 5] ** 2 - 0.2 * x[:, 6]], axis=1).astype(np.float32)
+    rows = np.zeros((m, 12), np.float32)
+    rows[:, :7] = x
+    rows[:, 7:11] = y
+    samples = torch.as_tensor(rows, device="cuda")
+    untrained = surrogate.ReturnMapSurrogate(surrogate.init_theta(1), np.zeros(8, np.float32), np.ones(8, np.float32),
+                                             np.zeros(4, np.float32), np.ones(4, np.float32))
+    before = untrained.evaluate(samples[:5000])
+    model, report = surrogate.ReturnMapSurrogate.fit(samples, epochs=10, batch=1024, lr=3e-3, seed=1, holdout=0.1, log=print)
+    for k in ("eps_ret_1", "eps_ret_2", "eps_ret_3", "porosity_increment"):
+        print(k, before[k]["mean_abs"], "->", report[k]["mean_abs"])
+        assert report[k]["mean_abs"] < 0.1 * before[k]["mean_abs"]
+    assert model.meta["train_mse"][-1] < 0.1 * model.meta["train_mse"][0]
+
+
+def _yielding_gas_dough():
+    return scenes.strain_rate_field(scenes.demo_gas(subdivisions=2), np.diag([1.5, -0.75, 0.2]))
+
+
+def test_guided_return_map_matches_exact_with_fewer_evaluations():
+    spec = _yielding_gas_dough()
+    steps = 40
+    exact = scenes.build_simulator(spec)
+    rows = surrogate.collect_samples(exact, steps, capacity=200000)
+    print("emitted rows:", rows.shape[0], flush=True)
+    assert rows.shape[0] > 2000
+    model, report = surrogate.ReturnMapSurrogate.fit(rows, epochs=30, batch=2048, lr=3e-3, seed=0, holdout=0.1)
+    print("held-out:", {k: v["mean_abs"] for k, v in report.items() if isinstance(v, dict)})
+
+    runs = {}
+    for name in ("exact", "guided"):
+        sim = scenes.build_simulator(spec)
+        if name == "guided":
+            assert surrogate.attach(sim, model, strict=False) == 1
+        surrogate.stats(sim, reset=True)
+        sim.Advance(steps)
+        runs[name] = (gpu_fields(sim), surrogate.stats(sim))
+    st_e, st_g = runs["exact"][1], runs["guided"][1]
+    print("exact stats", st_e, "guided stats", st_g)
+    assert st_g["return_maps"] == st_e["return_maps"] > 0
+    assert st_g["accepted"] >= 0.8 * st_g["return_maps"]
+    per_exact = st_e["evaluations"] / st_e["return_maps"]
+    per_guided = st_g["evaluations"] / st_g["return_maps"]
+    print(f"porosity-equation evaluations per return map: exact {per_exact:.2f}, guided {per_guided:.2f}")
+    assert per_guided < per_exact
+
+    o32, o64 = oracle_from_spec(spec, "f32"), oracle_from_spec(spec, "f64")
+    for _ in range(steps):
+        o32.Step()
+        o64.Step()
+    r32, r64 = oracle_fields(o32), oracle_fields(o64)
+    floors = field_scales(r32, o32.dx)
+    e_ref = field_errors(r32, r64, floors=floors)
+    for name in ("exact", "guided"):
+        e = field_errors(runs[name][0], r32, floors=floors)
+        bad = {k: (e[k], e_ref[k]) for k in e if e[k] > max(1e-5, 4.0 * e_ref[k])}
+        assert not bad, f"{name} run outside tolerance against the oracle: {bad}"
+    e = field_errors(runs["guided"][0], runs["exact"][0], floors=floors)
+    bad = {k: (e[k], e_ref[k]) for k in e if e[k] > max(1e-5, 4.0 * e_ref[k])}
+    assert not bad, f"guided run differs from the exact run: {bad}"
