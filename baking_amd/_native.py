@@ -102,6 +102,7 @@ SIGNATURES = {
     "bk_profile_end": ([P, P, P], c_int),
     "bk_profile_stage_count": ([], c_int32),
     "bk_profile_stage_name": ([I], c_char_p),
+    "bk_measure": ([POINTER(BkScene), POINTER(BkState), I, I, P, P], c_int),
     "bk_grid_untile": ([P, P, I, I, P], c_int),
     "bk_grid_untile_f64": ([P, P, I, P], c_int),
     "bk_ParticleToGrid": ([P, P, P, P, P, P, P, P, P, F, I, F, P, I, P], c_int),

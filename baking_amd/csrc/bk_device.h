@@ -103,12 +103,19 @@ __device__ __forceinline__ void grid_position(const float* x, const float o[3], 
 #pragma unroll
   for (int a = 0; a < 3; ++a) g[a] = __fdiv_rn(__fsub_rn(x[a], o[a]), dx);
 }
+// base = int(floor(g - 1/2)).  A NaN or a value outside [-1, 2^30] (a particle
+// far off the grid, e.g. after a blow-up that a flag reports) becomes -1, so
+// it is never taken for a valid base and no index is formed from it.
 __device__ __forceinline__ void stencil_base(const float g[3], int b[3]) {
 #pragma unroll
-  for (int a = 0; a < 3; ++a) b[a] = int(floorf(__fsub_rn(g[a], 0.5f)));
+  for (int a = 0; a < 3; ++a) {
+    const float f = floorf(__fsub_rn(g[a], 0.5f));
+    b[a] = (f >= -1.0f && f <= 1073741824.0f) ? int(f) : -1;
+  }
 }
+// base >= 0 and base + 2 < res, written without the addition so no base can overflow it.
 __device__ __forceinline__ bool valid_stencil(const int b[3], int res) {
-  return b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[0] + 2 < res && b[1] + 2 < res && b[2] + 2 < res;
+  return b[0] >= 0 && b[1] >= 0 && b[2] >= 0 && b[0] < res - 2 && b[1] < res - 2 && b[2] < res - 2;
 }
 __device__ __forceinline__ int64_t flatten_node(int i, int j, int k, int res) { return (int64_t(i) * res + j) * res + k; }
 

@@ -1081,6 +1081,50 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __r
   }
 }
 
+// ============================================================================ acceptance aggregates
+// Per-dough f64 sums (experiments/recording.py:46-112, physics/mpm.py:292-308):
+// [0] sum V0 det F, [1] sum capacity * T, [2] sum capacity, [3] sum W,
+// [4] sum water losses, [5] sum heat exchanges, [6] sum V0 * gas moles,
+// [7] sum particle mass.  A wavefront inside one dough reduces by shuffles and
+// adds once per sum; one straddling two doughs adds per lane.
+constexpr int kMeasureSums = 8;
+__global__ __launch_bounds__(256) void k_measure(bk_scene sc, bk_state s, int n, double* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[kMeasureSums];
+#pragma unroll
+  for (int k = 0; k < kMeasureSums; ++k) v[k] = 0.0;
+  int d = 0;
+  if (p < n) {
+    d = dough_of_particle(sc.dough_of, p);
+    const double V0 = sc.V0[p];
+    v[0] = V0 * m_det(m_cast<double>(m_load(s.F + 9 * p)));
+    if (s.T && sc.capacities) {
+      const double c = sc.capacities[p];
+      v[1] = c * s.T[p];
+      v[2] = c;
+    }
+    if (s.W) v[3] = s.W[p];
+    if (s.losses) v[4] = s.losses[p];
+    if (s.exchanges) v[5] = s.exchanges[p];
+    if (s.moles) v[6] = V0 * s.moles[p];
+    v[7] = s.masses ? double(s.masses[p]) : double(sc.m0[p]);
+  }
+  const int u = __builtin_amdgcn_readfirstlane(d);
+  if (__ballot(p < n && d != u) == 0ull) {
+#pragma unroll
+    for (int k = 0; k < kMeasureSums; ++k) {
+      double x = v[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+      if (__lane_id() == 0 && x != 0.0) unsafeAtomicAdd(out + u * kMeasureSums + k, x);
+    }
+  } else if (p < n) {
+#pragma unroll
+    for (int k = 0; k < kMeasureSums; ++k)
+      if (v[k] != 0.0) unsafeAtomicAdd(out + d * kMeasureSums + k, v[k]);
+  }
+}
+
 __global__ void k_untile_f32(const float* __restrict__ tiled, float* __restrict__ dense, int res, int comps) {
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   int64_t total = int64_t(res) * res * res;
@@ -1121,7 +1165,8 @@ static const char* kStageNames[kStageCount] = {"clear", "keys", "sort", "tiles",
                                                "pass_b2", "grid", "pass_c", "surrogate", "pass_c_deferred"};
 struct Profiler {
   int max_steps = 0, step = 0;
-  std::vector<hipEvent_t> ev;  // [step][stage][start/stop]
+  std::vector<hipEvent_t> ev;        // [step][stage][start/stop]
+  std::vector<uint8_t> recorded;     // [step][stage]: the stage ran (stages a substep skips have no events)
 };
 static std::unordered_map<const bk_workspace*, Profiler> g_prof;
 
@@ -1130,7 +1175,10 @@ struct StageTimer {
   hipStream_t st;
   int stage;
   StageTimer(Profiler* p, hipStream_t s, int k) : pr(p), st(s), stage(k) {
-    if (pr) (void)hipEventRecord(pr->ev[(size_t(pr->step) * kStageCount + stage) * 2 + 0], st);
+    if (pr) {
+      (void)hipEventRecord(pr->ev[(size_t(pr->step) * kStageCount + stage) * 2 + 0], st);
+      pr->recorded[size_t(pr->step) * kStageCount + stage] = 1;
+    }
   }
   ~StageTimer() {
     if (pr) (void)hipEventRecord(pr->ev[(size_t(pr->step) * kStageCount + stage) * 2 + 1], st);
@@ -1541,6 +1589,7 @@ int bk_profile_begin(bk_workspace* ws, int32_t max_steps) {
   Profiler& pr = g_prof[ws];
   for (hipEvent_t e : pr.ev) (void)hipEventDestroy(e);
   pr.ev.assign(size_t(max_steps) * kStageCount * 2, nullptr);
+  pr.recorded.assign(size_t(max_steps) * kStageCount, 0);
   for (auto& e : pr.ev) BK_HIP(hipEventCreate(&e));
   pr.max_steps = max_steps;
   pr.step = 0;
@@ -1558,10 +1607,12 @@ int bk_profile_end(bk_workspace* ws, float* ms_per_stage, int32_t* steps_recorde
   for (int k = 0; k < kStageCount; ++k) ms_per_stage[k] = 0.0f;
   for (int s = 0; s < pr->step; ++s)
     for (int k = 0; k < kStageCount; ++k) {
+      if (!pr->recorded[size_t(s) * kStageCount + k]) continue;
       hipEvent_t a = pr->ev[(size_t(s) * kStageCount + k) * 2], b = pr->ev[(size_t(s) * kStageCount + k) * 2 + 1];
       BK_HIP(hipEventSynchronize(b));
       float ms = 0.0f;
-      if (hipEventElapsedTime(&ms, a, b) == hipSuccess) ms_per_stage[k] += ms;
+      BK_HIP(hipEventElapsedTime(&ms, a, b));
+      ms_per_stage[k] += ms;
     }
   *steps_recorded = pr->step;
   for (hipEvent_t e : pr->ev) (void)hipEventDestroy(e);
@@ -1639,6 +1690,16 @@ void bk_graph_destroy(bk_graph* gr) {
   if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
   if (gr->graph) (void)hipGraphDestroy(gr->graph);
   delete gr;
+}
+
+int bk_measure(const bk_scene* scene, const bk_state* state, int32_t n, int32_t n_doughs, double* out, void* stream) {
+  if (!scene || !state || !out || n < 0 || n_doughs < 1 || !scene->V0 || !state->F) return fail("bk_measure: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  BK_HIP(hipMemsetAsync(out, 0, sizeof(double) * kMeasureSums * size_t(n_doughs), st));
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_measure, dim3(div_up(n, 256)), dim3(256), 0, st, *scene, *state, n, out);
+  BK_LAUNCH(measure);
+  return 0;
 }
 
 int bk_grid_untile(const float* tiled, float* dense, int32_t res, int32_t comps, void* stream) {

@@ -64,15 +64,14 @@ constexpr int kBlobBytes = kBlobW5 + 4 * kBlobTile;        // 29856
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// Round-to-nearest-even FP32 -> BF16 bits (NaN kept quiet).
-__device__ __forceinline__ short to_bf16(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return short((u >> 16) | 0x40);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return short(u >> 16);
-}
+// FP32 -> BF16 bits, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950).
+__device__ __forceinline__ short to_bf16(float x) { return __builtin_bit_cast(short, (__bf16)x); }
 
+// SiLU as used by training (IEEE exp and divide).
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + expf(-z)); }
+
+// SiLU for inference: hardware exp2 and reciprocal (a few ulps), its result is rounded to BF16 next.
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 // Evaluate the network for the 64 particles of one wavefront.  `xin` is this
 // wavefront's LDS staging area of 64 rows x 8 floats holding the normalised
@@ -117,7 +116,7 @@ __device__ inline void mlp_wave(const uint8_t* blob, float* xin) {
       for (int pt = 0; pt < 4; ++pt) {
         bf16x4 b;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) b[i] = to_bf16(silu(acc[mu][pt][i]));
+        for (int i = 0; i < 4; ++i) b[i] = to_bf16(silu_fast(acc[mu][pt][i]));
         h[mu][pt] = b;
       }
     const bf16x4* w = reinterpret_cast<const bf16x4*>(blob + kBlobW2 + layer * 16 * kBlobTile);
@@ -144,7 +143,7 @@ __device__ inline void mlp_wave(const uint8_t* blob, float* xin) {
     for (int pt = 0; pt < 4; ++pt) {
       bf16x4 b;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] = to_bf16(silu(acc[mu][pt][i]));
+      for (int i = 0; i < 4; ++i) b[i] = to_bf16(silu_fast(acc[mu][pt][i]));
       h[mu][pt] = b;
     }
   floatx4 out[4];

@@ -225,6 +225,18 @@ class BatchSimulator:
             lib.bk_graph_destroy(handle)
         self._graphs = {}
 
+    def _ensure_graphs(self):
+        """Capture both ping-pong directions now (capture launches nothing)."""
+        lib = _native.load()
+        for cur in (0, 1):
+            if cur not in self._graphs:
+                s_in, s_out = self._state_struct(cur), self._state_struct(1 - cur)
+                handle = lib.bk_batch_graph_create(self._workspace, ctypes.byref(self._scene), ctypes.byref(s_in),
+                                                   ctypes.byref(s_out), ctypes.byref(self._grid))
+                if not handle:
+                    _native.check(-2, "bk_batch_graph_create")
+                self._graphs[cur] = ctypes.c_void_p(handle)
+
     def _launch(self, stream=None, use_graph=None):
         lib = _native.load()
         use_graph = self.use_graphs if use_graph is None else use_graph

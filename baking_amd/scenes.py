@@ -8,7 +8,7 @@ Workloads (all SI units):
   moisture 0.4 + 0.05 * Gaussian random field (correlation length 0.02, seed 1), porosity U[0.03, 0.07],
   oven 473.15 K, dt = 5e-5.
 * ``batched_doughs`` -- 48 independent spheres, boxes and ellipsoids (half-extent 0.05-0.15, seed 2) on their own
-  64^3 or 128^3 grids, 3k-60k particles each (~1.0M total), physics as ``loaf_256``, dt = 5e-5.
+  64^3 or 128^3 grids, 3k-60k particles each (~1.0M total), material as ``loaf_256``, dt = 1.25e-5.
 
 Parameters the workload text leaves open are taken from the reference's
 example scene (``examples/oven_baking.txt``) and recorded in ``SceneSpec``.
@@ -231,14 +231,18 @@ def dough_shapes(count=48, seed=2, half_extent=(0.05, 0.15), max_particles=60000
     return out
 
 
-def dough_scene(shape, axes, res, seed, name="dough", p0=1000.0, porosity_range=(0.03, 0.07)):
+def dough_scene(shape, axes, res, seed, name="dough", p0=1000.0, porosity_range=(0.03, 0.07), dt=1.25e-5):
     """One dough resting on the tray of a unit-cube oven: sphere, box or ellipsoid with semi-axes ``axes``.
 
-    Physics as ``loaf_256`` (Poisson's ratio 0.45, pore gas at ambient p0 =
-    1000 Pa, oven 473.15 K, dt = 5e-5), moisture at the configured initial
-    saturation, porosity U[porosity_range] per particle; particles jittered 8
-    per cell from ``default_rng(seed)``, porosities from the same generator's
-    next draws.
+    Material as ``loaf_256`` (Poisson's ratio 0.45, pore gas at ambient p0 =
+    1000 Pa, oven 473.15 K), moisture at the configured initial saturation,
+    porosity U[porosity_range] per particle; particles jittered 8 per cell from
+    ``default_rng(seed)``, porosities from the same generator's next draws.
+    dt = 1.25e-5: a rounded dough on the tray compacts at its contact patch
+    (porosity down to ~1e-3 after 0.02 s), which stiffens its pore gas until
+    the gas-acoustic CFL check (gas.py:34-35) fails -- from t = 0.02 s at
+    dt = 5e-5 and within 1000 substeps at 2.5e-5; 1000 substeps at 1.25e-5
+    (12.5 ms of baking) stay clear of it.
     """
     dx = 1.0 / res
     axes = np.asarray(axes, dtype=np.float64)
@@ -268,23 +272,23 @@ def dough_scene(shape, axes, res, seed, name="dough", p0=1000.0, porosity_range=
     samples = make_samples(mesh, pos, dx, material.density)
     porosity = rng.uniform(*porosity_range, size=len(pos))
     T0 = 298.15
-    sim = SimulationConfig(grid_resolution=res, grid_spacing=dx, grid_origin=(0.0, 0.0, 0.0), time_step=5.0e-5,
+    sim = SimulationConfig(grid_resolution=res, grid_spacing=dx, grid_origin=(0.0, 0.0, 0.0), time_step=dt,
                            gravity=(0.0, -9.8, 0.0), boundary_cells=3, device_name="auto")
     return SceneSpec(name, mesh, samples, sim, material, _oven_ding(0.05, p0), _oven_transport(T0, 473.15), steps=1000,
                      porosities=porosity.astype(np.float32), gas_moles=p0 * porosity / (gas_constant * T0),
                      notes=dict(shape=shape, axes=axes.tolist(), res=res, seed=seed, p0=p0))
 
 
-def batched_doughs(count=48, seed=2, half_extent=(0.05, 0.15), max_particles=60000):
+def batched_doughs(count=48, seed=2, half_extent=(0.05, 0.15), max_particles=60000, dt=1.25e-5, p0=1000.0):
     """The batched workload: ``count`` independent doughs of uneven size (1k-60k particles each).
 
     Shapes and grids from ``dough_shapes(count, seed)``; dough i is seeded
     with ``default_rng([seed, i])``.  With the defaults: 48 doughs, ~1.0M
-    particles, 64^3 and 128^3 grids, 1000 substeps.
+    particles, 64^3 and 128^3 grids, dt = 1.25e-5, 1000 substeps.
     """
     specs = []
     for i, d in enumerate(dough_shapes(count, seed, half_extent, max_particles)):
-        specs.append(dough_scene(d["shape"], d["axes"], d["res"], seed=[seed, i], name=f"dough{i:02d}_{d['shape']}"))
+        specs.append(dough_scene(d["shape"], d["axes"], d["res"], seed=[seed, i], name=f"dough{i:02d}_{d['shape']}", dt=dt, p0=p0))
     return specs
 
 

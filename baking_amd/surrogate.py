@@ -302,23 +302,25 @@ def _simulators(sim):
     return getattr(sim, "doughs", None) or [sim]
 
 
-def attach(sim, model, strict=True):
+def attach(sim, model, strict=True, same_material=False):
     """Start the coupled solves of ``sim`` (MpmSimulator or BatchSimulator) from ``model``'s predictions.
 
-    Only doughs whose material parameters match the ones the model was trained
-    on use it (``strict``: raise if any does not).  Returns the number of doughs
-    using it.
+    Every dough with a coupled (gas) Ding return map uses it (``strict``: raise
+    if a dough has none).  The network only supplies the starting point of
+    each solve, whose accepted roots are the exact ones, so a dough whose
+    material parameters or time step differ from the training run's may use
+    it too (it may just need more Newton steps); ``same_material`` restricts it
+    to doughs whose parameters match.  Returns the number of doughs using it.
     """
-    import torch  # noqa: F401
     lib = _native.load()
     trained = model.meta.get("signature")
     using = []
     for s in _simulators(sim):
         ok = s.current_state.material_state is not None and s.material.ding_config.gas_enabled
-        if ok and trained is not None:
-            ok = material_signature(s) == trained
         if strict and not ok:
-            raise ValueError("the surrogate was trained for other material parameters (or the dough has no gas return map)")
+            raise ValueError("the dough has no coupled (gas) return map to start from the surrogate")
+        if ok and same_material and trained is not None:
+            ok = material_signature(s) == trained
         using.append(ok)
     sim._surrogate_blob = model.blob_tensor("cuda")
     _native.check(lib.bk_workspace_set_surrogate(sim._workspace, _native.ptr(sim._surrogate_blob)), "bk_workspace_set_surrogate")

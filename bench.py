@@ -57,14 +57,18 @@ STAGE_BYTES = {
 # pressure, saturation.
 DEFERRED_BYTES = 36 + 4 + 4 + 36 + 36 + 4 + 8 + 8 + 4 + 108 + 8 + 4 + 4 + 4 + 4 + 4 + 8 + 8
 SUBSTEP_BYTES = (552, 208)  # whole substep: particle state read + written once, grid work per active node
+BF16_PEAK_TFLOPS = 2500.0    # MI355X dense BF16 matrix peak, published
+MLP_MACS = 7 * 64 + 3 * 64 * 64 + 64 * 4  # multiply-accumulates of the surrogate network per row
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=None, help="timed substeps (default: 1000 batched_48, 2000 loaf_256, 1000 sphere_small)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="loaf_256", choices=["loaf_256", "sphere_small", "batched_48"])
+    ap.add_argument("--workload", default="batched_48", choices=["batched_48", "loaf_256", "sphere_small"])
+    ap.add_argument("--no-surrogate", action="store_true", help="exact return map only")
+    ap.add_argument("--model", default=None, help="surrogate model file (default: the packaged one)")
     ap.add_argument("--plain-steps", type=int, default=20, help="batched_48: substeps per dough of the plain comparator")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--profile-steps", type=int, default=20)
@@ -151,13 +155,40 @@ def roofline(stages, n, active_nodes, deferred, substep_ms, workload):
     return roof, dominant, dom_bytes, dom_ms
 
 
-def run_batched(args):
-    """The batched workload: plain one-at-a-time comparator, then the batch (autotuned, graph replay)."""
+def _timed(batch, steps, stream):
+    """Milliseconds of `steps` graph replays of the batch substep (HIP events on the launch stream)."""
     import torch
-    from baking_amd import scenes
+    batch._ensure_graphs()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(steps):
+        batch._launch()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop)
+
+
+def _relative_errors(ref, got, start):
+    """Max over doughs of |got - ref| / |ref| for the final aggregates, and of the same for their increments."""
+    out = {}
+    for key in ("particle_volume_m3", "temperature_mean_k", "water_mass_kg", "gas_total_mol"):
+        r, g, s0 = ref[key], got[key], start[key]
+        out[key] = float(np.max(np.abs(g - r) / np.maximum(np.abs(r), 1e-300)))
+        inc = np.abs(r - s0)
+        out[key + "_increment"] = float(np.max(np.abs(g - r) / np.maximum(inc, 1e-300))) if np.any(inc > 0) else 0.0
+    return out
+
+
+def run_batched(args):
+    """The batched workload: plain one-at-a-time comparator, then exact-batched and surrogate+batched from one state."""
+    import torch
+    from baking_amd import scenes, surrogate
     from baking_amd.autotune import autotune
+    from baking_amd.measure import measure
     from baking_amd.physics.batch import BatchSimulator
 
+    steps = args.steps or 1000
     t_build = time.perf_counter()
     specs = scenes.batched_doughs()
     sims = [scenes.build_simulator(s) for s in specs]
@@ -166,7 +197,7 @@ def run_batched(args):
     n = int(sum(counts))
     stream = torch.cuda.current_stream()
 
-    # plain comparator: each dough alone, single-dough substep launched kernel by kernel
+    # plain comparator: each dough alone, single-dough substep launched kernel by kernel, exact return map
     plain_ms = 0.0
     for s in sims:
         s.use_graphs = False
@@ -184,39 +215,58 @@ def run_batched(args):
     plain_rate = n * args.plain_steps / (plain_ms * 1e-3)
 
     batch = BatchSimulator(sims)
-    tuned = None if args.no_autotune else autotune(batch, log=None)
+    model = None
+    if not args.no_surrogate:
+        model = surrogate.ReturnMapSurrogate.load(args.model or surrogate.DEFAULT_MODEL)
+        surrogate.attach(batch, model)
+    tuned = None if args.no_autotune else autotune(batch)
+    if model is not None:
+        surrogate.detach(batch)
     for _ in range(args.warmup):
         batch._launch()
     torch.cuda.synchronize()
-    flags_before = batch.ReadFlags()
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        batch._launch()
-    stop.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(stop)
+    snapshot = batch._save_for_tuning()
+    start_aggr = measure(batch)
+
+    # exact return map, batched
+    surrogate.stats(batch, reset=True)
+    exact_ms = _timed(batch, steps, stream)
+    exact_aggr, exact_stats = measure(batch), surrogate.stats(batch, reset=True)
+    flags_exact = batch.ReadFlags()
+    result_method = "batched (exact return map)"
+    value_ms, stats_used, aggr_used = exact_ms, exact_stats, exact_aggr
+    accuracy = None
+    if model is not None:
+        batch._restore_after_tuning(snapshot)
+        surrogate.attach(batch, model)
+        value_ms = _timed(batch, steps, stream)
+        aggr_used, stats_used = measure(batch), surrogate.stats(batch, reset=True)
+        accuracy = _relative_errors(exact_aggr, aggr_used, start_aggr)
+        result_method = "surrogate+batched"
     flags = batch.ReadFlags()
-    batch.step_count += args.warmup + args.steps
-    batch._sync_doughs(args.warmup + args.steps)
+    del snapshot
 
     stages, wcounts = stage_profile(lib_launch=lambda: batch._launch(use_graph=False), workspace=batch._workspace,
                                     steps=args.profile_steps)
     active_nodes = int((batch.grid["m"] > 0).sum().item())
-    substep_ms = gpu_ms / args.steps
+    substep_ms = value_ms / steps
     roof, _, _, _ = roofline(stages, n, active_nodes, wcounts[2], substep_ms, "batched_48")
+    if "surrogate" in stages and stages["surrogate"] > 0 and wcounts[2] > 0:
+        flops = 2.0 * MLP_MACS * wcounts[2]
+        tf = flops / (stages["surrogate"] * 1e-3) / 1e12
+        roof["surrogate_mfma"] = {"bound": "mfma", "achieved": tf, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                  "frac": tf / BF16_PEAK_TFLOPS, "rows": wcounts[2], "avg_ms": stages["surrogate"]}
     residuals = batch._states[batch._cur].get("residual")
     yield_fraction = float((residuals > 0).float().mean().item()) if residuals is not None else 0.0
-    value = n * args.steps / (gpu_ms * 1e-3)
+    value = n * steps / (value_ms * 1e-3)
+    exact_rate = n * steps / (exact_ms * 1e-3)
+    per_map = lambda st: st["evaluations"] / st["return_maps"] if st["return_maps"] else 0.0  # noqa: E731
     result = {
         "metric": "particle-substeps/s on 1 MI355X, plain vs surrogate+batched at <=1e-3 rel err",
         "value": value,
         "unit": "particle-substeps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": substep_ms,
         "higher_is_better": True,
@@ -224,18 +274,25 @@ def run_batched(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "batched_48", "method": "batched (exact return map)", "hip_graph": True, "doughs": len(sims),
+        "config": {"workload": "batched_48", "method": result_method, "hip_graph": True, "doughs": len(sims),
                    "particles": n, "particles_per_dough": [min(counts), max(counts)],
-                   "grids": sorted(set(s.simulation_config.grid_resolution for s in sims)), "dt": 5e-5,
+                   "grids": sorted(set(s.simulation_config.grid_resolution for s in sims)),
+                   "dt": sims[0].simulation_config.time_step,
                    "active_stencil_nodes": active_nodes, "yield_fraction": yield_fraction, "deferred_return_maps": wcounts[2],
                    "tiles": wcounts[0], "active_node_blocks": wcounts[1], "tuning": batch.Tuning(),
                    "autotune": None if tuned is None else {"start": tuned["start"], "best_ms": tuned["ms"],
                                                            "trials": len(tuned["trials"])},
-                   "wall_s": wall, "build_s": build_s, "flags_max": int(flags.max()), "flags_before_max": int(flags_before.max())},
+                   "return_map_stats": {"exact": exact_stats, "used": stats_used,
+                                        "evaluations_per_map": {"exact": per_map(exact_stats), "used": per_map(stats_used)}},
+                   "wall_build_s": build_s,
+                   "flags_raised": {"exact": [[int(d), int(k)] for d, k in zip(*np.nonzero(flags_exact))],
+                                    "used": [[int(d), int(k)] for d, k in zip(*np.nonzero(flags))]}},
         "plain": {"value": plain_rate, "unit": "particle-substeps/s", "steps_per_dough": args.plain_steps,
                   "method": "each dough alone, single-dough substep launched kernel by kernel (no graph), exact return map",
                   "ms_total": plain_ms},
         "speedup_vs_plain": value / plain_rate,
+        "batched_exact": {"value": exact_rate, "ms_per_step": exact_ms / steps},
+        "accuracy_vs_exact": accuracy,
         "roofline": roof,
         "cpu_baseline": None,
     }
@@ -287,52 +344,82 @@ def main():
 
     if args.workload == "batched_48":
         return run_batched(args)
-    lib = _native.load()
+    from baking_amd import surrogate
+    from baking_amd.measure import measure
     spec = build_scene(args.workload)
+    steps = args.steps or (2000 if spec.name == "loaf_256" else 1000)
     sim = build_simulator(spec)
     sim.use_graphs = not args.no_graph
     n = sim.particle_data.particle_count
     stream = torch.cuda.current_stream()
+    model = None
+    if not args.no_surrogate and sim.current_state.material_state is not None and sim.material.ding_config.gas_enabled:
+        model = surrogate.ReturnMapSurrogate.load(args.model or surrogate.DEFAULT_MODEL)
+        if surrogate.attach(sim, model, strict=False) == 0:
+            model = None
     tuned = None
     if not args.no_autotune and sim.use_graphs:
         from baking_amd.autotune import autotune
         tuned = autotune(sim)
+    if model is not None:
+        surrogate.detach(sim)
 
     for _ in range(args.warmup):
         sim._launch_substep()
         sim._accept()
     torch.cuda.synchronize()
     flags_before = sim._read_flags()
+    snapshot = sim._save_for_tuning()
+    start_aggr = measure(sim)
 
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        sim._launch_substep()
-        sim._accept()
-    stop.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(stop)
+    def timed():
+        for direction in range(2):  # capture both ping-pong graphs outside the timed region
+            if sim.use_graphs:
+                sim._launch_substep()
+                sim._accept()
+        sim._restore_after_tuning(snapshot)
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        start.record(stream)
+        for _ in range(steps):
+            sim._launch_substep()
+            sim._accept()
+        stop.record(stream)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop), time.perf_counter() - t0
+
+    surrogate.stats(sim, reset=True)
+    exact_ms, wall = timed()
+    exact_aggr, exact_stats = measure(sim), surrogate.stats(sim, reset=True)
+    gpu_ms, aggr_used, stats_used, accuracy, method = exact_ms, exact_aggr, exact_stats, None, "plain (exact return map)"
+    if model is not None:
+        surrogate.attach(sim, model, strict=False)
+        surrogate.stats(sim, reset=True)
+        gpu_ms, wall = timed()
+        aggr_used, stats_used = measure(sim), surrogate.stats(sim, reset=True)
+        accuracy = _relative_errors(exact_aggr, aggr_used, start_aggr)
+        method = "plain + surrogate-started return map"
     flags = sim._read_flags()
+    del snapshot
 
     stages, counts = stage_profile(lib_launch=lambda: (sim._launch_substep(use_graph=False), sim._accept()),
                                    workspace=sim._workspace, steps=args.profile_steps)
     deferred = int(counts[2])
     active_nodes = int((sim.grid_state.tiled_masses > 0).sum().item())
-    roof, dominant, dom_bytes, dom_ms = roofline(stages, n, active_nodes, deferred, gpu_ms / args.steps, spec.name)
-    substep_ms = gpu_ms / args.steps
+    substep_ms = gpu_ms / steps
+    roof, dominant, dom_bytes, dom_ms = roofline(stages, n, active_nodes, deferred, substep_ms, spec.name)
     yield_fraction = float((sim.current_state.material_state.return_residuals > 0).float().mean().item()) \
         if sim.current_state.material_state is not None else 0.0
+    per_map = lambda st: st["evaluations"] / st["return_maps"] if st["return_maps"] else 0.0  # noqa: E731
 
-    value = n * args.steps / (gpu_ms * 1e-3)
+    value = n * steps / (gpu_ms * 1e-3)
     result = {
         "metric": "particle-substeps/s on 1 MI355X, plain vs surrogate+batched at <=1e-3 rel err",
         "value": value,
         "unit": "particle-substeps/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": substep_ms,
         "higher_is_better": True,
@@ -340,13 +427,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": spec.name, "method": "plain", "hip_graph": sim.use_graphs, "particles": n, "grid": spec.simulation.grid_resolution,
-                   "dt": spec.simulation.time_step, "active_stencil_nodes": active_nodes,
+        "config": {"workload": spec.name, "method": method, "hip_graph": sim.use_graphs, "particles": n,
+                   "grid": spec.simulation.grid_resolution, "dt": spec.simulation.time_step, "active_stencil_nodes": active_nodes,
                    "yield_fraction": yield_fraction, "deferred_return_maps": deferred,
                    "tiles": int(counts[0]), "active_node_blocks": int(counts[1]), "tuning": sim.Tuning(),
                    "autotune": None if tuned is None else {"start": tuned["start"], "best_ms": tuned["ms"],
                                                            "trials": len(tuned["trials"])},
+                   "return_map_stats": {"exact": exact_stats, "used": stats_used,
+                                        "evaluations_per_map": {"exact": per_map(exact_stats), "used": per_map(stats_used)}},
                    "wall_s": wall, "flags": flags, "flags_before_timing": flags_before},
+        "exact": {"value": n * steps / (exact_ms * 1e-3), "ms_per_step": exact_ms / steps},
+        "accuracy_vs_exact": accuracy,
         "roofline": roof,
         "cpu_baseline": None,
     }

@@ -243,6 +243,12 @@ int bk_profile_begin(bk_workspace* ws, int32_t max_steps);
 int bk_profile_end(bk_workspace* ws, float* ms_per_stage, int32_t* steps_recorded);
 int32_t bk_profile_stage_count(void);
 const char* bk_profile_stage_name(int32_t stage);
+/* Acceptance aggregates as f64 sums per dough (experiments/recording.py:46-112 Measure and the
+ * CheckState statistics of physics/mpm.py:292-308), out[8 * d + k] on the device:
+ * k = 0 sum V0 det F (particle volume), 1 sum capacity * T, 2 sum capacity (mean T = [1] / [2]),
+ * 3 sum W (water), 4 sum water losses, 5 sum heat exchanges, 6 sum V0 * gas moles, 7 sum particle mass.
+ * NULL state fields contribute 0 (mass falls back to scene.m0); scene.dough_of NULL = one dough. */
+int bk_measure(const bk_scene* scene, const bk_state* state, int32_t n, int32_t n_doughs, double* out, void* stream);
 /* Convert the tiled grid's mass/velocity/momentum to the reference's dense layout. */
 int bk_grid_untile(const float* tiled, float* dense, int32_t res, int32_t components, void* stream);
 int bk_grid_untile_f64(const double* tiled, double* dense, int32_t res, void* stream);
