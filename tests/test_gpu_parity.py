@@ -167,3 +167,30 @@ def test_free_translation_conserves_mass_and_shape():
     np.testing.assert_allclose(sim.vertex_positions.cpu().numpy(), verts0 + t * np.array(v0), atol=2.0e-6)
     np.testing.assert_allclose(stats["grid_mass"], m0, rtol=1.0e-6)
     assert stats["unmapped_mesh_vertices"] == 0
+
+
+def test_far_off_and_nonfinite_particles_are_flagged_not_indexed():
+    """Positions whose cell base overflows an int, or that are not finite, fail the stencil test in every
+    kernel (flag 1), so no node index is formed from them: the substep completes and the flag is raised."""
+    import ctypes
+    spec = scenes.sphere_small()
+    sim = scenes.build_simulator(spec)
+    x = sim.current_state.particle_positions
+    x[0, 0] = 1.0e30
+    x[1, 1] = float("inf")
+    x[2, 2] = float("nan")
+    x[3, 0] = -1.0e30
+    sim._launch_substeps(2)  # raw substeps: Advance would raise the reference's errors for the flags
+    torch.cuda.synchronize()
+    assert sim._read_flags()[1] == 1
+    lib = _native.load()
+    keys = torch.empty(4, dtype=torch.int32, device="cuda")
+    flags = torch.zeros(2, dtype=torch.int32, device="cuda")
+    o, keep = _native.host_f32(spec.simulation.grid_origin)
+    bad = torch.tensor([[1e30, 0.5, 0.5], [0.5, float("inf"), 0.5], [0.5, 0.5, float("nan")], [0.5, 0.5, 0.5]],
+                       dtype=torch.float32, device="cuda")
+    _native.check(lib.bk_cell_keys(_native.ptr(bad), 4, o, spec.simulation.grid_spacing, spec.simulation.grid_resolution, 1,
+                                   _native.ptr(keys), _native.ptr(flags), _native.stream_ptr()), "bk_cell_keys")
+    assert flags.cpu().numpy()[1] == 1
+    assert keys.cpu().numpy()[:3].tolist() == [0, 0, 0]
+    del ctypes
