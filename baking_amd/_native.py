@@ -66,6 +66,13 @@ class BkGrid(ctypes.Structure):
     _fields_ = [(name, c_void_p) for name in GRID_FIELDS]
 
 
+ADJOINT_FIELDS = ["x", "v", "C", "F", "Fe", "tau", "porosity"]
+
+
+class BkAdjoint(ctypes.Structure):
+    _fields_ = [(name, c_void_p) for name in ADJOINT_FIELDS]
+
+
 # Exported symbols and their argument types; the ABI test checks this list
 # against the header and the library.
 P = c_void_p
@@ -134,6 +141,10 @@ SIGNATURES = {
     "bk_workspace_set_surrogate": ([P, P], c_int),
     "bk_workspace_set_samples": ([P, P, I], c_int),
     "bk_workspace_stats": ([P, P, I, P], c_int),
+    "bk_adjoint_substep": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid),
+                            POINTER(BkAdjoint), P, P, P], c_int),
+    "bk_UpdateDing_adjoint": ([P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, F, F, F, F, I, P, P, P, F, I, P], c_int),
+    "bk_overstress_adjoint": ([P, P, P, I, P], c_int),
 }
 
 _lib = None

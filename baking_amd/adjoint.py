@@ -1,0 +1,198 @@
+"""Reverse-mode gradients through the fused substep: a checkpointed ``torch.autograd.Function``.
+
+The reference differentiates a rollout by recording ``MpmSimulator.Step`` on a
+``wp.Tape`` in history mode (``physics/mpm.py:18-22, 163-225``; used by
+``tests/test_warp_autodiff.py`` and ``tests/test_ding.py:81-170``), which keeps
+every substep's state and grid alive.  Here the forward runs the fused HIP
+substep (graphs included) and keeps only a checkpoint of the state every
+``checkpoint_every`` substeps.  The backward walks the segments in reverse:
+it recomputes one segment from its checkpoint, keeping each substep's state
+and grid, then runs the library's adjoint kernels (``bk_adjoint_substep``:
+G2P with the F update and the Ding return map, the grid update, P2G with the
+neo-Hookean stress) substep by substep in reverse.  Adjoints are FP64.
+
+Covered: the mechanical substep with the neo-Hookean model, or the Ding model
+without pore gas; gradients with respect to the initial positions,
+velocities, affine velocities and deformation gradients, the Ding elastic
+gradients, porosities and Kirchhoff stresses, and the Ding viscosity
+parameter.  Transport (heat and water) and pore gas have no adjoint here;
+``rollout`` raises for them.
+
+    out = rollout(sim, steps, {"v": v0})          # tensors that may require grad
+    loss = (out["x"][:, 0] - x0[:, 0]).mean()
+    loss.backward()
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _native
+from .launch import Kernel
+from .physics.materials.ding import DingMaterial
+from .physics.mpm import state_field_tensors
+
+# Adjoint of UpdateDing without pore gas (ding.py:140-221), arguments as UpdateDing's inputs, then the
+# adjoints of its outputs Fe, tau, porosity (read), then of old_Fe, old_porosity, F_total, C_affine and
+# the viscosity parameter (accumulated).
+UpdateDingAdjoint = Kernel("bk_UpdateDing_adjoint",
+                           ["f32", "f32", "i32", "f32", "f32", "f32", "f32", "f64", "f64", "f64", "f64", "f64", "f64", "f64",
+                            "f64", "f", "f", "f", "f", "i", "v4", "v4", "v4", "f"])
+# Implicit-function adjoint of the overstress solve (ding.py:90-100): (q2, p2, df, mf) rows, overstress adjoints -> 4 adjoints.
+OverstressAdjoint = Kernel("bk_overstress_adjoint", ["f64", "f64", "f64"])
+
+MECHANICAL_FIELDS = ("x", "v", "C", "F")
+DING_FIELDS = ("Fe", "porosity", "tau")
+_WIDTH = dict(x=3, v=3, C=9, F=9, Fe=9, tau=9, porosity=1)
+
+
+def _fields_of(sim):
+    return MECHANICAL_FIELDS + (DING_FIELDS if isinstance(sim.material, DingMaterial) else ())
+
+
+def _check_supported(sim):
+    if getattr(sim, "transport", None) is not None:
+        raise NotImplementedError("rollout: heat and water transport have no adjoint in this library")
+    if isinstance(sim.material, DingMaterial) and sim.material.ding_config.gas_enabled:
+        raise NotImplementedError("rollout: the Ding model with pore gas has no adjoint in this library")
+
+
+def _clone_state(state):
+    return {f: t.clone() for f, t in state_field_tensors(state).items()}
+
+
+def _load_state(state, saved):
+    for f, t in state_field_tensors(state).items():
+        t.copy_(saved[f])
+
+
+def _grid_clone(sim):
+    g = sim.grid_state
+    return (g.tiled_masses.clone(), g.tiled_momenta.clone(), g.tiled_velocities.clone())
+
+
+def _bk_state(saved):
+    s = _native.BkState()
+    for f, t in saved.items():
+        setattr(s, f, t.data_ptr())
+    return s
+
+
+def _bk_grid(grid):
+    g = _native.BkGrid()
+    g.m, g.p, g.v = (t.data_ptr() for t in grid)
+    return g
+
+
+def _run(sim, count):
+    for _ in range(count):
+        sim._launch_substep()
+        sim._accept()
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sim, steps, every, fields, visc, *inputs):
+        lib = _native.load()
+        state = state_field_tensors(sim.current_state)
+        for f, t in zip(fields, inputs):
+            state[f].copy_(t.detach().reshape(state[f].shape).to(state[f].dtype))
+        ding = isinstance(sim.material, DingMaterial)
+        if ding:
+            sim.material.viscosity_parameter.copy_(visc.detach().reshape(1).to(torch.float32))
+        flags_before = sim._read_flags()
+        checkpoints = []
+        for start in range(0, steps, every):
+            checkpoints.append(_clone_state(sim.current_state))
+            _run(sim, min(every, steps - start))
+        flags = sim._read_flags()
+        if any(f and not b for f, b in zip(flags, flags_before)):
+            sim._sync_api_flags(flags)
+            sim._raise_for(flags)
+            raise RuntimeError(f"rollout: the forward substeps raised error flags {flags}")
+        ctx.sim, ctx.steps, ctx.every, ctx.fields, ctx.checkpoints = sim, steps, every, fields, checkpoints
+        ctx.shapes = [t.shape for t in inputs]
+        ctx.dtypes = [t.dtype for t in inputs]
+        ctx.visc_shape, ctx.visc_dtype = visc.shape, visc.dtype
+        ctx.lib = lib
+        final = state_field_tensors(sim.current_state)
+        return tuple(final[f].clone() for f in fields)
+
+    @staticmethod
+    def backward(ctx, *grads):
+        sim, lib, fields = ctx.sim, ctx.lib, ctx.fields
+        n = sim.particle_data.particle_count
+        dev = sim.current_state.particle_positions.device
+        adj = {f: torch.zeros(n * _WIDTH[f], dtype=torch.float64, device=dev) for f in _fields_of(sim)}
+        for f, g in zip(fields, grads):
+            if g is not None:
+                adj[f] += g.detach().reshape(-1).to(torch.float64)
+        bk_adj = _native.BkAdjoint()
+        for f, t in adj.items():
+            setattr(bk_adj, f, t.data_ptr())
+        param = torch.zeros(1, dtype=torch.float64, device=dev)
+        nodes = sim.grid_state.tiled_masses.numel()
+        scratch = torch.empty(7 * nodes, dtype=torch.float64, device=dev)
+        # the simulator is left where the forward ended
+        keep = (sim.current_state, sim.next_state, sim.step_count, _clone_state(sim.current_state), sim._read_flags())
+        stream = _native.stream_ptr()
+        for seg in reversed(range(len(ctx.checkpoints))):
+            count = min(ctx.every, ctx.steps - seg * ctx.every)
+            _load_state(sim.current_state, ctx.checkpoints[seg])
+            states, grids = [ctx.checkpoints[seg]], []
+            for _ in range(count):
+                sim._launch_substep()
+                grids.append(_grid_clone(sim))
+                sim._accept()
+                states.append(_clone_state(sim.current_state))
+            for k in reversed(range(count)):
+                s_in, s_out, grid = _bk_state(states[k]), _bk_state(states[k + 1]), _bk_grid(grids[k])
+                _native.check(lib.bk_adjoint_substep(ctypes.byref(sim._params), ctypes.byref(sim._scene), ctypes.byref(s_in),
+                                                     ctypes.byref(s_out), ctypes.byref(grid), ctypes.byref(bk_adj),
+                                                     _native.ptr(scratch), _native.ptr(param), stream), "bk_adjoint_substep")
+            del states, grids
+        sim.current_state, sim.next_state, sim.step_count, final, flags = keep
+        _load_state(sim.current_state, final)
+        sim._write_flags(flags)
+        out = [adj[f].reshape(shape).to(dtype) for f, shape, dtype in zip(fields, ctx.shapes, ctx.dtypes)]
+        visc_grad = param.reshape(ctx.visc_shape).to(ctx.visc_dtype) if ctx.visc_shape.numel() else None
+        return (None, None, None, None, visc_grad, *out)
+
+
+def rollout(sim, steps, inputs=None, viscosity_parameter=None, checkpoint_every=16):
+    """Run ``steps`` substeps of ``sim`` from ``inputs`` and return the final state as differentiable tensors.
+
+    ``inputs`` maps state fields (``x, v, C, F`` and, for the Ding model, ``Fe, porosity, tau``) to tensors of
+    the state's shapes (N x 3, N x 3, N x 3 x 3, ...); fields left out start from the simulator's current
+    state.  ``viscosity_parameter`` (Ding model, a one-element tensor) replaces the material's
+    ``viscosity_parameter``.  Returns {field: tensor} for every field; the simulator is left at the final
+    state, as after ``Advance(steps)``.
+    """
+    if steps < 1:
+        raise ValueError("steps must be positive")
+    if checkpoint_every < 1:
+        raise ValueError("checkpoint_every must be positive")
+    _check_supported(sim)
+    inputs = dict(inputs or {})
+    fields = _fields_of(sim)
+    unknown = set(inputs) - set(fields)
+    if unknown:
+        raise ValueError(f"rollout: no adjoint for fields {sorted(unknown)}; differentiable fields are {fields}")
+    current = state_field_tensors(sim.current_state)
+    tensors = []
+    for f in fields:
+        t = inputs.get(f)
+        if t is None:
+            t = current[f].detach().clone()
+        elif t.numel() != current[f].numel():
+            raise ValueError(f"rollout: input {f} has {t.numel()} elements, the state has {current[f].numel()}")
+        tensors.append(t)
+    if isinstance(sim.material, DingMaterial):
+        visc = viscosity_parameter if viscosity_parameter is not None else sim.material.viscosity_parameter.detach().clone()
+    else:
+        if viscosity_parameter is not None:
+            raise ValueError("rollout: viscosity_parameter needs the Ding model")
+        visc = torch.zeros(0, device=current["x"].device)
+    outs = _Rollout.apply(sim, int(steps), int(checkpoint_every), fields, visc, *tensors)
+    return dict(zip(fields, outs))

@@ -1,0 +1,742 @@
+// Adjoint (reverse-mode) kernels of one substep of the mechanical MLS-MPM
+// path: G2P with the F update, the grid update, P2G, the neo-Hookean stress,
+// and the Ding return map without pore gas (reference physics/transfer.py:36-109,
+// grid.py:20-38, materials/elastic.py:4-10, materials/ding.py:77-221).
+//
+// Given the adjoint of the substep's output state (in the bk_adjoint arrays),
+// the kernels replace it with the adjoint of its input state and add the
+// gradient of the Ding viscosity parameter.  They read the forward values of
+// that substep: the input and output states, the grid the forward left (mass,
+// momentum, velocity) and the stress P2G used.  Adjoints are FP64.
+//
+// The return map's adjoint differentiates the map itself, not its iterations:
+// the overstress by the implicit function theorem on its converged equation
+// (as the reference's own adjoint, ding.py:90-100), the principal values by
+// forward-mode dual numbers, and the rotation part by the closed forms of the
+// derivative of an isotropic tensor function of the trial gradient, written
+// with divided differences that stay finite (their limits) when two
+// principal stretches coincide.
+#include "bk_device.h"
+#include "../../include/baking_abi.h"
+
+using namespace bk;
+
+void bk_internal_set_error(const char* what);  // bk_last_error's text (bk_substep.hip)
+
+namespace {
+int fail(const char* what) {
+  bk_internal_set_error(what);
+  return -1;
+}
+inline int div_up(long long a, long long b) { return int((a + b - 1) / b); }
+}  // namespace
+
+// ------------------------------------------------------------------ dual numbers
+// Value and derivatives along 5 directions: the 3 principal stretches, the
+// porosity coefficient a and the viscosity.
+constexpr int kDir = 5;
+struct Dual {
+  double v;
+  double d[kDir];
+};
+__device__ __forceinline__ Dual dconst(double v) {
+  Dual r;
+  r.v = v;
+#pragma unroll
+  for (int k = 0; k < kDir; ++k) r.d[k] = 0.0;
+  return r;
+}
+__device__ __forceinline__ Dual dvar(double v, int k) {
+  Dual r = dconst(v);
+  r.d[k] = 1.0;
+  return r;
+}
+__device__ __forceinline__ Dual operator+(const Dual& a, const Dual& b) {
+  Dual r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < kDir; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+__device__ __forceinline__ Dual operator-(const Dual& a, const Dual& b) {
+  Dual r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < kDir; ++k) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+__device__ __forceinline__ Dual operator*(const Dual& a, const Dual& b) {
+  Dual r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < kDir; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+__device__ __forceinline__ Dual operator*(double s, const Dual& a) {
+  Dual r;
+  r.v = s * a.v;
+#pragma unroll
+  for (int k = 0; k < kDir; ++k) r.d[k] = s * a.d[k];
+  return r;
+}
+__device__ __forceinline__ Dual operator/(const Dual& a, const Dual& b) {
+  Dual r;
+  r.v = a.v / b.v;
+#pragma unroll
+  for (int k = 0; k < kDir; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+  return r;
+}
+__device__ __forceinline__ Dual dlog(const Dual& a) {
+  Dual r;
+  r.v = log(a.v);
+#pragma unroll
+  for (int k = 0; k < kDir; ++k) r.d[k] = a.d[k] / a.v;
+  return r;
+}
+__device__ __forceinline__ Dual dexp(const Dual& a) {
+  Dual r;
+  r.v = exp(a.v);
+#pragma unroll
+  for (int k = 0; k < kDir; ++k) r.d[k] = r.v * a.d[k];
+  return r;
+}
+
+// ------------------------------------------------------------------ principal part of the return map
+// From the principal stretches s, the porosity coefficient a and the viscosity
+// eta: the corrected principal Kirchhoff stresses cs and the returned principal
+// stretches e (ding.py:103-131), with their derivatives.
+struct Principal {
+  Dual cs[3], e[3];
+};
+__device__ inline Principal principal_return(const double s[3], double a, double eta, double mu, double lam, double y, double dt) {
+  Dual sv[3] = {dvar(s[0], 0), dvar(s[1], 1), dvar(s[2], 2)};
+  const Dual ad = dvar(a, 3), ev = dvar(eta, 4);
+  Dual eps[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) eps[i] = dlog(sv[i]);
+  const Dual tr = eps[0] + eps[1] + eps[2];
+  Dual ts[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) ts[i] = 2.0 * mu * eps[i] + lam * tr;
+  const Dual mean = (1.0 / 3.0) * (ts[0] + ts[1] + ts[2]);
+  Dual dev[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) dev[i] = ts[i] - mean;
+  const double y2 = y * y;
+  const Dual q2 = (1.5 / y2) * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+  const Dual p2 = (1.0 / y2) * (ad * mean * mean);
+  const Dual df = dconst(6.0 * mu * dt * y2) / ev;
+  const Dual mf = ((2.0 / 3.0) * (2.0 * mu + 3.0 * lam) * dt * y2) * ad / ev;
+  // overstress: converged Newton on R(g) = g + 1 - q2/(1+df g)^2 - p2/(1+mf g)^2 = 0, derivative by the IFT
+  Dual g = dconst(0.0);
+  if (q2.v + p2.v > 1.0) {
+    double root = 0.0;
+    for (int it = 0; it < 64; ++it) {
+      const double u = 1.0 + df.v * root, w = 1.0 + mf.v * root;
+      const double res = root + 1.0 - q2.v / (u * u) - p2.v / (w * w);
+      const double der = 1.0 + 2.0 * df.v * q2.v / (u * u * u) + 2.0 * mf.v * p2.v / (w * w * w);
+      const double next = root - res / der;
+      if (next == root) break;
+      root = next;
+    }
+    const double u = 1.0 + df.v * root, w = 1.0 + mf.v * root;
+    const double der = 1.0 + 2.0 * df.v * q2.v / (u * u * u) + 2.0 * mf.v * p2.v / (w * w * w);
+    g.v = root;
+#pragma unroll
+    for (int k = 0; k < kDir; ++k) {
+      const double dR = -q2.d[k] / (u * u) - p2.d[k] / (w * w) + 2.0 * q2.v * root * df.d[k] / (u * u * u) +
+                        2.0 * p2.v * root * mf.d[k] / (w * w * w);
+      g.d[k] = -dR / der;
+    }
+  }
+  const Dual one = dconst(1.0);
+  const Dual dden = one + df * g, mden = one + mf * g;
+  Principal P;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) P.cs[i] = dev[i] / dden + mean / mden;
+  const Dual cm = (1.0 / 3.0) * (P.cs[0] + P.cs[1] + P.cs[2]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) P.e[i] = dexp((P.cs[i] - cm) * dconst(1.0 / (2.0 * mu)) + cm * dconst(1.0 / (2.0 * mu + 3.0 * lam)));
+  return P;
+}
+
+// Divided difference (f_i - f_j) / (s_i - s_j) of a symmetric function of the
+// principal values, or its limit df_i/ds_i - df_i/ds_j when they coincide.
+__device__ __forceinline__ double divided(const Dual* f, const double* s, int i, int j) {
+  const double ds = s[i] - s[j];
+  if (fabs(ds) > 1e-6 * fmax(fabs(s[i]), fabs(s[j]))) return (f[i].v - f[j].v) / ds;
+  return f[i].d[i] - f[i].d[j];
+}
+
+typedef double dmat[9];
+__device__ __forceinline__ void mmul(const double* A, const double* B, double* R) {  // R = A B
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) R[i * 3 + j] = A[i * 3] * B[j] + A[i * 3 + 1] * B[3 + j] + A[i * 3 + 2] * B[6 + j];
+}
+__device__ __forceinline__ void mmul_tn(const double* A, const double* B, double* R) {  // R = A^T B
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) R[i * 3 + j] = A[i] * B[j] + A[3 + i] * B[3 + j] + A[6 + i] * B[6 + j];
+}
+__device__ __forceinline__ void mmul_nt(const double* A, const double* B, double* R) {  // R = A B^T
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) R[i * 3 + j] = A[i * 3] * B[j * 3] + A[i * 3 + 1] * B[j * 3 + 1] + A[i * 3 + 2] * B[j * 3 + 2];
+}
+__device__ __forceinline__ double mdet(const double* a) {
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) + a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+// inverse transpose
+__device__ __forceinline__ void minv_t(const double* a, double* r) {
+  const double d = mdet(a);
+  r[0] = (a[4] * a[8] - a[5] * a[7]) / d;
+  r[3] = (a[2] * a[7] - a[1] * a[8]) / d;
+  r[6] = (a[1] * a[5] - a[2] * a[4]) / d;
+  r[1] = (a[5] * a[6] - a[3] * a[8]) / d;
+  r[4] = (a[0] * a[8] - a[2] * a[6]) / d;
+  r[7] = (a[2] * a[3] - a[0] * a[5]) / d;
+  r[2] = (a[3] * a[7] - a[4] * a[6]) / d;
+  r[5] = (a[1] * a[6] - a[0] * a[7]) / d;
+  r[8] = (a[0] * a[4] - a[1] * a[3]) / d;
+}
+
+// Adjoint of the non-coupled Ding return map (ding.py:140-221 with gas off):
+// outputs Fe, tau and the porosity det(Fe^-1 F) - 1 + n0, adjoints gFe, gtau, gn.
+// Adds to gFe_old, gF (the new total gradient), gC (the new affine velocity),
+// gporosity (the old porosity) and geta0 (the viscosity parameter).
+__device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe_old, const double* F, const double* C, double porosity,
+                                           int gelled_in, double T, double visc_param, const double* gFe_in, const double* gtau,
+                                           double gn, double* gFe_old, double* gF, double* gC, double& gporosity, double& geta0) {
+  int gel = gelled_in;
+  double Teff = T;
+  double yv = P.yield_stress, vfac = 1.0;
+  if (P.temperature_enabled) {
+    if (T >= P.knots[3]) gel = 1;
+    if (gel) Teff = P.knots[3];
+    yv = double(P.yield_stress * temperature_factor(float(Teff), P.knots, P.yf));
+    vfac = double(temperature_factor(float(Teff), P.knots, P.vf));
+  }
+  const double ev = visc_param * vfac;
+  const double mu = P.mu, lam = P.lam, dt = P.dt;
+  double A[9], trial[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) A[i] = dt * C[i] + ((i % 4 == 0) ? 1.0 : 0.0);
+  mmul(A, Fe_old, trial);
+  mat3<double> Tm, Um, Vm;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Tm.a[i] = trial[i];
+  double s[3];
+  svd3(Tm, Um, s, Vm);
+  const double* U = Um.a;
+  const double* V = Vm.a;
+  // porosity coefficient and its derivative (ding.py:68-74; non-coupled: the old porosity)
+  double a = 0.0, da = 0.0;
+  if (porosity > 0.0) {
+    const double lg = log(1.0 / porosity);
+    a = 9.0 / (4.0 * lg * lg * (3.5 * porosity + 1.0));
+    da = a * (2.0 / (porosity * lg) - 3.5 / (3.5 * porosity + 1.0));
+  }
+  const Principal pr = principal_return(s, a, ev, mu, lam, yv, dt);
+  double e[3], cs[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    e[i] = pr.e[i].v;
+    cs[i] = pr.cs[i].v;
+  }
+  // Fe0 = U diag(e) V^T, and the output Fe (isochoric fit when a == 0)
+  double Fe0[9], tmp[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) tmp[i * 3 + j] = U[i * 3 + j] * e[j];
+  mmul_nt(tmp, V, Fe0);
+  double Fe[9], c = 1.0;
+  const double det_trial = mdet(trial), det_fe0 = mdet(Fe0);
+  if (a == 0.0) c = cbrt(det_trial / det_fe0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fe[i] = c * Fe0[i];
+  double gFe[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) gFe[i] = gFe_in[i];
+  // porosity output: n = det F / det Fe - 1 + n0 (constant 0 when n0 = 0 and the old porosity is 0)
+  if (gn != 0.0 && !(P.n0 == 0.0f && porosity == 0.0)) {
+    const double ratio = mdet(F) / mdet(Fe);
+    double fit[9], feit[9];
+    minv_t(F, fit);
+    minv_t(Fe, feit);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      gF[i] += gn * ratio * fit[i];
+      gFe[i] -= gn * ratio * feit[i];
+    }
+  }
+  // isochoric fit
+  double gFe0[9], gTrial[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (a == 0.0) {
+    double cbar = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) cbar += gFe[i] * Fe0[i];
+    double tit[9], fe0it[9];
+    minv_t(trial, tit);
+    minv_t(Fe0, fe0it);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      gFe0[i] = c * gFe[i] - cbar * c / 3.0 * fe0it[i];
+      gTrial[i] += cbar * c / 3.0 * tit[i];
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gFe0[i] = gFe[i];
+  }
+  // into the principal frame
+  double Ne[9], Nt[9], t1[9];
+  mmul_tn(U, gFe0, t1);
+  mmul(t1, V, Ne);
+  mmul_tn(U, gtau, t1);
+  mmul(t1, U, Nt);
+  double M[9];
+  double gscal[kDir] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int k = 0; k < kDir; ++k) gscal[k] += Ne[i * 4] * pr.e[i].d[k] + Nt[i * 4] * pr.cs[i].d[k];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) M[i * 4] = gscal[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = i + 1; j < 3; ++j) {
+      const double al = divided(pr.e, s, i, j), be = (e[i] + e[j]) / (s[i] + s[j]);
+      const double pp = 0.5 * (al + be), pm = 0.5 * (al - be);
+      const double kap = divided(pr.cs, s, i, j) / (s[i] + s[j]);
+      const double nt = Nt[i * 3 + j] + Nt[j * 3 + i];
+      M[i * 3 + j] = pp * Ne[i * 3 + j] + pm * Ne[j * 3 + i] + nt * kap * s[j];
+      M[j * 3 + i] = pm * Ne[i * 3 + j] + pp * Ne[j * 3 + i] + nt * kap * s[i];
+    }
+  // back to the trial gradient: gTrial += U M V^T
+  double UM[9], G[9];
+  mmul(U, M, UM);
+  mmul_nt(UM, V, G);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) gTrial[i] += G[i];
+  gporosity += gscal[3] * da;
+  geta0 += gscal[4] * vfac;
+  // trial = (I + dt C) Fe_old
+  double t2[9];
+  mmul_tn(A, gTrial, t2);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) gFe_old[i] += t2[i];
+  mmul_nt(gTrial, Fe_old, t2);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) gC[i] += dt * t2[i];
+}
+
+// ------------------------------------------------------------------ transfers
+__device__ __forceinline__ void axis_weight_grads(float l, double w[3], double dw[3]) {
+  const double L = l;
+  w[0] = 0.5 * (1.5 - L) * (1.5 - L);
+  w[1] = 0.75 - (L - 1.0) * (L - 1.0);
+  w[2] = 0.5 * (L - 0.5) * (L - 0.5);
+  dw[0] = -(1.5 - L);
+  dw[1] = -2.0 * (L - 1.0);
+  dw[2] = L - 0.5;
+}
+
+struct AdjArgs {
+  bk_params P;
+  bk_state in, out;
+  bk_grid g;
+  const float* masses;      // particle masses P2G used
+  const float* V0;
+  const float* visc_param;  // per dough (one value)
+  bk_adjoint adj;
+  double* gm;  // grid adjoints (tiled nodes): mass, momentum (3), velocity (3)
+  double* gp;
+  double* gv;
+  double* param_grad;  // [0] viscosity parameter
+  int n, nb;
+};
+
+// Per particle: the Ding return map's adjoint (its outputs' adjoints arrive in
+// adj.Fe / adj.tau / adj.porosity), then G2P's: scatter the velocity adjoint to
+// the grid and leave the input state's partial adjoints in the adj arrays.
+__global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double geta = 0.0;
+  if (p < a.n) {
+    const bk_params& P = a.P;
+    const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+    double gx[3], gv[3], gC[9], gF[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      gx[i] = a.adj.x[3 * p + i];
+      gv[i] = a.adj.v[3 * p + i];
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      gC[i] = a.adj.C[9 * p + i];
+      gF[i] = a.adj.F[9 * p + i];
+    }
+    double gFe_old[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gpor = 0.0;
+    if (P.model == 1) {
+      DingParams D;
+      D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
+      for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
+      D.temperature_enabled = P.temperature_enabled;
+      D.coupled = 0;
+      D.bracketed = 0;
+      double Fe_old[9], Fn[9], Cn[9], gFe[9], gtau[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        Fe_old[i] = a.in.Fe[9 * p + i];
+        Fn[i] = a.out.F[9 * p + i];
+        Cn[i] = a.out.C[9 * p + i];
+        gFe[i] = a.adj.Fe[9 * p + i];
+        gtau[i] = a.adj.tau[9 * p + i];
+      }
+      ding_return_adjoint(D, Fe_old, Fn, Cn, double(a.in.porosity[p]), a.in.gelled[p], double(a.in.T_mat[p]), double(a.visc_param[0]),
+                          gFe, gtau, a.adj.porosity[p], gFe_old, gF, gC, gpor, geta);
+    }
+    // G2P (transfer.py:73-109)
+    const float dx = P.dx;
+    const double s4 = 4.0 / (double(dx) * double(dx));
+    const double dt = P.dt;
+    float g3[3];
+    grid_position(a.in.x + 3 * p, o, dx, g3);
+    int b[3];
+    stencil_base(g3, b);
+    double gx_in[3] = {gx[0], gx[1], gx[2]};
+    double gF_in[9];
+    double Cnew[9], Fold[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      Cnew[i] = a.out.C[9 * p + i];
+      Fold[i] = a.in.F[9 * p + i];
+    }
+    // F' = (I + dt C') F  ->  gF_in = (I + dt C')^T gF ; gC' += dt gF F^T
+    double Ainc[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Ainc[i] = dt * Cnew[i] + ((i % 4 == 0) ? 1.0 : 0.0);
+    mmul_tn(Ainc, gF, gF_in);
+    double t[9];
+    mmul_nt(gF, Fold, t);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) gC[i] += dt * t[i];
+    // x' = x + dt v'
+#pragma unroll
+    for (int i = 0; i < 3; ++i) gv[i] += dt * gx[i];
+    if (valid_stencil(b, P.res)) {
+      const float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+      double wx[3], wy[3], wz[3], dwx[3], dwy[3], dwz[3];
+      axis_weight_grads(l[0], wx, dwx);
+      axis_weight_grads(l[1], wy, dwy);
+      axis_weight_grads(l[2], wz, dwz);
+      const double inv_dx = 1.0 / double(dx);
+      for (int ox = 0; ox < 3; ++ox)
+        for (int oy = 0; oy < 3; ++oy)
+          for (int oz = 0; oz < 3; ++oz) {
+            const double w = wx[ox] * wy[oy] * wz[oz];
+            const double dw[3] = {dwx[ox] * wy[oy] * wz[oz] * inv_dx, wx[ox] * dwy[oy] * wz[oz] * inv_dx, wx[ox] * wy[oy] * dwz[oz] * inv_dx};
+            const int64_t node = tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, a.nb);
+            const double vi[3] = {a.g.v[3 * node], a.g.v[3 * node + 1], a.g.v[3 * node + 2]};
+            const double dd[3] = {(double(ox) - l[0]) * dx, (double(oy) - l[1]) * dx, (double(oz) - l[2]) * dx};
+            // value the node velocity contributes: v' . gv + s4 v^T gC dd
+            double gCdd[3], gCtv[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              gCdd[r] = gC[r * 3] * dd[0] + gC[r * 3 + 1] * dd[1] + gC[r * 3 + 2] * dd[2];
+              gCtv[r] = gC[r] * vi[0] + gC[3 + r] * vi[1] + gC[6 + r] * vi[2];
+            }
+            const double val = vi[0] * gv[0] + vi[1] * gv[1] + vi[2] * gv[2] + s4 * (vi[0] * gCdd[0] + vi[1] * gCdd[1] + vi[2] * gCdd[2]);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+              gx_in[r] += dw[r] * val - w * s4 * gCtv[r];
+              unsafeAtomicAdd(a.gv + 3 * node + r, w * (gv[r] + s4 * gCdd[r]));
+            }
+          }
+    }
+    // leave the input state's partial adjoints: x (from G2P), F (from the F update), Ding Fe and porosity
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      a.adj.x[3 * p + i] = gx_in[i];
+      a.adj.v[3 * p + i] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      a.adj.F[9 * p + i] = gF_in[i];
+      a.adj.C[9 * p + i] = 0.0;
+      if (P.model == 1) {
+        a.adj.Fe[9 * p + i] = gFe_old[i];
+        a.adj.tau[9 * p + i] = 0.0;
+      }
+    }
+    if (P.model == 1) a.adj.porosity[p] = gpor;
+  }
+  // viscosity-parameter gradient: wavefront sum, one atomic per wavefront
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) geta += __shfl_xor(geta, off, 64);
+  if (__lane_id() == 0 && geta != 0.0) unsafeAtomicAdd(a.param_grad, geta);
+}
+
+// Per node: the grid update's adjoint (grid.py:20-38).
+__global__ __launch_bounds__(256) void k_adj_grid(AdjArgs a, int64_t nodes) {
+  const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (node >= nodes) return;
+  const double m = a.g.m[node];
+  if (!(m > 0.0)) return;
+  const bk_params& P = a.P;
+  const int64_t blk = node >> 6;
+  const int loc = int(node & 63);
+  const int nb = a.nb;
+  const int bz = int(blk % nb), by = int((blk / nb) % nb), bx = int(blk / (int64_t(nb) * nb));
+  const int c[3] = {4 * bx + (loc >> 4), 4 * by + ((loc >> 2) & 3), 4 * bz + (loc & 3)};
+  double gmass = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double pk = a.g.p[3 * node + k];
+    // the forward zeroed this component (a boundary node whose velocity pointed outwards) iff it stored 0 there
+    const bool boundary = c[k] < P.boundary_cells || c[k] >= P.res - P.boundary_cells;
+    const bool zeroed = boundary && a.g.v[3 * node + k] == 0.0f;
+    const double gvk = zeroed ? 0.0 : a.gv[3 * node + k];
+    a.gp[3 * node + k] = gvk / m;
+    gmass -= gvk * pk / (m * m);
+  }
+  a.gm[node] = gmass;
+}
+
+// Per particle: P2G's adjoint (transfer.py:36-70) and, for the neo-Hookean model,
+// the stress's (elastic.py:4-10); completes the input state's adjoint.
+__global__ __launch_bounds__(128) void k_adj_p2g(AdjArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  const bk_params& P = a.P;
+  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+  const float dx = P.dx;
+  float g3[3];
+  grid_position(a.in.x + 3 * p, o, dx, g3);
+  int b[3];
+  stencil_base(g3, b);
+  if (!valid_stencil(b, P.res)) return;
+  const float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+  double wx[3], wy[3], wz[3], dwx[3], dwy[3], dwz[3];
+  axis_weight_grads(l[0], wx, dwx);
+  axis_weight_grads(l[1], wy, dwy);
+  axis_weight_grads(l[2], wz, dwz);
+  const double mass = a.masses[p];
+  const double s_tau = double(P.dt) * double(a.V0[p]) * 4.0 / (double(dx) * double(dx));
+  // the stress P2G used: the state's for the Ding model, mu (F F^T - I) + lam log(J) I of the input F otherwise
+  double tau[9], F[9];
+  bool stressed = true;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = a.in.F[9 * p + i];
+  if (P.model == 1) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) tau[i] = a.in.tau[9 * p + i];
+  } else {
+    const double J = mdet(F);
+    stressed = isfinite(J) && J > 0.0;
+    mmul_nt(F, F, tau);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) tau[i] = stressed ? double(P.mu) * (tau[i] - ((i % 4 == 0) ? 1.0 : 0.0)) + ((i % 4 == 0) ? double(P.lam) * log(J) : 0.0) : 0.0;
+  }
+  double Am[9], vv[3];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Am[i] = mass * double(a.in.C[9 * p + i]) - s_tau * tau[i];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) vv[i] = a.in.v[3 * p + i];
+  double gx[3] = {0, 0, 0}, gvp[3] = {0, 0, 0}, gA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const double inv_dx = 1.0 / double(dx);
+  for (int ox = 0; ox < 3; ++ox)
+    for (int oy = 0; oy < 3; ++oy)
+      for (int oz = 0; oz < 3; ++oz) {
+        const double w = wx[ox] * wy[oy] * wz[oz];
+        const double dw[3] = {dwx[ox] * wy[oy] * wz[oz] * inv_dx, wx[ox] * dwy[oy] * wz[oz] * inv_dx, wx[ox] * wy[oy] * dwz[oz] * inv_dx};
+        const int64_t node = tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, a.nb);
+        const double gpi[3] = {a.gp[3 * node], a.gp[3 * node + 1], a.gp[3 * node + 2]};
+        const double gmi = a.gm[node];
+        const double dd[3] = {(double(ox) - l[0]) * dx, (double(oy) - l[1]) * dx, (double(oz) - l[2]) * dx};
+        double Add[3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) Add[r] = Am[r * 3] * dd[0] + Am[r * 3 + 1] * dd[1] + Am[r * 3 + 2] * dd[2];
+        const double val = gmi * mass + gpi[0] * (mass * vv[0] + Add[0]) + gpi[1] * (mass * vv[1] + Add[1]) + gpi[2] * (mass * vv[2] + Add[2]);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          // d dd / dx = -I: -(A^T gp)_r
+          const double Atg = Am[r] * gpi[0] + Am[3 + r] * gpi[1] + Am[6 + r] * gpi[2];
+          gx[r] += dw[r] * val - w * Atg;
+          gvp[r] += w * mass * gpi[r];
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int cc = 0; cc < 3; ++cc) gA[r * 3 + cc] += w * gpi[r] * dd[cc];
+      }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    a.adj.x[3 * p + r] += gx[r];
+    a.adj.v[3 * p + r] = gvp[r];
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) a.adj.C[9 * p + i] = mass * gA[i];
+  double gtau[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) gtau[i] = -s_tau * gA[i];
+  if (P.model == 1) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.adj.tau[9 * p + i] = gtau[i];
+  } else if (stressed) {
+    // neo-Hookean Kirchhoff stress mu (F F^T - I) + lam log(J) I: gF += mu (gtau + gtau^T) F + lam tr(gtau) F^-T
+    double fit[9], sym[9], t[9];
+    minv_t(F, fit);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) sym[i * 3 + j] = gtau[i * 3 + j] + gtau[j * 3 + i];
+    mmul(sym, F, t);
+    const double tr = gtau[0] + gtau[4] + gtau[8];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.adj.F[9 * p + i] += double(P.mu) * t[i] + double(P.lam) * tr * fit[i];
+  }
+}
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, const bk_state* out, const bk_grid* grid,
+                       const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream) {
+  if (!P || !sc || !in || !out || !grid || !adj || !grid_scratch || !param_grad) return fail("bk_adjoint_substep: null argument");
+  if (P->transport_enabled || (P->model == 1 && P->gas_enabled))
+    return fail("bk_adjoint_substep: the adjoint covers the mechanical substep without transport or pore gas");
+  if (P->n < 1 || P->res < 4) return fail("bk_adjoint_substep: bad sizes");
+  hipStream_t st = (hipStream_t)stream;
+  AdjArgs a;
+  a.P = *P;
+  a.in = *in;
+  a.out = *out;
+  a.g = *grid;
+  a.masses = sc->m0;
+  a.V0 = sc->V0;
+  a.visc_param = sc->visc_param;
+  a.adj = *adj;
+  a.nb = (P->res + 3) / 4;
+  a.n = P->n;
+  const int64_t nodes = int64_t(a.nb) * a.nb * a.nb * 64;
+  a.gm = grid_scratch;
+  a.gp = grid_scratch + nodes;
+  a.gv = grid_scratch + 4 * nodes;
+  a.param_grad = param_grad;
+  if (hipMemsetAsync(grid_scratch, 0, sizeof(double) * 7 * size_t(nodes), st) != hipSuccess) return fail("bk_adjoint_substep: clear");
+  if (P->model == 1 && (!sc->visc_param || !in->Fe || !adj->Fe || !adj->tau || !adj->porosity))
+    return fail("bk_adjoint_substep: the Ding adjoint needs Fe, tau and porosity arrays");
+  hipLaunchKernelGGL(k_adj_g2p, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
+  hipLaunchKernelGGL(k_adj_grid, dim3(div_up(nodes, 256)), dim3(256), 0, st, a, nodes);
+  hipLaunchKernelGGL(k_adj_p2g, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : fail("bk_adjoint_substep: launch failed");
+}
+
+int bk_overstress_adjoint(const double* args4, const double* gbar, double* out4, int32_t n, void* stream);
+
+}  // extern "C"
+
+// IFT adjoint of the overstress solve alone (ding.py:90-100), for scalar checks.
+__global__ void k_overstress_adjoint(const double* args4, const double* gbar, double* out4, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double q2 = args4[4 * i], p2 = args4[4 * i + 1], df = args4[4 * i + 2], mf = args4[4 * i + 3];
+  double r[4] = {0, 0, 0, 0};
+  if (q2 + p2 > 1.0) {
+    double g = 0.0;
+    for (int it = 0; it < 64; ++it) {
+      const double u = 1.0 + df * g, w = 1.0 + mf * g;
+      const double res = g + 1.0 - q2 / (u * u) - p2 / (w * w);
+      const double der = 1.0 + 2.0 * df * q2 / (u * u * u) + 2.0 * mf * p2 / (w * w * w);
+      const double next = g - res / der;
+      if (next == g) break;
+      g = next;
+    }
+    const double u = 1.0 + df * g, w = 1.0 + mf * g;
+    const double der = 1.0 + 2.0 * df * q2 / (u * u * u) + 2.0 * mf * p2 / (w * w * w);
+    const double gb = gbar[i];
+    r[0] = gb / (der * u * u);
+    r[1] = gb / (der * w * w);
+    r[2] = -gb * 2.0 * q2 * g / (der * u * u * u);
+    r[3] = -gb * 2.0 * p2 * g / (der * w * w * w);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) out4[4 * i + k] = r[k];
+}
+
+extern "C" int bk_overstress_adjoint(const double* args4, const double* gbar, double* out4, int32_t n, void* stream) {
+  if (n < 0 || !args4 || !gbar || !out4) return fail("bk_overstress_adjoint: bad argument");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_overstress_adjoint, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, args4, gbar, out4, n);
+  return hipGetLastError() == hipSuccess ? 0 : fail("bk_overstress_adjoint: launch failed");
+}
+
+// Adjoint of UpdateDing for particles without pore gas (one thread each).
+__global__ __launch_bounds__(128) void k_update_ding_adjoint(DingParams D, const float* old_Fe, const float* old_porosity, const int32_t* old_gelled,
+                                      const float* F_total, const float* C_affine, const float* temperatures,
+                                      const float* viscosity_parameter, const double* adj_Fe, const double* adj_tau,
+                                      const double* adj_porosity, double* adj_old_Fe, double* adj_old_porosity, double* adj_F_total,
+                                      double* adj_C_affine, double* adj_viscosity_parameter, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double geta = 0.0;
+  if (p < n) {
+    double Fe_old[9], Fn[9], Cn[9], gFe[9], gtau[9], gFe_old[9], gF[9], gC[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      Fe_old[i] = old_Fe[9 * p + i];
+      Fn[i] = F_total[9 * p + i];
+      Cn[i] = C_affine[9 * p + i];
+      gFe[i] = adj_Fe ? adj_Fe[9 * p + i] : 0.0;
+      gtau[i] = adj_tau ? adj_tau[9 * p + i] : 0.0;
+      gFe_old[i] = gF[i] = gC[i] = 0.0;
+    }
+    double gpor = 0.0;
+    ding_return_adjoint(D, Fe_old, Fn, Cn, double(old_porosity[p]), old_gelled[p], double(temperatures[p]),
+                        double(viscosity_parameter[0]), gFe, gtau, adj_porosity ? adj_porosity[p] : 0.0, gFe_old, gF, gC, gpor, geta);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      if (adj_old_Fe) adj_old_Fe[9 * p + i] += gFe_old[i];
+      if (adj_F_total) adj_F_total[9 * p + i] += gF[i];
+      if (adj_C_affine) adj_C_affine[9 * p + i] += gC[i];
+    }
+    if (adj_old_porosity) adj_old_porosity[p] += gpor;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) geta += __shfl_xor(geta, off, 64);
+  if (adj_viscosity_parameter && __lane_id() == 0 && geta != 0.0) unsafeAtomicAdd(adj_viscosity_parameter, geta);
+}
+
+extern "C" int bk_UpdateDing_adjoint(const float* old_Fe, const float* old_porosity, const int32_t* old_gelled, const float* F_total,
+                                     const float* C_affine, const float* temperatures, const float* viscosity_parameter,
+                                     const double* adj_Fe, const double* adj_tau, const double* adj_porosity, double* adj_old_Fe,
+                                     double* adj_old_porosity, double* adj_F_total, double* adj_C_affine,
+                                     double* adj_viscosity_parameter, float mu, float lam, float yield_stress, float n0,
+                                     int32_t temperature_enabled, const float* knots4_host, const float* yield4_host,
+                                     const float* visc4_host, float dt, int32_t n, void* stream) {
+  if (n < 0) return fail("bk_UpdateDing_adjoint: negative count");
+  if (n == 0) return 0;
+  if (!old_Fe || !old_porosity || !old_gelled || !F_total || !C_affine || !temperatures || !viscosity_parameter)
+    return fail("bk_UpdateDing_adjoint: null input");
+  if (temperature_enabled && (!knots4_host || !yield4_host || !visc4_host))
+    return fail("bk_UpdateDing_adjoint: temperature tables missing");
+  DingParams D;
+  D.mu = mu; D.lam = lam; D.yield_stress = yield_stress; D.n0 = n0; D.dt = dt;
+  for (int i = 0; i < 4; ++i) {
+    D.knots[i] = temperature_enabled ? knots4_host[i] : 0.0f;
+    D.yf[i] = temperature_enabled ? yield4_host[i] : 1.0f;
+    D.vf[i] = temperature_enabled ? visc4_host[i] : 1.0f;
+  }
+  D.temperature_enabled = temperature_enabled;
+  D.coupled = 0;
+  D.bracketed = 0;
+  hipLaunchKernelGGL(k_update_ding_adjoint, dim3(div_up(n, 128)), dim3(128), 0, (hipStream_t)stream, D, old_Fe, old_porosity,
+                     old_gelled, F_total, C_affine, temperatures, viscosity_parameter, adj_Fe, adj_tau, adj_porosity, adj_old_Fe,
+                     adj_old_porosity, adj_F_total, adj_C_affine, adj_viscosity_parameter, n);
+  return hipGetLastError() == hipSuccess ? 0 : fail("bk_UpdateDing_adjoint: launch failed");
+}

@@ -362,3 +362,29 @@ def demo_heated(viscosity=1e8, yield_stress=100.0, dt=1e-4):
                      DingConfig(temperature_enabled=True, porosity=.2, yield_stress=yield_stress, viscosity=viscosity),
                      TransportConfig(enabled=True, water_enabled=False, initial_temperature_k=300.), steps=100)
     return strain_rate_field(spec, np.diag([1.5, -.75, .2]))
+
+
+def demo_autodiff(velocity=0.2):
+    """Elastic demo cube on a 20^3 grid (dx 0.05, dt 1e-3, no gravity) moving at ``velocity`` along x
+    (reference tests/test_warp_autodiff.py:24-33)."""
+    from .geometry.mesh_loader import CreateDemoMesh
+    from .geometry.particle_sampling import SampleParticles
+    mesh = CreateDemoMesh(1)
+    samples = SampleParticles(mesh, 1000.0, 4)
+    sim = SimulationConfig(grid_resolution=20, grid_spacing=0.05, time_step=0.001, gravity=(0.0, 0.0, 0.0), device_name="auto")
+    n = len(samples.particle_positions)
+    v = np.zeros((n, 3), np.float32)
+    v[:, 0] = velocity
+    return SceneSpec("demo_autodiff", mesh, samples, sim, MaterialConfig(), DingConfig(), TransportConfig(), steps=3, velocities=v)
+
+
+def demo_viscous_gradient(viscosity=1000.0, yield_stress=5.0, porosity=0.0):
+    """Ding demo cube (dt 1e-3, no gravity) in a diag(1.5, -0.75, 0.2) strain-rate flow
+    (reference tests/test_ding.py:142-151)."""
+    from .geometry.mesh_loader import CreateDemoMesh
+    from .geometry.particle_sampling import SampleParticles
+    mesh = CreateDemoMesh(1)
+    sim = SimulationConfig(time_step=0.001, gravity=(0.0, 0.0, 0.0), device_name="auto")
+    spec = SceneSpec("demo_viscous_gradient", mesh, SampleParticles(mesh, 1000.0, 4), sim, MaterialConfig(model="ding2019"),
+                     DingConfig(yield_stress=yield_stress, viscosity=viscosity, porosity=porosity), TransportConfig(), steps=5)
+    return strain_rate_field(spec, np.diag([1.5, -0.75, 0.2]))

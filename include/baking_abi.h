@@ -327,6 +327,35 @@ int bk_porosity_coefficient(const float* porosity, float* out, int32_t n, void* 
 int bk_solve_porosity(const float* args6, const double* plastic_volume, float* out_porosity, double* out_prediction,
                       int32_t solver, int32_t n, void* stream);
 
+/* ---------------------------------------------------------------- adjoint (reverse mode)
+ * Replaces the reference's wp.Tape over MpmSimulator.Step in history mode
+ * (physics/mpm.py:18-22, 163-225; tests/test_warp_autodiff.py, tests/test_ding.py:81-170):
+ * adjoints of one mechanical substep, neo-Hookean or Ding without pore gas and
+ * without transport.  Adjoint arrays are FP64 and particle-major like the state. */
+typedef struct bk_adjoint {
+  double *x, *v, *C, *F;      /* N x 3, N x 3, N x 9, N x 9 */
+  double *Fe, *tau, *porosity; /* Ding model only: N x 9, N x 9, N */
+} bk_adjoint;
+/* In: adj = the adjoint of the substep's output state `out`, the grid the
+ * forward substep from `in` left (mass, momentum, velocity; single dough,
+ * tiled nodes).  Out: adj = the adjoint of `in`; param_grad[0] += the
+ * gradient of the Ding viscosity parameter (scene.visc_param).  grid_scratch:
+ * 7 doubles per tiled node, cleared by the call. */
+int bk_adjoint_substep(const bk_params* params, const bk_scene* scene, const bk_state* in, const bk_state* out, const bk_grid* grid,
+                       const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream);
+/* Adjoint of UpdateDing without pore gas (physics/materials/ding.py:140-221; arguments as bk_UpdateDing):
+ * given adjoints of its outputs Fe, tau and porosity (any may be NULL = 0), ADDS the adjoints of
+ * old_Fe, old_porosity, F_total, C_affine (NULL = not wanted) and of the viscosity parameter (one double). */
+int bk_UpdateDing_adjoint(const float* old_Fe, const float* old_porosity, const int32_t* old_gelled, const float* F_total,
+                          const float* C_affine, const float* temperatures, const float* viscosity_parameter, const double* adj_Fe,
+                          const double* adj_tau, const double* adj_porosity, double* adj_old_Fe, double* adj_old_porosity,
+                          double* adj_F_total, double* adj_C_affine, double* adj_viscosity_parameter, float mu, float lam,
+                          float yield_stress, float n0, int32_t temperature_enabled, const float* knots4_host,
+                          const float* yield4_host, const float* visc4_host, float dt, int32_t n, void* stream);
+/* Implicit-function adjoint of the overstress solve (ding.py:90-100): args4 = (q2, p2, dev_factor,
+ * mean_factor) per element, gbar = adjoint of the overstress; out4 = adjoints of the four arguments. */
+int bk_overstress_adjoint(const double* args4, const double* gbar, double* out4, int32_t n, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,238 @@
+"""GPU checks of the reverse-mode path (``baking_amd.adjoint``): adjoint kernels behind a checkpointed autograd Function.
+
+Pins, each against the FP64 oracle (the same scene run in double precision):
+
+* the reference's own gradient tests, with their tolerances:
+  d mean(x_final - x0)_x / d v0 = 0.003 on the elastic demo cube
+  (tests/test_warp_autodiff.py:54-60, rtol 0.01); d mean(tau_xx) / d eta
+  through 5 MPM substeps of the Ding cube (tests/test_ding.py:142-170, rtol
+  0.03); the same through 4 single-point return maps (tests/test_ding.py:81-113,
+  rtol 0.02) -- each also against a tighter FP64 central difference;
+* the implicit-function adjoint of the overstress solve (ding.py:90-100)
+  against central differences of the oracle's FP64 overstress solve;
+* a random linear functional of the whole final state after 20 substeps
+  (checkpoints every 8, so three recompute segments) differentiated with
+  respect to 32 random coordinates of the initial state, against FP64
+  central differences (eps = 1e-5 of the coordinate's scale) of the oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.harness import OraclePoint, oracle_from_spec
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("needs a ROCm GPU", allow_module_level=True)
+
+from baking_amd import adjoint, scenes  # noqa: E402
+from baking_amd.config import DingConfig, MaterialConfig  # noqa: E402
+from baking_amd.physics.materials.ding import DingMaterial  # noqa: E402
+from baking_amd.physics.state import AllocateState  # noqa: E402
+
+
+def _oracle_run(spec, steps, setup=None):
+    o = oracle_from_spec(spec, "f64")
+    if setup is not None:
+        setup(o)
+    for _ in range(steps):
+        o.Step()
+    return o
+
+
+def test_velocity_gradient_closed_form():
+    """tests/test_warp_autodiff.py: 3 substeps of the elastic cube at v0 = 0.2 along x; the loss moves rigidly."""
+    spec = scenes.demo_autodiff(0.2)
+    sim = scenes.build_simulator(spec)
+    x0 = sim.current_state.particle_positions.clone()
+    n = x0.shape[0]
+    vp = torch.tensor([0.2], device="cuda", requires_grad=True)
+    v0 = vp * torch.tensor([1.0, 0.0, 0.0], device="cuda").expand(n, 3)
+    out = adjoint.rollout(sim, 3, {"v": v0})
+    loss = (out["x"][:, 0] - x0[:, 0]).mean()
+    loss.backward()
+    grad = float(vp.grad)
+    x0n = spec.samples.particle_positions.astype(np.float32).astype(np.float64)
+
+    def loss_at(value):
+        def setup(o):
+            o.state.v[...] = 0.0
+            o.state.v[:, 0] = value
+        o = _oracle_run(spec, 3, setup)
+        return float(np.mean(o.state.x[:, 0] - x0n[:, 0]))
+
+    fd_ref = (loss_at(0.3) - loss_at(0.1)) / 0.2
+    fd = (loss_at(0.2 + 1e-6) - loss_at(0.2 - 1e-6)) / 2e-6
+    print(f"gradient {grad:.9g}, oracle FD (eps 0.1) {fd_ref:.9g}, oracle FD (eps 1e-6) {fd:.9g}")
+    np.testing.assert_allclose(grad, fd_ref, rtol=0.01, atol=1e-6)
+    np.testing.assert_allclose(grad, 0.003, rtol=0.01, atol=1e-6)
+    np.testing.assert_allclose(grad, fd, rtol=1e-5)
+    # the simulator is left at the final state
+    assert sim.step_count == 3
+    torch.testing.assert_close(sim.current_state.particle_positions, out["x"].detach())
+
+
+@pytest.mark.parametrize("porosity", [0.0, 0.2])
+def test_mpm_viscosity_gradient(porosity):
+    """tests/test_ding.py:142-170: d mean(tau_xx) / d eta after 5 substeps of the yielding Ding cube."""
+    spec = scenes.demo_viscous_gradient(1000.0, porosity=porosity)
+    sim = scenes.build_simulator(spec)
+    eta = torch.tensor([1000.0], device="cuda", requires_grad=True)
+    out = adjoint.rollout(sim, 5, viscosity_parameter=eta)
+    loss = out["tau"][:, 0, 0].mean()
+    loss.backward()
+    grad = float(eta.grad)
+
+    def loss_at(value):
+        def setup(o):
+            o.visc_param[0] = value
+        o = _oracle_run(spec, 5, setup)
+        return float(np.mean(o.state.ding["tau"][:, 0, 0]))
+
+    fd_ref = (loss_at(1010.0) - loss_at(990.0)) / 20.0
+    fd = (loss_at(1000.0 + 1e-3) - loss_at(1000.0 - 1e-3)) / 2e-3
+    print(f"porosity {porosity}: gradient {grad:.9g}, oracle FD (eps 10) {fd_ref:.9g}, oracle FD (eps 1e-3) {fd:.9g}")
+    assert abs(fd_ref) > 1e-7
+    np.testing.assert_allclose(grad, fd_ref, rtol=0.03, atol=1e-7)
+    np.testing.assert_allclose(grad, fd, rtol=1e-3)
+
+
+def _point_gradient(viscosity):
+    """Four single-point return maps at prescribed total gradients, then the adjoint chain (tests/test_ding.py:81-100)."""
+    ding = DingConfig(yield_stress=500.0, viscosity=viscosity)
+    mc = MaterialConfig()
+    mat = DingMaterial(mc, ding, "cuda")
+    state = AllocateState(1, "cuda")
+    state.deformation_gradients.copy_(torch.eye(3, device="cuda")[None])
+    mat.InitializeState(state)
+    states = [state]
+    dt = 0.01
+    for k in range(4):
+        F = np.diag([1.15 + 0.01 * k, 0.93, 1.02])
+        out = AllocateState(1, "cuda")
+        inc = F @ np.linalg.inv(states[-1].deformation_gradients.cpu().numpy()[0].astype(np.float64))
+        out.deformation_gradients.copy_(torch.as_tensor(F, dtype=torch.float32)[None])
+        out.affine_velocities.copy_(torch.as_tensor((inc - np.eye(3)) / dt, dtype=torch.float32)[None])
+        mat.AdvanceState(states[-1], out, dt)
+        states.append(out)
+    loss = float(states[-1].material_state.kirchhoff_stresses[0, 0, 0])
+    z = lambda *s: torch.zeros(*s, dtype=torch.float64, device="cuda")  # noqa: E731
+    g_Fe, g_tau, g_n = z(9), z(9), z(1)
+    g_tau[0] = 1.0
+    g_eta = z(1)
+    for k in reversed(range(4)):
+        old, new = states[k].material_state, states[k + 1]
+        gFe_old, gn_old = z(9), z(1)
+        adjoint.UpdateDingAdjoint(1, old.elastic_gradients, old.porosities, old.gelled, new.deformation_gradients,
+                                  new.affine_velocities, old.temperatures, mat.viscosity_parameter, g_Fe, g_tau, g_n, gFe_old, gn_old,
+                                  z(9), z(9), g_eta, mc.shear_modulus, mc.lame_lambda, ding.yield_stress, ding.porosity,
+                                  int(ding.temperature_enabled), ding.temperature_knots_k, ding.yield_factors,
+                                  ding.viscosity_factors, dt)
+        g_Fe, g_tau, g_n = gFe_old, z(9), gn_old
+    return loss, float(g_eta)
+
+
+def test_point_viscosity_gradient():
+    """tests/test_ding.py:81-113 on one particle: d tau_xx / d eta at eta = 1e10."""
+    loss, grad = _point_gradient(1e10)
+
+    def loss_at(value):
+        p = OraclePoint(MaterialConfig(), DingConfig(yield_stress=500.0, viscosity=value), precision="f64")
+        for k in range(4):
+            s = p.step(np.diag([1.15 + 0.01 * k, 0.93, 1.02]))
+        return float(s["tau"][0, 0, 0])
+
+    fd_ref = (loss_at(1e10 + 1e8) - loss_at(1e10 - 1e8)) / 2e8
+    fd = (loss_at(1e10 + 1e4) - loss_at(1e10 - 1e4)) / 2e4
+    print(f"loss {loss:.6g}: gradient {grad:.9g}, oracle FD (eps 1e8) {fd_ref:.9g}, oracle FD (eps 1e4) {fd:.9g}")
+    assert abs(fd_ref) > 1e-12
+    np.testing.assert_allclose(grad, fd_ref, rtol=0.02, atol=1e-12)
+    np.testing.assert_allclose(grad, fd, rtol=1e-3)
+
+
+def test_overstress_adjoint_matches_oracle_differences():
+    rng = np.random.default_rng(5)
+    m = 64
+    args = np.stack([rng.uniform(1.2, 500.0, m), rng.uniform(0.0, 50.0, m), 10 ** rng.uniform(-2, 1, m),
+                     10 ** rng.uniform(-4, 0, m)], axis=1)
+    args[:4, :2] = [[0.3, 0.2], [0.5, 0.49], [0.0, 0.9], [0.1, 0.1]]  # inside the yield surface: zero adjoint
+    gbar = rng.normal(size=m)
+    out = torch.zeros(m * 4, dtype=torch.float64, device="cuda")
+    adjoint.OverstressAdjoint(m, torch.as_tensor(args.reshape(-1), device="cuda"), torch.as_tensor(gbar, device="cuda"), out)
+    got = out.cpu().numpy().reshape(m, 4)
+    solve = O.lib().oracle_overstress64
+    want = np.zeros((m, 4))
+    for i in range(m):
+        for k in range(4):
+            h = 1e-7 * max(abs(args[i, k]), 1e-3)
+            ap, am = args[i].copy(), args[i].copy()
+            ap[k] += h
+            am[k] -= h
+            want[i, k] = gbar[i] * (solve(*ap) - solve(*am)) / (2 * h)
+    assert np.all(got[:4] == 0.0)
+    np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-9 * np.abs(want).max())
+
+
+def _state_scales(fields):
+    return dict(x=0.05, v=0.5, C=2.0, F=0.1, Fe=0.1, porosity=0.05, tau=50.0)
+
+
+@pytest.mark.parametrize("case", ["neo_hookean", "ding", "ding_porous"])
+def test_rollout_gradient_matches_fp64_differences(case):
+    """32 random initial-state coordinates of a random linear functional of the state after 20 substeps."""
+    if case == "neo_hookean":
+        spec = scenes.strain_rate_field(scenes.scaled(scenes.demo_autodiff(0.0), time_step=5e-4), np.diag([1.0, -0.6, 0.3]))
+    else:
+        spec = scenes.demo_viscous_gradient(1000.0, porosity=0.2 if case == "ding_porous" else 0.0)
+        spec = scenes.scaled(spec, time_step=5e-4)
+    steps = 20
+    sim = scenes.build_simulator(spec)
+    fields = adjoint._fields_of(sim)
+    from baking_amd.physics.mpm import state_field_tensors
+    start = {f: t.detach().clone() for f, t in state_field_tensors(sim.current_state).items() if f in fields}
+    inputs = {f: start[f].clone().requires_grad_(True) for f in fields}
+    out = adjoint.rollout(sim, steps, inputs, checkpoint_every=8)
+    rng = np.random.default_rng(7)
+    scales = _state_scales(fields)
+    weights = {f: rng.normal(size=tuple(out[f].shape)) / scales[f] for f in fields}
+    loss = sum((out[f].double() * torch.as_tensor(weights[f], device="cuda")).sum() for f in fields)
+    loss.backward()
+    grads = {f: inputs[f].grad.double().cpu().numpy() for f in fields}
+
+    o_fields = dict(x=("x", None), v=("v", None), C=("C", None), F=("F", None), Fe=("Fe", "ding"), porosity=("porosity", "ding"),
+                    tau=("tau", "ding"))
+    start_np = {f: start[f].cpu().numpy().astype(np.float64) for f in fields}
+
+    def view(o, f):
+        name, part = o_fields[f]
+        return o.state.ding[name] if part else getattr(o.state, name)
+
+    def loss_at(field, index, delta):
+        def setup(o):
+            for f in fields:
+                view(o, f)[...] = start_np[f]
+            view(o, field)[index] += delta
+        o = _oracle_run(spec, steps, setup)
+        return sum(float((view(o, f) * weights[f]).sum()) for f in fields)
+
+    picks = []
+    n = start["x"].shape[0]
+    # with n0 = 0 a zero porosity stays 0 (ding.py:207-209): no neighbourhood to difference in
+    pickable = [f for f in fields if not (f == "porosity" and spec.ding.porosity == 0.0)]
+    for _ in range(32):
+        f = pickable[rng.integers(len(pickable))]
+        idx = (int(rng.integers(n)),) + tuple(int(rng.integers(s)) for s in start[f].shape[1:])
+        picks.append((f, idx))
+    rows = []
+    for f, idx in picks:
+        h = 1e-5 * scales[f]
+        fd = (loss_at(f, idx, h) - loss_at(f, idx, -h)) / (2 * h)
+        rows.append((f, idx, grads[f][idx], fd))
+    scale = max(abs(r[3]) for r in rows)
+    worst = max(abs(r[2] - r[3]) / (abs(r[3]) + 1e-3 * scale) for r in rows)
+    for r in rows:
+        print(f"{r[0]:9s} {str(r[1]):14s} adjoint {r[2]: .8e}  FD {r[3]: .8e}")
+    print(f"{case}: worst relative difference {worst:.2e} (floor 1e-3 of the largest gradient {scale:.3e})")
+    assert worst < 2e-3
