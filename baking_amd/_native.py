@@ -66,7 +66,7 @@ class BkGrid(ctypes.Structure):
     _fields_ = [(name, c_void_p) for name in GRID_FIELDS]
 
 
-ADJOINT_FIELDS = ["x", "v", "C", "F", "Fe", "tau", "porosity"]
+ADJOINT_FIELDS = ["x", "v", "C", "F", "Fe", "tau", "porosity", "moles"]
 
 
 class BkAdjoint(ctypes.Structure):
@@ -145,6 +145,7 @@ SIGNATURES = {
                             POINTER(BkAdjoint), P, P, P], c_int),
     "bk_UpdateDing_adjoint": ([P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, F, F, F, F, I, P, P, P, F, I, P], c_int),
     "bk_overstress_adjoint": ([P, P, P, I, P], c_int),
+    "bk_solve_porosity_adjoint": ([P, P, P, P, I, P], c_int),
 }
 
 _lib = None

@@ -11,12 +11,16 @@ and grid, then runs the library's adjoint kernels (``bk_adjoint_substep``:
 G2P with the F update and the Ding return map, the grid update, P2G with the
 neo-Hookean stress) substep by substep in reverse.  Adjoints are FP64.
 
-Covered: the mechanical substep with the neo-Hookean model, or the Ding model
-without pore gas; gradients with respect to the initial positions,
+Covered: the substep with the neo-Hookean model, or the Ding model with or
+without pore gas (the coupled porosity return and the gas pressure and
+source included); gradients with respect to the initial positions,
 velocities, affine velocities and deformation gradients, the Ding elastic
-gradients, porosities and Kirchhoff stresses, and the Ding viscosity
-parameter.  Transport (heat and water) and pore gas have no adjoint here;
-``rollout`` raises for them.
+gradients, porosities, Kirchhoff stresses and gas moles, and the Ding
+viscosity and gas production parameters.  Heat and water transport is
+differentiated only where it leaves temperature and water unchanged
+(conductivity, heat transfer and tray heat transfer 0, water off, as the
+reference's gas gradient test, tests/test_gas.py:120-160); ``rollout`` raises
+for transport that changes them.
 
     out = rollout(sim, steps, {"v": v0})          # tensors that may require grad
     loss = (out["x"][:, 0] - x0[:, 0]).mean()
@@ -44,18 +48,29 @@ OverstressAdjoint = Kernel("bk_overstress_adjoint", ["f64", "f64", "f64"])
 
 MECHANICAL_FIELDS = ("x", "v", "C", "F")
 DING_FIELDS = ("Fe", "porosity", "tau")
-_WIDTH = dict(x=3, v=3, C=9, F=9, Fe=9, tau=9, porosity=1)
+GAS_FIELDS = ("moles",)
+_WIDTH = dict(x=3, v=3, C=9, F=9, Fe=9, tau=9, porosity=1, moles=1)
+
+
+def _gas(sim):
+    return isinstance(sim.material, DingMaterial) and sim.material.ding_config.gas_enabled
 
 
 def _fields_of(sim):
-    return MECHANICAL_FIELDS + (DING_FIELDS if isinstance(sim.material, DingMaterial) else ())
+    if not isinstance(sim.material, DingMaterial):
+        return MECHANICAL_FIELDS
+    return MECHANICAL_FIELDS + DING_FIELDS + (GAS_FIELDS if _gas(sim) else ())
 
 
 def _check_supported(sim):
-    if getattr(sim, "transport", None) is not None:
-        raise NotImplementedError("rollout: heat and water transport have no adjoint in this library")
-    if isinstance(sim.material, DingMaterial) and sim.material.ding_config.gas_enabled:
-        raise NotImplementedError("rollout: the Ding model with pore gas has no adjoint in this library")
+    t = getattr(sim, "transport", None)
+    if t is not None:
+        c = t.config
+        frozen = (c.conductivity == 0.0 and c.heat_transfer == 0.0 and c.tray_heat_transfer == 0.0 and not c.water_enabled
+                  and float(t.heat_transfer_parameter.detach().cpu()[0]) == 0.0)
+        if not frozen:
+            raise NotImplementedError("rollout: heat and water transport is differentiated only where it leaves temperature and "
+                                      "water unchanged (conductivity, heat_transfer and tray_heat_transfer 0, water off)")
 
 
 def _clone_state(state):
@@ -93,7 +108,7 @@ def _run(sim, count):
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, sim, steps, every, fields, visc, *inputs):
+    def forward(ctx, sim, steps, every, fields, visc, prod, *inputs):
         lib = _native.load()
         state = state_field_tensors(sim.current_state)
         for f, t in zip(fields, inputs):
@@ -101,6 +116,8 @@ class _Rollout(torch.autograd.Function):
         ding = isinstance(sim.material, DingMaterial)
         if ding:
             sim.material.viscosity_parameter.copy_(visc.detach().reshape(1).to(torch.float32))
+        if _gas(sim):
+            sim.material.production_parameter.copy_(prod.detach().reshape(1).to(torch.float32))
         flags_before = sim._read_flags()
         checkpoints = []
         for start in range(0, steps, every):
@@ -114,7 +131,7 @@ class _Rollout(torch.autograd.Function):
         ctx.sim, ctx.steps, ctx.every, ctx.fields, ctx.checkpoints = sim, steps, every, fields, checkpoints
         ctx.shapes = [t.shape for t in inputs]
         ctx.dtypes = [t.dtype for t in inputs]
-        ctx.visc_shape, ctx.visc_dtype = visc.shape, visc.dtype
+        ctx.params = [(visc.shape, visc.dtype), (prod.shape, prod.dtype)]
         ctx.lib = lib
         final = state_field_tensors(sim.current_state)
         return tuple(final[f].clone() for f in fields)
@@ -131,7 +148,7 @@ class _Rollout(torch.autograd.Function):
         bk_adj = _native.BkAdjoint()
         for f, t in adj.items():
             setattr(bk_adj, f, t.data_ptr())
-        param = torch.zeros(1, dtype=torch.float64, device=dev)
+        param = torch.zeros(2, dtype=torch.float64, device=dev)
         nodes = sim.grid_state.tiled_masses.numel()
         scratch = torch.empty(7 * nodes, dtype=torch.float64, device=dev)
         # the simulator is left where the forward ended
@@ -156,18 +173,19 @@ class _Rollout(torch.autograd.Function):
         _load_state(sim.current_state, final)
         sim._write_flags(flags)
         out = [adj[f].reshape(shape).to(dtype) for f, shape, dtype in zip(fields, ctx.shapes, ctx.dtypes)]
-        visc_grad = param.reshape(ctx.visc_shape).to(ctx.visc_dtype) if ctx.visc_shape.numel() else None
-        return (None, None, None, None, visc_grad, *out)
+        pgrads = [param[k:k + 1].reshape(shape).to(dtype) if shape.numel() else None
+                  for k, (shape, dtype) in enumerate(ctx.params)]
+        return (None, None, None, None, *pgrads, *out)
 
 
-def rollout(sim, steps, inputs=None, viscosity_parameter=None, checkpoint_every=16):
+def rollout(sim, steps, inputs=None, viscosity_parameter=None, production_parameter=None, checkpoint_every=16):
     """Run ``steps`` substeps of ``sim`` from ``inputs`` and return the final state as differentiable tensors.
 
-    ``inputs`` maps state fields (``x, v, C, F`` and, for the Ding model, ``Fe, porosity, tau``) to tensors of
-    the state's shapes (N x 3, N x 3, N x 3 x 3, ...); fields left out start from the simulator's current
-    state.  ``viscosity_parameter`` (Ding model, a one-element tensor) replaces the material's
-    ``viscosity_parameter``.  Returns {field: tensor} for every field; the simulator is left at the final
-    state, as after ``Advance(steps)``.
+    ``inputs`` maps state fields (``x, v, C, F``; for the Ding model also ``Fe, porosity, tau``, and ``moles``
+    with pore gas) to tensors of the state's shapes (N x 3, N x 3, N x 3 x 3, ...); fields left out start
+    from the simulator's current state.  ``viscosity_parameter`` (Ding model) and ``production_parameter``
+    (pore gas), one-element tensors, replace the material's parameters.  Returns {field: tensor} for every
+    field; the simulator is left at the final state, as after ``Advance(steps)``.
     """
     if steps < 1:
         raise ValueError("steps must be positive")
@@ -188,11 +206,18 @@ def rollout(sim, steps, inputs=None, viscosity_parameter=None, checkpoint_every=
         elif t.numel() != current[f].numel():
             raise ValueError(f"rollout: input {f} has {t.numel()} elements, the state has {current[f].numel()}")
         tensors.append(t)
+    dev = current["x"].device
     if isinstance(sim.material, DingMaterial):
         visc = viscosity_parameter if viscosity_parameter is not None else sim.material.viscosity_parameter.detach().clone()
     else:
         if viscosity_parameter is not None:
             raise ValueError("rollout: viscosity_parameter needs the Ding model")
-        visc = torch.zeros(0, device=current["x"].device)
-    outs = _Rollout.apply(sim, int(steps), int(checkpoint_every), fields, visc, *tensors)
+        visc = torch.zeros(0, device=dev)
+    if _gas(sim):
+        prod = production_parameter if production_parameter is not None else sim.material.production_parameter.detach().clone()
+    else:
+        if production_parameter is not None:
+            raise ValueError("rollout: production_parameter needs the Ding model with pore gas")
+        prod = torch.zeros(0, device=dev)
+    outs = _Rollout.apply(sim, int(steps), int(checkpoint_every), fields, visc, prod, *tensors)
     return dict(zip(fields, outs))

@@ -101,16 +101,61 @@ __device__ __forceinline__ Dual dexp(const Dual& a) {
   return r;
 }
 
-// ------------------------------------------------------------------ principal part of the return map
-// From the principal stretches s, the porosity coefficient a and the viscosity
-// eta: the corrected principal Kirchhoff stresses cs and the returned principal
-// stretches e (ding.py:103-131), with their derivatives.
-struct Principal {
-  Dual cs[3], e[3];
+// ------------------------------------------------------------------ coupled porosity root (porous_return.py:64-100)
+// Derivatives of the converged end-state porosity n (the root of
+// n = prediction(n), porous_return.py:34-40) with respect to q, p,
+// dev_factor, mean_scale, exponent and plastic_volume (initial_porosity: 1/R'),
+// by the implicit function theorem on that equation, as the reference's adjoint.
+struct PorosityDerivs {
+  double q, p, df, ms, ex, pv, n0;
 };
-__device__ inline Principal principal_return(const double s[3], double a, double eta, double mu, double lam, double y, double dt) {
+__device__ inline PorosityDerivs porosity_root_derivatives(double n, double q, double p, double d, double h, double ex, double pv) {
+  const double a = pressure_coefficient64(n);
+  double da = 0.0;
+  if (n > 0.0) {
+    const double lg = log(1.0 / n);
+    da = a * (2.0 / (n * lg) - 3.5 / (3.5 * n + 1.0));
+  }
+  const double g = overstress64(q, a * p, d, h * a);
+  const double u = 1.0 + d * g, v = 1.0 + h * a * g;
+  const double der = 1.0 + 2.0 * d * q / (u * u * u) + 2.0 * h * a * a * p / (v * v * v);
+  double r_q = 0.0, r_p = 0.0, r_d = 0.0, r_h = 0.0, r_a = 0.0;
+  if (q + a * p > 1.0) {
+    r_q = 1.0 / (der * u * u);
+    r_p = a / (der * v * v);
+    r_d = -2.0 * q * g / (der * u * u * u);
+    r_h = -2.0 * a * a * p * g / (der * v * v * v);
+    r_a = p * (1.0 - h * a * g) / (der * v * v * v);
+  }
+  const double expo = exp(ex * (1.0 - 1.0 / v));
+  const double nv = pv * expo;
+  const double pref = nv * ex / (v * v);
+  const double s = 1.0 / (1.0 - pref * h * da * (g + a * r_a));
+  PorosityDerivs D;
+  D.q = s * pref * h * a * r_q;
+  D.p = s * pref * h * a * r_p;
+  D.df = s * pref * h * a * r_d;
+  D.ms = s * pref * (a * g + h * a * r_h);
+  D.ex = s * nv * (1.0 - 1.0 / v);
+  D.pv = s * expo;
+  D.n0 = s;
+  return D;
+}
+
+// ------------------------------------------------------------------ principal part of the return map
+// From the principal stretches s, the old porosity and the viscosity eta: the
+// corrected principal Kirchhoff stresses cs, the returned principal stretches
+// e and the porosity n the map used (ding.py:103-131, 183-192), with their
+// derivatives along (s0, s1, s2, old porosity, eta).  Non-coupled, n is the
+// old porosity; coupled, the end-state root (porous_return.py:43-61) with the
+// derivatives above.
+struct Principal {
+  Dual cs[3], e[3], n;
+};
+__device__ inline Principal principal_return(const double s[3], double n_old, double eta, double mu, double lam, double y, double dt,
+                                             bool coupled, float n0, bool bracketed) {
   Dual sv[3] = {dvar(s[0], 0), dvar(s[1], 1), dvar(s[2], 2)};
-  const Dual ad = dvar(a, 3), ev = dvar(eta, 4);
+  const Dual ev = dvar(eta, 4);
   Dual eps[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) eps[i] = dlog(sv[i]);
@@ -124,9 +169,33 @@ __device__ inline Principal principal_return(const double s[3], double a, double
   for (int i = 0; i < 3; ++i) dev[i] = ts[i] - mean;
   const double y2 = y * y;
   const Dual q2 = (1.5 / y2) * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
-  const Dual p2 = (1.0 / y2) * (ad * mean * mean);
+  const Dual pm = (1.0 / y2) * (mean * mean);
   const Dual df = dconst(6.0 * mu * dt * y2) / ev;
-  const Dual mf = ((2.0 / 3.0) * (2.0 * mu + 3.0 * lam) * dt * y2) * ad / ev;
+  const Dual ms = dconst((2.0 / 3.0) * (2.0 * mu + 3.0 * lam) * dt * y2) / ev;
+  Principal P;
+  P.n = dvar(n_old, 3);
+  if (coupled) {
+    const Dual ex = (3.0 / (2.0 * mu + 3.0 * lam)) * mean;
+    const double opv = 1.0 - double(n0) + n_old;
+    const float root = bracketed ? solve_porosity_bracketed(float(q2.v), float(pm.v), float(df.v), float(ms.v), float(ex.v), opv, n0)
+                                 : solve_porosity(float(q2.v), float(pm.v), float(df.v), float(ms.v), float(ex.v), opv, n0);
+    const PorosityDerivs D = porosity_root_derivatives(double(root), q2.v, pm.v, df.v, ms.v, ex.v, opv);
+    P.n.v = root;
+#pragma unroll
+    for (int k = 0; k < kDir; ++k)
+      P.n.d[k] = D.q * q2.d[k] + D.p * pm.d[k] + D.df * df.d[k] + D.ms * ms.d[k] + D.ex * ex.d[k] + (k == 3 ? D.pv : 0.0);
+  }
+  // porosity coefficient a(n) (ding.py:68-74)
+  Dual ad = dconst(0.0);
+  if (P.n.v > 0.0) {
+    const double nn = P.n.v, lg = log(1.0 / nn);
+    ad.v = 9.0 / (4.0 * lg * lg * (3.5 * nn + 1.0));
+    const double da = ad.v * (2.0 / (nn * lg) - 3.5 / (3.5 * nn + 1.0));
+#pragma unroll
+    for (int k = 0; k < kDir; ++k) ad.d[k] = da * P.n.d[k];
+  }
+  const Dual p2 = ad * pm;
+  const Dual mf = ms * ad;
   // overstress: converged Newton on R(g) = g + 1 - q2/(1+df g)^2 - p2/(1+mf g)^2 = 0, derivative by the IFT
   Dual g = dconst(0.0);
   if (q2.v + p2.v > 1.0) {
@@ -151,7 +220,6 @@ __device__ inline Principal principal_return(const double s[3], double a, double
   }
   const Dual one = dconst(1.0);
   const Dual dden = one + df * g, mden = one + mf * g;
-  Principal P;
 #pragma unroll
   for (int i = 0; i < 3; ++i) P.cs[i] = dev[i] / dden + mean / mden;
   const Dual cm = (1.0 / 3.0) * (P.cs[0] + P.cs[1] + P.cs[2]);
@@ -204,10 +272,13 @@ __device__ __forceinline__ void minv_t(const double* a, double* r) {
   r[8] = (a[0] * a[4] - a[1] * a[3]) / d;
 }
 
-// Adjoint of the non-coupled Ding return map (ding.py:140-221 with gas off):
-// outputs Fe, tau and the porosity det(Fe^-1 F) - 1 + n0, adjoints gFe, gtau, gn.
-// Adds to gFe_old, gF (the new total gradient), gC (the new affine velocity),
-// gporosity (the old porosity) and geta0 (the viscosity parameter).
+// Adjoint of the Ding return map (ding.py:140-221).  Outputs Fe, tau and the
+// porosity, with adjoints gFe, gtau, gn.  Non-coupled: the porosity output is
+// det(Fe^-1 F) - 1 + n0 (constant 0 when n0 = 0 and the old porosity is 0) and
+// Fe takes the isochoric fit when a = 0; coupled: the porosity output is the
+// end-state root and Fe is rescaled to det F / (1 - n0 + n).  Adds to gFe_old,
+// gF (the new total gradient), gC (the new affine velocity), gporosity (the
+// old porosity) and geta0 (the viscosity parameter).
 __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe_old, const double* F, const double* C, double porosity,
                                            int gelled_in, double T, double visc_param, const double* gFe_in, const double* gtau,
                                            double gn, double* gFe_old, double* gF, double* gC, double& gporosity, double& geta0) {
@@ -233,37 +304,36 @@ __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe
   svd3(Tm, Um, s, Vm);
   const double* U = Um.a;
   const double* V = Vm.a;
-  // porosity coefficient and its derivative (ding.py:68-74; non-coupled: the old porosity)
-  double a = 0.0, da = 0.0;
-  if (porosity > 0.0) {
-    const double lg = log(1.0 / porosity);
-    a = 9.0 / (4.0 * lg * lg * (3.5 * porosity + 1.0));
-    da = a * (2.0 / (porosity * lg) - 3.5 / (3.5 * porosity + 1.0));
-  }
-  const Principal pr = principal_return(s, a, ev, mu, lam, yv, dt);
+  const bool coupled = P.coupled != 0;
+  const Principal pr = principal_return(s, porosity, ev, mu, lam, yv, dt, coupled, P.n0, P.bracketed != 0);
   double e[3], cs[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     e[i] = pr.e[i].v;
     cs[i] = pr.cs[i].v;
   }
-  // Fe0 = U diag(e) V^T, and the output Fe (isochoric fit when a == 0)
+  // Fe0 = U diag(e) V^T, and the output Fe = c Fe0
   double Fe0[9], tmp[9];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) tmp[i * 3 + j] = U[i * 3 + j] * e[j];
   mmul_nt(tmp, V, Fe0);
-  double Fe[9], c = 1.0;
-  const double det_trial = mdet(trial), det_fe0 = mdet(Fe0);
-  if (a == 0.0) c = cbrt(det_trial / det_fe0);
+  const bool isochoric = !coupled && pr.n.v <= 0.0;  // a = 0
+  const double target = 1.0 - double(P.n0) + pr.n.v;
+  double c = 1.0;
+  if (coupled) c = cbrt(mdet(F) / (target * mdet(Fe0)));
+  else if (isochoric) c = cbrt(mdet(trial) / mdet(Fe0));
+  double Fe[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) Fe[i] = c * Fe0[i];
   double gFe[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) gFe[i] = gFe_in[i];
-  // porosity output: n = det F / det Fe - 1 + n0 (constant 0 when n0 = 0 and the old porosity is 0)
-  if (gn != 0.0 && !(P.n0 == 0.0f && porosity == 0.0)) {
+  double gnroot = 0.0;  // adjoint of the porosity the map used
+  if (coupled) {
+    gnroot += gn;
+  } else if (gn != 0.0 && !(P.n0 == 0.0f && porosity == 0.0)) {
     const double ratio = mdet(F) / mdet(Fe);
     double fit[9], feit[9];
     minv_t(F, fit);
@@ -274,20 +344,23 @@ __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe
       gFe[i] -= gn * ratio * feit[i];
     }
   }
-  // isochoric fit
+  // the volume fit: c = (det X / det Fe0)^(1/3), X = trial (isochoric) or F / target (coupled)
   double gFe0[9], gTrial[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (a == 0.0) {
+  if (coupled || isochoric) {
     double cbar = 0.0;
 #pragma unroll
     for (int i = 0; i < 9; ++i) cbar += gFe[i] * Fe0[i];
-    double tit[9], fe0it[9];
-    minv_t(trial, tit);
+    const double k3 = cbar * c / 3.0;
+    double xit[9], fe0it[9];
+    minv_t(coupled ? F : trial, xit);
     minv_t(Fe0, fe0it);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-      gFe0[i] = c * gFe[i] - cbar * c / 3.0 * fe0it[i];
-      gTrial[i] += cbar * c / 3.0 * tit[i];
+      gFe0[i] = c * gFe[i] - k3 * fe0it[i];
+      if (coupled) gF[i] += k3 * xit[i];
+      else gTrial[i] += k3 * xit[i];
     }
+    if (coupled) gnroot -= k3 / target;
   } else {
 #pragma unroll
     for (int i = 0; i < 9; ++i) gFe0[i] = gFe[i];
@@ -304,6 +377,8 @@ __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int k = 0; k < kDir; ++k) gscal[k] += Ne[i * 4] * pr.e[i].d[k] + Nt[i * 4] * pr.cs[i].d[k];
+#pragma unroll
+  for (int k = 0; k < kDir; ++k) gscal[k] += gnroot * pr.n.d[k];
 #pragma unroll
   for (int i = 0; i < 3; ++i) M[i * 4] = gscal[i];
 #pragma unroll
@@ -323,7 +398,7 @@ __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe
   mmul_nt(UM, V, G);
 #pragma unroll
   for (int i = 0; i < 9; ++i) gTrial[i] += G[i];
-  gporosity += gscal[3] * da;
+  gporosity += gscal[3];
   geta0 += gscal[4] * vfac;
   // trial = (I + dt C) Fe_old
   double t2[9];
@@ -353,11 +428,12 @@ struct AdjArgs {
   const float* masses;      // particle masses P2G used
   const float* V0;
   const float* visc_param;  // per dough (one value)
+  const float* prod_param;  // gas production parameter (one value)
   bk_adjoint adj;
   double* gm;  // grid adjoints (tiled nodes): mass, momentum (3), velocity (3)
   double* gp;
   double* gv;
-  double* param_grad;  // [0] viscosity parameter
+  double* param_grad;  // [0] viscosity parameter, [1] gas production parameter
   int n, nb;
 };
 
@@ -387,8 +463,8 @@ __global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
       D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
       for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
       D.temperature_enabled = P.temperature_enabled;
-      D.coupled = 0;
-      D.bracketed = 0;
+      D.coupled = P.gas_enabled;
+      D.bracketed = P.return_solver;
       double Fe_old[9], Fn[9], Cn[9], gFe[9], gtau[9];
 #pragma unroll
       for (int i = 0; i < 9; ++i) {
@@ -398,7 +474,8 @@ __global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
         gFe[i] = a.adj.Fe[9 * p + i];
         gtau[i] = a.adj.tau[9 * p + i];
       }
-      ding_return_adjoint(D, Fe_old, Fn, Cn, double(a.in.porosity[p]), a.in.gelled[p], double(a.in.T_mat[p]), double(a.visc_param[0]),
+      const float T_local = P.transport_enabled ? a.out.T_mat[p] : a.in.T_mat[p];
+      ding_return_adjoint(D, Fe_old, Fn, Cn, double(a.in.porosity[p]), a.in.gelled[p], double(T_local), double(a.visc_param[0]),
                           gFe, gtau, a.adj.porosity[p], gFe_old, gF, gC, gpor, geta);
     }
     // G2P (transfer.py:73-109)
@@ -508,11 +585,11 @@ __global__ __launch_bounds__(256) void k_adj_grid(AdjArgs a, int64_t nodes) {
   a.gm[node] = gmass;
 }
 
-// Per particle: P2G's adjoint (transfer.py:36-70) and, for the neo-Hookean model,
-// the stress's (elastic.py:4-10); completes the input state's adjoint.
-__global__ __launch_bounds__(128) void k_adj_p2g(AdjArgs a) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
+// Particle p of P2G's adjoint (transfer.py:36-70) with the stress's: neo-Hookean
+// (elastic.py:4-10), or the Ding state's plus the pore-gas pressure term
+// (gas.py:16-35) and the gas source (thermal_water.py:296-330).  Completes the
+// input state's adjoint; returns p's share of the production-parameter gradient.
+__device__ inline double adj_p2g_particle(const AdjArgs& a, int p) {
   const bk_params& P = a.P;
   const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
   const float dx = P.dx;
@@ -520,22 +597,38 @@ __global__ __launch_bounds__(128) void k_adj_p2g(AdjArgs a) {
   grid_position(a.in.x + 3 * p, o, dx, g3);
   int b[3];
   stencil_base(g3, b);
-  if (!valid_stencil(b, P.res)) return;
+  if (!valid_stencil(b, P.res)) return 0.0;
   const float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
   double wx[3], wy[3], wz[3], dwx[3], dwy[3], dwz[3];
   axis_weight_grads(l[0], wx, dwx);
   axis_weight_grads(l[1], wy, dwy);
   axis_weight_grads(l[2], wz, dwz);
-  const double mass = a.masses[p];
+  const double mass = P.transport_enabled ? double(a.out.masses[p]) : double(a.masses[p]);
   const double s_tau = double(P.dt) * double(a.V0[p]) * 4.0 / (double(dx) * double(dx));
   // the stress P2G used: the state's for the Ding model, mu (F F^T - I) + lam log(J) I of the input F otherwise
   double tau[9], F[9];
   bool stressed = true;
 #pragma unroll
   for (int i = 0; i < 9; ++i) F[i] = a.in.F[9 * p + i];
+  // pore gas: tau += J (p_ambient - p) I = (J p_ambient - N R T / n) I, N the substep's new moles
+  bool gas = false;
+  double J = 0.0, moles = 0.0, Tg = 0.0, nv = 0.0;
   if (P.model == 1) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) tau[i] = a.in.tau[9 * p + i];
+    if (P.gas_enabled) {
+      nv = a.in.porosity[p];
+      J = mdet(F);
+      moles = a.out.moles[p];
+      Tg = P.transport_enabled ? a.out.T_mat[p] : a.in.T_mat[p];
+      gas = nv > 0.0 && nv < 1.0 && J > 0.0;
+      if (gas) {
+        const double add = J * double(P.p_ambient) - moles * kGasConstant * Tg / nv;
+        tau[0] += add;
+        tau[4] += add;
+        tau[8] += add;
+      }
+    }
   } else {
     const double J = mdet(F);
     stressed = isfinite(J) && J > 0.0;
@@ -585,9 +678,28 @@ __global__ __launch_bounds__(128) void k_adj_p2g(AdjArgs a) {
   double gtau[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) gtau[i] = -s_tau * gA[i];
+  double gprod = 0.0;
   if (P.model == 1) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) a.adj.tau[9 * p + i] = gtau[i];
+    if (P.gas_enabled) {
+      double gmoles = a.adj.moles[p];  // adjoint of the new moles
+      if (gas) {
+        const double tr = gtau[0] + gtau[4] + gtau[8];
+        double fit[9];
+        minv_t(F, fit);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) a.adj.F[9 * p + i] += tr * double(P.p_ambient) * J * fit[i];
+        a.adj.porosity[p] += tr * moles * kGasConstant * Tg / (nv * nv);
+        gmoles -= tr * kGasConstant * Tg / nv;
+      }
+      // N_new = N_old + dt rho alpha exp(-z^2), z = ((T_old + T_new) / 2 - T_peak) / width
+      a.adj.moles[p] = gmoles;
+      if (P.transport_enabled && !P.settling) {
+        const double z = (0.5 * (a.in.T[p] + a.out.T[p]) - double(P.gas_peak)) / double(P.gas_width);
+        gprod = gmoles * double(P.dt * P.density) * exp(-(z * z));
+      }
+    }
   } else if (stressed) {
     // neo-Hookean Kirchhoff stress mu (F F^T - I) + lam log(J) I: gF += mu (gtau + gtau^T) F + lam tr(gtau) F^-T
     double fit[9], sym[9], t[9];
@@ -601,6 +713,15 @@ __global__ __launch_bounds__(128) void k_adj_p2g(AdjArgs a) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) a.adj.F[9 * p + i] += double(P.mu) * t[i] + double(P.lam) * tr * fit[i];
   }
+  return gprod;
+}
+
+__global__ __launch_bounds__(128) void k_adj_p2g(AdjArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double gprod = p < a.n ? adj_p2g_particle(a, p) : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) gprod += __shfl_xor(gprod, off, 64);
+  if (__lane_id() == 0 && gprod != 0.0) unsafeAtomicAdd(a.param_grad + 1, gprod);
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -609,8 +730,11 @@ extern "C" {
 int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, const bk_state* out, const bk_grid* grid,
                        const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream) {
   if (!P || !sc || !in || !out || !grid || !adj || !grid_scratch || !param_grad) return fail("bk_adjoint_substep: null argument");
-  if (P->transport_enabled || (P->model == 1 && P->gas_enabled))
-    return fail("bk_adjoint_substep: the adjoint covers the mechanical substep without transport or pore gas");
+  if (P->transport_enabled && (P->kappa != 0.0f || P->tray_h != 0.0f || P->water_enabled || P->settling))
+    return fail("bk_adjoint_substep: transport is differentiated only where it leaves temperature and water unchanged "
+                "(conductivity, tray and heat transfer 0, water off)");
+  if (P->model == 1 && P->gas_enabled && (!P->transport_enabled || !in->moles || !out->moles || !adj->moles || !sc->prod_param))
+    return fail("bk_adjoint_substep: the pore-gas adjoint needs transport, moles and the production parameter");
   if (P->n < 1 || P->res < 4) return fail("bk_adjoint_substep: bad sizes");
   hipStream_t st = (hipStream_t)stream;
   AdjArgs a;
@@ -621,6 +745,7 @@ int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* i
   a.masses = sc->m0;
   a.V0 = sc->V0;
   a.visc_param = sc->visc_param;
+  a.prod_param = sc->prod_param;
   a.adj = *adj;
   a.nb = (P->res + 3) / 4;
   a.n = P->n;
@@ -739,4 +864,26 @@ extern "C" int bk_UpdateDing_adjoint(const float* old_Fe, const float* old_poros
                      old_gelled, F_total, C_affine, temperatures, viscosity_parameter, adj_Fe, adj_tau, adj_porosity, adj_old_Fe,
                      adj_old_porosity, adj_F_total, adj_C_affine, adj_viscosity_parameter, n);
   return hipGetLastError() == hipSuccess ? 0 : fail("bk_UpdateDing_adjoint: launch failed");
+}
+
+// Adjoint of the coupled porosity solve alone (porous_return.py:64-100), at the bracketed solve's root.
+__global__ __launch_bounds__(256) void k_solve_porosity_adjoint(const float* args6, const double* pv, const double* gbar, double* out7, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* q = args6 + 6 * i;
+  const float root = solve_porosity_bracketed(q[0], q[1], q[2], q[3], q[4], pv[i], q[5]);
+  const PorosityDerivs D = porosity_root_derivatives(double(root), q[0], q[1], q[2], q[3], q[4], pv[i]);
+  const double g = gbar[i];
+  const double r[7] = {D.q, D.p, D.df, D.ms, D.ex, D.n0, D.pv};
+#pragma unroll
+  for (int k = 0; k < 7; ++k) out7[7 * i + k] = g * r[k];
+}
+
+extern "C" int bk_solve_porosity_adjoint(const float* args6, const double* plastic_volume, const double* gbar, double* out7, int32_t n,
+                                         void* stream) {
+  if (n < 0 || (n > 0 && (!args6 || !plastic_volume || !gbar || !out7))) return fail("bk_solve_porosity_adjoint: bad argument");
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_solve_porosity_adjoint, dim3(div_up(n, 256)), dim3(256), 0, (hipStream_t)stream, args6, plastic_volume, gbar,
+                     out7, n);
+  return hipGetLastError() == hipSuccess ? 0 : fail("bk_solve_porosity_adjoint: launch failed");
 }

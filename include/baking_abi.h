@@ -329,17 +329,23 @@ int bk_solve_porosity(const float* args6, const double* plastic_volume, float* o
 
 /* ---------------------------------------------------------------- adjoint (reverse mode)
  * Replaces the reference's wp.Tape over MpmSimulator.Step in history mode
- * (physics/mpm.py:18-22, 163-225; tests/test_warp_autodiff.py, tests/test_ding.py:81-170):
- * adjoints of one mechanical substep, neo-Hookean or Ding without pore gas and
- * without transport.  Adjoint arrays are FP64 and particle-major like the state. */
+ * (physics/mpm.py:18-22, 163-225; tests/test_warp_autodiff.py, tests/test_ding.py:81-170,
+ * tests/test_gas.py:131-160): adjoints of one substep with the neo-Hookean or
+ * the Ding model, the latter with pore gas and the coupled porosity return.
+ * Transport is differentiated only where it leaves temperature and water
+ * unchanged (conductivity, tray and heat transfer 0, water off -- the caller
+ * guarantees the zero heat-transfer parameter).  Adjoint arrays are FP64 and
+ * particle-major like the state. */
 typedef struct bk_adjoint {
   double *x, *v, *C, *F;      /* N x 3, N x 3, N x 9, N x 9 */
   double *Fe, *tau, *porosity; /* Ding model only: N x 9, N x 9, N */
+  double* moles;               /* Ding with pore gas only: N */
 } bk_adjoint;
 /* In: adj = the adjoint of the substep's output state `out`, the grid the
  * forward substep from `in` left (mass, momentum, velocity; single dough,
  * tiled nodes).  Out: adj = the adjoint of `in`; param_grad[0] += the
- * gradient of the Ding viscosity parameter (scene.visc_param).  grid_scratch:
+ * gradient of the Ding viscosity parameter (scene.visc_param), param_grad[1]
+ * += that of the gas production parameter (scene.prod_param).  grid_scratch:
  * 7 doubles per tiled node, cleared by the call. */
 int bk_adjoint_substep(const bk_params* params, const bk_scene* scene, const bk_state* in, const bk_state* out, const bk_grid* grid,
                        const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream);
@@ -355,6 +361,11 @@ int bk_UpdateDing_adjoint(const float* old_Fe, const float* old_porosity, const 
 /* Implicit-function adjoint of the overstress solve (ding.py:90-100): args4 = (q2, p2, dev_factor,
  * mean_factor) per element, gbar = adjoint of the overstress; out4 = adjoints of the four arguments. */
 int bk_overstress_adjoint(const double* args4, const double* gbar, double* out4, int32_t n, void* stream);
+/* Implicit-function adjoint of the coupled porosity solve (porous_return.py:64-100): arguments as
+ * bk_solve_porosity, gbar = adjoint of the solved porosity; out7 = adjoints of (q, p, dev_factor,
+ * mean_scale, exponent, initial_porosity, plastic_volume) at the bracketed solve's root. */
+int bk_solve_porosity_adjoint(const float* args6, const double* plastic_volume, const double* gbar, double* out7, int32_t n,
+                              void* stream);
 
 #ifdef __cplusplus
 }

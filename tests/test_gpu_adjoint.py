@@ -8,8 +8,13 @@ Pins, each against the FP64 oracle (the same scene run in double precision):
   through 5 MPM substeps of the Ding cube (tests/test_ding.py:142-170, rtol
   0.03); the same through 4 single-point return maps (tests/test_ding.py:81-113,
   rtol 0.02) -- each also against a tighter FP64 central difference;
-* the implicit-function adjoint of the overstress solve (ding.py:90-100)
-  against central differences of the oracle's FP64 overstress solve;
+* the reference's gas-production gradient, d mean(det F) / d alpha through
+  20 substeps of the gas-producing porous cube (tests/test_gas.py:120-160,
+  rtol 0.05), with the coupled porosity return and the pore-gas stress;
+* the implicit-function adjoints of the overstress solve (ding.py:90-100) and
+  of the coupled porosity solve (porous_return.py:64-100) against central
+  differences of the oracle's FP64 overstress and of the FP64 root of its
+  porosity-consistency equation (porous_return.py:34-40);
 * a random linear functional of the whole final state after 20 substeps
   (checkpoints every 8, so three recompute segments) differentiated with
   respect to 32 random coordinates of the initial state, against FP64
@@ -175,15 +180,98 @@ def test_overstress_adjoint_matches_oracle_differences():
     np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-9 * np.abs(want).max())
 
 
+def _porosity_root(q, p, df, ms, ex, pv, n0):
+    """FP64 root of n = prediction(n) (the oracle's porous_return.py:34-40) by 200 bisection steps."""
+    lo, hi = 0.0, 1.0 - 1e-9
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if mid < O.porosity_prediction(mid, q, p, df, ms, ex, pv, n0):
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def test_solve_porosity_adjoint_matches_oracle_differences():
+    rng = np.random.default_rng(3)
+    rows, pvs = [], []
+    while len(rows) < 24:  # interior roots with the trial state outside the yield surface
+        n0 = np.float32(rng.uniform(0.05, 0.3))
+        pv = 1.0 - float(n0) + rng.uniform(0.5 * n0, 1.5 * n0)
+        q, p = np.float32(10 ** rng.uniform(0, 2.5)), np.float32(10 ** rng.uniform(0, 3))
+        df, ms = np.float32(10 ** rng.uniform(-2, 0.5)), np.float32(10 ** rng.uniform(-2, 0.5))
+        ex = np.float32(rng.uniform(-0.05, 0.05))
+        if q + O.porosity_coefficient(pv - 1.0 + float(n0)) * p <= 1.0:
+            continue
+        if not 0.01 < _porosity_root(q, p, df, ms, ex, pv, n0) < 0.99:
+            continue
+        rows.append([q, p, df, ms, ex, n0])
+        pvs.append(pv)
+    args = np.asarray(rows, np.float32)
+    pvs = np.asarray(pvs)
+    m = len(rows)
+    gbar = rng.normal(size=m)
+    out = torch.zeros(m * 7, dtype=torch.float64, device="cuda")
+    from baking_amd import _native
+    lib = _native.load()
+    a_d, pv_d, g_d = (torch.as_tensor(v, device="cuda") for v in (args.reshape(-1), pvs, gbar))
+    _native.check(lib.bk_solve_porosity_adjoint(_native.ptr(a_d), _native.ptr(pv_d), _native.ptr(g_d), _native.ptr(out), m,
+                                                _native.stream_ptr()), "bk_solve_porosity_adjoint")
+    got = out.cpu().numpy().reshape(m, 7)
+    want = np.zeros((m, 7))
+    for i in range(m):
+        for k in range(7):
+            if k < 6:  # float arguments: differences between exactly representable neighbours
+                x = float(args[i, k])
+                up, dn = np.float32(x + 1e-4 * abs(x)), np.float32(x - 1e-4 * abs(x))
+                ap, am = list(args[i]), list(args[i])
+                ap[k], am[k] = up, dn
+                h = float(up) - float(dn)
+                fd = (_porosity_root(*ap[:5], pvs[i], ap[5]) - _porosity_root(*am[:5], pvs[i], am[5])) / h
+            else:
+                h = 1e-7
+                fd = (_porosity_root(*args[i, :5], pvs[i] + h, args[i, 5]) - _porosity_root(*args[i, :5], pvs[i] - h, args[i, 5])) / (2 * h)
+            want[i, k] = gbar[i] * fd
+    err = np.abs(got - want) / (np.abs(want) + 1e-6 * np.abs(want).max())
+    print("worst relative difference", err.max())
+    np.testing.assert_allclose(got, want, rtol=1e-3, atol=1e-6 * np.abs(want).max())
+
+
+def test_gas_production_gradient():
+    """tests/test_gas.py:131-160: d mean(det F) / d alpha after 20 substeps of the gas-producing porous cube."""
+    spec = scenes.demo_gas(rate=0.005)
+    sim = scenes.build_simulator(spec)
+    alpha = torch.tensor([0.005], device="cuda", requires_grad=True)
+    out = adjoint.rollout(sim, 20, production_parameter=alpha)
+    loss = torch.linalg.det(out["F"].double()).mean()
+    loss.backward()
+    grad = float(alpha.grad)
+
+    def loss_at(value):
+        def setup(o):
+            o.prod_param[0] = value
+        o = _oracle_run(spec, 20, setup)
+        return float(np.linalg.det(o.state.F).mean())
+
+    fd_ref = (loss_at(0.0051) - loss_at(0.0049)) / 0.0002
+    fd = (loss_at(0.005 + 1e-7) - loss_at(0.005 - 1e-7)) / 2e-7
+    print(f"gradient {grad:.9g}, oracle FD (eps 1e-4) {fd_ref:.9g}, oracle FD (eps 1e-7) {fd:.9g}")
+    assert abs(fd_ref) > 1e-4
+    np.testing.assert_allclose(grad, fd_ref, rtol=0.05, atol=1e-4)
+    np.testing.assert_allclose(grad, fd, rtol=2e-3)
+
+
 def _state_scales(fields):
-    return dict(x=0.05, v=0.5, C=2.0, F=0.1, Fe=0.1, porosity=0.05, tau=50.0)
+    return dict(x=0.05, v=0.5, C=2.0, F=0.1, Fe=0.1, porosity=0.05, tau=50.0, moles=0.01)
 
 
-@pytest.mark.parametrize("case", ["neo_hookean", "ding", "ding_porous"])
+@pytest.mark.parametrize("case", ["neo_hookean", "ding", "ding_porous", "ding_gas"])
 def test_rollout_gradient_matches_fp64_differences(case):
     """32 random initial-state coordinates of a random linear functional of the state after 20 substeps."""
     if case == "neo_hookean":
         spec = scenes.strain_rate_field(scenes.scaled(scenes.demo_autodiff(0.0), time_step=5e-4), np.diag([1.0, -0.6, 0.3]))
+    elif case == "ding_gas":
+        spec = scenes.strain_rate_field(scenes.demo_gas(rate=0.005), np.diag([0.5, -0.25, 0.1]))
     else:
         spec = scenes.demo_viscous_gradient(1000.0, porosity=0.2 if case == "ding_porous" else 0.0)
         spec = scenes.scaled(spec, time_step=5e-4)
@@ -202,7 +290,7 @@ def test_rollout_gradient_matches_fp64_differences(case):
     grads = {f: inputs[f].grad.double().cpu().numpy() for f in fields}
 
     o_fields = dict(x=("x", None), v=("v", None), C=("C", None), F=("F", None), Fe=("Fe", "ding"), porosity=("porosity", "ding"),
-                    tau=("tau", "ding"))
+                    tau=("tau", "ding"), moles=("moles", "ding"))
     start_np = {f: start[f].cpu().numpy().astype(np.float64) for f in fields}
 
     def view(o, f):
