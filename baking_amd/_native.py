@@ -141,7 +141,7 @@ SIGNATURES = {
     "bk_workspace_set_surrogate": ([P, P], c_int),
     "bk_workspace_set_samples": ([P, P, I], c_int),
     "bk_workspace_stats": ([P, P, I, P], c_int),
-    "bk_adjoint_substep": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid),
+    "bk_adjoint_substep": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P,
                             POINTER(BkAdjoint), P, P, P], c_int),
     "bk_UpdateDing_adjoint": ([P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, F, F, F, F, I, P, P, P, F, I, P], c_int),
     "bk_overstress_adjoint": ([P, P, P, I, P], c_int),

@@ -6,10 +6,12 @@ The reference differentiates a rollout by recording ``MpmSimulator.Step`` on a
 every substep's state and grid alive.  Here the forward runs the fused HIP
 substep (graphs included) and keeps only a checkpoint of the state every
 ``checkpoint_every`` substeps.  The backward walks the segments in reverse:
-it recomputes one segment from its checkpoint, keeping each substep's state
-and grid, then runs the library's adjoint kernels (``bk_adjoint_substep``:
-G2P with the F update and the Ding return map, the grid update, P2G with the
-neo-Hookean stress) substep by substep in reverse.  Adjoints are FP64.
+it recomputes one segment from its checkpoint, keeping each substep's input
+state, then, substep by substep in reverse, reruns that substep's forward
+(which leaves its grid and its sorted P2G tiles in the workspace) and runs
+the library's adjoint kernels on it (``bk_adjoint_substep``: the Ding return
+map, G2P with the F update, the grid update, P2G with the stress).  Adjoints
+are FP64.
 
 Covered: the substep with the neo-Hookean model, or the Ding model with or
 without pore gas (the coupled porosity return and the gas pressure and
@@ -82,24 +84,6 @@ def _load_state(state, saved):
         t.copy_(saved[f])
 
 
-def _grid_clone(sim):
-    g = sim.grid_state
-    return (g.tiled_masses.clone(), g.tiled_momenta.clone(), g.tiled_velocities.clone())
-
-
-def _bk_state(saved):
-    s = _native.BkState()
-    for f, t in saved.items():
-        setattr(s, f, t.data_ptr())
-    return s
-
-
-def _bk_grid(grid):
-    g = _native.BkGrid()
-    g.m, g.p, g.v = (t.data_ptr() for t in grid)
-    return g
-
-
 def _run(sim, count):
     for _ in range(count):
         sim._launch_substep()
@@ -150,25 +134,28 @@ class _Rollout(torch.autograd.Function):
             setattr(bk_adj, f, t.data_ptr())
         param = torch.zeros(2, dtype=torch.float64, device=dev)
         nodes = sim.grid_state.tiled_masses.numel()
-        scratch = torch.empty(7 * nodes, dtype=torch.float64, device=dev)
+        scratch = torch.zeros(7 * nodes, dtype=torch.float64, device=dev)
         # the simulator is left where the forward ended
         keep = (sim.current_state, sim.next_state, sim.step_count, _clone_state(sim.current_state), sim._read_flags())
         stream = _native.stream_ptr()
         for seg in reversed(range(len(ctx.checkpoints))):
             count = min(ctx.every, ctx.steps - seg * ctx.every)
             _load_state(sim.current_state, ctx.checkpoints[seg])
-            states, grids = [ctx.checkpoints[seg]], []
-            for _ in range(count):
-                sim._launch_substep()
-                grids.append(_grid_clone(sim))
-                sim._accept()
+            states = [ctx.checkpoints[seg]]
+            for _ in range(count - 1):
+                _run(sim, 1)
                 states.append(_clone_state(sim.current_state))
             for k in reversed(range(count)):
-                s_in, s_out, grid = _bk_state(states[k]), _bk_state(states[k + 1]), _bk_grid(grids[k])
+                if k != count - 1:
+                    _load_state(sim.current_state, states[k])
+                # the substep's forward again: its output, grid and P2G tiles for the adjoint
+                sim._launch_substep()
+                s_in, s_out = sim._state_struct(sim.current_state), sim._state_struct(sim.next_state)
                 _native.check(lib.bk_adjoint_substep(ctypes.byref(sim._params), ctypes.byref(sim._scene), ctypes.byref(s_in),
-                                                     ctypes.byref(s_out), ctypes.byref(grid), ctypes.byref(bk_adj),
-                                                     _native.ptr(scratch), _native.ptr(param), stream), "bk_adjoint_substep")
-            del states, grids
+                                                     ctypes.byref(s_out), ctypes.byref(sim._grid), sim._workspace,
+                                                     ctypes.byref(bk_adj), _native.ptr(scratch), _native.ptr(param), stream),
+                              "bk_adjoint_substep")
+            del states
         sim.current_state, sim.next_state, sim.step_count, final, flags = keep
         _load_state(sim.current_state, final)
         sim._write_flags(flags)

@@ -17,11 +17,13 @@
 // with divided differences that stay finite (their limits) when two
 // principal stretches coincide.
 #include "bk_device.h"
+#include "bk_workspace.h"
 #include "../../include/baking_abi.h"
 
 using namespace bk;
 
 void bk_internal_set_error(const char* what);  // bk_last_error's text (bk_substep.hip)
+int bk_internal_tile_launch(bk_workspace* ws, const bk_dough** doughs, int* n_doughs, int* grid_cap);  // bk_substep.hip
 
 namespace {
 int fail(const char* what) {
@@ -81,16 +83,18 @@ __device__ __forceinline__ Dual operator*(double s, const Dual& a) {
 }
 __device__ __forceinline__ Dual operator/(const Dual& a, const Dual& b) {
   Dual r;
-  r.v = a.v / b.v;
+  const double inv = 1.0 / b.v;
+  r.v = a.v * inv;
 #pragma unroll
-  for (int k = 0; k < kDir; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+  for (int k = 0; k < kDir; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) * inv;
   return r;
 }
 __device__ __forceinline__ Dual dlog(const Dual& a) {
   Dual r;
   r.v = log(a.v);
+  const double inv = 1.0 / a.v;
 #pragma unroll
-  for (int k = 0; k < kDir; ++k) r.d[k] = a.d[k] / a.v;
+  for (int k = 0; k < kDir; ++k) r.d[k] = a.d[k] * inv;
   return r;
 }
 __device__ __forceinline__ Dual dexp(const Dual& a) {
@@ -109,14 +113,39 @@ __device__ __forceinline__ Dual dexp(const Dual& a) {
 struct PorosityDerivs {
   double q, p, df, ms, ex, pv, n0;
 };
-__device__ inline PorosityDerivs porosity_root_derivatives(double n, double q, double p, double d, double h, double ex, double pv) {
-  const double a = pressure_coefficient64(n);
-  double da = 0.0;
+// Overstress of the converged Newton iteration on R(g) = g + 1 - q2/(1+df g)^2 - p2/(1+mf g)^2
+// (0 inside the yield surface), stopped one step after |R| falls below 1e-13 of its scale.
+__device__ inline double overstress_converged(double q2, double p2, double df, double mf) {
+  double root = 0.0;
+  if (q2 + p2 > 1.0) {
+    const double tol = 1e-13 * (1.0 + q2 + p2);
+    for (int it = 0; it < 64; ++it) {
+      const double u = 1.0 + df * root, w = 1.0 + mf * root;
+      const double iu = 1.0 / u, iw = 1.0 / w;
+      const double res = root + 1.0 - q2 * iu * iu - p2 * iw * iw;
+      const double der = 1.0 + 2.0 * df * q2 * iu * iu * iu + 2.0 * mf * p2 * iw * iw * iw;
+      root -= res / der;
+      if (fabs(res) < tol) break;
+    }
+  }
+  return root;
+}
+
+// a(n) (ding.py:68-74, porous_return.py:11-16) and its derivative.
+__device__ __forceinline__ void coefficient_and_slope(double n, double& a, double& da) {
+  a = pressure_coefficient64(n);
+  da = 0.0;
   if (n > 0.0) {
     const double lg = log(1.0 / n);
     da = a * (2.0 / (n * lg) - 3.5 / (3.5 * n + 1.0));
   }
-  const double g = overstress64(q, a * p, d, h * a);
+}
+
+// g = the overstress at the root (overstress_converged(q, a p, d, h a)).
+__device__ inline PorosityDerivs porosity_root_derivatives(double n, double g, double q, double p, double d, double h, double ex,
+                                                           double pv) {
+  double a, da;
+  coefficient_and_slope(n, a, da);
   const double u = 1.0 + d * g, v = 1.0 + h * a * g;
   const double der = 1.0 + 2.0 * d * q / (u * u * u) + 2.0 * h * a * a * p / (v * v * v);
   double r_q = 0.0, r_p = 0.0, r_d = 0.0, r_h = 0.0, r_a = 0.0;
@@ -147,13 +176,13 @@ __device__ inline PorosityDerivs porosity_root_derivatives(double n, double q, d
 // corrected principal Kirchhoff stresses cs, the returned principal stretches
 // e and the porosity n the map used (ding.py:103-131, 183-192), with their
 // derivatives along (s0, s1, s2, old porosity, eta).  Non-coupled, n is the
-// old porosity; coupled, the end-state root (porous_return.py:43-61) with the
-// derivatives above.
+// old porosity; coupled, the end-state root the forward solved
+// (porous_return.py:43-61, its output porosity), with the derivatives above.
 struct Principal {
   Dual cs[3], e[3], n;
 };
-__device__ inline Principal principal_return(const double s[3], double n_old, double eta, double mu, double lam, double y, double dt,
-                                             bool coupled, float n0, bool bracketed) {
+__device__ inline Principal principal_return(const double s[3], double n_old, double n_root, double eta, double mu, double lam, double y,
+                                             double dt, bool coupled, float n0) {
   Dual sv[3] = {dvar(s[0], 0), dvar(s[1], 1), dvar(s[2], 2)};
   const Dual ev = dvar(eta, 4);
   Dual eps[3];
@@ -170,61 +199,51 @@ __device__ inline Principal principal_return(const double s[3], double n_old, do
   const double y2 = y * y;
   const Dual q2 = (1.5 / y2) * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
   const Dual pm = (1.0 / y2) * (mean * mean);
-  const Dual df = dconst(6.0 * mu * dt * y2) / ev;
-  const Dual ms = dconst((2.0 / 3.0) * (2.0 * mu + 3.0 * lam) * dt * y2) / ev;
+  const Dual rev = dconst(1.0) / ev;
+  const Dual df = (6.0 * mu * dt * y2) * rev;
+  const Dual ms = ((2.0 / 3.0) * (2.0 * mu + 3.0 * lam) * dt * y2) * rev;
   Principal P;
-  P.n = dvar(n_old, 3);
+  P.n = dvar(coupled ? n_root : n_old, 3);
+  if (coupled) P.n.d[3] = 0.0;
+  double av, da;
+  coefficient_and_slope(P.n.v, av, da);
+  // one overstress solve serves the return map and, coupled, the porosity root's derivatives
+  const double gv = overstress_converged(q2.v, av * pm.v, df.v, ms.v * av);
   if (coupled) {
     const Dual ex = (3.0 / (2.0 * mu + 3.0 * lam)) * mean;
     const double opv = 1.0 - double(n0) + n_old;
-    const float root = bracketed ? solve_porosity_bracketed(float(q2.v), float(pm.v), float(df.v), float(ms.v), float(ex.v), opv, n0)
-                                 : solve_porosity(float(q2.v), float(pm.v), float(df.v), float(ms.v), float(ex.v), opv, n0);
-    const PorosityDerivs D = porosity_root_derivatives(double(root), q2.v, pm.v, df.v, ms.v, ex.v, opv);
-    P.n.v = root;
+    const PorosityDerivs D = porosity_root_derivatives(P.n.v, gv, q2.v, pm.v, df.v, ms.v, ex.v, opv);
 #pragma unroll
     for (int k = 0; k < kDir; ++k)
       P.n.d[k] = D.q * q2.d[k] + D.p * pm.d[k] + D.df * df.d[k] + D.ms * ms.d[k] + D.ex * ex.d[k] + (k == 3 ? D.pv : 0.0);
   }
-  // porosity coefficient a(n) (ding.py:68-74)
-  Dual ad = dconst(0.0);
-  if (P.n.v > 0.0) {
-    const double nn = P.n.v, lg = log(1.0 / nn);
-    ad.v = 9.0 / (4.0 * lg * lg * (3.5 * nn + 1.0));
-    const double da = ad.v * (2.0 / (nn * lg) - 3.5 / (3.5 * nn + 1.0));
+  Dual ad = dconst(av);
 #pragma unroll
-    for (int k = 0; k < kDir; ++k) ad.d[k] = da * P.n.d[k];
-  }
+  for (int k = 0; k < kDir; ++k) ad.d[k] = da * P.n.d[k];
   const Dual p2 = ad * pm;
   const Dual mf = ms * ad;
-  // overstress: converged Newton on R(g) = g + 1 - q2/(1+df g)^2 - p2/(1+mf g)^2 = 0, derivative by the IFT
+  // overstress derivative by the IFT on its converged equation (ding.py:90-100)
   Dual g = dconst(0.0);
   if (q2.v + p2.v > 1.0) {
-    double root = 0.0;
-    for (int it = 0; it < 64; ++it) {
-      const double u = 1.0 + df.v * root, w = 1.0 + mf.v * root;
-      const double res = root + 1.0 - q2.v / (u * u) - p2.v / (w * w);
-      const double der = 1.0 + 2.0 * df.v * q2.v / (u * u * u) + 2.0 * mf.v * p2.v / (w * w * w);
-      const double next = root - res / der;
-      if (next == root) break;
-      root = next;
-    }
-    const double u = 1.0 + df.v * root, w = 1.0 + mf.v * root;
-    const double der = 1.0 + 2.0 * df.v * q2.v / (u * u * u) + 2.0 * mf.v * p2.v / (w * w * w);
-    g.v = root;
+    const double u = 1.0 + df.v * gv, w = 1.0 + mf.v * gv;
+    const double iu = 1.0 / u, iw = 1.0 / w;
+    const double der = 1.0 + 2.0 * df.v * q2.v * iu * iu * iu + 2.0 * mf.v * p2.v * iw * iw * iw;
+    const double ider = 1.0 / der;
+    g.v = gv;
 #pragma unroll
     for (int k = 0; k < kDir; ++k) {
-      const double dR = -q2.d[k] / (u * u) - p2.d[k] / (w * w) + 2.0 * q2.v * root * df.d[k] / (u * u * u) +
-                        2.0 * p2.v * root * mf.d[k] / (w * w * w);
-      g.d[k] = -dR / der;
+      const double dR = -q2.d[k] * iu * iu - p2.d[k] * iw * iw + 2.0 * q2.v * gv * df.d[k] * iu * iu * iu +
+                        2.0 * p2.v * gv * mf.d[k] * iw * iw * iw;
+      g.d[k] = -dR * ider;
     }
   }
   const Dual one = dconst(1.0);
-  const Dual dden = one + df * g, mden = one + mf * g;
+  const Dual rdden = one / (one + df * g), rmden = one / (one + mf * g);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) P.cs[i] = dev[i] / dden + mean / mden;
+  for (int i = 0; i < 3; ++i) P.cs[i] = dev[i] * rdden + mean * rmden;
   const Dual cm = (1.0 / 3.0) * (P.cs[0] + P.cs[1] + P.cs[2]);
 #pragma unroll
-  for (int i = 0; i < 3; ++i) P.e[i] = dexp((P.cs[i] - cm) * dconst(1.0 / (2.0 * mu)) + cm * dconst(1.0 / (2.0 * mu + 3.0 * lam)));
+  for (int i = 0; i < 3; ++i) P.e[i] = dexp((1.0 / (2.0 * mu)) * (P.cs[i] - cm) + (1.0 / (2.0 * mu + 3.0 * lam)) * cm);
   return P;
 }
 
@@ -280,8 +299,9 @@ __device__ __forceinline__ void minv_t(const double* a, double* r) {
 // gF (the new total gradient), gC (the new affine velocity), gporosity (the
 // old porosity) and geta0 (the viscosity parameter).
 __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe_old, const double* F, const double* C, double porosity,
-                                           int gelled_in, double T, double visc_param, const double* gFe_in, const double* gtau,
-                                           double gn, double* gFe_old, double* gF, double* gC, double& gporosity, double& geta0) {
+                                           double n_root, int gelled_in, double T, double visc_param, const double* gFe_in,
+                                           const double* gtau, double gn, double* gFe_old, double* gF, double* gC, double& gporosity,
+                                           double& geta0) {
   int gel = gelled_in;
   double Teff = T;
   double yv = P.yield_stress, vfac = 1.0;
@@ -305,7 +325,7 @@ __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe
   const double* U = Um.a;
   const double* V = Vm.a;
   const bool coupled = P.coupled != 0;
-  const Principal pr = principal_return(s, porosity, ev, mu, lam, yv, dt, coupled, P.n0, P.bracketed != 0);
+  const Principal pr = principal_return(s, porosity, n_root, ev, mu, lam, yv, dt, coupled, P.n0);
   double e[3], cs[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -437,12 +457,57 @@ struct AdjArgs {
   int n, nb;
 };
 
-// Per particle: the Ding return map's adjoint (its outputs' adjoints arrive in
-// adj.Fe / adj.tau / adj.porosity), then G2P's: scatter the velocity adjoint to
-// the grid and leave the input state's partial adjoints in the adj arrays.
-__global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
+// Per particle (Ding model): the return map's adjoint.  Its outputs' adjoints
+// arrive in adj.Fe / adj.tau / adj.porosity; adds the new F's and C's
+// adjoints to adj.F / adj.C and leaves the old Fe's and porosity's in
+// adj.Fe / adj.porosity (adj.tau cleared for P2G).
+__global__ __launch_bounds__(128) void k_adj_ding(AdjArgs a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   double geta = 0.0;
+  if (p < a.n) {
+    const bk_params& P = a.P;
+    DingParams D;
+    D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
+    for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
+    D.temperature_enabled = P.temperature_enabled;
+    D.coupled = P.gas_enabled;
+    D.bracketed = P.return_solver;
+    double Fe_old[9], Fn[9], Cn[9], gFe[9], gtau[9], gFe_old[9], gF[9], gC[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      Fe_old[i] = a.in.Fe[9 * p + i];
+      Fn[i] = a.out.F[9 * p + i];
+      Cn[i] = a.out.C[9 * p + i];
+      gFe[i] = a.adj.Fe[9 * p + i];
+      gtau[i] = a.adj.tau[9 * p + i];
+      gFe_old[i] = gF[i] = gC[i] = 0.0;
+    }
+    double gpor = 0.0;
+    const float T_local = P.transport_enabled ? a.out.T_mat[p] : a.in.T_mat[p];
+    ding_return_adjoint(D, Fe_old, Fn, Cn, double(a.in.porosity[p]), double(a.out.porosity[p]), a.in.gelled[p], double(T_local),
+                        double(a.visc_param[0]), gFe, gtau, a.adj.porosity[p], gFe_old, gF, gC, gpor, geta);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      a.adj.F[9 * p + i] += gF[i];
+      a.adj.C[9 * p + i] += gC[i];
+      a.adj.Fe[9 * p + i] = gFe_old[i];
+      a.adj.tau[9 * p + i] = 0.0;
+    }
+    a.adj.porosity[p] = gpor;
+  }
+  // viscosity-parameter gradient: wavefront sum, one atomic per wavefront
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) geta += __shfl_xor(geta, off, 64);
+  if (__lane_id() == 0 && geta != 0.0) unsafeAtomicAdd(a.param_grad, geta);
+}
+
+// Per particle: G2P's adjoint with the F update (transfer.py:73-109), its
+// gather part.  Leaves the input state's partial adjoints of x and F in
+// adj.x / adj.F, and the adjoints of the new velocity and affine velocity
+// (with the position and F-update terms added) in adj.v / adj.C for the
+// scatter to the grid (k_adj_scatter).
+__global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < a.n) {
     const bk_params& P = a.P;
     const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
@@ -456,27 +521,6 @@ __global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
     for (int i = 0; i < 9; ++i) {
       gC[i] = a.adj.C[9 * p + i];
       gF[i] = a.adj.F[9 * p + i];
-    }
-    double gFe_old[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gpor = 0.0;
-    if (P.model == 1) {
-      DingParams D;
-      D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
-      for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
-      D.temperature_enabled = P.temperature_enabled;
-      D.coupled = P.gas_enabled;
-      D.bracketed = P.return_solver;
-      double Fe_old[9], Fn[9], Cn[9], gFe[9], gtau[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) {
-        Fe_old[i] = a.in.Fe[9 * p + i];
-        Fn[i] = a.out.F[9 * p + i];
-        Cn[i] = a.out.C[9 * p + i];
-        gFe[i] = a.adj.Fe[9 * p + i];
-        gtau[i] = a.adj.tau[9 * p + i];
-      }
-      const float T_local = P.transport_enabled ? a.out.T_mat[p] : a.in.T_mat[p];
-      ding_return_adjoint(D, Fe_old, Fn, Cn, double(a.in.porosity[p]), a.in.gelled[p], double(T_local), double(a.visc_param[0]),
-                          gFe, gtau, a.adj.porosity[p], gFe_old, gF, gC, gpor, geta);
     }
     // G2P (transfer.py:73-109)
     const float dx = P.dx;
@@ -513,6 +557,7 @@ __global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
       axis_weight_grads(l[1], wy, dwy);
       axis_weight_grads(l[2], wz, dwz);
       const double inv_dx = 1.0 / double(dx);
+#pragma unroll 1
       for (int ox = 0; ox < 3; ++ox)
         for (int oy = 0; oy < 3; ++oy)
           for (int oz = 0; oz < 3; ++oz) {
@@ -530,36 +575,95 @@ __global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
             }
             const double val = vi[0] * gv[0] + vi[1] * gv[1] + vi[2] * gv[2] + s4 * (vi[0] * gCdd[0] + vi[1] * gCdd[1] + vi[2] * gCdd[2]);
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-              gx_in[r] += dw[r] * val - w * s4 * gCtv[r];
-              unsafeAtomicAdd(a.gv + 3 * node + r, w * (gv[r] + s4 * gCdd[r]));
-            }
+            for (int r = 0; r < 3; ++r) gx_in[r] += dw[r] * val - w * s4 * gCtv[r];
           }
     }
-    // leave the input state's partial adjoints: x (from G2P), F (from the F update), Ding Fe and porosity
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       a.adj.x[3 * p + i] = gx_in[i];
-      a.adj.v[3 * p + i] = 0.0;
+      a.adj.v[3 * p + i] = gv[i];
     }
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       a.adj.F[9 * p + i] = gF_in[i];
-      a.adj.C[9 * p + i] = 0.0;
-      if (P.model == 1) {
-        a.adj.Fe[9 * p + i] = gFe_old[i];
-        a.adj.tau[9 * p + i] = 0.0;
-      }
+      a.adj.C[9 * p + i] = gC[i];
     }
-    if (P.model == 1) a.adj.porosity[p] = gpor;
   }
-  // viscosity-parameter gradient: wavefront sum, one atomic per wavefront
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) geta += __shfl_xor(geta, off, 64);
-  if (__lane_id() == 0 && geta != 0.0) unsafeAtomicAdd(a.param_grad, geta);
 }
 
-// Per node: the grid update's adjoint (grid.py:20-38).
+// G2P's adjoint, its scatter part: node velocity adjoints
+// sum_p w_ip (gv_p + 4/dx^2 gC_p (x_i - x_p)), over the P2G tiles of the
+// forward substep just run from the same positions (the workspace's sorted
+// particles, one particle block per tile).  As the forward P2G does, each tile
+// accumulates in LDS (FP64) and flushes one global atomic per node and
+// component, instead of 81 global atomics per particle.
+__global__ __launch_bounds__(kTileThreads) void k_adj_scatter(const bk_dough* __restrict__ doughs, AdjArgs a, bk_workspace* ws) {
+  __shared__ double acc[3][kRegionNodes];
+  const int ntiles = ws->counters[0];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bk_params& P = a.P;
+  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+  const float dx = P.dx;
+  const double s4 = 4.0 / (double(dx) * double(dx));
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tg = tile_at(doughs, ws, t);
+    for (int i = threadIdx.x; i < 3 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
+    __syncthreads();
+    const int S = (tg.count + 63) / 64;
+    for (int sub = w; sub < S; sub += 4) {
+      const int j = lane * S + sub;
+      if (j >= tg.count) continue;
+      const int p = ws->sorted_vals[tg.start + j];
+      float g3[3];
+      grid_position(a.in.x + 3 * p, o, dx, g3);
+      int b[3];
+      stencil_base(g3, b);
+      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.g.bx || (b[1] >> 2) != tg.g.by || (b[2] >> 2) != tg.g.bz) continue;
+      const float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+      float wxf[3], wyf[3], wzf[3];
+      axis_weights(l[0], wxf);
+      axis_weights(l[1], wyf);
+      axis_weights(l[2], wzf);
+      double gv[3], gC[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) gv[i] = a.adj.v[3 * p + i];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) gC[i] = s4 * a.adj.C[9 * p + i];
+      const int li = b[0] - 4 * tg.g.bx, lj = b[1] - 4 * tg.g.by, lk = b[2] - 4 * tg.g.bz;
+#pragma unroll
+      for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+          for (int oz = 0; oz < 3; ++oz) {
+            const double wt = double(wxf[ox]) * double(wyf[oy]) * double(wzf[oz]);
+            const double dd[3] = {(double(ox) - l[0]) * dx, (double(oy) - l[1]) * dx, (double(oz) - l[2]) * dx};
+            const int ri = region_index(li + ox, lj + oy, lk + oz);
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+              atomicAdd(&acc[r][ri], wt * (gv[r] + gC[r * 3] * dd[0] + gC[r * 3 + 1] * dd[1] + gC[r * 3 + 2] * dd[2]));
+          }
+    }
+    __syncthreads();
+    for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
+      const int li = ri / 36, lj = (ri / 6) % 6, lk = ri % 6;
+      const int i = 4 * tg.g.bx + li, j = 4 * tg.g.by + lj, k = 4 * tg.g.bz + lk;
+      if (i >= P.res || j >= P.res || k >= P.res) continue;
+      const int64_t node = tg.g.node_base + tiled_node(i, j, k, tg.g.nb);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double v = acc[r][ri];
+        if (v != 0.0) unsafeAtomicAdd(a.gv + 3 * node + r, v);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Per node: the grid update's adjoint (grid.py:20-38).  Consumes the velocity
+// adjoints G2P's adjoint scattered and sets them back to zero, so the scratch
+// is clear for the next call without a pass over the whole grid: only nodes
+// with mass receive scatters (every stencil node of a particle has mass).
 __global__ __launch_bounds__(256) void k_adj_grid(AdjArgs a, int64_t nodes) {
   const int64_t node = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (node >= nodes) return;
@@ -579,6 +683,7 @@ __global__ __launch_bounds__(256) void k_adj_grid(AdjArgs a, int64_t nodes) {
     const bool boundary = c[k] < P.boundary_cells || c[k] >= P.res - P.boundary_cells;
     const bool zeroed = boundary && a.g.v[3 * node + k] == 0.0f;
     const double gvk = zeroed ? 0.0 : a.gv[3 * node + k];
+    a.gv[3 * node + k] = 0.0;
     a.gp[3 * node + k] = gvk / m;
     gmass -= gvk * pk / (m * m);
   }
@@ -728,8 +833,13 @@ __global__ __launch_bounds__(128) void k_adj_p2g(AdjArgs a) {
 extern "C" {
 
 int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, const bk_state* out, const bk_grid* grid,
-                       const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream) {
-  if (!P || !sc || !in || !out || !grid || !adj || !grid_scratch || !param_grad) return fail("bk_adjoint_substep: null argument");
+                       bk_workspace* ws, const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream) {
+  if (!P || !sc || !in || !out || !grid || !ws || !adj || !grid_scratch || !param_grad)
+    return fail("bk_adjoint_substep: null argument");
+  const bk_dough* doughs = nullptr;
+  int n_doughs = 0, grid_cap = 0;
+  if (bk_internal_tile_launch(ws, &doughs, &n_doughs, &grid_cap)) return -1;
+  if (n_doughs != 1) return fail("bk_adjoint_substep: the workspace must hold one dough (the simulator's own)");
   if (P->transport_enabled && (P->kappa != 0.0f || P->tray_h != 0.0f || P->water_enabled || P->settling))
     return fail("bk_adjoint_substep: transport is differentiated only where it leaves temperature and water unchanged "
                 "(conductivity, tray and heat transfer 0, water off)");
@@ -754,10 +864,11 @@ int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* i
   a.gp = grid_scratch + nodes;
   a.gv = grid_scratch + 4 * nodes;
   a.param_grad = param_grad;
-  if (hipMemsetAsync(grid_scratch, 0, sizeof(double) * 7 * size_t(nodes), st) != hipSuccess) return fail("bk_adjoint_substep: clear");
   if (P->model == 1 && (!sc->visc_param || !in->Fe || !adj->Fe || !adj->tau || !adj->porosity))
     return fail("bk_adjoint_substep: the Ding adjoint needs Fe, tau and porosity arrays");
+  if (P->model == 1) hipLaunchKernelGGL(k_adj_ding, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
   hipLaunchKernelGGL(k_adj_g2p, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
+  hipLaunchKernelGGL(k_adj_scatter, dim3(grid_cap), dim3(kTileThreads), 0, st, doughs, a, ws);
   hipLaunchKernelGGL(k_adj_grid, dim3(div_up(nodes, 256)), dim3(256), 0, st, a, nodes);
   hipLaunchKernelGGL(k_adj_p2g, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : fail("bk_adjoint_substep: launch failed");
@@ -822,7 +933,7 @@ __global__ __launch_bounds__(128) void k_update_ding_adjoint(DingParams D, const
       gFe_old[i] = gF[i] = gC[i] = 0.0;
     }
     double gpor = 0.0;
-    ding_return_adjoint(D, Fe_old, Fn, Cn, double(old_porosity[p]), old_gelled[p], double(temperatures[p]),
+    ding_return_adjoint(D, Fe_old, Fn, Cn, double(old_porosity[p]), 0.0, old_gelled[p], double(temperatures[p]),
                         double(viscosity_parameter[0]), gFe, gtau, adj_porosity ? adj_porosity[p] : 0.0, gFe_old, gF, gC, gpor, geta);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -872,7 +983,10 @@ __global__ __launch_bounds__(256) void k_solve_porosity_adjoint(const float* arg
   if (i >= n) return;
   const float* q = args6 + 6 * i;
   const float root = solve_porosity_bracketed(q[0], q[1], q[2], q[3], q[4], pv[i], q[5]);
-  const PorosityDerivs D = porosity_root_derivatives(double(root), q[0], q[1], q[2], q[3], q[4], pv[i]);
+  double ac, dac;
+  coefficient_and_slope(double(root), ac, dac);
+  const double g0 = overstress_converged(q[0], ac * double(q[1]), q[2], double(q[3]) * ac);
+  const PorosityDerivs D = porosity_root_derivatives(double(root), g0, q[0], q[1], q[2], q[3], q[4], pv[i]);
   const double g = gbar[i];
   const double r[7] = {D.q, D.p, D.df, D.ms, D.ex, D.n0, D.pv};
 #pragma unroll

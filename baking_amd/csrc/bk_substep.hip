@@ -31,6 +31,7 @@
 //            porosity solve starts from the network's prediction
 #include "bk_device.h"
 #include "bk_surrogate.h"
+#include "bk_workspace.h"
 #include "../../include/baking_abi.h"
 
 #include <stddef.h>
@@ -67,9 +68,6 @@ int check_hip(hipError_t e, const char* where) {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per workgroup
-constexpr int kTileThreads = 256;                     // P2G workgroup size
-constexpr int kRegion = 6;                            // 4 cells + 2-node stencil reach
-constexpr int kRegionNodes = kRegion * kRegion * kRegion;
 constexpr int kParticleThreads = 256;                 // launch bound of the per-particle kernels
 
 inline int div_up(long long a, long long b) { return int((a + b - 1) / b); }
@@ -80,37 +78,6 @@ inline int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v
 // The library's one error text (bk_last_error), also set by the other sources.
 void bk_internal_set_error(const char* what) { g_last_error = what; }
 
-struct bk_workspace {
-  int max_n = 0, total_blocks = 0, max_doughs = 0, n_vertices = 0, n_slots = 0;
-  bk_dough* doughs = nullptr;                   // device copy of the dough table
-  uint32_t* keys_raw = nullptr;                 // cell keys in particle order
-  uint32_t* keys_s[2] = {nullptr, nullptr};     // radix scratch pairs
-  int32_t* vals_s[2] = {nullptr, nullptr};
-  uint32_t* sorted_keys = nullptr;              // sorted keys / particle indices
-  int32_t* sorted_vals = nullptr;
-  int32_t* hist = nullptr;
-  int32_t* digit_total = nullptr;
-  int32_t* blk_start = nullptr;                 // per batch block: first sorted particle
-  int32_t* blk_end = nullptr;
-  int32_t* node_flag = nullptr;                 // per batch block: active this substep
-  int32_t* blocks = nullptr;                    // non-empty particle blocks
-  int4* tiles = nullptr;                        // P2G tiles: (block, dough, first sorted particle, count)
-  int32_t* active = nullptr;                    // active node blocks
-  int32_t* counters = nullptr;                  // [0] tiles, [1] active blocks, [2] deferred, [3] particle blocks
-  int32_t* queue = nullptr;                     // particles whose coupled porosity needs the full solve
-  float* tau = nullptr;
-  float* vertices = nullptr;
-  double* slot_exposed = nullptr;
-  double* slot_contact = nullptr;
-  int32_t* flags = nullptr;                     // BK_FLAG_COUNT per dough
-  float* features = nullptr;                    // surrogate inputs of the queued particles, kMlpRow per queue slot
-  float* predictions = nullptr;                 // surrogate outputs, kMlpOut per queue slot
-  const uint8_t* mlp = nullptr;                 // surrogate inference blob (caller-owned), or null
-  float* samples = nullptr;                     // emitted (features, exact outputs) rows of kSampleRow floats, or null
-  int32_t sample_cap = 0;
-  unsigned long long* stats = nullptr;          // [0] porosity evaluations [1] guided solves accepted [2] guided solves
-                                                // rejected [3] queued return maps; summed over substeps
-};
 constexpr int kSampleRow = 12;  // 7 features, 4 exact outputs, 1 unused
 
 // Host side of a workspace: a copy of the device descriptor, the uploaded
@@ -148,23 +115,6 @@ __device__ __forceinline__ int dough_of_block(const bk_dough* __restrict__ D, in
     if (D[mid].block_offset <= b) lo = mid; else hi = mid - 1;
   }
   return lo;
-}
-
-// Position of batch block b inside its dough's grid.
-struct BlockGeom {
-  int d, nb, bx, by, bz;
-  int64_t node_base;  // first node of the dough's grid in the batch grid
-};
-__device__ __forceinline__ BlockGeom block_geom(const bk_dough* __restrict__ D, int d, int b) {
-  BlockGeom g;
-  g.d = d;
-  g.nb = (D[d].params.res + 3) / 4;
-  int lb = b - D[d].block_offset;
-  g.bz = lb % g.nb;
-  g.by = (lb / g.nb) % g.nb;
-  g.bx = lb / (g.nb * g.nb);
-  g.node_base = int64_t(D[d].block_offset) << 6;
-  return g;
 }
 
 __device__ __forceinline__ int32_t* dough_flags(const bk_workspace* ws, int d) { return ws->flags + d * BK_FLAG_COUNT; }
@@ -505,25 +455,6 @@ __device__ __forceinline__ Robin robin_terms(const bk_params& P, float heat_para
   if (P.water_enabled) r.exch += double(P.latent * P.loss_coef) * r.sat * s.exposed;
   return r;
 }
-
-// One P2G tile: up to `chunk` sorted particles of one particle block.  Every
-// field is wave-uniform.
-struct Tile {
-  int d, start, count;
-  BlockGeom g;
-};
-__device__ __forceinline__ Tile tile_at(const bk_dough* __restrict__ doughs, const bk_workspace* ws, int t) {
-  const int4 e = ws->tiles[t];
-  Tile T;
-  const int blk = __builtin_amdgcn_readfirstlane(e.x);
-  T.d = __builtin_amdgcn_readfirstlane(e.y);
-  T.start = __builtin_amdgcn_readfirstlane(e.z);
-  T.count = __builtin_amdgcn_readfirstlane(e.w);
-  T.g = block_geom(doughs, T.d, blk);
-  return T;
-}
-
-__device__ __forceinline__ int region_index(int li, int lj, int lk) { return (li * kRegion + lj) * kRegion + lk; }
 
 // In both P2G passes, lanes of one wave take a tile's particles S apart
 // (S = ceil(count / 64)) so that, with ~8 particles per cell, neighbouring
@@ -1155,6 +1086,18 @@ static HostWs* host_view(bk_workspace* ws) {
   HostWs h;
   if (ws_host(ws, h.dev)) return nullptr;
   return &(g_ws_cache[ws] = h);
+}
+
+// For the adjoint's tiled scatter (bk_adjoint.hip): the device dough table of
+// a workspace, its dough count and its P2G launch width.  The tiles are those
+// of the last substep the workspace ran.
+int bk_internal_tile_launch(bk_workspace* ws, const bk_dough** doughs, int* n_doughs, int* grid_cap) {
+  HostWs* hw = host_view(ws);
+  if (!hw) return fail("workspace: unknown handle");
+  *doughs = hw->dev.doughs;
+  *n_doughs = int(hw->doughs.size());
+  *grid_cap = hw->tuning.grid_cap;
+  return 0;
 }
 
 // Optional per-stage timing: HIP events recorded around each stage of the

@@ -1,0 +1,78 @@
+// Device layout of the substep workspace and the tile helpers shared by the
+// substep kernels (bk_substep.hip) and the adjoint's tiled scatter
+// (bk_adjoint.hip).  Internal: not part of the C ABI.
+#pragma once
+#include "bk_device.h"
+#include "../../include/baking_abi.h"
+
+constexpr int kTileThreads = 256;  // P2G workgroup size
+constexpr int kRegion = 6;         // 4 cells + 2-node stencil reach
+constexpr int kRegionNodes = kRegion * kRegion * kRegion;
+
+struct bk_workspace {
+  int max_n = 0, total_blocks = 0, max_doughs = 0, n_vertices = 0, n_slots = 0;
+  bk_dough* doughs = nullptr;                   // device copy of the dough table
+  uint32_t* keys_raw = nullptr;                 // cell keys in particle order
+  uint32_t* keys_s[2] = {nullptr, nullptr};     // radix scratch pairs
+  int32_t* vals_s[2] = {nullptr, nullptr};
+  uint32_t* sorted_keys = nullptr;              // sorted keys / particle indices
+  int32_t* sorted_vals = nullptr;
+  int32_t* hist = nullptr;
+  int32_t* digit_total = nullptr;
+  int32_t* blk_start = nullptr;                 // per batch block: first sorted particle
+  int32_t* blk_end = nullptr;
+  int32_t* node_flag = nullptr;                 // per batch block: active this substep
+  int32_t* blocks = nullptr;                    // non-empty particle blocks
+  int4* tiles = nullptr;                        // P2G tiles: (block, dough, first sorted particle, count)
+  int32_t* active = nullptr;                    // active node blocks
+  int32_t* counters = nullptr;                  // [0] tiles, [1] active blocks, [2] deferred, [3] particle blocks
+  int32_t* queue = nullptr;                     // particles whose coupled porosity needs the full solve
+  float* tau = nullptr;
+  float* vertices = nullptr;
+  double* slot_exposed = nullptr;
+  double* slot_contact = nullptr;
+  int32_t* flags = nullptr;                     // BK_FLAG_COUNT per dough
+  float* features = nullptr;                    // surrogate inputs of the queued particles, kMlpRow per queue slot
+  float* predictions = nullptr;                 // surrogate outputs, kMlpOut per queue slot
+  const uint8_t* mlp = nullptr;                 // surrogate inference blob (caller-owned), or null
+  float* samples = nullptr;                     // emitted (features, exact outputs) rows of kSampleRow floats, or null
+  int32_t sample_cap = 0;
+  unsigned long long* stats = nullptr;          // [0] porosity evaluations [1] guided solves accepted [2] guided solves
+                                                // rejected [3] queued return maps; summed over substeps
+};
+// Position of batch block b inside its dough's grid.
+struct BlockGeom {
+  int d, nb, bx, by, bz;
+  int64_t node_base;  // first node of the dough's grid in the batch grid
+};
+__device__ __forceinline__ BlockGeom block_geom(const bk_dough* __restrict__ D, int d, int b) {
+  BlockGeom g;
+  g.d = d;
+  g.nb = (D[d].params.res + 3) / 4;
+  int lb = b - D[d].block_offset;
+  g.bz = lb % g.nb;
+  g.by = (lb / g.nb) % g.nb;
+  g.bx = lb / (g.nb * g.nb);
+  g.node_base = int64_t(D[d].block_offset) << 6;
+  return g;
+}
+
+// One P2G tile: up to `chunk` sorted particles of one particle block.  Every
+// field is wave-uniform.
+struct Tile {
+  int d, start, count;
+  BlockGeom g;
+};
+__device__ __forceinline__ Tile tile_at(const bk_dough* __restrict__ doughs, const bk_workspace* ws, int t) {
+  const int4 e = ws->tiles[t];
+  Tile T;
+  const int blk = __builtin_amdgcn_readfirstlane(e.x);
+  T.d = __builtin_amdgcn_readfirstlane(e.y);
+  T.start = __builtin_amdgcn_readfirstlane(e.z);
+  T.count = __builtin_amdgcn_readfirstlane(e.w);
+  T.g = block_geom(doughs, T.d, blk);
+  return T;
+}
+
+__device__ __forceinline__ int region_index(int li, int lj, int lk) { return (li * kRegion + lj) * kRegion + lk; }
+

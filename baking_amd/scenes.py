@@ -388,3 +388,12 @@ def demo_viscous_gradient(viscosity=1000.0, yield_stress=5.0, porosity=0.0):
     spec = SceneSpec("demo_viscous_gradient", mesh, SampleParticles(mesh, 1000.0, 4), sim, MaterialConfig(model="ding2019"),
                      DingConfig(yield_stress=yield_stress, viscosity=viscosity, porosity=porosity), TransportConfig(), steps=5)
     return strain_rate_field(spec, np.diag([1.5, -0.75, 0.2]))
+
+
+def loaf_256_frozen_transport(**kwargs):
+    """``loaf_256`` with the heat and water transport frozen (conductivity, heat transfer and tray heat
+    transfer 0, water off): the temperature stays at 298.15 K, pore gas and the coupled return map stay on.
+    The gradient workload: every part of the substep that the adjoint differentiates runs at full size."""
+    spec = loaf_256(**kwargs)
+    t = replace(spec.transport, water_enabled=False, conductivity=0.0, heat_transfer=0.0, tray_heat_transfer=0.0)
+    return replace(spec, name="loaf_256_frozen_transport", transport=t)

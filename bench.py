@@ -7,6 +7,11 @@ particle):
 
 * ``loaf_256`` (default): the 0.30 x 0.15 x 0.12 loaf on a 256^3 grid, ~0.72M
   particles, one simulator.
+* ``adjoint_loaf``: gradients (``baking_amd.adjoint.rollout``) through 64
+  substeps of ``loaf_256`` with its transport frozen (temperature and water
+  constant; pore gas and the coupled return map on): forward with checkpoints
+  every 16 substeps, then the backward (segment recompute + adjoint kernels).
+  Prints its own JSON line (gradient particle-substeps/s), not the headline.
 * ``batched_48``: 48 independent doughs of 3k-60k particles (~1.0M) on their
   own 64^3 / 128^3 grids, advanced as one batch (one launch per stage over
   every dough, one HIP graph per substep, autotuned launch shapes).  Also
@@ -66,7 +71,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed substeps (default: 1000 batched_48, 2000 loaf_256, 1000 sphere_small)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="batched_48", choices=["batched_48", "loaf_256", "sphere_small"])
+    ap.add_argument("--workload", default="batched_48", choices=["batched_48", "loaf_256", "sphere_small", "adjoint_loaf"])
     ap.add_argument("--no-surrogate", action="store_true", help="exact return map only")
     ap.add_argument("--model", default=None, help="surrogate model file (default: the packaged one)")
     ap.add_argument("--plain-steps", type=int, default=20, help="batched_48: substeps per dough of the plain comparator")
@@ -329,6 +334,99 @@ def copy_probe():
     print(json.dumps({"copy_probe_bytes": a.numel() * 4, "repeats": 5}))
 
 
+# Adjoint substep (k_adj_g2p, k_adj_grid, k_adj_p2g), algorithmic bytes per particle: the forward state it
+# reads (in: x, F, Fe, porosity, gelled, T_mat, C, v, tau, moles, mass, V0; out: F, C, T, moles) and the FP64
+# adjoint state read and written (x, v, C, F, Fe, tau, porosity, moles); per active node: the seven FP64 grid
+# adjoints written and read once and the forward grid (m, p, v) read.
+ADJOINT_PARTICLE_BYTES = (12 + 36 + 36 + 4 + 4 + 4 + 36 + 12 + 36 + 8 + 4 + 4) + (36 + 36 + 16 + 8) + 2 * (24 + 24 + 72 + 72 + 72 + 72 + 8 + 8)
+ADJOINT_NODE_BYTES = 2 * 56 + 28
+
+
+def run_adjoint(args):
+    """Gradient throughput: rollout forward (checkpointed) + backward on the frozen-transport loaf."""
+    import ctypes
+    import torch
+    from baking_amd import _native, adjoint
+    from baking_amd.scenes import build_simulator, loaf_256_frozen_transport
+    spec = loaf_256_frozen_transport()
+    sim = build_simulator(spec)
+    n = sim.particle_data.particle_count
+    steps = args.steps or 64
+    for _ in range(args.warmup):
+        sim._launch_substep()
+        sim._accept()
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(0)
+    w = torch.as_tensor(rng.normal(size=(n, 3)), device="cuda")
+    v0 = sim.current_state.particle_velocities.detach().clone().requires_grad_(True)
+    eta = sim.material.viscosity_parameter.detach().clone().requires_grad_(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = adjoint.rollout(sim, steps, {"v": v0}, viscosity_parameter=eta)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss = (out["x"].double() * w).sum()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd, bwd = t1 - t0, t2 - t1
+    # the adjoint kernels alone: repeated calls on one substep's (in, out, grid)
+    lib = _native.load()
+    sim._launch_substep()  # the forward substep whose grid and P2G tiles the adjoint uses
+    torch.cuda.synchronize()
+    g = sim.grid_state
+    active_nodes = int((g.tiled_masses > 0).sum().item())
+    adj = {f: torch.as_tensor(rng.normal(size=n * adjoint._WIDTH[f]) * 1e-3, device="cuda") for f in adjoint._fields_of(sim)}
+    bk_adj = _native.BkAdjoint()
+    for f, t in adj.items():
+        setattr(bk_adj, f, t.data_ptr())
+    scratch = torch.zeros(7 * g.tiled_masses.numel(), dtype=torch.float64, device="cuda")
+    param = torch.zeros(2, dtype=torch.float64, device="cuda")
+    st_in, st_out = sim._state_struct(sim.current_state), sim._state_struct(sim.next_state)
+    stream = torch.cuda.current_stream()
+
+    def call():
+        _native.check(lib.bk_adjoint_substep(ctypes.byref(sim._params), ctypes.byref(sim._scene), ctypes.byref(st_in),
+                                             ctypes.byref(st_out), ctypes.byref(sim._grid), sim._workspace, ctypes.byref(bk_adj),
+                                             _native.ptr(scratch), _native.ptr(param), _native.stream_ptr()), "bk_adjoint_substep")
+    call()
+    reps = 20
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record(stream)
+    for _ in range(reps):
+        call()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    adj_ms = start.elapsed_time(stop) / reps
+    adj_bytes = n * ADJOINT_PARTICLE_BYTES + active_nodes * ADJOINT_NODE_BYTES
+    result = {
+        "metric": "gradient particle-substeps/s on 1 MI355X (forward with checkpoints + backward)",
+        "value": n * steps / (fwd + bwd),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": (fwd + bwd) / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 state, f64 adjoint",
+        "data": "synthetic",
+        "config": {"workload": spec.name, "particles": n, "grid": spec.simulation.grid_resolution, "dt": spec.simulation.time_step,
+                   "checkpoint_every": 16, "active_stencil_nodes": active_nodes,
+                   "gradient_of": "sum(w * x_final), w ~ N(0,1), w.r.t. v0 and the viscosity parameter"},
+        "forward_ms_per_substep": fwd / steps * 1e3,
+        "backward_ms_per_substep": bwd / steps * 1e3,
+        "adjoint_kernels": {"ms_per_substep": adj_ms, "algorithmic_bytes": adj_bytes,
+                            "achieved_GBs": adj_bytes / (adj_ms * 1e-3) / 1e9, "peak_GBs": HBM_PEAK_GBS,
+                            "frac": adj_bytes / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "viscosity_gradient": float(eta.grad),
+        "v0_gradient_norm": float(v0.grad.norm()),
+    }
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -344,6 +442,8 @@ def main():
 
     if args.workload == "batched_48":
         return run_batched(args)
+    if args.workload == "adjoint_loaf":
+        return run_adjoint(args)
     from baking_amd import surrogate
     from baking_amd.measure import measure
     spec = build_scene(args.workload)

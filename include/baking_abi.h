@@ -341,14 +341,17 @@ typedef struct bk_adjoint {
   double *Fe, *tau, *porosity; /* Ding model only: N x 9, N x 9, N */
   double* moles;               /* Ding with pore gas only: N */
 } bk_adjoint;
-/* In: adj = the adjoint of the substep's output state `out`, the grid the
- * forward substep from `in` left (mass, momentum, velocity; single dough,
- * tiled nodes).  Out: adj = the adjoint of `in`; param_grad[0] += the
+/* In: adj = the adjoint of the substep's output state `out`; the grid and the
+ * workspace `ws` as the forward substep from `in` to `out` left them, with
+ * nothing run on `ws` since (single dough, tiled nodes; the adjoint's
+ * scatter reuses that substep's sorted P2G tiles).  Out: adj = the adjoint
+ * of `in`; param_grad[0] += the
  * gradient of the Ding viscosity parameter (scene.visc_param), param_grad[1]
  * += that of the gas production parameter (scene.prod_param).  grid_scratch:
- * 7 doubles per tiled node, cleared by the call. */
+ * 7 doubles per tiled node, all zero before the first call; each call leaves
+ * it ready for the next (no pass over the whole grid). */
 int bk_adjoint_substep(const bk_params* params, const bk_scene* scene, const bk_state* in, const bk_state* out, const bk_grid* grid,
-                       const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream);
+                       bk_workspace* ws, const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream);
 /* Adjoint of UpdateDing without pore gas (physics/materials/ding.py:140-221; arguments as bk_UpdateDing):
  * given adjoints of its outputs Fe, tau and porosity (any may be NULL = 0), ADDS the adjoints of
  * old_Fe, old_porosity, F_total, C_affine (NULL = not wanted) and of the viscosity parameter (one double). */
