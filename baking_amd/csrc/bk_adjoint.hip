@@ -599,6 +599,7 @@ __global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
 // component, instead of 81 global atomics per particle.
 __global__ __launch_bounds__(kTileThreads) void k_adj_scatter(const bk_dough* __restrict__ doughs, AdjArgs a, bk_workspace* ws) {
   __shared__ double acc[3][kRegionNodes];
+  __shared__ int cell_lo[64], cell_hi[64];
   const int ntiles = ws->counters[0];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bk_params& P = a.P;
@@ -608,11 +609,14 @@ __global__ __launch_bounds__(kTileThreads) void k_adj_scatter(const bk_dough* __
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tg = tile_at(doughs, ws, t);
     for (int i = threadIdx.x; i < 3 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
+    if (threadIdx.x < 64) {
+      cell_lo[threadIdx.x] = 0;
+      cell_hi[threadIdx.x] = 0;
+    }
     __syncthreads();
-    const int S = (tg.count + 63) / 64;
-    for (int sub = w; sub < S; sub += 4) {
-      const int j = lane * S + sub;
-      if (j >= tg.count) continue;
+    tile_cell_ranges(ws, tg, cell_lo, cell_hi);
+    const int hi = cell_hi[lane];
+    for (int j = cell_lo[lane] + w; j < hi; j += 4) {
       const int p = ws->sorted_vals[tg.start + j];
       float g3[3];
       grid_position(a.in.x + 3 * p, o, dx, g3);

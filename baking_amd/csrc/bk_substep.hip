@@ -388,7 +388,7 @@ __global__ __launch_bounds__(256) void k_reconstruct(const float* __restrict__ x
                                                      const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
                                                      const int32_t* __restrict__ parts, const float* __restrict__ wts,
                                                      const float* __restrict__ ref_offs, float* __restrict__ out, int nv) {
-  int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+  int v = xcd_block() * 4 + (threadIdx.x >> 6);
   if (v >= nv) return;
   reconstruct_vertex_wave(v, threadIdx.x & 63, x, F, ref_vertices, offs, parts, wts, ref_offs, out);
 }
@@ -456,14 +456,15 @@ __device__ __forceinline__ Robin robin_terms(const bk_params& P, float heat_para
   return r;
 }
 
-// In both P2G passes, lanes of one wave take a tile's particles S apart
-// (S = ceil(count / 64)) so that, with ~8 particles per cell, neighbouring
-// lanes sit in different cells and their LDS atomics hit different nodes.
+// In both P2G passes lane c of every wave takes the particles of cell c of
+// the tile's block (tile_cell_ranges), so one LDS atomic instruction touches
+// 64 distinct nodes.
 
 // ============================================================================ pass A: thermal P2G
 __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_grid g,
                                                          bk_workspace* ws) {
   __shared__ double acc[7][kRegionNodes];
+  __shared__ int cell_lo[64], cell_hi[64];
   const int ntiles = ws->counters[0];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -473,11 +474,14 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
     const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
     const float heat_param = P.settling ? 0.0f : sc.heat_param[tg.d];
     for (int i = threadIdx.x; i < 7 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
+    if (threadIdx.x < 64) {
+      cell_lo[threadIdx.x] = 0;
+      cell_hi[threadIdx.x] = 0;
+    }
     __syncthreads();
-    const int S = (tg.count + 63) / 64;
-    for (int sub = w; sub < S; sub += 4) {
-      int j = lane * S + sub;
-      if (j >= tg.count) continue;
+    tile_cell_ranges(ws, tg, cell_lo, cell_hi);
+    const int hi = cell_hi[lane];
+    for (int j = cell_lo[lane] + w; j < hi; j += 4) {
       int p = ws->sorted_vals[tg.start + j];
       float g3[3];
       grid_position(in.x + 3 * p, o, P.dx, g3);
@@ -590,7 +594,7 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
 // ============================================================================ pass B1: scalar G2P, gas, stresses
 __global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in,
                                                               bk_state out, bk_grid g, bk_workspace* ws, int n) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  int p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   per_dough(dough_of_particle(sc.dough_of, p), [&](int d) {
     const bk_params& P = doughs[d].params;
@@ -697,6 +701,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(const bk_dough* __rest
   // f64 LDS accumulators: ds_add_f64 sustains far higher throughput than ds_add_f32 on gfx950
   // for this scatter, and each node's sum is rounded to float once, at the flush.
   __shared__ double acc[4][kRegionNodes];
+  __shared__ int cell_lo[64], cell_hi[64];
   const int ntiles = ws->counters[0];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -705,11 +710,14 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(const bk_dough* __rest
     const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
     const float* masses = P.transport_enabled ? out.masses : sc.m0;
     for (int i = threadIdx.x; i < 4 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
+    if (threadIdx.x < 64) {
+      cell_lo[threadIdx.x] = 0;
+      cell_hi[threadIdx.x] = 0;
+    }
     __syncthreads();
-    const int S = (tg.count + 63) / 64;
-    for (int sub = w; sub < S; sub += 4) {
-      int j = lane * S + sub;
-      if (j >= tg.count) continue;
+    tile_cell_ranges(ws, tg, cell_lo, cell_hi);
+    const int hi = cell_hi[lane];
+    for (int j = cell_lo[lane] + w; j < hi; j += 4) {
       int p = ws->sorted_vals[tg.start + j];
       float g3[3];
       grid_position(in.x + 3 * p, o, P.dx, g3);

@@ -5,6 +5,23 @@
 #include "bk_device.h"
 #include "../../include/baking_abi.h"
 
+// Workgroups are dispatched to the 8 XCDs of an MI355X round-robin (workgroup b
+// to XCD b % 8), and each XCD has its own L2.  xcd_block() renumbers them so
+// that every XCD runs one contiguous range of logical workgroups: neighbouring
+// particles and mesh vertices -- which gather the same grid nodes and
+// particles -- then share one L2 instead of being spread over all eight.  A
+// bijection on [0, gridDim.x) for any grid size.  Used by the gather-bound
+// kernels (pass B1, mesh reconstruction): measured on the batched workload it
+// speeds them up, while it slows the tiled scatters (their halo flushes then
+// collide on one XCD) and the deferred return maps (spatially clustered work
+// lands on a few XCDs), which keep the round-robin order.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block() {
+  const int b = int(blockIdx.x), G = int(gridDim.x);
+  const int x = b % kXcds, i = b / kXcds, per = G / kXcds, rem = G % kXcds;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
 constexpr int kTileThreads = 256;  // P2G workgroup size
 constexpr int kRegion = 6;         // 4 cells + 2-node stencil reach
 constexpr int kRegionNodes = kRegion * kRegion * kRegion;
@@ -75,4 +92,22 @@ __device__ __forceinline__ Tile tile_at(const bk_dough* __restrict__ doughs, con
 }
 
 __device__ __forceinline__ int region_index(int li, int lj, int lk) { return (li * kRegion + lj) * kRegion + lk; }
+
+// A tile's particles are sorted by cell (the low 6 bits of the key).  Lanes of
+// the tiled scatters take one cell each (lane = cell index), and the waves of
+// the workgroup take its particles by rank within the cell.  Then the lanes of
+// one LDS atomic address 64 distinct stencil nodes: no address conflicts, and
+// with the (36, 6, 1) region strides the f64 atomics spread evenly over the
+// banks.  Fills lo[c], hi[c] with the tile-relative range of cell c (empty:
+// lo = hi = 0).  Call with the whole workgroup after zeroing lo and hi and a
+// barrier; ends with a barrier.
+__device__ __forceinline__ void tile_cell_ranges(const bk_workspace* ws, const Tile& tg, int* lo, int* hi) {
+  const uint32_t* k = ws->sorted_keys + tg.start;
+  for (int j = threadIdx.x; j < tg.count; j += blockDim.x) {
+    const uint32_t key = k[j];
+    if (j == 0 || k[j - 1] != key) lo[key & 63] = j;
+    if (j == tg.count - 1 || k[j + 1] != key) hi[key & 63] = j + 1;
+  }
+  __syncthreads();
+}
 
