@@ -66,7 +66,7 @@ class BkGrid(ctypes.Structure):
     _fields_ = [(name, c_void_p) for name in GRID_FIELDS]
 
 
-ADJOINT_FIELDS = ["x", "v", "C", "F", "Fe", "tau", "porosity", "moles"]
+ADJOINT_FIELDS = ["x", "v", "C", "F", "Fe", "tau", "porosity", "moles", "T", "W", "losses", "exchanges"]
 
 
 class BkAdjoint(ctypes.Structure):
@@ -141,6 +141,7 @@ SIGNATURES = {
     "bk_workspace_set_surrogate": ([P, P], c_int),
     "bk_workspace_set_samples": ([P, P, I], c_int),
     "bk_workspace_stats": ([P, P, I, P], c_int),
+    "bk_adjoint_scratch_size": ([POINTER(BkParams)], c_int64),
     "bk_adjoint_substep": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P,
                             POINTER(BkAdjoint), P, P, P], c_int),
     "bk_UpdateDing_adjoint": ([P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, F, F, F, F, I, P, P, P, F, I, P], c_int),

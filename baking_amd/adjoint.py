@@ -13,16 +13,16 @@ the library's adjoint kernels on it (``bk_adjoint_substep``: the Ding return
 map, G2P with the F update, the grid update, P2G with the stress).  Adjoints
 are FP64.
 
-Covered: the substep with the neo-Hookean model, or the Ding model with or
-without pore gas (the coupled porosity return and the gas pressure and
-source included); gradients with respect to the initial positions,
-velocities, affine velocities and deformation gradients, the Ding elastic
-gradients, porosities, Kirchhoff stresses and gas moles, and the Ding
-viscosity and gas production parameters.  Heat and water transport is
-differentiated only where it leaves temperature and water unchanged
-(conductivity, heat transfer and tray heat transfer 0, water off, as the
-reference's gas gradient test, tests/test_gas.py:120-160); ``rollout`` raises
-for transport that changes them.
+Covered: the substep with the neo-Hookean model or the Ding model (pore gas,
+the coupled porosity return and the temperature-dependent yield stress and
+viscosity included) and the heat and water transport (mesh areas, Robin
+terms, thermal P2G, FV diffusion, scalar G2P, gas source, water stress).
+Gradients with respect to the initial positions, velocities, affine
+velocities and deformation gradients, the Ding elastic gradients,
+porosities, Kirchhoff stresses and gas moles, the temperatures, water
+masses and loss/exchange accumulators, and the Ding viscosity, gas
+production and heat-transfer parameters.  Not covered: settling substeps
+(not provided by the forward either) and batched doughs.
 
     out = rollout(sim, steps, {"v": v0})          # tensors that may require grad
     loss = (out["x"][:, 0] - x0[:, 0]).mean()
@@ -51,28 +51,23 @@ OverstressAdjoint = Kernel("bk_overstress_adjoint", ["f64", "f64", "f64"])
 MECHANICAL_FIELDS = ("x", "v", "C", "F")
 DING_FIELDS = ("Fe", "porosity", "tau")
 GAS_FIELDS = ("moles",)
-_WIDTH = dict(x=3, v=3, C=9, F=9, Fe=9, tau=9, porosity=1, moles=1)
+TRANSPORT_FIELDS = ("T", "W", "losses", "exchanges")
+_WIDTH = dict(x=3, v=3, C=9, F=9, Fe=9, tau=9, porosity=1, moles=1, T=1, W=1, losses=1, exchanges=1)
 
 
 def _gas(sim):
     return isinstance(sim.material, DingMaterial) and sim.material.ding_config.gas_enabled
 
 
+def _transport(sim):
+    return getattr(sim, "transport", None) is not None
+
+
 def _fields_of(sim):
-    if not isinstance(sim.material, DingMaterial):
-        return MECHANICAL_FIELDS
-    return MECHANICAL_FIELDS + DING_FIELDS + (GAS_FIELDS if _gas(sim) else ())
-
-
-def _check_supported(sim):
-    t = getattr(sim, "transport", None)
-    if t is not None:
-        c = t.config
-        frozen = (c.conductivity == 0.0 and c.heat_transfer == 0.0 and c.tray_heat_transfer == 0.0 and not c.water_enabled
-                  and float(t.heat_transfer_parameter.detach().cpu()[0]) == 0.0)
-        if not frozen:
-            raise NotImplementedError("rollout: heat and water transport is differentiated only where it leaves temperature and "
-                                      "water unchanged (conductivity, heat_transfer and tray_heat_transfer 0, water off)")
+    fields = MECHANICAL_FIELDS
+    if isinstance(sim.material, DingMaterial):
+        fields += DING_FIELDS + (GAS_FIELDS if _gas(sim) else ())
+    return fields + (TRANSPORT_FIELDS if _transport(sim) else ())
 
 
 def _clone_state(state):
@@ -92,7 +87,7 @@ def _run(sim, count):
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, sim, steps, every, fields, visc, prod, *inputs):
+    def forward(ctx, sim, steps, every, fields, visc, prod, heat, *inputs):
         lib = _native.load()
         state = state_field_tensors(sim.current_state)
         for f, t in zip(fields, inputs):
@@ -102,6 +97,8 @@ class _Rollout(torch.autograd.Function):
             sim.material.viscosity_parameter.copy_(visc.detach().reshape(1).to(torch.float32))
         if _gas(sim):
             sim.material.production_parameter.copy_(prod.detach().reshape(1).to(torch.float32))
+        if _transport(sim):
+            sim.transport.heat_transfer_parameter.copy_(heat.detach().reshape(1).to(torch.float32))
         flags_before = sim._read_flags()
         checkpoints = []
         for start in range(0, steps, every):
@@ -115,7 +112,7 @@ class _Rollout(torch.autograd.Function):
         ctx.sim, ctx.steps, ctx.every, ctx.fields, ctx.checkpoints = sim, steps, every, fields, checkpoints
         ctx.shapes = [t.shape for t in inputs]
         ctx.dtypes = [t.dtype for t in inputs]
-        ctx.params = [(visc.shape, visc.dtype), (prod.shape, prod.dtype)]
+        ctx.params = [(visc.shape, visc.dtype), (prod.shape, prod.dtype), (heat.shape, heat.dtype)]
         ctx.lib = lib
         final = state_field_tensors(sim.current_state)
         return tuple(final[f].clone() for f in fields)
@@ -132,9 +129,8 @@ class _Rollout(torch.autograd.Function):
         bk_adj = _native.BkAdjoint()
         for f, t in adj.items():
             setattr(bk_adj, f, t.data_ptr())
-        param = torch.zeros(2, dtype=torch.float64, device=dev)
-        nodes = sim.grid_state.tiled_masses.numel()
-        scratch = torch.zeros(7 * nodes, dtype=torch.float64, device=dev)
+        param = torch.zeros(3, dtype=torch.float64, device=dev)
+        scratch = torch.zeros(int(lib.bk_adjoint_scratch_size(ctypes.byref(sim._params))), dtype=torch.float64, device=dev)
         # the simulator is left where the forward ended
         keep = (sim.current_state, sim.next_state, sim.step_count, _clone_state(sim.current_state), sim._read_flags())
         stream = _native.stream_ptr()
@@ -162,23 +158,25 @@ class _Rollout(torch.autograd.Function):
         out = [adj[f].reshape(shape).to(dtype) for f, shape, dtype in zip(fields, ctx.shapes, ctx.dtypes)]
         pgrads = [param[k:k + 1].reshape(shape).to(dtype) if shape.numel() else None
                   for k, (shape, dtype) in enumerate(ctx.params)]
-        return (None, None, None, None, *pgrads, *out)
+        return (None, None, None, None, *pgrads, *out)  # sim, steps, every, fields; parameters; fields
 
 
-def rollout(sim, steps, inputs=None, viscosity_parameter=None, production_parameter=None, checkpoint_every=16):
+def rollout(sim, steps, inputs=None, viscosity_parameter=None, production_parameter=None, heat_transfer_parameter=None,
+            checkpoint_every=16):
     """Run ``steps`` substeps of ``sim`` from ``inputs`` and return the final state as differentiable tensors.
 
-    ``inputs`` maps state fields (``x, v, C, F``; for the Ding model also ``Fe, porosity, tau``, and ``moles``
-    with pore gas) to tensors of the state's shapes (N x 3, N x 3, N x 3 x 3, ...); fields left out start
-    from the simulator's current state.  ``viscosity_parameter`` (Ding model) and ``production_parameter``
-    (pore gas), one-element tensors, replace the material's parameters.  Returns {field: tensor} for every
-    field; the simulator is left at the final state, as after ``Advance(steps)``.
+    ``inputs`` maps state fields (``x, v, C, F``; for the Ding model also ``Fe, porosity, tau``, ``moles``
+    with pore gas; with transport ``T, W, losses, exchanges``) to tensors of the state's shapes (N x 3,
+    N x 3, N x 3 x 3, ...); fields left out start from the simulator's current state.
+    ``viscosity_parameter`` (Ding model), ``production_parameter`` (pore gas) and
+    ``heat_transfer_parameter`` (transport), one-element tensors, replace the material's and the
+    transport's parameters.  Returns {field: tensor} for every field; the simulator is left at the final
+    state, as after ``Advance(steps)``.
     """
     if steps < 1:
         raise ValueError("steps must be positive")
     if checkpoint_every < 1:
         raise ValueError("checkpoint_every must be positive")
-    _check_supported(sim)
     inputs = dict(inputs or {})
     fields = _fields_of(sim)
     unknown = set(inputs) - set(fields)
@@ -206,5 +204,12 @@ def rollout(sim, steps, inputs=None, viscosity_parameter=None, production_parame
         if production_parameter is not None:
             raise ValueError("rollout: production_parameter needs the Ding model with pore gas")
         prod = torch.zeros(0, device=dev)
-    outs = _Rollout.apply(sim, int(steps), int(checkpoint_every), fields, visc, prod, *tensors)
+    if _transport(sim):
+        heat = (heat_transfer_parameter if heat_transfer_parameter is not None
+                else sim.transport.heat_transfer_parameter.detach().clone())
+    else:
+        if heat_transfer_parameter is not None:
+            raise ValueError("rollout: heat_transfer_parameter needs transport")
+        heat = torch.zeros(0, device=dev)
+    outs = _Rollout.apply(sim, int(steps), int(checkpoint_every), fields, visc, prod, heat, *tensors)
     return dict(zip(fields, outs))

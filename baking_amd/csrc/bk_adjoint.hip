@@ -23,7 +23,8 @@
 using namespace bk;
 
 void bk_internal_set_error(const char* what);  // bk_last_error's text (bk_substep.hip)
-int bk_internal_tile_launch(bk_workspace* ws, const bk_dough** doughs, int* n_doughs, int* grid_cap);  // bk_substep.hip
+int bk_internal_tile_launch(bk_workspace* ws, const bk_dough** doughs, int* n_doughs, int* grid_cap,
+                            const bk_workspace** device_view);  // bk_substep.hip
 
 namespace {
 int fail(const char* what) {
@@ -34,9 +35,9 @@ inline int div_up(long long a, long long b) { return int((a + b - 1) / b); }
 }  // namespace
 
 // ------------------------------------------------------------------ dual numbers
-// Value and derivatives along 5 directions: the 3 principal stretches, the
-// porosity coefficient a and the viscosity.
-constexpr int kDir = 5;
+// Value and derivatives along 6 directions: the 3 principal stretches, the
+// old porosity, the viscosity and the yield stress.
+constexpr int kDir = 6;
 struct Dual {
   double v;
   double d[kDir];
@@ -175,7 +176,7 @@ __device__ inline PorosityDerivs porosity_root_derivatives(double n, double g, d
 // From the principal stretches s, the old porosity and the viscosity eta: the
 // corrected principal Kirchhoff stresses cs, the returned principal stretches
 // e and the porosity n the map used (ding.py:103-131, 183-192), with their
-// derivatives along (s0, s1, s2, old porosity, eta).  Non-coupled, n is the
+// derivatives along (s0, s1, s2, old porosity, eta, yield stress y).  Non-coupled, n is the
 // old porosity; coupled, the end-state root the forward solved
 // (porous_return.py:43-61, its output porosity), with the derivatives above.
 struct Principal {
@@ -196,12 +197,14 @@ __device__ inline Principal principal_return(const double s[3], double n_old, do
   Dual dev[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) dev[i] = ts[i] - mean;
-  const double y2 = y * y;
-  const Dual q2 = (1.5 / y2) * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
-  const Dual pm = (1.0 / y2) * (mean * mean);
+  const Dual yd = dvar(y, 5);
+  const Dual y2 = yd * yd;
+  const Dual iy2 = dconst(1.0) / y2;
+  const Dual q2 = 1.5 * ((dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]) * iy2);
+  const Dual pm = (mean * mean) * iy2;
   const Dual rev = dconst(1.0) / ev;
-  const Dual df = (6.0 * mu * dt * y2) * rev;
-  const Dual ms = ((2.0 / 3.0) * (2.0 * mu + 3.0 * lam) * dt * y2) * rev;
+  const Dual df = (6.0 * mu * dt) * (y2 * rev);
+  const Dual ms = ((2.0 / 3.0) * (2.0 * mu + 3.0 * lam) * dt) * (y2 * rev);
   Principal P;
   P.n = dvar(coupled ? n_root : n_old, 3);
   if (coupled) P.n.d[3] = 0.0;
@@ -245,6 +248,16 @@ __device__ inline Principal principal_return(const double s[3], double n_old, do
 #pragma unroll
   for (int i = 0; i < 3; ++i) P.e[i] = dexp((1.0 / (2.0 * mu)) * (P.cs[i] - cm) + (1.0 / (2.0 * mu + 3.0 * lam)) * cm);
   return P;
+}
+
+// Slope of the piecewise-linear temperature factor (ding.py:56-65) at t: 0 below the
+// first knot and from the last knot on.
+__device__ __forceinline__ double temperature_slope(double t, const float k[4], const float f[4]) {
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (t >= double(k[i]) && t < double(k[i + 1])) s = (double(f[i + 1]) - double(f[i])) / (double(k[i + 1]) - double(k[i]));
+  return s;
 }
 
 // Divided difference (f_i - f_j) / (s_i - s_j) of a symmetric function of the
@@ -301,7 +314,7 @@ __device__ __forceinline__ void minv_t(const double* a, double* r) {
 __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe_old, const double* F, const double* C, double porosity,
                                            double n_root, int gelled_in, double T, double visc_param, const double* gFe_in,
                                            const double* gtau, double gn, double* gFe_old, double* gF, double* gC, double& gporosity,
-                                           double& geta0) {
+                                           double& geta0, double& gT) {
   int gel = gelled_in;
   double Teff = T;
   double yv = P.yield_stress, vfac = 1.0;
@@ -392,7 +405,7 @@ __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe
   mmul_tn(U, gtau, t1);
   mmul(t1, U, Nt);
   double M[9];
-  double gscal[kDir] = {0, 0, 0, 0, 0};
+  double gscal[kDir] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -420,6 +433,10 @@ __device__ inline void ding_return_adjoint(const DingParams& P, const double* Fe
   for (int i = 0; i < 9; ++i) gTrial[i] += G[i];
   gporosity += gscal[3];
   geta0 += gscal[4] * vfac;
+  // yield stress and viscosity factors of the temperature (ding.py:56-65, 160-170); a gelled particle
+  // takes them at the last knot, where they no longer depend on it
+  if (P.temperature_enabled && !gel)
+    gT += gscal[4] * visc_param * temperature_slope(T, P.knots, P.vf) + gscal[5] * double(P.yield_stress) * temperature_slope(T, P.knots, P.yf);
   // trial = (I + dt C) Fe_old
   double t2[9];
   mmul_tn(A, gTrial, t2);
@@ -453,8 +470,21 @@ struct AdjArgs {
   double* gm;  // grid adjoints (tiled nodes): mass, momentum (3), velocity (3)
   double* gp;
   double* gv;
-  double* param_grad;  // [0] viscosity parameter, [1] gas production parameter
+  double* param_grad;  // [0] viscosity parameter, [1] gas production parameter, [2] heat-transfer parameter
   int n, nb;
+  // transport (when on): forward scene and workspace data
+  const double* capacities;
+  const float* heat_param;
+  const int32_t *particle_slot, *slot_face_offsets, *slot_faces, *faces;
+  const int32_t *bind_offsets, *bind_particles;
+  const float *bind_weights, *bind_ref_offsets;
+  const double *slot_exposed, *slot_contact;
+  const float* vertices;
+  // transport adjoint scratch: per tiled node, per particle, per slot, per vertex
+  double *accT, *accM, *ghr, *gwr, *gV, *gC, *gE, *gW, *gH, *gL;
+  double *tcbar, *mrbar;
+  double *gSexp, *gScon;
+  double* gvert;
 };
 
 // Per particle (Ding model): the return map's adjoint.  Its outputs' adjoints
@@ -482,10 +512,11 @@ __global__ __launch_bounds__(128) void k_adj_ding(AdjArgs a) {
       gtau[i] = a.adj.tau[9 * p + i];
       gFe_old[i] = gF[i] = gC[i] = 0.0;
     }
-    double gpor = 0.0;
+    double gpor = 0.0, gT = 0.0;
     const float T_local = P.transport_enabled ? a.out.T_mat[p] : a.in.T_mat[p];
     ding_return_adjoint(D, Fe_old, Fn, Cn, double(a.in.porosity[p]), double(a.out.porosity[p]), a.in.gelled[p], double(T_local),
-                        double(a.visc_param[0]), gFe, gtau, a.adj.porosity[p], gFe_old, gF, gC, gpor, geta);
+                        double(a.visc_param[0]), gFe, gtau, a.adj.porosity[p], gFe_old, gF, gC, gpor, geta, gT);
+    if (P.transport_enabled) a.adj.T[p] += gT;  // T_mat = float(T_new)
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       a.adj.F[9 * p + i] += gF[i];
@@ -695,10 +726,13 @@ __global__ __launch_bounds__(256) void k_adj_grid(AdjArgs a, int64_t nodes) {
 }
 
 // Particle p of P2G's adjoint (transfer.py:36-70) with the stress's: neo-Hookean
-// (elastic.py:4-10), or the Ding state's plus the pore-gas pressure term
-// (gas.py:16-35) and the gas source (thermal_water.py:296-330).  Completes the
-// input state's adjoint; returns p's share of the production-parameter gradient.
-__device__ inline double adj_p2g_particle(const AdjArgs& a, int p) {
+// (elastic.py:4-10) or the Ding state's, plus the pore-gas pressure term
+// (gas.py:16-35) and the water stress (thermal_water.py:248-253) when
+// transport is on.  Completes the mechanical part of the input state's
+// adjoint; the mass and the new temperature and water feed adj.W / adj.T,
+// which hold the adjoints of the substep's new T and W until the transport
+// adjoint (k_adj_b1) turns them into the old ones.
+__device__ inline void adj_p2g_particle(const AdjArgs& a, int p) {
   const bk_params& P = a.P;
   const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
   const float dx = P.dx;
@@ -706,30 +740,31 @@ __device__ inline double adj_p2g_particle(const AdjArgs& a, int p) {
   grid_position(a.in.x + 3 * p, o, dx, g3);
   int b[3];
   stencil_base(g3, b);
-  if (!valid_stencil(b, P.res)) return 0.0;
+  if (!valid_stencil(b, P.res)) return;
   const float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
   double wx[3], wy[3], wz[3], dwx[3], dwy[3], dwz[3];
   axis_weight_grads(l[0], wx, dwx);
   axis_weight_grads(l[1], wy, dwy);
   axis_weight_grads(l[2], wz, dwz);
-  const double mass = P.transport_enabled ? double(a.out.masses[p]) : double(a.masses[p]);
+  const bool transport = P.transport_enabled != 0;
+  const double mass = transport ? double(a.out.masses[p]) : double(a.masses[p]);
   const double s_tau = double(P.dt) * double(a.V0[p]) * 4.0 / (double(dx) * double(dx));
   // the stress P2G used: the state's for the Ding model, mu (F F^T - I) + lam log(J) I of the input F otherwise
   double tau[9], F[9];
   bool stressed = true;
 #pragma unroll
   for (int i = 0; i < 9; ++i) F[i] = a.in.F[9 * p + i];
+  const double J = mdet(F);
   // pore gas: tau += J (p_ambient - p) I = (J p_ambient - N R T / n) I, N the substep's new moles
   bool gas = false;
-  double J = 0.0, moles = 0.0, Tg = 0.0, nv = 0.0;
+  double moles = 0.0, Tg = 0.0, nv = 0.0;
   if (P.model == 1) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) tau[i] = a.in.tau[9 * p + i];
     if (P.gas_enabled) {
       nv = a.in.porosity[p];
-      J = mdet(F);
       moles = a.out.moles[p];
-      Tg = P.transport_enabled ? a.out.T_mat[p] : a.in.T_mat[p];
+      Tg = transport ? a.out.T_mat[p] : a.in.T_mat[p];
       gas = nv > 0.0 && nv < 1.0 && J > 0.0;
       if (gas) {
         const double add = J * double(P.p_ambient) - moles * kGasConstant * Tg / nv;
@@ -739,18 +774,29 @@ __device__ inline double adj_p2g_particle(const AdjArgs& a, int p) {
       }
     }
   } else {
-    const double J = mdet(F);
     stressed = isfinite(J) && J > 0.0;
     mmul_nt(F, F, tau);
 #pragma unroll
     for (int i = 0; i < 9; ++i) tau[i] = stressed ? double(P.mu) * (tau[i] - ((i % 4 == 0) ? 1.0 : 0.0)) + ((i % 4 == 0) ? double(P.lam) * log(J) : 0.0) : 0.0;
   }
-  double Am[9], vv[3];
+  // water stress: tau -= J c (s - s0) I with s = W_new / (rho_w V0 J): k = c (W_new / (rho_w V0) - J s0)
+  const double cw = (transport && P.water_enabled) ? double(P.water_stress) : 0.0;
+  const double rw_v0 = double(P.rho_w) * double(a.V0[p]);
+  if (cw != 0.0) {
+    const double k = cw * (a.out.W[p] / rw_v0 - J * double(P.s0));
+    tau[0] -= k;
+    tau[4] -= k;
+    tau[8] -= k;
+  }
+  double Am[9], vv[3], Cp[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) Am[i] = mass * double(a.in.C[9 * p + i]) - s_tau * tau[i];
+  for (int i = 0; i < 9; ++i) {
+    Cp[i] = a.in.C[9 * p + i];
+    Am[i] = mass * Cp[i] - s_tau * tau[i];
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i) vv[i] = a.in.v[3 * p + i];
-  double gx[3] = {0, 0, 0}, gvp[3] = {0, 0, 0}, gA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double gx[3] = {0, 0, 0}, gvp[3] = {0, 0, 0}, gA[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, gmass = 0.0;
   const double inv_dx = 1.0 / double(dx);
   for (int ox = 0; ox < 3; ++ox)
     for (int oy = 0; oy < 3; ++oy)
@@ -765,6 +811,13 @@ __device__ inline double adj_p2g_particle(const AdjArgs& a, int p) {
 #pragma unroll
         for (int r = 0; r < 3; ++r) Add[r] = Am[r * 3] * dd[0] + Am[r * 3 + 1] * dd[1] + Am[r * 3 + 2] * dd[2];
         const double val = gmi * mass + gpi[0] * (mass * vv[0] + Add[0]) + gpi[1] * (mass * vv[1] + Add[1]) + gpi[2] * (mass * vv[2] + Add[2]);
+        if (transport) {
+          // d/d mass of m_i = w m and p_i = w (m v + (m C - ...) dd)
+          double vcd = gmi;
+#pragma unroll
+          for (int r = 0; r < 3; ++r) vcd += gpi[r] * (vv[r] + Cp[r * 3] * dd[0] + Cp[r * 3 + 1] * dd[1] + Cp[r * 3 + 2] * dd[2]);
+          gmass += w * vcd;
+        }
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
           // d dd / dx = -I: -(A^T gp)_r
@@ -787,71 +840,460 @@ __device__ inline double adj_p2g_particle(const AdjArgs& a, int p) {
   double gtau[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) gtau[i] = -s_tau * gA[i];
-  double gprod = 0.0;
+  const double tr = gtau[0] + gtau[4] + gtau[8];
+  double gJ = 0.0;  // adjoint of det F from the gas and water terms
+  if (transport) {
+    // masses = dry + W_new; the water stress through W_new
+    a.adj.W[p] += gmass - tr * cw / rw_v0;
+    gJ += tr * cw * double(P.s0);
+  }
   if (P.model == 1) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) a.adj.tau[9 * p + i] = gtau[i];
-    if (P.gas_enabled) {
-      double gmoles = a.adj.moles[p];  // adjoint of the new moles
-      if (gas) {
-        const double tr = gtau[0] + gtau[4] + gtau[8];
-        double fit[9];
-        minv_t(F, fit);
-#pragma unroll
-        for (int i = 0; i < 9; ++i) a.adj.F[9 * p + i] += tr * double(P.p_ambient) * J * fit[i];
-        a.adj.porosity[p] += tr * moles * kGasConstant * Tg / (nv * nv);
-        gmoles -= tr * kGasConstant * Tg / nv;
-      }
-      // N_new = N_old + dt rho alpha exp(-z^2), z = ((T_old + T_new) / 2 - T_peak) / width
-      a.adj.moles[p] = gmoles;
-      if (P.transport_enabled && !P.settling) {
-        const double z = (0.5 * (a.in.T[p] + a.out.T[p]) - double(P.gas_peak)) / double(P.gas_width);
-        gprod = gmoles * double(P.dt * P.density) * exp(-(z * z));
-      }
+    if (gas) {
+      gJ += tr * double(P.p_ambient);
+      a.adj.porosity[p] += tr * moles * kGasConstant * Tg / (nv * nv);
+      a.adj.moles[p] -= tr * kGasConstant * Tg / nv;  // adjoint of the new moles
+      if (transport) a.adj.T[p] -= tr * moles * kGasConstant / nv;  // T_mat = float(T_new)
     }
   } else if (stressed) {
     // neo-Hookean Kirchhoff stress mu (F F^T - I) + lam log(J) I: gF += mu (gtau + gtau^T) F + lam tr(gtau) F^-T
-    double fit[9], sym[9], t[9];
-    minv_t(F, fit);
+    double sym[9], t[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) sym[i * 3 + j] = gtau[i * 3 + j] + gtau[j * 3 + i];
     mmul(sym, F, t);
-    const double tr = gtau[0] + gtau[4] + gtau[8];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) a.adj.F[9 * p + i] += double(P.mu) * t[i] + double(P.lam) * tr * fit[i];
+    for (int i = 0; i < 9; ++i) a.adj.F[9 * p + i] += double(P.mu) * t[i];
+    gJ += double(P.lam) * tr / J;
   }
-  return gprod;
+  if (gJ != 0.0) {
+    double fit[9];
+    minv_t(F, fit);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.adj.F[9 * p + i] += gJ * J * fit[i];
+  }
 }
 
 __global__ __launch_bounds__(128) void k_adj_p2g(AdjArgs a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  double gprod = p < a.n ? adj_p2g_particle(a, p) : 0.0;
+  if (p < a.n) adj_p2g_particle(a, p);
+}
+
+// ------------------------------------------------------------------ transport adjoint
+// The forward transport of one substep (thermal_water.py:95-260, fused as the
+// substep's mesh, pass A, diffusion and pass B1 stages), in reverse:
+//   B1   T' = T + sum_i w dT_i;  W' = W (1 + sum_{W_i > 0} w dW_i / W_i);
+//        N' = N + dt rho alpha exp(-z^2), z = ((T + T')/2 - T_peak) / width
+//   diff dT_i = dt hr_i / C_i, dW_i = dt wr_i over the 6-neighbour FV stencil
+//   A    V_i, C_i, E_i, W_i, H_i, L_i = sum_p w (V0 J, cap, cap T, W, heat, loss)
+//        with the Robin heat and loss of each particle's exposed / contact area
+//   mesh areas of the boundary faces from the vertices each particle binds.
+
+// Particle p: the B1 adjoint.  In: adj.T / adj.W / adj.moles hold the adjoints
+// of the new T, W and moles.  Out: those of the old ones (the thermal P2G
+// adjoint adds more), the scatter weights of dT and W_i-scaled dW, the
+// positions' share through the gather weights, and the production parameter's
+// gradient (returned).
+__device__ inline double adj_b1_particle(const AdjArgs& a, int p) {
+  const bk_params& P = a.P;
+  const double Told = a.in.T[p], Tnew = a.out.T[p], Wold = a.in.W[p];
+  double gTn = a.adj.T[p], gWn = a.adj.W[p];
+  double gT = 0.0, gprod = 0.0;
+  if (P.model == 1 && P.gas_enabled && !P.settling) {
+    const double gN = a.adj.moles[p];  // N' = N + source: the old moles' adjoint is the same
+    const double z = (0.5 * (Told + Tnew) - double(P.gas_peak)) / double(P.gas_width);
+    const double e = exp(-(z * z));
+    const double src = double(P.dt * P.density) * double(a.prod_param[0]) * e;
+    gprod = gN * double(P.dt * P.density) * e;
+    const double gz = gN * src * (-2.0 * z) * 0.5 / double(P.gas_width);
+    gT += gz;
+    gTn += gz;
+  }
+  // gather: tc = sum w dT_i, mr = sum_{W_i > 0} w dW_i / W_i
+  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+  float g3[3];
+  grid_position(a.in.x + 3 * p, o, P.dx, g3);
+  int b[3];
+  stencil_base(g3, b);
+  double mr = 0.0, gx[3] = {0.0, 0.0, 0.0};
+  if (valid_stencil(b, P.res)) {
+    const float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+    double wx[3], wy[3], wz[3], dwx[3], dwy[3], dwz[3];
+    axis_weight_grads(l[0], wx, dwx);
+    axis_weight_grads(l[1], wy, dwy);
+    axis_weight_grads(l[2], wz, dwz);
+    const double inv_dx = 1.0 / double(P.dx);
+    const double gmr = gWn * Wold;
+    for (int ox = 0; ox < 3; ++ox)
+      for (int oy = 0; oy < 3; ++oy)
+        for (int oz = 0; oz < 3; ++oz) {
+          const int64_t node = tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, a.nb);
+          const double nw = a.g.W[node];
+          const double q = nw > 0.0 ? a.g.dW[node] / nw : 0.0;
+          mr += wx[ox] * wy[oy] * wz[oz] * q;
+          const double val = gTn * a.g.dT[node] + gmr * q;
+          gx[0] += dwx[ox] * wy[oy] * wz[oz] * inv_dx * val;
+          gx[1] += wx[ox] * dwy[oy] * wz[oz] * inv_dx * val;
+          gx[2] += wx[ox] * wy[oy] * dwz[oz] * inv_dx * val;
+        }
+  }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) a.adj.x[3 * p + r] += gx[r];
+  a.tcbar[p] = gTn;         // adjoint of tc
+  a.mrbar[p] = gWn * Wold;  // adjoint of mr
+  a.adj.T[p] = gT + gTn;    // T' = T + tc
+  a.adj.W[p] = gWn * (1.0 + mr);
+  return gprod;
+}
+
+__global__ __launch_bounds__(128) void k_adj_b1(AdjArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double gprod = p < a.n ? adj_b1_particle(a, p) : 0.0;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) gprod += __shfl_xor(gprod, off, 64);
   if (__lane_id() == 0 && gprod != 0.0) unsafeAtomicAdd(a.param_grad + 1, gprod);
 }
 
+// The B1 gathers' scatter: per node sum_p w tc_bar and sum_p w mr_bar over the
+// forward substep's P2G tiles (one cell per lane, FP64 LDS, one global atomic
+// per node and sum); k_adj_diffuse_a applies the per-node factors.
+__global__ __launch_bounds__(kTileThreads) void k_adj_scatter_b1(const bk_dough* __restrict__ doughs, AdjArgs a, bk_workspace* ws) {
+  __shared__ double acc[2][kRegionNodes];
+  __shared__ int cell_lo[64], cell_hi[64];
+  const int ntiles = ws->counters[0];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bk_params& P = a.P;
+  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const Tile tg = tile_at(doughs, ws, t);
+    for (int i = threadIdx.x; i < 2 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
+    if (threadIdx.x < 64) {
+      cell_lo[threadIdx.x] = 0;
+      cell_hi[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    tile_cell_ranges(ws, tg, cell_lo, cell_hi);
+    const int hi = cell_hi[lane];
+    for (int j = cell_lo[lane] + w; j < hi; j += 4) {
+      const int p = ws->sorted_vals[tg.start + j];
+      float g3[3];
+      grid_position(a.in.x + 3 * p, o, P.dx, g3);
+      int b[3];
+      stencil_base(g3, b);
+      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.g.bx || (b[1] >> 2) != tg.g.by || (b[2] >> 2) != tg.g.bz) continue;
+      const float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+      float wxf[3], wyf[3], wzf[3];
+      axis_weights(l[0], wxf);
+      axis_weights(l[1], wyf);
+      axis_weights(l[2], wzf);
+      const double tb = a.tcbar[p], mb = a.mrbar[p];
+      const int li = b[0] - 4 * tg.g.bx, lj = b[1] - 4 * tg.g.by, lk = b[2] - 4 * tg.g.bz;
+#pragma unroll
+      for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+          for (int oz = 0; oz < 3; ++oz) {
+            const double wt = double(wxf[ox]) * double(wyf[oy]) * double(wzf[oz]);
+            const int ri = region_index(li + ox, lj + oy, lk + oz);
+            atomicAdd(&acc[0][ri], wt * tb);
+            atomicAdd(&acc[1][ri], wt * mb);
+          }
+    }
+    __syncthreads();
+    for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
+      const int li = ri / 36, lj = (ri / 6) % 6, lk = ri % 6;
+      const int i = 4 * tg.g.bx + li, j = 4 * tg.g.by + lj, k = 4 * tg.g.bz + lk;
+      if (i >= P.res || j >= P.res || k >= P.res) continue;
+      const int64_t node = tg.g.node_base + tiled_node(i, j, k, tg.g.nb);
+      if (acc[0][ri] != 0.0) unsafeAtomicAdd(a.accT + node, acc[0][ri]);
+      if (acc[1][ri] != 0.0) unsafeAtomicAdd(a.accM + node, acc[1][ri]);
+    }
+    __syncthreads();
+  }
+}
+
+// Per active node with capacity: the diffusion step's own adjoint,
+// dT_i = dt hr_i / C_i and dW_i = dt wr_i (thermal_water.py:150-190), and the
+// W_i of B1's dW_i / W_i.  Leaves hr_bar, wr_bar for the neighbour gather and
+// starts the C and W adjoints; clears the scatter sums.
+__global__ __launch_bounds__(256) void k_adj_diffuse_a(AdjArgs a, bk_workspace* ws) {
+  const int nact = ws->counters[1];
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
+    const int blk = __builtin_amdgcn_readfirstlane(ws->active[t >> 6]);
+    const int64_t node = (int64_t(blk) << 6) | (t & 63);
+    const double Ci = a.g.C[node];
+    if (!(Ci > 0.0)) continue;
+    const double gdT = a.accT[node], sM = a.accM[node];
+    a.accT[node] = 0.0;
+    a.accM[node] = 0.0;
+    const double Wi = a.g.W[node];
+    const double gdW = Wi > 0.0 ? sM / Wi : 0.0;
+    const double dt = double(a.P.dt);
+    a.ghr[node] = dt * gdT / Ci;
+    a.gwr[node] = dt * gdW;
+    a.gC[node] = -gdT * a.g.dT[node] / Ci;
+    a.gW[node] = Wi > 0.0 ? -sM * a.g.dW[node] / (Wi * Wi) : 0.0;
+  }
+}
+
+// Per active node with capacity: the FV fluxes' adjoint, gathered from the
+// node's own terms and its neighbours' (whose sums hold (T_i - T_j) and
+// (S_i - S_j) with the same symmetric fill), then T = E / C and S = W / (rho_w V).
+__global__ __launch_bounds__(256) void k_adj_diffuse_b(AdjArgs a, bk_workspace* ws) {
+  const int nact = ws->counters[1];
+  const bk_params& P = a.P;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
+    const int blk = __builtin_amdgcn_readfirstlane(ws->active[t >> 6]);
+    const int loc = t & 63;
+    const int64_t node = (int64_t(blk) << 6) | loc;
+    const double Ci = a.g.C[node];
+    if (!(Ci > 0.0)) continue;
+    const int nb = a.nb;
+    const int bz = blk % nb, by = (blk / nb) % nb, bx = blk / (nb * nb);
+    const int c[3] = {4 * bx + (loc >> 4), 4 * by + ((loc >> 2) & 3), 4 * bz + (loc & 3)};
+    const double rho_w = double(P.rho_w);
+    const double dx3 = double(P.dx * P.dx * P.dx);
+    const double hk = double(P.kappa * P.dx);
+    const double wk = double(P.diffusivity * P.rho_w * P.dx);
+    const double Vi = a.g.V[node], Wi = a.g.W[node];
+    const double Ti = a.g.E[node] / Ci, Si = Wi / (rho_w * Vi);
+    const double hri = a.ghr[node], wri = a.gwr[node];
+    double gTi = 0.0, gSi = 0.0, gVi = 0.0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax)
+#pragma unroll
+      for (int side = 0; side < 2; ++side) {
+        int q[3] = {c[0], c[1], c[2]};
+        q[ax] += 2 * side - 1;
+        if (q[ax] < 0 || q[ax] >= P.res) continue;
+        const int64_t other = tiled_node(q[0], q[1], q[2], nb);
+        const double Cj = a.g.C[other];
+        if (!(Cj > 0.0)) continue;
+        const double Vj = a.g.V[other];
+        const double Tj = a.g.E[other] / Cj;
+        const double fill = 2.0 * Vi * Vj / (Vi + Vj) / dx3;
+        const double dfill = 2.0 * Vj * Vj / ((Vi + Vj) * (Vi + Vj)) / dx3;  // d fill / d V_i
+        const double hrj = a.ghr[other];
+        // heat: node i's term hk fill (Tj - Ti), node j's term hk fill (Ti - Tj)
+        gTi += hk * fill * (hrj - hri);
+        gVi += hk * dfill * (Tj - Ti) * (hri - hrj);
+        if (P.water_enabled) {
+          const double Sj = a.g.W[other] / (rho_w * Vj);
+          const double wrj = a.gwr[other];
+          const double tm = 0.5 * (Ti + Tj), sm = 0.5 * (Si + Sj);
+          const double mob = wk * fill * tm * sm;
+          // node i's term mob (Sj - Si), node j's term mob (Si - Sj): both through mob's dependence on i
+          const double A = (Sj - Si) * (wri - wrj);
+          gSi += mob * (wrj - wri) + A * wk * fill * tm * 0.5;
+          gTi += A * wk * fill * 0.5 * sm;
+          gVi += A * wk * dfill * tm * sm;
+        }
+      }
+    a.gH[node] = hri;
+    a.gL[node] = -wri;
+    a.gE[node] = gTi / Ci;
+    a.gC[node] += -gTi * Ti / Ci;
+    a.gW[node] += gSi / (rho_w * Vi);
+    a.gV[node] = gVi - gSi * Si / Vi;
+  }
+}
+
+// Particle p: the thermal P2G's adjoint (thermal_water.py:95-137) with the
+// Robin terms' (108-120) -- also those of B1's loss and exchange accumulators,
+// which use the same inputs.  Adds to adj.T / adj.W / adj.x / adj.F, sets the
+// adjoints of p's exposed and contact areas, returns the heat-transfer
+// parameter's gradient.
+__device__ inline double adj_pass_a_particle(const AdjArgs& a, int p) {
+  const bk_params& P = a.P;
+  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+  const double T = a.in.T[p], W = a.in.W[p], cap = a.capacities[p];
+  double F[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = a.in.F[9 * p + i];
+  const double J = mdet(F);
+  const double vol = double(a.V0[p]) * J;
+  const int slot = a.particle_slot[p];
+  const double Sexp = slot >= 0 ? a.slot_exposed[slot] : 0.0, Scon = slot >= 0 ? a.slot_contact[slot] : 0.0;
+  const double h = P.settling ? 0.0 : double(a.heat_param[0]);
+  const double rho_w = double(P.rho_w);
+  const double sat = W / (rho_w * vol);
+  const double beta = P.water_enabled ? double(P.loss_coef) : 0.0;
+  const double loss = beta * T * sat * Sexp;
+  const double heat = h * Sexp * (double(P.T_oven) - T) + double(P.tray_h) * Scon * (double(P.T_tray) - T) - double(P.latent) * loss;
+  // accumulators: losses' = losses + dt loss, exchanges' = exchanges + dt heat
+  double gheat = double(P.dt) * a.adj.exchanges[p], gloss = double(P.dt) * a.adj.losses[p];
+  double gvol = 0.0, gT = 0.0, gW = 0.0, gx[3] = {0.0, 0.0, 0.0};
+  float g3[3];
+  grid_position(a.in.x + 3 * p, o, P.dx, g3);
+  int b[3];
+  stencil_base(g3, b);
+  if (valid_stencil(b, P.res)) {
+    const float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
+    double wx[3], wy[3], wz[3], dwx[3], dwy[3], dwz[3];
+    axis_weight_grads(l[0], wx, dwx);
+    axis_weight_grads(l[1], wy, dwy);
+    axis_weight_grads(l[2], wz, dwz);
+    const double inv_dx = 1.0 / double(P.dx);
+    for (int ox = 0; ox < 3; ++ox)
+      for (int oy = 0; oy < 3; ++oy)
+        for (int oz = 0; oz < 3; ++oz) {
+          const int64_t node = tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, a.nb);
+          const double w = wx[ox] * wy[oy] * wz[oz];
+          const double gVn = a.gV[node], gCn = a.gC[node], gEn = a.gE[node], gWn = a.gW[node], gHn = a.gH[node], gLn = a.gL[node];
+          gvol += w * gVn;
+          gT += w * cap * gEn;
+          gW += w * gWn;
+          gheat += w * gHn;
+          gloss += w * gLn;
+          const double val = vol * gVn + cap * gCn + cap * T * gEn + W * gWn + heat * gHn + loss * gLn;
+          gx[0] += dwx[ox] * wy[oy] * wz[oz] * inv_dx * val;
+          gx[1] += wx[ox] * dwy[oy] * wz[oz] * inv_dx * val;
+          gx[2] += wx[ox] * wy[oy] * dwz[oz] * inv_dx * val;
+        }
+  }
+  // Robin terms
+  gloss -= double(P.latent) * gheat;
+  gT += -(h * Sexp + double(P.tray_h) * Scon) * gheat + beta * sat * Sexp * gloss;
+  const double gsat = beta * T * Sexp * gloss;
+  const double gSexp = h * (double(P.T_oven) - T) * gheat + beta * T * sat * gloss;
+  const double gScon = double(P.tray_h) * (double(P.T_tray) - T) * gheat;
+  const double gh = Sexp * (double(P.T_oven) - T) * gheat;
+  gW += gsat / (rho_w * vol);
+  gvol -= gsat * sat / vol;
+  a.adj.T[p] += gT;
+  a.adj.W[p] += gW;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) a.adj.x[3 * p + r] += gx[r];
+  if (gvol != 0.0) {
+    double fit[9];
+    minv_t(F, fit);
+    const double gJ = gvol * double(a.V0[p]);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a.adj.F[9 * p + i] += gJ * J * fit[i];
+  }
+  if (slot >= 0) {
+    a.gSexp[slot] = gSexp;
+    a.gScon[slot] = gScon;
+  }
+  return P.settling ? 0.0 : gh;
+}
+
+__global__ __launch_bounds__(128) void k_adj_pass_a(AdjArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double gh = p < a.n ? adj_pass_a_particle(a, p) : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) gh += __shfl_xor(gh, off, 64);
+  if (__lane_id() == 0 && gh != 0.0) unsafeAtomicAdd(a.param_grad + 2, gh);
+}
+
+// Per owner slot: the boundary faces' areas (thermal_water.py:77-92) in reverse,
+// area = |(b - a) x (c - a)| / 2 summed into the slot's exposed or contact total
+// (the contact test is a discrete choice): vertex adjoints, FP64 atomics.
+__global__ __launch_bounds__(256) void k_adj_faces(AdjArgs a, int nslots) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const bk_params& P = a.P;
+  const double ge = a.gSexp[s], gc = a.gScon[s];
+  if (ge == 0.0 && gc == 0.0) return;
+  const float floor_h = P.origin[1] + float(P.boundary_cells - 1) * P.dx;
+  const double fcd = double(floor_h + P.contact_distance);
+  for (int k = a.slot_face_offsets[s]; k < a.slot_face_offsets[s + 1]; ++k) {
+    const int32_t* f = a.faces + 3 * a.slot_faces[k];
+    double Pv[3][3];
+    for (int q = 0; q < 3; ++q)
+      for (int r = 0; r < 3; ++r) Pv[q][r] = double(a.vertices[3 * f[q] + r]);
+    double e1[3], e2[3];
+    for (int r = 0; r < 3; ++r) { e1[r] = Pv[1][r] - Pv[0][r]; e2[r] = Pv[2][r] - Pv[0][r]; }
+    const double nx = e1[1] * e2[2] - e1[2] * e2[1], ny = e1[2] * e2[0] - e1[0] * e2[2], nz = e1[0] * e2[1] - e1[1] * e2[0];
+    const double len = sqrt(nx * nx + ny * ny + nz * nz);
+    const double cy = (Pv[0][1] + Pv[1][1] + Pv[2][1]) / 3.0;
+    const bool contact = cy <= fcd && ny < -0.2 * len;
+    const double ga = contact ? gc : ge;
+    if (ga == 0.0 || !(len > 0.0)) continue;
+    const double u[3] = {nx / len, ny / len, nz / len};
+    // d area / d e1 = (e2 x u) / 2, d area / d e2 = (u x e1) / 2
+    const double d1[3] = {0.5 * (e2[1] * u[2] - e2[2] * u[1]), 0.5 * (e2[2] * u[0] - e2[0] * u[2]), 0.5 * (e2[0] * u[1] - e2[1] * u[0])};
+    const double d2[3] = {0.5 * (u[1] * e1[2] - u[2] * e1[1]), 0.5 * (u[2] * e1[0] - u[0] * e1[2]), 0.5 * (u[0] * e1[1] - u[1] * e1[0])};
+    for (int r = 0; r < 3; ++r) {
+      unsafeAtomicAdd(a.gvert + 3 * f[1] + r, ga * d1[r]);
+      unsafeAtomicAdd(a.gvert + 3 * f[2] + r, ga * d2[r]);
+      unsafeAtomicAdd(a.gvert + 3 * f[0] + r, -ga * (d1[r] + d2[r]));
+    }
+  }
+}
+
+// One wavefront per vertex: mesh reconstruction's adjoint (mesh_tracking.py:13-38).
+// v = (1 - sum_b w_b) anchor + sum_b w_b (x_b + F_b r_b), anchor = x_s + F_s r_s of
+// the first binding.  Scatters to the bound particles' x and F (FP64 atomics) and
+// clears the vertex adjoint.
+__global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs a, int nv) {
+  const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (v >= nv) return;
+  const double gv[3] = {a.gvert[3 * v], a.gvert[3 * v + 1], a.gvert[3 * v + 2]};
+  if (gv[0] == 0.0 && gv[1] == 0.0 && gv[2] == 0.0) return;
+  const int s = a.bind_offsets[v], e = a.bind_offsets[v + 1];
+  double wsum = 0.0;
+  for (int k = s + lane; k < e; k += 64) {
+    const int pi = a.bind_particles[k];
+    const double wk = a.bind_weights[k];
+    wsum += wk;
+    const float* r = a.bind_ref_offsets + 3 * k;
+    for (int q = 0; q < 3; ++q) {
+      unsafeAtomicAdd(a.adj.x + 3 * pi + q, wk * gv[q]);
+      for (int c = 0; c < 3; ++c) unsafeAtomicAdd(a.adj.F + 9 * pi + 3 * q + c, wk * gv[q] * double(r[c]));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off, 64);
+  if (lane == 0) {
+    if (s < e) {
+      const int ps = a.bind_particles[s];
+      const double ca = 1.0 - wsum;
+      const float* r = a.bind_ref_offsets + 3 * s;
+      for (int q = 0; q < 3; ++q) {
+        unsafeAtomicAdd(a.adj.x + 3 * ps + q, ca * gv[q]);
+        for (int c = 0; c < 3; ++c) unsafeAtomicAdd(a.adj.F + 9 * ps + 3 * q + c, ca * gv[q] * double(r[c]));
+      }
+    }
+    a.gvert[3 * v] = a.gvert[3 * v + 1] = a.gvert[3 * v + 2] = 0.0;
+  }
+}
+
 // ------------------------------------------------------------------ C ABI
 extern "C" {
 
+int64_t bk_adjoint_scratch_size(const bk_params* P) {
+  if (!P || P->res < 4 || P->n < 1) return -1;
+  const int64_t nb = (P->res + 3) / 4, nodes = nb * nb * nb * 64;
+  int64_t size = 7 * nodes;
+  if (P->transport_enabled) size += 10 * nodes + 2 * int64_t(P->n) + 2 * int64_t(P->n_slots) + 3 * int64_t(P->n_vertices);
+  return size;
+}
+
 int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, const bk_state* out, const bk_grid* grid,
-                       bk_workspace* ws, const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream) {
-  if (!P || !sc || !in || !out || !grid || !ws || !adj || !grid_scratch || !param_grad)
-    return fail("bk_adjoint_substep: null argument");
+                       bk_workspace* ws, const bk_adjoint* adj, double* scratch, double* param_grad, void* stream) {
+  if (!P || !sc || !in || !out || !grid || !ws || !adj || !scratch || !param_grad) return fail("bk_adjoint_substep: null argument");
   const bk_dough* doughs = nullptr;
+  const bk_workspace* wsv = nullptr;
   int n_doughs = 0, grid_cap = 0;
-  if (bk_internal_tile_launch(ws, &doughs, &n_doughs, &grid_cap)) return -1;
+  if (bk_internal_tile_launch(ws, &doughs, &n_doughs, &grid_cap, &wsv)) return -1;
   if (n_doughs != 1) return fail("bk_adjoint_substep: the workspace must hold one dough (the simulator's own)");
-  if (P->transport_enabled && (P->kappa != 0.0f || P->tray_h != 0.0f || P->water_enabled || P->settling))
-    return fail("bk_adjoint_substep: transport is differentiated only where it leaves temperature and water unchanged "
-                "(conductivity, tray and heat transfer 0, water off)");
+  if (P->n < 1 || P->res < 4) return fail("bk_adjoint_substep: bad sizes");
+  if (P->settling) return fail("bk_adjoint_substep: settling substeps have no adjoint");
+  if (P->model == 1 && (!sc->visc_param || !in->Fe || !adj->Fe || !adj->tau || !adj->porosity))
+    return fail("bk_adjoint_substep: the Ding adjoint needs Fe, tau and porosity arrays");
   if (P->model == 1 && P->gas_enabled && (!P->transport_enabled || !in->moles || !out->moles || !adj->moles || !sc->prod_param))
     return fail("bk_adjoint_substep: the pore-gas adjoint needs transport, moles and the production parameter");
-  if (P->n < 1 || P->res < 4) return fail("bk_adjoint_substep: bad sizes");
+  const bool transport = P->transport_enabled != 0;
+  if (transport && (!adj->T || !adj->W || !adj->losses || !adj->exchanges || !in->T || !in->W || !out->T || !out->W ||
+                    !out->masses || !sc->capacities || !sc->heat_param || !grid->V || !grid->dT))
+    return fail("bk_adjoint_substep: the transport adjoint needs T, W, losses and exchanges arrays and the thermal grid");
   hipStream_t st = (hipStream_t)stream;
-  AdjArgs a;
+  AdjArgs a = {};
   a.P = *P;
   a.in = *in;
   a.out = *out;
@@ -864,17 +1306,47 @@ int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* i
   a.nb = (P->res + 3) / 4;
   a.n = P->n;
   const int64_t nodes = int64_t(a.nb) * a.nb * a.nb * 64;
-  a.gm = grid_scratch;
-  a.gp = grid_scratch + nodes;
-  a.gv = grid_scratch + 4 * nodes;
+  a.gm = scratch;
+  a.gp = scratch + nodes;
+  a.gv = scratch + 4 * nodes;
   a.param_grad = param_grad;
-  if (P->model == 1 && (!sc->visc_param || !in->Fe || !adj->Fe || !adj->tau || !adj->porosity))
-    return fail("bk_adjoint_substep: the Ding adjoint needs Fe, tau and porosity arrays");
+  if (transport) {
+    double* t = scratch + 7 * nodes;
+    double** grid_arrays[10] = {&a.accT, &a.accM, &a.ghr, &a.gwr, &a.gV, &a.gC, &a.gE, &a.gW, &a.gH, &a.gL};
+    for (int k = 0; k < 10; ++k) { *grid_arrays[k] = t; t += nodes; }
+    a.tcbar = t; t += P->n;
+    a.mrbar = t; t += P->n;
+    a.gSexp = t; t += P->n_slots;
+    a.gScon = t; t += P->n_slots;
+    a.gvert = t;
+    a.capacities = sc->capacities;
+    a.heat_param = sc->heat_param;
+    a.particle_slot = sc->particle_slot;
+    a.slot_face_offsets = sc->slot_face_offsets;
+    a.slot_faces = sc->slot_faces;
+    a.faces = sc->faces;
+    a.bind_offsets = sc->bind_offsets;
+    a.bind_particles = sc->bind_particles;
+    a.bind_weights = sc->bind_weights;
+    a.bind_ref_offsets = sc->bind_ref_offsets;
+    a.slot_exposed = wsv->slot_exposed;
+    a.slot_contact = wsv->slot_contact;
+    a.vertices = wsv->vertices;
+  }
   if (P->model == 1) hipLaunchKernelGGL(k_adj_ding, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
   hipLaunchKernelGGL(k_adj_g2p, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
   hipLaunchKernelGGL(k_adj_scatter, dim3(grid_cap), dim3(kTileThreads), 0, st, doughs, a, ws);
   hipLaunchKernelGGL(k_adj_grid, dim3(div_up(nodes, 256)), dim3(256), 0, st, a, nodes);
   hipLaunchKernelGGL(k_adj_p2g, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
+  if (transport) {
+    hipLaunchKernelGGL(k_adj_b1, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
+    hipLaunchKernelGGL(k_adj_scatter_b1, dim3(grid_cap), dim3(kTileThreads), 0, st, doughs, a, ws);
+    hipLaunchKernelGGL(k_adj_diffuse_a, dim3(grid_cap), dim3(256), 0, st, a, ws);
+    hipLaunchKernelGGL(k_adj_diffuse_b, dim3(grid_cap), dim3(256), 0, st, a, ws);
+    hipLaunchKernelGGL(k_adj_pass_a, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
+    if (P->n_slots > 0) hipLaunchKernelGGL(k_adj_faces, dim3(div_up(P->n_slots, 256)), dim3(256), 0, st, a, P->n_slots);
+    if (P->n_vertices > 0) hipLaunchKernelGGL(k_adj_vertices, dim3(div_up(P->n_vertices, 4)), dim3(256), 0, st, a, P->n_vertices);
+  }
   return hipGetLastError() == hipSuccess ? 0 : fail("bk_adjoint_substep: launch failed");
 }
 
@@ -936,9 +1408,10 @@ __global__ __launch_bounds__(128) void k_update_ding_adjoint(DingParams D, const
       gtau[i] = adj_tau ? adj_tau[9 * p + i] : 0.0;
       gFe_old[i] = gF[i] = gC[i] = 0.0;
     }
-    double gpor = 0.0;
+    double gpor = 0.0, gT = 0.0;
     ding_return_adjoint(D, Fe_old, Fn, Cn, double(old_porosity[p]), 0.0, old_gelled[p], double(temperatures[p]),
-                        double(viscosity_parameter[0]), gFe, gtau, adj_porosity ? adj_porosity[p] : 0.0, gFe_old, gF, gC, gpor, geta);
+                        double(viscosity_parameter[0]), gFe, gtau, adj_porosity ? adj_porosity[p] : 0.0, gFe_old, gF, gC, gpor, geta,
+                        gT);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
       if (adj_old_Fe) adj_old_Fe[9 * p + i] += gFe_old[i];

@@ -1096,15 +1096,18 @@ static HostWs* host_view(bk_workspace* ws) {
   return &(g_ws_cache[ws] = h);
 }
 
-// For the adjoint's tiled scatter (bk_adjoint.hip): the device dough table of
-// a workspace, its dough count and its P2G launch width.  The tiles are those
-// of the last substep the workspace ran.
-int bk_internal_tile_launch(bk_workspace* ws, const bk_dough** doughs, int* n_doughs, int* grid_cap) {
+// For the adjoint (bk_adjoint.hip): the device dough table of a workspace,
+// its dough count, its P2G launch width and its device arrays.  The tiles,
+// active blocks, vertices and slot areas are those of the last substep the
+// workspace ran.
+int bk_internal_tile_launch(bk_workspace* ws, const bk_dough** doughs, int* n_doughs, int* grid_cap,
+                            const bk_workspace** device_view) {
   HostWs* hw = host_view(ws);
   if (!hw) return fail("workspace: unknown handle");
   *doughs = hw->dev.doughs;
   *n_doughs = int(hw->doughs.size());
   *grid_cap = hw->tuning.grid_cap;
+  *device_view = &hw->dev;  // host copy of the device descriptor: device addresses of the workspace arrays
   return 0;
 }
 

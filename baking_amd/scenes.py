@@ -397,3 +397,18 @@ def loaf_256_frozen_transport(**kwargs):
     spec = loaf_256(**kwargs)
     t = replace(spec.transport, water_enabled=False, conductivity=0.0, heat_transfer=0.0, tray_heat_transfer=0.0)
     return replace(spec, name="loaf_256_frozen_transport", transport=t)
+
+
+def demo_transport(heat_transfer=20.0, gas=True, water_stress=0.0, spacing=0.025, dt=1e-4, subdivisions=2):
+    """Ding dough cube with full heat and water transport (reference tests/test_transport.py:12-20 with the
+    TransportConfig of its heat-transfer gradient test, 96-121): 40^3 grid, dt 1e-4, no gravity, pore gas at
+    porosity 0.2, temperature-dependent yield stress and viscosity."""
+    from .geometry.mesh_loader import CreateDemoMesh
+    from .geometry.particle_sampling import SampleParticles
+    mesh = CreateDemoMesh(subdivisions)
+    sim = SimulationConfig(time_step=dt, gravity=(0.0, 0.0, 0.0), grid_spacing=spacing, grid_resolution=int(round(1 / spacing)),
+                           device_name="auto")
+    ding = DingConfig(gas_enabled=gas, porosity=0.2 if gas else 0.0, gas_production_rate=0.001, temperature_enabled=True)
+    transport = TransportConfig(enabled=True, heat_transfer=heat_transfer, water_stress=water_stress)
+    return SceneSpec("demo_transport", mesh, SampleParticles(mesh, 1000.0, 4), sim, MaterialConfig(model="ding2019", young_modulus=10000.0),
+                     ding, transport, steps=4)

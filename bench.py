@@ -380,8 +380,8 @@ def run_adjoint(args):
     bk_adj = _native.BkAdjoint()
     for f, t in adj.items():
         setattr(bk_adj, f, t.data_ptr())
-    scratch = torch.zeros(7 * g.tiled_masses.numel(), dtype=torch.float64, device="cuda")
-    param = torch.zeros(2, dtype=torch.float64, device="cuda")
+    scratch = torch.zeros(int(lib.bk_adjoint_scratch_size(ctypes.byref(sim._params))), dtype=torch.float64, device="cuda")
+    param = torch.zeros(3, dtype=torch.float64, device="cuda")
     st_in, st_out = sim._state_struct(sim.current_state), sim._state_struct(sim.next_state)
     stream = torch.cuda.current_stream()
 

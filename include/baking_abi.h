@@ -330,28 +330,30 @@ int bk_solve_porosity(const float* args6, const double* plastic_volume, float* o
 /* ---------------------------------------------------------------- adjoint (reverse mode)
  * Replaces the reference's wp.Tape over MpmSimulator.Step in history mode
  * (physics/mpm.py:18-22, 163-225; tests/test_warp_autodiff.py, tests/test_ding.py:81-170,
- * tests/test_gas.py:131-160): adjoints of one substep with the neo-Hookean or
- * the Ding model, the latter with pore gas and the coupled porosity return.
- * Transport is differentiated only where it leaves temperature and water
- * unchanged (conductivity, tray and heat transfer 0, water off -- the caller
- * guarantees the zero heat-transfer parameter).  Adjoint arrays are FP64 and
- * particle-major like the state. */
+ * tests/test_gas.py:131-160, tests/test_transport.py:96-121,
+ * tests/test_ding_thermal.py:166-191): adjoints of one substep with the
+ * neo-Hookean or the Ding model (pore gas and the coupled porosity return
+ * included) and the heat and water transport (mesh areas, Robin terms,
+ * thermal P2G, FV diffusion, scalar G2P, gas source).  Adjoint arrays are
+ * FP64 and particle-major like the state. */
 typedef struct bk_adjoint {
-  double *x, *v, *C, *F;      /* N x 3, N x 3, N x 9, N x 9 */
-  double *Fe, *tau, *porosity; /* Ding model only: N x 9, N x 9, N */
-  double* moles;               /* Ding with pore gas only: N */
+  double *x, *v, *C, *F;                   /* N x 3, N x 3, N x 9, N x 9 */
+  double *Fe, *tau, *porosity;              /* Ding model only: N x 9, N x 9, N */
+  double* moles;                            /* Ding with pore gas only: N */
+  double *T, *W, *losses, *exchanges;       /* transport only: N each */
 } bk_adjoint;
 /* In: adj = the adjoint of the substep's output state `out`; the grid and the
  * workspace `ws` as the forward substep from `in` to `out` left them, with
- * nothing run on `ws` since (single dough, tiled nodes; the adjoint's
- * scatter reuses that substep's sorted P2G tiles).  Out: adj = the adjoint
- * of `in`; param_grad[0] += the
- * gradient of the Ding viscosity parameter (scene.visc_param), param_grad[1]
- * += that of the gas production parameter (scene.prod_param).  grid_scratch:
- * 7 doubles per tiled node, all zero before the first call; each call leaves
- * it ready for the next (no pass over the whole grid). */
+ * nothing run on `ws` since (single dough, tiled nodes; the adjoint reuses
+ * that substep's sorted P2G tiles, active blocks, vertices and areas).
+ * Out: adj = the adjoint of `in`; param_grad[0] += the gradient of the Ding
+ * viscosity parameter (scene.visc_param), [1] += the gas production
+ * parameter's (scene.prod_param), [2] += the heat-transfer parameter's
+ * (scene.heat_param).  scratch: bk_adjoint_scratch_size(params) doubles, all
+ * zero before the first call; each call leaves it ready for the next. */
+int64_t bk_adjoint_scratch_size(const bk_params* params);
 int bk_adjoint_substep(const bk_params* params, const bk_scene* scene, const bk_state* in, const bk_state* out, const bk_grid* grid,
-                       bk_workspace* ws, const bk_adjoint* adj, double* grid_scratch, double* param_grad, void* stream);
+                       bk_workspace* ws, const bk_adjoint* adj, double* scratch, double* param_grad, void* stream);
 /* Adjoint of UpdateDing without pore gas (physics/materials/ding.py:140-221; arguments as bk_UpdateDing):
  * given adjoints of its outputs Fe, tau and porosity (any may be NULL = 0), ADDS the adjoints of
  * old_Fe, old_porosity, F_total, C_affine (NULL = not wanted) and of the viscosity parameter (one double). */

@@ -11,6 +11,11 @@ Pins, each against the FP64 oracle (the same scene run in double precision):
 * the reference's gas-production gradient, d mean(det F) / d alpha through
   20 substeps of the gas-producing porous cube (tests/test_gas.py:120-160,
   rtol 0.05), with the coupled porosity return and the pore-gas stress;
+* the reference's transport gradients: d mean(T) / d h (heat transfer)
+  through 4 substeps with heat and water transport and pore gas
+  (tests/test_transport.py:96-121, rtol 3e-3), and d mean(tau_xx) / d eta
+  through 5 substeps of the heated porous cube (tests/test_ding_thermal.py:
+  166-191, rtol 0.03);
 * the implicit-function adjoints of the overstress solve (ding.py:90-100) and
   of the coupled porosity solve (porous_return.py:64-100) against central
   differences of the oracle's FP64 overstress and of the FP64 root of its
@@ -261,17 +266,72 @@ def test_gas_production_gradient():
     np.testing.assert_allclose(grad, fd, rtol=2e-3)
 
 
+def test_heat_transfer_gradient():
+    """tests/test_transport.py:96-121: d mean(T) / d h after 4 substeps with heat and water transport and pore gas."""
+    spec = scenes.demo_transport(20.0)
+    sim = scenes.build_simulator(spec)
+    h = torch.tensor([20.0], device="cuda", requires_grad=True)
+    out = adjoint.rollout(sim, 4, heat_transfer_parameter=h)
+    loss = out["T"].mean()
+    loss.backward()
+    grad = float(h.grad)
+
+    def loss_at(value):
+        def setup(o):
+            o.heat_param[0] = value
+        o = _oracle_run(spec, 4, setup)
+        return float(o.state.transport["T"].mean())
+
+    fd_ref = (loss_at(22.0) - loss_at(18.0)) / 4.0
+    fd = (loss_at(20.5) - loss_at(19.5)) / 1.0
+    print(f"gradient {grad:.9g}, oracle FD (eps 2) {fd_ref:.9g}, oracle FD (eps 0.5) {fd:.9g}")
+    assert grad > 0
+    np.testing.assert_allclose(grad, fd_ref, rtol=3e-3, atol=1e-10)
+    np.testing.assert_allclose(grad, fd, rtol=1e-3)
+
+
+def test_heated_porous_viscosity_gradient():
+    """tests/test_ding_thermal.py:166-191: d mean(tau_xx) / d eta after 5 substeps of the heated porous cube."""
+    spec = scenes.demo_heated(viscosity=1000.0, yield_stress=5.0, dt=1e-3)
+    sim = scenes.build_simulator(spec)
+    eta = torch.tensor([1000.0], device="cuda", requires_grad=True)
+    out = adjoint.rollout(sim, 5, viscosity_parameter=eta)
+    loss = out["tau"][:, 0, 0].mean()
+    loss.backward()
+    grad = float(eta.grad)
+
+    def loss_at(value):
+        def setup(o):
+            o.visc_param[0] = value
+        o = _oracle_run(spec, 5, setup)
+        return float(np.mean(o.state.ding["tau"][:, 0, 0]))
+
+    fd_ref = (loss_at(1010.0) - loss_at(990.0)) / 20.0
+    fd = (loss_at(1000.0 + 1e-3) - loss_at(1000.0 - 1e-3)) / 2e-3
+    print(f"gradient {grad:.9g}, oracle FD (eps 10) {fd_ref:.9g}, oracle FD (eps 1e-3) {fd:.9g}")
+    assert abs(fd_ref) > 1e-7
+    np.testing.assert_allclose(grad, fd_ref, rtol=0.03, atol=1e-7)
+    np.testing.assert_allclose(grad, fd, rtol=1e-3)
+
+
 def _state_scales(fields):
-    return dict(x=0.05, v=0.5, C=2.0, F=0.1, Fe=0.1, porosity=0.05, tau=50.0, moles=0.01)
+    # typical magnitudes (or variations) of each field in these scenes: the random functional weights
+    # each about equally, and central differences step by 1e-5 of them
+    return dict(x=0.05, v=0.5, C=2.0, F=0.1, Fe=0.1, porosity=0.05, tau=50.0, moles=10.0, T=1.0, W=0.05, losses=1e-8,
+                exchanges=0.1)
 
 
-@pytest.mark.parametrize("case", ["neo_hookean", "ding", "ding_porous", "ding_gas"])
+@pytest.mark.parametrize("case", ["neo_hookean", "ding", "ding_porous", "ding_gas", "transport", "heated"])
 def test_rollout_gradient_matches_fp64_differences(case):
     """32 random initial-state coordinates of a random linear functional of the state after 20 substeps."""
     if case == "neo_hookean":
         spec = scenes.strain_rate_field(scenes.scaled(scenes.demo_autodiff(0.0), time_step=5e-4), np.diag([1.0, -0.6, 0.3]))
     elif case == "ding_gas":
         spec = scenes.strain_rate_field(scenes.demo_gas(rate=0.005), np.diag([0.5, -0.25, 0.1]))
+    elif case == "transport":
+        spec = scenes.strain_rate_field(scenes.demo_transport(20.0, water_stress=10000.0), np.diag([0.5, -0.25, 0.1]))
+    elif case == "heated":
+        spec = scenes.demo_heated(viscosity=1000.0, yield_stress=5.0, dt=5e-4)
     else:
         spec = scenes.demo_viscous_gradient(1000.0, porosity=0.2 if case == "ding_porous" else 0.0)
         spec = scenes.scaled(spec, time_step=5e-4)
@@ -290,12 +350,13 @@ def test_rollout_gradient_matches_fp64_differences(case):
     grads = {f: inputs[f].grad.double().cpu().numpy() for f in fields}
 
     o_fields = dict(x=("x", None), v=("v", None), C=("C", None), F=("F", None), Fe=("Fe", "ding"), porosity=("porosity", "ding"),
-                    tau=("tau", "ding"), moles=("moles", "ding"))
+                    tau=("tau", "ding"), moles=("moles", "ding"), T=("T", "transport"), W=("W", "transport"),
+                    losses=("losses", "transport"), exchanges=("exchanges", "transport"))
     start_np = {f: start[f].cpu().numpy().astype(np.float64) for f in fields}
 
     def view(o, f):
         name, part = o_fields[f]
-        return o.state.ding[name] if part else getattr(o.state, name)
+        return getattr(o.state, part)[name] if part else getattr(o.state, name)
 
     def loss_at(field, index, delta):
         def setup(o):
@@ -318,9 +379,16 @@ def test_rollout_gradient_matches_fp64_differences(case):
         h = 1e-5 * scales[f]
         fd = (loss_at(f, idx, h) - loss_at(f, idx, -h)) / (2 * h)
         rows.append((f, idx, grads[f][idx], fd))
-    scale = max(abs(r[3]) for r in rows)
-    worst = max(abs(r[2] - r[3]) / (abs(r[3]) + 1e-3 * scale) for r in rows)
+    # The adjoint differentiates the float32 trajectory, the differences the FP64 one: agreement to ~1e-4
+    # of a field's gradient scale is that precision.  A coordinate whose gradient is far below its
+    # field's scale is therefore compared with a floor of 0.1 of the field's largest picked gradient;
+    # a missing or wrong term shows up as O(1) relative errors on many coordinates.
+    scale = {f: max(abs(r[3]) for r in rows if r[0] == f) for f in {r[0] for r in rows}}
+    rel = [abs(r[2] - r[3]) / (abs(r[3]) + 0.1 * scale[r[0]]) for r in rows]
+    plain = [abs(r[2] - r[3]) / max(abs(r[3]), 1e-300) for r in rows]
     for r in rows:
         print(f"{r[0]:9s} {str(r[1]):14s} adjoint {r[2]: .8e}  FD {r[3]: .8e}")
-    print(f"{case}: worst relative difference {worst:.2e} (floor 1e-3 of the largest gradient {scale:.3e})")
-    assert worst < 2e-3
+    print(f"{case}: worst relative difference {max(rel):.2e} (floor 0.1 of the field's largest gradient), "
+          f"median plain relative difference {np.median(plain):.2e}")
+    assert max(rel) < 2e-3
+    assert np.median(plain) < 1e-4
