@@ -66,7 +66,8 @@ class BkGrid(ctypes.Structure):
     _fields_ = [(name, c_void_p) for name in GRID_FIELDS]
 
 
-ADJOINT_FIELDS = ["x", "v", "C", "F", "Fe", "tau", "porosity", "moles", "T", "W", "losses", "exchanges"]
+ADJOINT_FIELDS = ["x", "v", "C", "F", "Fe", "tau", "porosity", "moles", "T", "W", "losses", "exchanges", "rev_offsets", "rev_bindings",
+                  "rev_vertices", "anchor_coef"]
 
 
 class BkAdjoint(ctypes.Structure):

@@ -70,13 +70,44 @@ def _fields_of(sim):
     return fields + (TRANSPORT_FIELDS if _transport(sim) else ())
 
 
+def _state_list(state):
+    return list(state_field_tensors(state).values())
+
+
 def _clone_state(state):
-    return {f: t.clone() for f, t in state_field_tensors(state).items()}
+    """A copy of every state array (one multi-tensor copy)."""
+    src = _state_list(state)
+    dst = [torch.empty_like(t) for t in src]
+    torch._foreach_copy_(dst, src)
+    return dst
 
 
 def _load_state(state, saved):
-    for f, t in state_field_tensors(state).items():
-        t.copy_(saved[f])
+    torch._foreach_copy_(_state_list(state), saved)
+
+
+def _binding_reverse(sim):
+    """The mesh bindings indexed by particle, for the gather in the mesh adjoint (cached per simulator)."""
+    rev = getattr(sim, "_adjoint_bindings", None)
+    if rev is not None:
+        return rev
+    sc = sim._scene_tensors
+    offs = sc["bind_offsets"].long()
+    parts = sc["bind_particles"].long()
+    n, nv = sim.particle_data.particle_count, offs.numel() - 1
+    counts = offs[1:] - offs[:-1]
+    vertex_of = torch.repeat_interleave(torch.arange(nv, device=offs.device), counts)
+    order = torch.argsort(parts, stable=True)
+    rev_offsets = torch.zeros(n + 1, dtype=torch.long, device=offs.device)
+    rev_offsets[1:] = torch.cumsum(torch.bincount(parts, minlength=n), 0)
+    # per-vertex weight sums as differences of one prefix sum (bindings are grouped by vertex)
+    csum = torch.zeros(parts.numel() + 1, dtype=torch.float64, device=offs.device)
+    csum[1:] = torch.cumsum(sc["bind_weights"].double(), 0)
+    wsum = csum[offs[1:]] - csum[offs[:-1]]
+    rev = dict(rev_offsets=rev_offsets.int().contiguous(), rev_bindings=order.int().contiguous(),
+               rev_vertices=vertex_of[order].int().contiguous(), anchor_coef=(1.0 - wsum).contiguous())
+    sim._adjoint_bindings = rev
+    return rev
 
 
 def _run(sim, count):
@@ -129,29 +160,36 @@ class _Rollout(torch.autograd.Function):
         bk_adj = _native.BkAdjoint()
         for f, t in adj.items():
             setattr(bk_adj, f, t.data_ptr())
+        if _transport(sim):
+            for f, t in _binding_reverse(sim).items():
+                setattr(bk_adj, f, t.data_ptr())
         param = torch.zeros(3, dtype=torch.float64, device=dev)
         scratch = torch.zeros(int(lib.bk_adjoint_scratch_size(ctypes.byref(sim._params))), dtype=torch.float64, device=dev)
         # the simulator is left where the forward ended
         keep = (sim.current_state, sim.next_state, sim.step_count, _clone_state(sim.current_state), sim._read_flags())
         stream = _native.stream_ptr()
+        # the ping-pong pair keeps its arrays throughout: one bk_state each
+        structs = {id(s): sim._state_struct(s) for s in (sim.current_state, sim.next_state)}
+        bufs = [[torch.empty_like(t) for t in _state_list(sim.current_state)] for _ in range(ctx.every - 1)]
         for seg in reversed(range(len(ctx.checkpoints))):
             count = min(ctx.every, ctx.steps - seg * ctx.every)
             _load_state(sim.current_state, ctx.checkpoints[seg])
             states = [ctx.checkpoints[seg]]
-            for _ in range(count - 1):
+            for k in range(count - 1):
                 _run(sim, 1)
-                states.append(_clone_state(sim.current_state))
+                torch._foreach_copy_(bufs[k], _state_list(sim.current_state))
+                states.append(bufs[k])
             for k in reversed(range(count)):
                 if k != count - 1:
                     _load_state(sim.current_state, states[k])
                 # the substep's forward again: its output, grid and P2G tiles for the adjoint
                 sim._launch_substep()
-                s_in, s_out = sim._state_struct(sim.current_state), sim._state_struct(sim.next_state)
+                s_in, s_out = structs[id(sim.current_state)], structs[id(sim.next_state)]
                 _native.check(lib.bk_adjoint_substep(ctypes.byref(sim._params), ctypes.byref(sim._scene), ctypes.byref(s_in),
                                                      ctypes.byref(s_out), ctypes.byref(sim._grid), sim._workspace,
                                                      ctypes.byref(bk_adj), _native.ptr(scratch), _native.ptr(param), stream),
                               "bk_adjoint_substep")
-            del states
+        del bufs
         sim.current_state, sim.next_state, sim.step_count, final, flags = keep
         _load_state(sim.current_state, final)
         sim._write_flags(flags)

@@ -1225,42 +1225,39 @@ __global__ __launch_bounds__(256) void k_adj_faces(AdjArgs a, int nslots) {
   }
 }
 
-// One wavefront per vertex: mesh reconstruction's adjoint (mesh_tracking.py:13-38).
-// v = (1 - sum_b w_b) anchor + sum_b w_b (x_b + F_b r_b), anchor = x_s + F_s r_s of
-// the first binding.  Scatters to the bound particles' x and F (FP64 atomics) and
-// clears the vertex adjoint.
-__global__ __launch_bounds__(256) void k_adj_vertices(AdjArgs a, int nv) {
-  const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (v >= nv) return;
-  const double gv[3] = {a.gvert[3 * v], a.gvert[3 * v + 1], a.gvert[3 * v + 2]};
-  if (gv[0] == 0.0 && gv[1] == 0.0 && gv[2] == 0.0) return;
-  const int s = a.bind_offsets[v], e = a.bind_offsets[v + 1];
-  double wsum = 0.0;
-  for (int k = s + lane; k < e; k += 64) {
-    const int pi = a.bind_particles[k];
-    const double wk = a.bind_weights[k];
-    wsum += wk;
+// Per particle: mesh reconstruction's adjoint (mesh_tracking.py:13-38),
+// v = (1 - sum_b w_b) anchor + sum_b w_b (x_b + F_b r_b) with anchor = x_s + F_s r_s
+// of the vertex's first binding, gathered over the particle's own bindings
+// (the reverse index adj.rev_*): no atomics.
+__global__ __launch_bounds__(256) void k_adj_bindings(AdjArgs a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  double gx[3] = {0.0, 0.0, 0.0}, gF[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool any = false;
+  for (int idx = a.adj.rev_offsets[p]; idx < a.adj.rev_offsets[p + 1]; ++idx) {
+    const int k = a.adj.rev_bindings[idx], v = a.adj.rev_vertices[idx];
+    const double gv[3] = {a.gvert[3 * v], a.gvert[3 * v + 1], a.gvert[3 * v + 2]};
+    if (gv[0] == 0.0 && gv[1] == 0.0 && gv[2] == 0.0) continue;
+    const double c = double(a.bind_weights[k]) + (k == a.bind_offsets[v] ? a.adj.anchor_coef[v] : 0.0);
     const float* r = a.bind_ref_offsets + 3 * k;
-    for (int q = 0; q < 3; ++q) {
-      unsafeAtomicAdd(a.adj.x + 3 * pi + q, wk * gv[q]);
-      for (int c = 0; c < 3; ++c) unsafeAtomicAdd(a.adj.F + 9 * pi + 3 * q + c, wk * gv[q] * double(r[c]));
-    }
-  }
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) wsum += __shfl_xor(wsum, off, 64);
-  if (lane == 0) {
-    if (s < e) {
-      const int ps = a.bind_particles[s];
-      const double ca = 1.0 - wsum;
-      const float* r = a.bind_ref_offsets + 3 * s;
-      for (int q = 0; q < 3; ++q) {
-        unsafeAtomicAdd(a.adj.x + 3 * ps + q, ca * gv[q]);
-        for (int c = 0; c < 3; ++c) unsafeAtomicAdd(a.adj.F + 9 * ps + 3 * q + c, ca * gv[q] * double(r[c]));
-      }
+    for (int q = 0; q < 3; ++q) {
+      gx[q] += c * gv[q];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) gF[3 * q + e] += c * gv[q] * double(r[e]);
     }
-    a.gvert[3 * v] = a.gvert[3 * v + 1] = a.gvert[3 * v + 2] = 0.0;
+    any = true;
   }
+  if (!any) return;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) a.adj.x[3 * p + q] += gx[q];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) a.adj.F[9 * p + i] += gF[i];
+}
+
+__global__ void k_adj_clear_vertices(AdjArgs a, int nv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * nv) a.gvert[i] = 0.0;
 }
 
 // ------------------------------------------------------------------ C ABI
@@ -1292,6 +1289,8 @@ int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* i
   if (transport && (!adj->T || !adj->W || !adj->losses || !adj->exchanges || !in->T || !in->W || !out->T || !out->W ||
                     !out->masses || !sc->capacities || !sc->heat_param || !grid->V || !grid->dT))
     return fail("bk_adjoint_substep: the transport adjoint needs T, W, losses and exchanges arrays and the thermal grid");
+  if (transport && P->n_vertices > 0 && (!adj->rev_offsets || !adj->rev_bindings || !adj->rev_vertices || !adj->anchor_coef))
+    return fail("bk_adjoint_substep: the transport adjoint needs the reverse binding index (adj.rev_*, adj.anchor_coef)");
   hipStream_t st = (hipStream_t)stream;
   AdjArgs a = {};
   a.P = *P;
@@ -1345,7 +1344,10 @@ int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* i
     hipLaunchKernelGGL(k_adj_diffuse_b, dim3(grid_cap), dim3(256), 0, st, a, ws);
     hipLaunchKernelGGL(k_adj_pass_a, dim3(div_up(a.n, 128)), dim3(128), 0, st, a);
     if (P->n_slots > 0) hipLaunchKernelGGL(k_adj_faces, dim3(div_up(P->n_slots, 256)), dim3(256), 0, st, a, P->n_slots);
-    if (P->n_vertices > 0) hipLaunchKernelGGL(k_adj_vertices, dim3(div_up(P->n_vertices, 4)), dim3(256), 0, st, a, P->n_vertices);
+    if (P->n_vertices > 0) {
+      hipLaunchKernelGGL(k_adj_bindings, dim3(div_up(a.n, 256)), dim3(256), 0, st, a);
+      hipLaunchKernelGGL(k_adj_clear_vertices, dim3(div_up(3 * int64_t(P->n_vertices), 256)), dim3(256), 0, st, a, P->n_vertices);
+    }
   }
   return hipGetLastError() == hipSuccess ? 0 : fail("bk_adjoint_substep: launch failed");
 }

@@ -390,15 +390,6 @@ def demo_viscous_gradient(viscosity=1000.0, yield_stress=5.0, porosity=0.0):
     return strain_rate_field(spec, np.diag([1.5, -0.75, 0.2]))
 
 
-def loaf_256_frozen_transport(**kwargs):
-    """``loaf_256`` with the heat and water transport frozen (conductivity, heat transfer and tray heat
-    transfer 0, water off): the temperature stays at 298.15 K, pore gas and the coupled return map stay on.
-    The gradient workload: every part of the substep that the adjoint differentiates runs at full size."""
-    spec = loaf_256(**kwargs)
-    t = replace(spec.transport, water_enabled=False, conductivity=0.0, heat_transfer=0.0, tray_heat_transfer=0.0)
-    return replace(spec, name="loaf_256_frozen_transport", transport=t)
-
-
 def demo_transport(heat_transfer=20.0, gas=True, water_stress=0.0, spacing=0.025, dt=1e-4, subdivisions=2):
     """Ding dough cube with full heat and water transport (reference tests/test_transport.py:12-20 with the
     TransportConfig of its heat-transfer gradient test, 96-121): 40^3 grid, dt 1e-4, no gravity, pore gas at

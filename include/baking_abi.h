@@ -341,6 +341,11 @@ typedef struct bk_adjoint {
   double *Fe, *tau, *porosity;              /* Ding model only: N x 9, N x 9, N */
   double* moles;                            /* Ding with pore gas only: N */
   double *T, *W, *losses, *exchanges;       /* transport only: N each */
+  /* transport only, static per scene: the mesh bindings (scene.bind_*) indexed by particle --
+   * rev_offsets[N+1], rev_bindings / rev_vertices[B] (binding index and its vertex, grouped by
+   * bind_particles) -- and anchor_coef[V] = 1 - sum of the vertex's binding weights */
+  const int32_t *rev_offsets, *rev_bindings, *rev_vertices;
+  const double* anchor_coef;
 } bk_adjoint;
 /* In: adj = the adjoint of the substep's output state `out`; the grid and the
  * workspace `ws` as the forward substep from `in` to `out` left them, with
