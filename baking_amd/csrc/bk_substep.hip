@@ -17,7 +17,7 @@
 //   tiles    bounds of each non-empty particle block in the sorted order;
 //            activate the 2x2x2 node blocks each reaches; cut blocks into P2G
 //            tiles of at most `tile_chunk` particles
-//   mesh     reconstruct mesh vertices, then boundary-face areas per owner slot
+//   mesh     reconstruct the boundary faces' vertices, then their areas per owner slot
 //   pass A   thermal P2G (7 f64 scalars) accumulated per tile in LDS, flushed
 //            with one global f64 atomic per touched node and scalar
 //   diffuse  6-neighbour FV heat/water fluxes per active node block
@@ -149,7 +149,7 @@ __global__ void k_cell_keys(const float* __restrict__ x, int n, float ox, float 
 __global__ void k_batch_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of,
                              const float* __restrict__ x, int n, uint32_t* __restrict__ keys, bk_workspace* ws) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) { ws->counters[0] = 0; ws->counters[1] = 0; ws->counters[2] = 0; ws->counters[3] = 0; }
+  if (p == 0) { ws->counters[0] = 0; ws->counters[1] = 0; ws->counters[2] = 0; ws->counters[3] = 0; ws->counters[6] += 1; }
   if (p >= n) return;
   per_dough(dough_of_particle(dough_of, p), [&](int d) {
     const bk_params& P = doughs[d].params;
@@ -383,13 +383,29 @@ __global__ void k_clear_blocks(bk_workspace* ws, bk_grid g, int thermal) {
 }
 
 // ============================================================================ mesh + surface areas
-// geometry/mesh_tracking.py:13-38, one wavefront per vertex (4 vertices per 256-thread workgroup).
+// Only the boundary faces' areas enter the substep (the Robin terms), so only
+// their vertices are reconstructed: each slot marks its faces' vertices with
+// the substep count, and the reconstruction skips the vertices left unmarked.
+__global__ void k_mark_face_vertices(bk_scene sc, bk_workspace* ws, int nslots) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nslots) return;
+  const int32_t stamp = ws->counters[6];
+  for (int k = sc.slot_face_offsets[s]; k < sc.slot_face_offsets[s + 1]; ++k) {
+    const int32_t* f = sc.faces + 3 * sc.slot_faces[k];
+    ws->vert_mark[f[0]] = stamp;
+    ws->vert_mark[f[1]] = stamp;
+    ws->vert_mark[f[2]] = stamp;
+  }
+}
+
+// geometry/mesh_tracking.py:13-38, one wavefront per marked vertex (4 vertices per 256-thread workgroup).
 __global__ __launch_bounds__(256) void k_reconstruct(const float* __restrict__ x, const float* __restrict__ F,
                                                      const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
                                                      const int32_t* __restrict__ parts, const float* __restrict__ wts,
-                                                     const float* __restrict__ ref_offs, float* __restrict__ out, int nv) {
+                                                     const float* __restrict__ ref_offs, float* __restrict__ out, int nv,
+                                                     const bk_workspace* ws) {
   int v = xcd_block() * 4 + (threadIdx.x >> 6);
-  if (v >= nv) return;
+  if (v >= nv || ws->vert_mark[v] != ws->counters[6]) return;
   reconstruct_vertex_wave(v, threadIdx.x & 63, x, F, ref_vertices, offs, parts, wts, ref_offs, out);
 }
 
@@ -1182,10 +1198,12 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
   if (hw->thermal) {
     {  // 5. mesh reconstruction and surface areas
       StageTimer tm(pr, st, kStageMesh);
-      if (hw->n_vertices > 0)
+      if (hw->n_vertices > 0 && hw->n_slots > 0) {
+        hipLaunchKernelGGL(k_mark_face_vertices, dim3(div_up(hw->n_slots, 256)), dim3(256), 0, st, *sc, ws, hw->n_slots);
         hipLaunchKernelGGL(k_reconstruct, dim3(div_up(hw->n_vertices, 4)), dim3(256), 0, st, in->x, in->F, sc->ref_vertices,
                            sc->bind_offsets, sc->bind_particles, sc->bind_weights, sc->bind_ref_offsets, h->vertices,
-                           hw->n_vertices);
+                           hw->n_vertices, ws);
+      }
       if (hw->n_slots > 0)
         hipLaunchKernelGGL(k_slot_areas, dim3(div_up(hw->n_slots, 256)), dim3(256), 0, st, D, *sc, h->vertices, ws, hw->n_slots);
       BK_LAUNCH(mesh);
@@ -1391,6 +1409,7 @@ bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blo
   alloc((void**)&h.tau, n * 36);
   alloc((void**)&h.queue, n * 4);
   alloc((void**)&h.vertices, size_t(n_vertices) * 12);
+  alloc((void**)&h.vert_mark, size_t(n_vertices) * 4);
   alloc((void**)&h.slot_exposed, size_t(n_slots) * 8);
   alloc((void**)&h.slot_contact, size_t(n_slots) * 8);
   alloc((void**)&h.flags, size_t(BK_FLAG_COUNT) * max_doughs * 4);
@@ -1425,7 +1444,8 @@ void bk_workspace_destroy(bk_workspace* ws) {
   if (ws_host(ws, h)) return;
   void* ptrs[] = {h.doughs, h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1],
                   h.hist, h.digit_total, h.blk_start, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
-                  h.queue, h.tau, h.vertices, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions, h.stats};
+                  h.queue, h.tau, h.vertices, h.vert_mark, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions,
+                  h.stats};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   (void)hipFree(ws);

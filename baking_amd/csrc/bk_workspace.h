@@ -42,10 +42,12 @@ struct bk_workspace {
   int32_t* blocks = nullptr;                    // non-empty particle blocks
   int4* tiles = nullptr;                        // P2G tiles: (block, dough, first sorted particle, count)
   int32_t* active = nullptr;                    // active node blocks
-  int32_t* counters = nullptr;                  // [0] tiles, [1] active blocks, [2] deferred, [3] particle blocks
+  int32_t* counters = nullptr;                  // [0] tiles, [1] active blocks, [2] deferred, [3] particle blocks,
+                                                // [4] emitted samples, [6] substeps run
   int32_t* queue = nullptr;                     // particles whose coupled porosity needs the full solve
   float* tau = nullptr;
-  float* vertices = nullptr;
+  float* vertices = nullptr;                    // reconstructed positions of the boundary faces' vertices
+  int32_t* vert_mark = nullptr;                 // per vertex: substep count when a boundary face last used it
   double* slot_exposed = nullptr;
   double* slot_contact = nullptr;
   int32_t* flags = nullptr;                     // BK_FLAG_COUNT per dough

@@ -154,7 +154,8 @@ int bk_workspace_set_doughs(bk_workspace* ws, const bk_dough* table, int32_t n_d
 int bk_workspace_set_tuning(bk_workspace* ws, const bk_tuning* tuning);
 int bk_workspace_get_tuning(bk_workspace* ws, bk_tuning* tuning);
 int32_t* bk_workspace_flags(bk_workspace* ws);          /* device int32[BK_FLAG_COUNT * max_doughs], dough-major */
-float* bk_workspace_vertices(bk_workspace* ws);         /* device float[V*3]: last reconstructed mesh */
+float* bk_workspace_vertices(bk_workspace* ws);         /* device float[V*3]: the last substep's positions of the
+                                                           vertices its boundary faces use (the others are not updated) */
 /* Copies {tiles, active node blocks, deferred return maps} of the last substep; synchronises. */
 int bk_workspace_counts(bk_workspace* ws, int32_t* out3, void* stream);
 /* Blocking copy of the flag block to host int32[8] after the work already on `stream`; and the reverse. */
