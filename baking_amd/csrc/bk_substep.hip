@@ -514,6 +514,8 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
       Robin r = robin_terms(P, heat_param, Jf, sc.V0[p], T, W, particle_surface(sc, ws, p));
       double cap = sc.capacities[p];
       const int li = b[0] - 4 * tg.g.bx, lj = b[1] - 4 * tg.g.by, lk = b[2] - 4 * tg.g.bz;
+      // the Robin terms are zero off the surface (no exposed or contact area): most particles skip their adds
+      const bool surface = r.heat != 0.0 || r.loss != 0.0 || r.exch != 0.0;
 #pragma unroll
       for (int ox = 0; ox < 3; ++ox)
 #pragma unroll
@@ -526,9 +528,11 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
             atomicAdd(&acc[1][ri], wt * cap);
             atomicAdd(&acc[2][ri], wt * cap * T);
             atomicAdd(&acc[3][ri], wt * W);
-            atomicAdd(&acc[4][ri], wt * r.heat);
-            atomicAdd(&acc[5][ri], wt * r.loss);
-            atomicAdd(&acc[6][ri], wt * r.exch);
+            if (surface) {
+              atomicAdd(&acc[4][ri], wt * r.heat);
+              atomicAdd(&acc[5][ri], wt * r.loss);
+              atomicAdd(&acc[6][ri], wt * r.exch);
+            }
           }
     }
     __syncthreads();
