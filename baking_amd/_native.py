@@ -74,6 +74,18 @@ class BkAdjoint(ctypes.Structure):
     _fields_ = [(name, c_void_p) for name in ADJOINT_FIELDS]
 
 
+class BkSettling(ctypes.Structure):
+    """Settling controller record of one dough (``bk_settling``)."""
+    _fields_ = [("auto_mode", c_int32), ("settling_steps", c_int32), ("time_step", c_double), ("min_duration_s", c_double),
+                ("hold_duration_s", c_double), ("speed_tolerance", c_double), ("rate_tolerance", c_double),
+                ("phase", c_int32), ("reason", c_int32), ("step_count", c_int64), ("reference_step", c_int64),
+                ("quiet_steps", c_int64), ("baking_start_step", c_int64), ("plastic_rate", c_double),
+                ("diagnostics", c_float * 3), ("ref_slot", c_int32), ("copy_pending", c_int32), ("reserved", c_int32)]
+
+
+SETTLING_REASONS = ("pending", "fixed_duration", "quasi_static", "timeout")
+
+
 # Exported symbols and their argument types; the ABI test checks this list
 # against the header and the library.
 P = c_void_p
@@ -142,6 +154,10 @@ SIGNATURES = {
     "bk_workspace_set_surrogate": ([P, P], c_int),
     "bk_workspace_set_samples": ([P, P, I], c_int),
     "bk_workspace_stats": ([P, P, I, P], c_int),
+    "bk_workspace_settling_set": ([P, I, I, POINTER(BkSettling), P], c_int),
+    "bk_workspace_settling_get": ([P, I, I, POINTER(BkSettling), P], c_int),
+    "bk_workspace_settling_reference": ([P, P], c_int),
+    "bk_SettlingMetrics": ([P, P, P, P, P, F, I, P], c_int),
     "bk_adjoint_scratch_size": ([POINTER(BkParams)], c_int64),
     "bk_adjoint_substep": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P,
                             POINTER(BkAdjoint), P, P, P], c_int),

@@ -21,8 +21,11 @@ Gradients with respect to the initial positions, velocities, affine
 velocities and deformation gradients, the Ding elastic gradients,
 porosities, Kirchhoff stresses and gas moles, the temperatures, water
 masses and loss/exchange accumulators, and the Ding viscosity, gas
-production and heat-transfer parameters.  Not covered: settling substeps
-(not provided by the forward either) and batched doughs.
+production and heat-transfer parameters.  Resting substeps before baking
+(``physics/mpm.py:57-109``) are differentiated with the phase the forward's
+settling controller gave them; the stopping decision itself has no
+derivative, as in the reference (``physics/quasistatic.py:1-6``).  Not
+covered: batched doughs.
 
     out = rollout(sim, steps, {"v": v0})          # tensors that may require grad
     loss = (out["x"][:, 0] - x0[:, 0]).mean()
@@ -116,6 +119,18 @@ def _run(sim, count):
         sim._accept()
 
 
+def _run_forward(sim, count):
+    """``count`` substeps as ``Advance`` runs them: while the dough rests, the settling controller decides
+    when baking starts, and a timeout raises at the reference's substep."""
+    while count > 0:
+        chunk = sim._settling_chunk(count)
+        resting = sim.IsSettling
+        _run(sim, chunk)
+        if resting:
+            sim._pull_settling()
+        count -= chunk
+
+
 class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, sim, steps, every, fields, visc, prod, heat, *inputs):
@@ -131,16 +146,22 @@ class _Rollout(torch.autograd.Function):
         if _transport(sim):
             sim.transport.heat_transfer_parameter.copy_(heat.detach().reshape(1).to(torch.float32))
         flags_before = sim._read_flags()
+        first, resting = sim.step_count, sim.IsSettling
         checkpoints = []
         for start in range(0, steps, every):
             checkpoints.append(_clone_state(sim.current_state))
-            _run(sim, min(every, steps - start))
+            _run_forward(sim, min(every, steps - start))
         flags = sim._read_flags()
         if any(f and not b for f, b in zip(flags, flags_before)):
             sim._sync_api_flags(flags)
             sim._raise_for(flags)
             raise RuntimeError(f"rollout: the forward substeps raised error flags {flags}")
         ctx.sim, ctx.steps, ctx.every, ctx.fields, ctx.checkpoints = sim, steps, every, fields, checkpoints
+        # substeps [0, rest) of the rollout rested (the stopping decision itself has no derivative)
+        if resting:
+            ctx.rest = steps if sim._baking_start_step is None else max(0, min(steps, sim._baking_start_step - first))
+        else:
+            ctx.rest = 0
         ctx.shapes = [t.shape for t in inputs]
         ctx.dtypes = [t.dtype for t in inputs]
         ctx.params = [(visc.shape, visc.dtype), (prod.shape, prod.dtype), (heat.shape, heat.dtype)]
@@ -166,7 +187,10 @@ class _Rollout(torch.autograd.Function):
         param = torch.zeros(3, dtype=torch.float64, device=dev)
         scratch = torch.zeros(int(lib.bk_adjoint_scratch_size(ctypes.byref(sim._params))), dtype=torch.float64, device=dev)
         # the simulator is left where the forward ended
-        keep = (sim.current_state, sim.next_state, sim.step_count, _clone_state(sim.current_state), sim._read_flags())
+        keep = (sim.current_state, sim.next_state, sim.step_count, _clone_state(sim.current_state), sim._read_flags(),
+                sim._params)
+        # each substep runs again with its own phase, without the settling controller
+        phase_params = (sim._make_params(settling=0), sim._make_params(settling=1))
         stream = _native.stream_ptr()
         # the ping-pong pair keeps its arrays throughout: one bk_state each
         structs = {id(s): sim._state_struct(s) for s in (sim.current_state, sim.next_state)}
@@ -176,12 +200,14 @@ class _Rollout(torch.autograd.Function):
             _load_state(sim.current_state, ctx.checkpoints[seg])
             states = [ctx.checkpoints[seg]]
             for k in range(count - 1):
+                sim._params = phase_params[int(seg * ctx.every + k < ctx.rest)]
                 _run(sim, 1)
                 torch._foreach_copy_(bufs[k], _state_list(sim.current_state))
                 states.append(bufs[k])
             for k in reversed(range(count)):
                 if k != count - 1:
                     _load_state(sim.current_state, states[k])
+                sim._params = phase_params[int(seg * ctx.every + k < ctx.rest)]
                 # the substep's forward again: its output, grid and P2G tiles for the adjoint
                 sim._launch_substep()
                 s_in, s_out = structs[id(sim.current_state)], structs[id(sim.next_state)]
@@ -190,7 +216,7 @@ class _Rollout(torch.autograd.Function):
                                                      ctypes.byref(bk_adj), _native.ptr(scratch), _native.ptr(param), stream),
                               "bk_adjoint_substep")
         del bufs
-        sim.current_state, sim.next_state, sim.step_count, final, flags = keep
+        sim.current_state, sim.next_state, sim.step_count, final, flags, sim._params = keep
         _load_state(sim.current_state, final)
         sim._write_flags(flags)
         out = [adj[f].reshape(shape).to(dtype) for f, shape, dtype in zip(fields, ctx.shapes, ctx.dtypes)]

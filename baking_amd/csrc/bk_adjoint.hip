@@ -1280,7 +1280,8 @@ int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* i
   if (bk_internal_tile_launch(ws, &doughs, &n_doughs, &grid_cap, &wsv)) return -1;
   if (n_doughs != 1) return fail("bk_adjoint_substep: the workspace must hold one dough (the simulator's own)");
   if (P->n < 1 || P->res < 4) return fail("bk_adjoint_substep: bad sizes");
-  if (P->settling) return fail("bk_adjoint_substep: settling substeps have no adjoint");
+  if (P->settling == 2)
+    return fail("bk_adjoint_substep: pass the substep's phase (settling 0 baking, 1 resting), not the controller mode 2");
   if (P->model == 1 && (!sc->visc_param || !in->Fe || !adj->Fe || !adj->tau || !adj->porosity))
     return fail("bk_adjoint_substep: the Ding adjoint needs Fe, tau and porosity arrays");
   if (P->model == 1 && P->gas_enabled && (!P->transport_enabled || !in->moles || !out->moles || !adj->moles || !sc->prod_param))
@@ -1294,6 +1295,12 @@ int bk_adjoint_substep(const bk_params* P, const bk_scene* sc, const bk_state* i
   hipStream_t st = (hipStream_t)stream;
   AdjArgs a = {};
   a.P = *P;
+  if (a.P.settling == 1) {  // a resting substep (thermal_water.py:306, 325-326, 333)
+    a.P.tray_h = 0.0f;
+    a.P.loss_coef = 0.0f;
+    a.P.kappa = 0.0f;
+    a.P.diffusivity = 0.0f;
+  }
   a.in = *in;
   a.out = *out;
   a.g = *grid;

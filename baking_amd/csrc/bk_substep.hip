@@ -29,6 +29,8 @@
 //   surrogate (optional) the return-map network on the queued particles' features
 //   pass C'  the queued return maps, densely; with the surrogate, each coupled
 //            porosity solve starts from the network's prediction
+//   settling (doughs still resting before baking) the rest metrics and the
+//            rest/bake decision on the device (bk_settling.hip)
 #include "bk_device.h"
 #include "bk_surrogate.h"
 #include "bk_workspace.h"
@@ -85,8 +87,8 @@ constexpr int kSampleRow = 12;  // 7 features, 4 exact outputs, 1 unused
 struct HostWs {
   bk_workspace dev;
   std::vector<bk_dough> doughs;
-  int n = 0, n_vertices = 0, n_slots = 0, key_bits = 6;
-  bool thermal = false, gas = false, surrogate = false;
+  int n = 0, n_vertices = 0, n_slots = 0, key_bits = 6, max_dough_n = 0;
+  bool thermal = false, gas = false, surrogate = false, settle = false;
   bk_tuning tuning{4096, 256, 4096};
 };
 static std::mutex g_ws_mutex;
@@ -458,17 +460,21 @@ __device__ __forceinline__ Surface particle_surface(const bk_scene& sc, const bk
 struct Robin {
   double vol, sat, loss, heat, exch;
 };
-__device__ __forceinline__ Robin robin_terms(const bk_params& P, float heat_param, float Jf, float V0, double T, double W,
-                                             const Surface& s) {
+// A resting substep has no oven, tray or surface-loss terms (thermal_water.py:306, 325-326).
+__device__ __forceinline__ Robin robin_terms(const bk_params& P, bool rest, float heat_param, float Jf, float V0, double T,
+                                             double W, const Surface& s) {
+  const float h = rest ? 0.0f : heat_param;
+  const float tray = rest ? 0.0f : P.tray_h;
+  const float loss = rest ? 0.0f : P.loss_coef;
   Robin r;
   r.vol = double(V0) * double(Jf);
   r.sat = W / (double(P.rho_w) * r.vol);
   r.loss = 0.0;
-  if (P.water_enabled) r.loss = double(P.loss_coef) * T * r.sat * s.exposed;
-  r.heat = double(heat_param) * s.exposed * (double(P.T_oven) - T) + double(P.tray_h) * s.contact * (double(P.T_tray) - T) -
+  if (P.water_enabled) r.loss = double(loss) * T * r.sat * s.exposed;
+  r.heat = double(h) * s.exposed * (double(P.T_oven) - T) + double(tray) * s.contact * (double(P.T_tray) - T) -
            double(P.latent) * r.loss;
-  r.exch = double(heat_param) * s.exposed + double(P.tray_h) * s.contact;
-  if (P.water_enabled) r.exch += double(P.latent * P.loss_coef) * r.sat * s.exposed;
+  r.exch = double(h) * s.exposed + double(tray) * s.contact;
+  if (P.water_enabled) r.exch += double(P.latent * loss) * r.sat * s.exposed;
   return r;
 }
 
@@ -488,7 +494,8 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
     const bk_params& P = doughs[tg.d].params;
     if (!P.transport_enabled) continue;  // a dough without transport in a batch with some (uniform per tile)
     const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
-    const float heat_param = P.settling ? 0.0f : sc.heat_param[tg.d];
+    const bool rest = resting(P, ws, tg.d);
+    const float heat_param = sc.heat_param[tg.d];
     for (int i = threadIdx.x; i < 7 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
     if (threadIdx.x < 64) {
       cell_lo[threadIdx.x] = 0;
@@ -511,7 +518,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
       axis_weights(l[2], wz);
       float Jf = m_det(m_load(in.F + 9 * p));
       double T = in.T[p], W = in.W[p];
-      Robin r = robin_terms(P, heat_param, Jf, sc.V0[p], T, W, particle_surface(sc, ws, p));
+      Robin r = robin_terms(P, rest, heat_param, Jf, sc.V0[p], T, W, particle_surface(sc, ws, p));
       double cap = sc.capacities[p];
       const int li = b[0] - 4 * tg.g.bx, lj = b[1] - 4 * tg.g.by, lk = b[2] - 4 * tg.g.bz;
       // the Robin terms are zero off the surface (no exposed or contact area): most particles skip their adds
@@ -569,8 +576,9 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
       const double rho_w = double(P.rho_w);
       const double dt = double(P.dt);
       const double dx3 = double(P.dx * P.dx * P.dx);
-      const double hk = double(P.kappa * P.dx);
-      const double wk = double(P.diffusivity * P.rho_w * P.dx);
+      const bool rest = resting(P, ws, bg.d);
+      const double hk = rest ? 0.0 : double(P.kappa * P.dx);
+      const double wk = rest ? 0.0 : double(P.diffusivity * P.rho_w * P.dx);
       int c[3] = {4 * bg.bx + (loc >> 4), 4 * bg.by + ((loc >> 2) & 3), 4 * bg.bz + (loc & 3)};
       double Vi = g.V[node];
       double Ti = g.E[node] / Ci;
@@ -653,8 +661,8 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __
             }
       }
       double T_old = in.T[p], W_old = in.W[p];
-      const float heat_param = P.settling ? 0.0f : sc.heat_param[d];
-      Robin r = robin_terms(P, heat_param, Jf, sc.V0[p], T_old, W_old, particle_surface(sc, ws, p));
+      const bool rest = resting(P, ws, d);
+      Robin r = robin_terms(P, rest, sc.heat_param[d], Jf, sc.V0[p], T_old, W_old, particle_surface(sc, ws, p));
       double T_new = T_old + tc;
       double W_new = W_old * (1.0 + mr);
       double dry = sc.dry[p];
@@ -670,7 +678,7 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __
       sat = W_new / (double(P.rho_w) * double(sc.V0[p]) * double(Jf));
       if (P.model == 1 && P.gas_enabled) {
         moles = in.moles[p];
-        if (!P.settling) {
+        if (!rest) {
           double tm = 0.5 * (T_old + T_new);
           double z = (tm - double(P.gas_peak)) / double(P.gas_width);
           moles += double(P.dt * P.density) * double(sc.prod_param[d]) * exp(-(z * z));
@@ -1134,9 +1142,9 @@ int bk_internal_tile_launch(bk_workspace* ws, const bk_dough** doughs, int* n_do
 // Optional per-stage timing: HIP events recorded around each stage of the
 // substep on the launch stream (no extra synchronisation while recording).
 enum { kStageClear, kStageKeys, kStageSort, kStageTiles, kStageMesh, kStagePassA, kStageDiffuse, kStagePassB1, kStagePassB2,
-       kStageGrid, kStagePassC, kStageSurrogate, kStagePassCDeferred, kStageCount };
+       kStageGrid, kStagePassC, kStageSurrogate, kStagePassCDeferred, kStageSettling, kStageCount };
 static const char* kStageNames[kStageCount] = {"clear", "keys", "sort", "tiles", "mesh", "pass_a", "diffuse", "pass_b1",
-                                               "pass_b2", "grid", "pass_c", "surrogate", "pass_c_deferred"};
+                                               "pass_b2", "grid", "pass_c", "surrogate", "pass_c_deferred", "settling"};
 struct Profiler {
   int max_steps = 0, step = 0;
   std::vector<hipEvent_t> ev;        // [step][stage][start/stop]
@@ -1253,6 +1261,11 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, D, *sc, *in, *out, ws);
     BK_LAUNCH(pass_c_deferred);
   }
+  if (hw->settle) {  // 13. settling controller (rest metrics and the rest/bake decision)
+    StageTimer tm(pr, st, kStageSettling);
+    int rc = bk_internal_settling_launch(ws, D, nd, hw->max_dough_n, out->v, out->C, in->Fp, out->Fp, st);
+    if (rc) return rc;
+  }
   if (pr) ++pr->step;
   return 0;
 }
@@ -1277,9 +1290,11 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
   if (!table || nd < 1 || nd > h.max_doughs) return fail("bk_workspace_set_doughs: dough count exceeds the workspace");
   if (same_table(hw, table, nd)) return 0;
   long long n = 0, blocks = 0, nv = 0, ns = 0;
-  bool thermal = false, gas = false, surrogate = false;
+  int max_dough_n = 0;
+  bool thermal = false, gas = false, surrogate = false, settle = false;
   for (int d = 0; d < nd; ++d) {
     const bk_dough& D = table[d];
+    if (D.params.settling < 0 || D.params.settling > 2) return fail("bk_workspace_set_doughs: params.settling must be 0, 1 or 2");
     if (D.params.res < 4 || D.params.n < 1 || !(D.params.dx > 0.0f))
       return fail("bk_workspace_set_doughs: every dough needs res >= 4, n >= 1 and dx > 0");
     if (D.particle_offset != n || D.block_offset != blocks)
@@ -1292,6 +1307,8 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
     thermal |= D.params.transport_enabled != 0;
     gas |= D.params.model == 1 && D.params.gas_enabled != 0;
     surrogate |= D.params.surrogate != 0;
+    settle |= D.params.settling == 2;
+    max_dough_n = D.params.n > max_dough_n ? D.params.n : max_dough_n;
   }
   if (n > h.max_n || blocks > h.total_blocks || nv > h.n_vertices || ns > h.n_slots)
     return fail("bk_workspace_set_doughs: batch exceeds the workspace (particles, blocks, vertices or slots)");
@@ -1302,6 +1319,17 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
     BK_HIP(hipMemset(h.node_flag, 0, size_t(h.total_blocks) * 4));
     BK_HIP(hipMemset(h.counters, 0, 64));
   }
+  if (settle) {  // the settling controller's partials: 3 floats per dough and run of 256 particles
+    const int64_t need = int64_t(nd) * ((max_dough_n + 255) / 256) * 3;
+    if (need > hw->dev.settle_part_cap) {
+      if (hw->dev.settle_part) BK_HIP(hipFree(hw->dev.settle_part));
+      hw->dev.settle_part = nullptr;
+      hw->dev.settle_part_cap = 0;
+      BK_HIP(hipMalloc(&hw->dev.settle_part, size_t(need) * sizeof(float)));
+      hw->dev.settle_part_cap = need;
+      BK_HIP(hipMemcpy(ws, &hw->dev, sizeof(bk_workspace), hipMemcpyHostToDevice));
+    }
+  }
   hw->doughs.assign(table, table + nd);
   hw->n = int(n);
   hw->n_vertices = int(nv);
@@ -1309,6 +1337,8 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
   hw->thermal = thermal;
   hw->gas = gas;
   hw->surrogate = surrogate;
+  hw->settle = settle;
+  hw->max_dough_n = max_dough_n;
   int bits = 6;
   while ((1ll << bits) < blocks * 64) ++bits;
   hw->key_bits = bits;
@@ -1420,6 +1450,7 @@ bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blo
   alloc((void**)&h.features, n * kMlpRow * 4);
   alloc((void**)&h.predictions, n * kMlpOut * 4);
   alloc((void**)&h.stats, 4 * sizeof(unsigned long long));
+  alloc((void**)&h.settle, sizeof(bk_settling) * size_t(max_doughs));
   if (!ok || hipMemcpy(ws, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
     fail("bk_workspace_create: allocation failed");
     return nullptr;
@@ -1449,7 +1480,7 @@ void bk_workspace_destroy(bk_workspace* ws) {
   void* ptrs[] = {h.doughs, h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1],
                   h.hist, h.digit_total, h.blk_start, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
                   h.queue, h.tau, h.vertices, h.vert_mark, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions,
-                  h.stats};
+                  h.stats, h.settle, h.settle_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   (void)hipFree(ws);
@@ -1459,6 +1490,45 @@ int bk_workspace_set_doughs(bk_workspace* ws, const bk_dough* table, int32_t n_d
   HostWs* hw = host_view(ws);
   if (!hw) return fail("bk_workspace_set_doughs: bad workspace");
   return upload_doughs(ws, hw, table, n_doughs);
+}
+
+int bk_workspace_settling_reference(bk_workspace* ws, float* reference) {
+  HostWs* hw = host_view(ws);
+  if (!hw) return fail("bk_workspace_settling_reference: bad workspace");
+  BK_HIP(hipDeviceSynchronize());
+  hw->dev.settle_ref = reference;
+  BK_HIP(hipMemcpy(ws, &hw->dev, sizeof(bk_workspace), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int bk_workspace_settling_set(bk_workspace* ws, int32_t first, int32_t count, const bk_settling* records, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !records || count < 1) return fail("bk_workspace_settling_set: bad argument");
+  if (first < 0 || (long long)first + count > hw->dev.max_doughs)
+    return fail("bk_workspace_settling_set: dough range outside the workspace");
+  std::vector<bk_settling> r(records, records + count);
+  for (bk_settling& x : r) {
+    if (x.phase != 0 && x.phase != 1) return fail("bk_workspace_settling_set: phase must be 0 or 1");
+    if (x.ref_slot != 0 && x.ref_slot != 1) return fail("bk_workspace_settling_set: ref_slot must be 0 or 1");
+    if (!(x.time_step > 0.0)) return fail("bk_workspace_settling_set: time_step must be positive");
+    if (x.auto_mode && !hw->dev.settle_ref)
+      return fail("bk_workspace_settling_set: auto mode needs a reference buffer (bk_workspace_settling_reference)");
+  }
+  hipStream_t st = (hipStream_t)stream;
+  BK_HIP(hipMemcpyAsync(hw->dev.settle + first, r.data(), sizeof(bk_settling) * size_t(count), hipMemcpyHostToDevice, st));
+  BK_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+int bk_workspace_settling_get(bk_workspace* ws, int32_t first, int32_t count, bk_settling* records, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !records || count < 1) return fail("bk_workspace_settling_get: bad argument");
+  if (first < 0 || (long long)first + count > hw->dev.max_doughs)
+    return fail("bk_workspace_settling_get: dough range outside the workspace");
+  hipStream_t st = (hipStream_t)stream;
+  BK_HIP(hipMemcpyAsync(records, hw->dev.settle + first, sizeof(bk_settling) * size_t(count), hipMemcpyDeviceToHost, st));
+  BK_HIP(hipStreamSynchronize(st));
+  return 0;
 }
 
 int bk_workspace_set_tuning(bk_workspace* ws, const bk_tuning* tuning) {

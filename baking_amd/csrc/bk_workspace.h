@@ -58,7 +58,23 @@ struct bk_workspace {
   int32_t sample_cap = 0;
   unsigned long long* stats = nullptr;          // [0] porosity evaluations [1] guided solves accepted [2] guided solves
                                                 // rejected [3] queued return maps; summed over substeps
+  bk_settling* settle = nullptr;                // settling controller record per dough
+  float* settle_ref = nullptr;                  // caller's [2][max_n][9] settling reference plastic gradients, or null
+  float* settle_part = nullptr;                 // settling metric maxima per dough and run of 256 particles
+  int64_t settle_part_cap = 0;                  // floats in settle_part
 };
+
+// Whether dough d rests this substep (physics/thermal_water.py:296-355 with
+// settling=True): params.settling 1 always, 2 while its controller record is in
+// phase 1.
+__device__ __forceinline__ bool resting(const bk_params& P, const bk_workspace* ws, int d) {
+  return P.settling == 1 || (P.settling == 2 && ws->settle[d].phase != 0);
+}
+
+// Enqueue the settling controller after a substep with input plastic
+// gradients Fp_in and output (v, C, Fp) (bk_settling.hip).
+int bk_internal_settling_launch(bk_workspace* ws, const bk_dough* doughs, int n_doughs, int max_dough_n, const float* v,
+                                const float* C, const float* Fp_in, const float* Fp, hipStream_t st);
 // Position of batch block b inside its dough's grid.
 struct BlockGeom {
   int d, nb, bx, by, bz;

@@ -14,6 +14,12 @@ state, next state and grid are views into the batch buffers, so the
 per-dough reference API (``CheckState``, ``CaptureSnapshot``, ``UpdateMesh``,
 ``Masses``, the state tensors) keeps working on each dough; step the doughs
 through the batch only.
+
+Doughs that are still resting before baking (``physics/mpm.py:57-109``) keep
+resting in the batch: each has its own settling-controller record in the
+batch workspace, and ``Advance`` ends a chunk of substeps where a dough's
+rest could time out, as ``MpmSimulator.Advance`` does.  Changes of a dough's
+``transport.config`` after packing are not followed.
 """
 from __future__ import annotations
 
@@ -61,7 +67,7 @@ class BatchSimulator:
 
         table = (_native.BkDough * nd)()
         for d, s in enumerate(sims):
-            table[d].params = s._make_params(settling=False)
+            table[d].params = s._make_params()  # settling 2 for a dough still resting
             table[d].particle_offset = int(self.particle_offsets[d])
             table[d].block_offset = int(self.block_offsets[d])
         self._table = table
@@ -73,6 +79,65 @@ class BatchSimulator:
         if tuning is not None:
             self.SetTuning(**tuning)
         self._flags_host = torch.zeros(nd * 8, dtype=torch.int32).pin_memory()
+        self._settle_reference = None
+        if any(s.IsSettling for s in sims):
+            self._pack_settling()
+
+    def _pack_settling(self):
+        """Move the resting doughs' controller records and reference plastic gradients into the batch."""
+        lib = _native.load()
+        nd, n = self.dough_count, self.particle_count
+        self._settle_reference = torch.zeros(2, n, 3, 3, dtype=torch.float32, device=self.device_name)
+        _native.check(lib.bk_workspace_settling_reference(self._workspace, _native.ptr(self._settle_reference)),
+                      "bk_workspace_settling_reference")
+        records = (_native.BkSettling * nd)()
+        for d, s in enumerate(self.doughs):
+            sl = self.dough_slice(d)
+            if s.IsSettling and s._settle_record is not None:
+                ctypes.memmove(ctypes.byref(records[d]), ctypes.byref(s._settle_record), ctypes.sizeof(_native.BkSettling))
+                if s._settle_reference is not None:
+                    self._settle_reference[:, sl] = s._settle_reference
+            else:
+                records[d].phase, records[d].time_step = 0, s.simulation_config.time_step
+            if s._settle_reference is not None:
+                s._settle_reference = self._settle_reference[:, sl]
+        _native.check(lib.bk_workspace_settling_set(self._workspace, 0, nd, records, _native.stream_ptr()),
+                      "bk_workspace_settling_set")
+
+    def _read_settling(self):
+        records = (_native.BkSettling * self.dough_count)()
+        _native.check(_native.load().bk_workspace_settling_get(self._workspace, 0, self.dough_count, records,
+                                                               _native.stream_ptr()), "bk_workspace_settling_get")
+        return records
+
+    def _pull_settling(self, resting):
+        """Mirror the records of the doughs that rested into their simulators; raise for timed-out rests."""
+        records = self._read_settling()
+        timed_out = []
+        for d in resting:
+            s, rec = self.doughs[d], records[d]
+            if rec.step_count != s.step_count if rec.phase else rec.step_count > s.step_count:  # counts resting substeps
+                raise RuntimeError(f"settling controller of dough {d} is at substep {rec.step_count}, the dough at {s.step_count}")
+            s._settle_record = _native.BkSettling.from_buffer_copy(rec)
+            s._apply_settling(rec)
+            if rec.reason == 3 and rec.phase == 1:
+                timed_out.append(d)
+        if self._table_rests and not any(s.IsSettling for s in self.doughs):
+            self._refresh_table()  # every dough bakes: drop the controller from the substep
+        if timed_out:
+            raise RuntimeError(f"Settling did not reach quasi-static tolerances before settling_duration_s in doughs {timed_out}; "
+                               "baking was not started. Increase the maximum duration or review tolerances/material.")
+
+    @property
+    def _table_rests(self):
+        return any(self._table[d].params.settling == 2 for d in range(self.dough_count))
+
+    def _refresh_table(self):
+        for d, s in enumerate(self.doughs):
+            self._table[d].params = s._make_params()
+        _native.check(_native.load().bk_workspace_set_doughs(self._workspace, self._table, self.dough_count),
+                      "bk_workspace_set_doughs")
+        self._destroy_graphs()
 
     # ------------------------------------------------------------------ packing
     def _pack_state(self, which):
@@ -188,12 +253,9 @@ class BatchSimulator:
 
     def SetSurrogate(self, using):
         """Per-dough switch of the guided coupled-porosity solve (see baking_amd.surrogate.attach)."""
-        lib = _native.load()
         for d, on in enumerate(using):
             self.doughs[d]._surrogate_on = bool(on)
-            self._table[d].params.surrogate = int(bool(on))
-        _native.check(lib.bk_workspace_set_doughs(self._workspace, self._table, self.dough_count), "bk_workspace_set_doughs")
-        self._destroy_graphs()
+        self._refresh_table()
 
     # ------------------------------------------------------------------ tuning
     def SetTuning(self, tile_chunk=None, particle_block=None, grid_cap=None):
@@ -287,6 +349,17 @@ class BatchSimulator:
         """
         if step_count < 1:
             raise ValueError("step_count must be positive")
+        remaining = step_count
+        while True:
+            # a resting dough can time out only at its settling_steps: end the chunk there
+            chunk = min(s._settling_chunk(remaining) for s in self.doughs)
+            flags = self._advance_chunk(chunk, check, stream)
+            remaining -= chunk
+            if remaining == 0:
+                return flags
+
+    def _advance_chunk(self, step_count, check, stream):
+        resting = [d for d, s in enumerate(self.doughs) if s.IsSettling]
         before = self.ReadFlags() if check else None
         for _ in range(step_count):
             self._launch(stream)
@@ -300,6 +373,8 @@ class BatchSimulator:
             if new.any():
                 bad = {int(d): [_FLAG_NAMES[i] for i in np.flatnonzero(new[d, :7])] for d in np.flatnonzero(new.any(axis=1))}
                 raise RuntimeError(f"batched substep flags raised in doughs {bad}")
+        if resting:
+            self._pull_settling(resting)
         return flags
 
     def Step(self):
@@ -318,10 +393,18 @@ class BatchSimulator:
 
     def _save_for_tuning(self):
         flags = self.ReadFlags()
-        return self._cur, flags, [t.clone() for side in self._states for t in side.values()]
+        settling = None
+        if self._settle_reference is not None:
+            settling = (self._read_settling(), self._settle_reference.clone())
+        return self._cur, flags, [t.clone() for side in self._states for t in side.values()], settling
 
     def _restore_after_tuning(self, saved):
-        cur, flags, clones = saved
+        cur, flags, clones, settling = saved
+        if settling is not None:
+            records, reference = settling
+            self._settle_reference.copy_(reference)
+            _native.check(_native.load().bk_workspace_settling_set(self._workspace, 0, self.dough_count, records,
+                                                                   _native.stream_ptr()), "bk_workspace_settling_set")
         torch.cuda.synchronize()
         for dst, src in zip([t for side in self._states for t in side.values()], clones):
             dst.copy_(src)

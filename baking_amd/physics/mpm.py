@@ -15,6 +15,14 @@ Error semantics follow the reference:
   the flags once.  If any transport/Ding flag is set it restores the state it
   started from and replays substep by substep, so the error is raised at the
   same substep, with the same accepted state, as ``Step()`` would.
+
+Settling (``physics/mpm.py:57-109``): while the dough rests, the substep's
+parameters carry ``settling = 2`` and the library's settling controller --
+a per-dough record on the device -- reduces the rest metrics after every
+substep and starts baking, so resting substeps need no host round trip.  The
+host reads the record back after ``Step`` and at the end of each ``Advance``
+chunk; ``Advance`` ends a chunk at ``settling_steps`` so that a timeout raises
+at the reference's substep with the reference's accepted state.
 """
 from __future__ import annotations
 
@@ -77,8 +85,9 @@ class MpmSimulator:
         self.transport = None
         if transport_config is not None and transport_config.enabled:
             transport_config.Validate(material_config)
-            if transport_config.settling_duration_s > 0:
-                raise NotImplementedError("settling phases are not provided by this build (settling_duration_s must be 0)")
+            if (transport_config.settling_duration_s > 0 and transport_config.settling_auto
+                    and not isinstance(self.material, DingMaterial)):
+                raise ValueError("automatic settling measures the plastic-gradient rate and needs the ding2019 material")
             self.material.initial_temperature_k = transport_config.initial_temperature_k
             self.transport = ThermalWaterTransport(self, transport_config)
         self._workspace = None
@@ -86,16 +95,24 @@ class MpmSimulator:
 
     # ------------------------------------------------------------------ properties kept from the reference
     @property
+    def _settling_configured(self):
+        return self.transport is not None and self.transport.config.settling_duration_s > 0
+
+    @property
     def settling_steps(self):
-        return 0
+        if self._baking_start_step is not None and self._baking_start_step > 0:
+            return self._baking_start_step
+        if self.transport is None:
+            return 0
+        return int(math.ceil(self.transport.config.settling_duration_s / self.simulation_config.time_step - 1e-10))
 
     @property
     def IsSettling(self):
-        return False
+        return self._settling_configured and self._baking_start_step is None
 
     @property
     def baking_steps(self):
-        return self.step_count
+        return 0 if self.IsSettling else max(0, self.step_count - self.settling_steps)
 
     @property
     def baking_time(self):
@@ -141,11 +158,16 @@ class MpmSimulator:
         self.vertex_positions = torch.as_tensor(self.tet_mesh.vertex_positions.astype(np.float32), device=self.device_name)
         self.error_flags = torch.zeros(2, dtype=torch.int32, device=self.device_name)
         self.step_count = 0
-        self._baking_start_step = 0
-        self.settling_reason = "disabled"
+        self._baking_start_step = None if self._settling_configured else 0
+        self.settling_quiet_steps = 0
+        self.settling_reference_step = 0
+        self.settling_plastic_rate = 0.0
+        self.settling_reason = "pending" if self._baking_start_step is None else "disabled"
         self.settling_diagnostics = dict(max_speed_m_s=0.0, max_strain_rate_per_s=0.0, max_plastic_rate_per_s=0.0)
         self.mesh_step_count = 0
         self._build_native()
+        if self._settling_configured:
+            self._start_settling()
         if getattr(self, "result_recorder", None) is not None:
             self.result_recorder.Reset()
 
@@ -194,7 +216,9 @@ class MpmSimulator:
             setattr(scene, name, keep[name].data_ptr() if name in keep else None)
         self._scene = scene
         self._refresh_grid_struct()
-        self._params = self._make_params(settling=False)
+        self._settle_record = None
+        self._settle_reference = None
+        self._params = self._make_params()
 
     def _refresh_grid_struct(self):
         grid = _native.BkGrid()
@@ -213,7 +237,13 @@ class MpmSimulator:
         except Exception:
             pass
 
-    def _make_params(self, settling):
+    def _make_params(self, settling=None):
+        """bk_params of this simulator.  settling: 0 bake, 1 rest, 2 follow the settling controller;
+        None picks 2 while the dough rests and 0 once baking has started."""
+        if settling is None:
+            settling = 2 if self.IsSettling else 0
+        settling = int(settling)
+        rest = settling == 1
         sim, mat = self.simulation_config, self.material.material_config
         P = _native.BkParams()
         P.origin = (ctypes.c_float * 3)(*sim.grid_origin)
@@ -235,13 +265,13 @@ class MpmSimulator:
             P.p_ambient, P.gas_peak, P.gas_width = c.gas_ambient_pressure_pa, c.gas_reaction_temperature_k, c.gas_reaction_width_k
         if self.transport is not None:
             t = self.transport.config
-            P.transport_enabled, P.water_enabled, P.settling = 1, int(t.water_enabled), int(settling)
+            P.transport_enabled, P.water_enabled, P.settling = 1, int(t.water_enabled), settling
             P.T_oven, P.T_tray = t.oven_temperature_k, t.tray_temperature_k
-            P.tray_h = 0.0 if settling else t.tray_heat_transfer
-            P.loss_coef = 0.0 if settling else t.surface_water_loss
+            P.tray_h = 0.0 if rest else t.tray_heat_transfer
+            P.loss_coef = 0.0 if rest else t.surface_water_loss
             P.rho_w, P.latent = t.water_density, t.latent_heat
-            P.kappa = 0.0 if settling else t.conductivity
-            P.diffusivity = 0.0 if settling else t.water_diffusivity
+            P.kappa = 0.0 if rest else t.conductivity
+            P.diffusivity = 0.0 if rest else t.water_diffusivity
             P.s0, P.water_stress, P.contact_distance = t.initial_saturation, t.water_stress, t.contact_distance
             P.n_vertices = len(self.tet_mesh.vertex_positions)
             P.n_faces = len(self.transport.faces)
@@ -250,6 +280,112 @@ class MpmSimulator:
         P.return_solver = {"bisection": 0, "bracketed": 1}[self.return_solver]
         P.surrogate = int(self._surrogate_on)
         return P
+
+    # ------------------------------------------------------------------ settling controller (physics/mpm.py:77-109)
+    def _start_settling(self):
+        """Device record of a dough starting its rest at step 0; the reference plastic gradients are Fp now."""
+        t = self.transport.config
+        self._settle_reference = None
+        if isinstance(self.material, DingMaterial):
+            n = self.particle_data.particle_count
+            self._settle_reference = torch.zeros(2, n, 3, 3, dtype=torch.float32, device=self.device_name)
+            _native.check(_native.load().bk_workspace_settling_reference(self._workspace, _native.ptr(self._settle_reference)),
+                          "bk_workspace_settling_reference")
+        rec = _native.BkSettling()
+        rec.auto_mode = int(t.settling_auto)
+        rec.settling_steps = self.settling_steps
+        rec.time_step = self.simulation_config.time_step
+        rec.min_duration_s, rec.hold_duration_s = t.settling_min_duration_s, t.settling_hold_duration_s
+        rec.speed_tolerance, rec.rate_tolerance = t.settling_speed_tolerance, t.settling_rate_tolerance
+        rec.phase, rec.reason, rec.step_count, rec.baking_start_step = 1, 0, self.step_count, -1
+        self._write_settling(rec)
+        if isinstance(self.material, DingMaterial):
+            self.settling_reference_plastic.copy_(self.current_state.material_state.plastic_gradients)
+
+    def _write_settling(self, rec):
+        _native.check(_native.load().bk_workspace_settling_set(self._workspace, 0, 1, ctypes.byref(rec), _native.stream_ptr()),
+                      "bk_workspace_settling_set")
+        self._settle_record = _native.BkSettling.from_buffer_copy(rec)
+
+    @property
+    def settling_reference_plastic(self):
+        """The live (N, 3, 3) half of the controller's reference plastic gradients (None without a rest)."""
+        if self._settle_record is None or self._settle_reference is None:
+            return None
+        return self._settle_reference[self._settle_record.ref_slot]
+
+    def _save_settling(self):
+        """Record and live reference of the controller, to undo substeps that are replayed."""
+        if self._settle_record is None:
+            return None
+        return _native.BkSettling.from_buffer_copy(self._settle_record), self.settling_reference_plastic.clone()
+
+    def _restore_settling(self, saved):
+        if saved is None:
+            return
+        rec, ref = saved
+        self._write_settling(rec)
+        self.settling_reference_plastic.copy_(ref)
+        self._apply_settling(rec)
+
+    def _apply_settling(self, rec):
+        """Mirror the device record in the reference's simulator attributes."""
+        self.settling_quiet_steps = int(rec.quiet_steps)
+        self.settling_reference_step = int(rec.reference_step)
+        self.settling_plastic_rate = float(rec.plastic_rate)
+        self.settling_reason = _native.SETTLING_REASONS[rec.reason]
+        if rec.auto_mode:
+            d = rec.diagnostics
+            self.settling_diagnostics = dict(max_speed_m_s=float(d[0]), max_strain_rate_per_s=float(d[1]),
+                                             max_plastic_rate_per_s=float(d[2]))
+        if rec.phase == 0 and self._baking_start_step is None:
+            self._baking_start_step = int(rec.baking_start_step)
+            self._params = self._make_params()
+        elif rec.phase == 1 and self._baking_start_step is not None and self._settling_configured:
+            self._baking_start_step = None
+            self._params = self._make_params()
+
+    def _pull_settling(self):
+        """Read the record after the substeps already enqueued; raise the reference's timeout error."""
+        rec = _native.BkSettling()
+        _native.check(_native.load().bk_workspace_settling_get(self._workspace, 0, 1, ctypes.byref(rec), _native.stream_ptr()),
+                      "bk_workspace_settling_get")
+        if rec.step_count != self.step_count if rec.phase else rec.step_count > self.step_count:  # counts resting substeps
+            raise RuntimeError(f"settling controller is at substep {rec.step_count}, the simulator at {self.step_count}")
+        self._settle_record = rec
+        self._apply_settling(rec)
+        if rec.reason == 3 and rec.phase == 1:
+            raise RuntimeError('Settling did not reach quasi-static tolerances before settling_duration_s; '
+                               'baking was not started. Increase the maximum duration or review tolerances/material.')
+
+    def _sync_settling(self):
+        """Follow changes of ``transport.config`` made since the last substep, as the reference, which reads it
+        live: the rest/bake mode of the parameters and the controller's configuration."""
+        if self.transport is None:
+            return
+        mode = 2 if self.IsSettling else 0
+        if self._params.settling != mode:
+            self._params = self._make_params()
+        rec = self._settle_record
+        if not self.IsSettling or rec is None:
+            return
+        t = self.transport.config
+        want = (int(t.settling_auto), self.settling_steps, t.settling_min_duration_s, t.settling_hold_duration_s,
+                t.settling_speed_tolerance, t.settling_rate_tolerance)
+        have = (rec.auto_mode, rec.settling_steps, rec.min_duration_s, rec.hold_duration_s, rec.speed_tolerance, rec.rate_tolerance)
+        if want != have:
+            if want[0] and self._settle_reference is None:
+                raise ValueError("automatic settling measures the plastic-gradient rate and needs the ding2019 material")
+            new = _native.BkSettling.from_buffer_copy(rec)
+            (new.auto_mode, new.settling_steps, new.min_duration_s, new.hold_duration_s, new.speed_tolerance,
+             new.rate_tolerance) = want
+            self._write_settling(new)
+
+    def _settling_chunk(self, remaining):
+        """Substeps Advance may run before reading the controller: a timeout can only come at settling_steps."""
+        if not self.IsSettling or self.step_count + remaining < self.settling_steps:
+            return remaining
+        return max(1, self.settling_steps - self.step_count)
 
     @staticmethod
     def _state_struct(state):
@@ -270,13 +406,13 @@ class MpmSimulator:
 
     def SetSurrogate(self, on):
         self._surrogate_on = bool(on)
-        self._params = self._make_params(settling=False)
+        self._params = self._make_params()
 
     def SetReturnSolver(self, name):
         if name not in ("bisection", "bracketed"):
             raise ValueError("return solver must be 'bisection' or 'bracketed'")
         self.return_solver = name
-        self._params = self._make_params(settling=False)
+        self._params = self._make_params()
 
     def SetTuning(self, tile_chunk=None, particle_block=None, grid_cap=None):
         """Launch shapes of the fused substep (``bk_tuning``); captured graphs are dropped."""
@@ -302,10 +438,11 @@ class MpmSimulator:
     def _save_for_tuning(self):
         flags = self._read_flags()
         tensors = _state_tensors(self.current_state) + _state_tensors(self.next_state)
-        return self.current_state, self.next_state, self.step_count, flags, [t.clone() for t in tensors]
+        return (self.current_state, self.next_state, self.step_count, flags, [t.clone() for t in tensors],
+                self._save_settling())
 
     def _restore_after_tuning(self, saved):
-        current, nxt, steps, flags, clones = saved
+        current, nxt, steps, flags, clones, settling = saved
         torch.cuda.synchronize()
         self.current_state, self.next_state, self.step_count = current, nxt, steps
         for dst, src in zip(_state_tensors(current) + _state_tensors(nxt), clones):
@@ -314,6 +451,7 @@ class MpmSimulator:
         for t in self.thermal_grid.values():
             t.zero_()
         self._write_flags(flags)
+        self._restore_settling(settling)
 
     def _destroy_graphs(self):
         lib = _native.load()
@@ -376,11 +514,20 @@ class MpmSimulator:
             self.transport.error_flags.copy_(torch.tensor(flags[5:7], dtype=torch.int32))
 
     def Step(self):
+        self._sync_settling()
+        was_settling = self.IsSettling
+        saved = self._save_settling() if was_settling else None
         self._launch_substep()
         flags = self._read_flags()
         self._sync_api_flags(flags)
-        self._raise_for(flags)
+        try:
+            self._raise_for(flags)
+        except RuntimeError:
+            self._restore_settling(saved)  # the rejected substep never ran for the controller either
+            raise
         self._accept()
+        if was_settling:
+            self._pull_settling()
         if getattr(self, "result_recorder", None) is not None:
             self.result_recorder.Record()
 
@@ -392,10 +539,22 @@ class MpmSimulator:
             for _ in range(step_count):
                 self.Step()
             return self.CheckState() if check else None
+        remaining = step_count
+        while remaining > 0:
+            chunk = self._settling_chunk(remaining)
+            self._advance_chunk(chunk)
+            remaining -= chunk
+        self._sync_api_flags(self._read_flags())
+        return self.CheckState() if check else None
+
+    def _advance_chunk(self, count):
+        self._sync_settling()
+        was_settling = self.IsSettling
         flags_before = self._read_flags()
         start_state, start_next, start_steps = self.current_state, self.next_state, self.step_count
         saved = [t.clone() for t in _state_tensors(start_state)]
-        for _ in range(step_count):
+        saved_settling = self._save_settling() if was_settling else None
+        for _ in range(count):
             self._launch_substep()
             self._accept()
         flags = self._read_flags()
@@ -406,11 +565,13 @@ class MpmSimulator:
             for dst, src in zip(_state_tensors(start_state), saved):
                 dst.copy_(src)
             self._write_flags(flags_before)
-            for _ in range(step_count):
+            self._restore_settling(saved_settling)
+            for _ in range(count):
                 self.Step()
+            return
         del saved
-        self._sync_api_flags(self._read_flags())
-        return self.CheckState() if check else None
+        if was_settling:
+            self._pull_settling()
 
     def UpdateMesh(self, force_update=False):
         if force_update or self.mesh_step_count != self.step_count:

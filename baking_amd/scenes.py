@@ -403,3 +403,27 @@ def demo_transport(heat_transfer=20.0, gas=True, water_stress=0.0, spacing=0.025
     transport = TransportConfig(enabled=True, heat_transfer=heat_transfer, water_stress=water_stress)
     return SceneSpec("demo_transport", mesh, SampleParticles(mesh, 1000.0, 4), sim, MaterialConfig(model="ding2019", young_modulus=10000.0),
                      ding, transport, steps=4)
+
+
+def demo_resting(heat_transfer=20.0, settling_duration_s=0.0004):
+    """Fixed-duration rest before baking (reference tests/test_settling.py:8-12): the transport cube of
+    tests/test_transport.py:12-20 with pore gas, default water stress, gravity -9.81 and a 0.4 ms rest
+    (4 substeps at dt 1e-4)."""
+    spec = demo_transport(heat_transfer=heat_transfer, gas=True, water_stress=TransportConfig().water_stress)
+    spec.name = "demo_resting"
+    spec.simulation = replace(spec.simulation, gravity=(0.0, -9.81, 0.0))
+    spec.transport = replace(spec.transport, settling_duration_s=settling_duration_s)
+    spec.steps = 8
+    return spec
+
+
+def demo_auto_settling(heat_transfer=20.0, gravity=(0.0, 0.0, 0.0)):
+    """Automatic quasi-static detection (reference tests/test_auto_settling.py:9-16): at most 2 ms of rest,
+    0.2 ms minimum, 0.3 ms hold, speed tolerance 1e-4 m/s and rate tolerance 0.01 1/s."""
+    spec = demo_transport(heat_transfer=heat_transfer, gas=True, water_stress=TransportConfig().water_stress)
+    spec.name = "demo_auto_settling"
+    spec.simulation = replace(spec.simulation, gravity=tuple(gravity))
+    spec.transport = replace(spec.transport, settling_duration_s=0.002, settling_auto=True, settling_min_duration_s=0.0002,
+                             settling_hold_duration_s=0.0003, settling_speed_tolerance=1e-4, settling_rate_tolerance=0.01)
+    spec.steps = 8
+    return spec

@@ -56,6 +56,8 @@ STAGE_BYTES = {
     # reads x, F, Fe_old, porosity, gelled, T_mat, moles, W, V0 ; writes x, v, C, F, Fe, Fp, tau, residual,
     # porosity, porosity residual, T_mat, gelled, yield, viscosity, pressure, saturation; gathers v
     "pass_c": (12 + 36 + 36 + 4 + 4 + 4 + 8 + 8 + 4 + 96 + 108 + 8 + 16 + 8 + 8, 12),
+    # settling controller of a resting dough: reads v, C, the reference Fp and the new Fp
+    "settling": (12 + 36 + 36 + 36, 0),
 }
 # Deferred coupled-porosity return maps, per queued particle: reads Fe_old, porosity, gelled, F, C, T_mat,
 # moles, W, V0; writes Fe, Fp, tau, residual, porosity residual, porosity, T_mat, gelled, yield, viscosity,
@@ -71,7 +73,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed substeps (default: 1000 batched_48, 2000 loaf_256, 1000 sphere_small)")
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="batched_48", choices=["batched_48", "loaf_256", "sphere_small", "adjoint_loaf"])
+    ap.add_argument("--workload", default="batched_48",
+                    choices=["batched_48", "loaf_256", "sphere_small", "adjoint_loaf", "rest_loaf"])
     ap.add_argument("--no-surrogate", action="store_true", help="exact return map only")
     ap.add_argument("--model", default=None, help="surrogate model file (default: the packaged one)")
     ap.add_argument("--plain-steps", type=int, default=20, help="batched_48: substeps per dough of the plain comparator")
@@ -437,6 +440,67 @@ def run_adjoint(args):
     print(json.dumps(result))
 
 
+def run_rest(args):
+    """The loaf resting before baking (automatic settling with examples/oven_baking.txt:66-71): substep
+    throughput with the device-side settling controller, and the controller's own cost."""
+    import torch
+    from dataclasses import replace
+    from baking_amd.scenes import build_simulator, loaf_256
+    spec = loaf_256()
+    spec.transport = replace(spec.transport, settling_duration_s=5.0, settling_auto=True, settling_min_duration_s=0.1,
+                             settling_hold_duration_s=0.1, settling_speed_tolerance=0.001, settling_rate_tolerance=0.01)
+    sim = build_simulator(spec)
+    n = sim.particle_data.particle_count
+    # the rest cannot end before min_duration_s (2000 substeps at dt 5e-5): every timed substep rests
+    steps = args.steps or 1000
+    stream = torch.cuda.current_stream()
+    for _ in range(args.warmup):
+        sim._launch_substep()
+        sim._accept()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        sim._launch_substep()
+        sim._accept()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    gpu_ms = start.elapsed_time(stop)
+    sim._pull_settling()
+    resting = sim.IsSettling
+    stages, counts = stage_profile(lib_launch=lambda: (sim._launch_substep(use_graph=False), sim._accept()),
+                                   workspace=sim._workspace, steps=args.profile_steps)
+    sim._pull_settling()
+    settle_ms = stages.get("settling", 0.0)
+    settle_bytes = STAGE_BYTES["settling"][0] * n
+    result = {
+        "metric": "resting particle-substeps/s on 1 MI355X (settling controller on the device)",
+        "value": n * steps / (gpu_ms * 1e-3),
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": gpu_ms / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "rest_loaf", "particles": n, "grid": spec.simulation.grid_resolution,
+                   "dt": spec.simulation.time_step, "settling": "auto, examples/oven_baking.txt:66-71",
+                   "still_resting": resting, "quiet_steps": sim.settling_quiet_steps,
+                   "diagnostics": sim.settling_diagnostics, "tiles": int(counts[0])},
+        "settling_controller": {"kernels": ["k_settling_update", "k_settling_finish"], "ms_per_substep": settle_ms,
+                                "algorithmic_bytes": settle_bytes,
+                                "achieved_GBs": settle_bytes / (settle_ms * 1e-3) / 1e9 if settle_ms > 0 else 0.0,
+                                "peak_GBs": HBM_PEAK_GBS,
+                                "frac": settle_bytes / (settle_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if settle_ms > 0 else 0.0,
+                                "share_of_substep": settle_ms / (gpu_ms / steps)},
+        "stages_ms": stages,
+    }
+    print(json.dumps(result))
+
+
 def main():
     args = parse()
     if args.copy_probe:
@@ -454,6 +518,8 @@ def main():
         return run_batched(args)
     if args.workload == "adjoint_loaf":
         return run_adjoint(args)
+    if args.workload == "rest_loaf":
+        return run_rest(args)
     from baking_amd import surrogate
     from baking_amd.measure import measure
     spec = build_scene(args.workload)

@@ -59,7 +59,9 @@ typedef struct bk_params {
   int32_t temperature_enabled, gas_enabled;
   float knots[4], yield_factors[4], viscosity_factors[4];
   float p_ambient, gas_peak, gas_width;
-  /* transport (config.py:148-196) */
+  /* transport (config.py:148-196).  settling: 0 baking; 1 resting (thermal_water.py:296-355 with settling=True:
+   * no heat or tray transfer, no surface loss, no conduction or water diffusion, no gas production); 2 rest
+   * while the dough's settling controller (bk_settling) is in phase 1, then bake */
   int32_t transport_enabled, water_enabled, settling;
   float T_oven, T_tray, tray_h, loss_coef, rho_w, latent, kappa, diffusivity, s0, water_stress, contact_distance;
   /* sizes */
@@ -165,6 +167,45 @@ int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* strea
 int bk_workspace_read_flags_all(bk_workspace* ws, int32_t* host, int32_t n_doughs, void* stream);
 int bk_workspace_write_flags_all(bk_workspace* ws, const int32_t* host, int32_t n_doughs, void* stream);
 
+/* ---------------------------------------------------------------- settling controller
+ * The rest before baking (physics/mpm.py:57-109 settling_steps/IsSettling/UpdateSettling with
+ * physics/quasistatic.py:11-21 SettlingMetrics), one record per dough kept in the workspace.  While a dough's
+ * params.settling is 2, every substep of bk_substep / bk_batch_substep (and their graphs) rests if the record's
+ * phase is 1, and ends with the controller's kernels, which reduce the three settling metrics over the dough's particles
+ * (auto mode) and advances the record exactly as UpdateSettling advances the simulator: the host reads it only
+ * when it wants the outcome, never per substep.  Auto mode reads state.Fp (the Ding model).  On a timeout the
+ * record keeps phase 1 and sets reason 3; the caller raises the reference's RuntimeError. */
+typedef struct bk_settling {
+  /* configuration (TransportConfig, config.py:150-155) */
+  int32_t auto_mode;          /* settling_auto */
+  int32_t settling_steps;     /* ceil(settling_duration_s / time_step - 1e-10): fixed duration, or the timeout */
+  double time_step;
+  double min_duration_s, hold_duration_s, speed_tolerance, rate_tolerance;
+  /* state (the simulator attributes of physics/mpm.py:150-157) */
+  int32_t phase;              /* 1 while resting (IsSettling), 0 once baking has started */
+  int32_t reason;             /* settling_reason: 0 pending, 1 fixed_duration, 2 quasi_static, 3 timeout */
+  int64_t step_count;         /* substeps run, counted after each one as step_count is */
+  int64_t reference_step;     /* settling_reference_step */
+  int64_t quiet_steps;        /* settling_quiet_steps */
+  int64_t baking_start_step;  /* _baking_start_step, -1 while resting */
+  double plastic_rate;        /* settling_plastic_rate */
+  float diagnostics[3];       /* settling_diagnostics: max speed, max strain rate, max plastic rate */
+  int32_t ref_slot;           /* half of the reference buffer holding settling_reference_plastic */
+  int32_t copy_pending;       /* the last substep closed a plastic-rate window: the next one takes its input Fp as
+                                 the reference (and stores it in the ref_slot half) */
+  int32_t reserved;
+} bk_settling;
+
+/* Register the caller-owned reference buffer, device float[2][max_particles][9] in batch particle order: the
+ * two halves of settling_reference_plastic (record.ref_slot names the live one; a closed window writes the
+ * other half and flips ref_slot, so restoring a saved record restores its reference).  Synchronises the device.
+ * Auto mode needs it. */
+int bk_workspace_settling_reference(bk_workspace* ws, float* reference);
+/* Write the whole records of doughs [first, first + count) after the work already on `stream`; _get copies them
+ * back (blocking). */
+int bk_workspace_settling_set(bk_workspace* ws, int32_t first, int32_t count, const bk_settling* records, void* stream);
+int bk_workspace_settling_get(bk_workspace* ws, int32_t first, int32_t count, bk_settling* records, void* stream);
+
 /* ---------------------------------------------------------------- return-map surrogate (no reference counterpart)
  * A 7 -> 4 x 64 SiLU -> 4 network standing in for the start of the coupled (gas)
  * Ding return map (physics/materials/ding.py:168-221 with porous_return.py:43-61).
@@ -267,6 +308,9 @@ int bk_UpdateGrid(const float* node_masses, const float* node_momenta, float* no
 int bk_GridToParticle(const float* x_in, const float* F_in, const float* node_velocities, float* x_out, float* v_out, float* C_out,
                       float* F_out, const float* origin3_host, float dx, int32_t res, float dt, int32_t* flags, int32_t n,
                       void* stream);
+/* physics/quasistatic.py:11-21; metrics = float[3] running maxima (the caller zeroes them) */
+int bk_SettlingMetrics(const float* velocities, const float* affine, const float* old_plastic, const float* new_plastic,
+                       float* metrics, float time_step, int32_t n, void* stream);
 /* physics/materials/elastic.py:13-26 */
 int bk_ComputeElasticStress(const float* F, float* tau, float mu, float lam, int32_t* flags, int32_t n, void* stream);
 /* physics/materials/ding.py:140-221; ding flags = int32[3] */

@@ -660,6 +660,30 @@ static void Saturations(int n, const double* W, const R* V0, const R* F, double*
   for (int p = 0; p < n; ++p) sat[p] = W[p] / (double(rho_w) * double(V0[p]) * double(det(load_m(F + 9 * p))));
 }
 
+template <typename R> inline R frobenius_sq(const M3<R>& m) {
+  R s = R(0);
+  for (int i = 0; i < 9; ++i) s = s + m.a[i] * m.a[i];
+  return s;
+}
+
+template <typename R>
+static void SettlingMetrics(int n, const R* v, const R* A, const R* old_p, const R* new_p, R* metrics, R dt) {
+  // physics/quasistatic.py:11-21: running maxima of |v|, |sym C|_F and |Fp - Fp_ref|_F / (dt |Fp_ref|_F);
+  // metrics[] keeps the caller's starting values (zero in the reference) like its atomic_max targets.
+  for (int p = 0; p < n; ++p) {
+    const M3<R> a = load_m(A + 9 * p), o = load_m(old_p + 9 * p), w = load_m(new_p + 9 * p);
+    M3<R> s;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) s.a[i * 3 + j] = R(0.5) * (a.a[i * 3 + j] + a.a[j * 3 + i]);
+    const R speed = std::sqrt(v[3 * p] * v[3 * p] + v[3 * p + 1] * v[3 * p + 1] + v[3 * p + 2] * v[3 * p + 2]);
+    const R strain = std::sqrt(frobenius_sq(s));
+    const R plastic = std::sqrt(frobenius_sq(sub(w, o))) / (dt * std::sqrt(frobenius_sq(o)));
+    metrics[0] = std::max(metrics[0], speed);
+    metrics[1] = std::max(metrics[1], strain);
+    metrics[2] = std::max(metrics[2], plastic);
+  }
+}
+
 template <typename R>
 static void ReconstructMeshVertices(int nv, const R* x, const R* F, const R* ref_verts, const int* offs, const int* parts,
                                     const R* weights, const R* ref_offs, R* out) {
@@ -809,6 +833,9 @@ ORACLE_BOTH(LocalGas,
             (n, om, To, Tn, nm, pr, F, por, pa, de, pk, wi, dt, pro))
 ORACLE_BOTH(WaterStress, (int n, const float* F, const double* s, const float* os, float* ns, float s0, float k),
             (int n, const double* F, const double* s, const double* os, double* ns, double s0, double k), (n, F, s, os, ns, s0, k))
+ORACLE_BOTH(SettlingMetrics, (int n, const float* v, const float* A, const float* op, const float* np, float* m, float dt),
+            (int n, const double* v, const double* A, const double* op, const double* np, double* m, double dt),
+            (n, v, A, op, np, m, dt))
 ORACLE_BOTH(Saturations, (int n, const double* W, const float* V0, const float* F, double* s, float rw),
             (int n, const double* W, const double* V0, const double* F, double* s, double rw), (n, W, V0, F, s, rw))
 ORACLE_BOTH(ReconstructMeshVertices,

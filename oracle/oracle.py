@@ -371,10 +371,74 @@ class OracleSimulator:
         self.ding_flags = np.zeros(3, np.int32)
         self.transport_flags = np.zeros(2, np.int32)
         self.step_count = 0
+        # settling controller (physics/mpm.py:150-157)
+        self._baking_start_step = None if self.tr is not None and self.tr.settling_duration_s > 0 else 0
+        self.settling_quiet_steps = 0
+        self.settling_reference_step = 0
+        self.settling_plastic_rate = 0.0
+        self.settling_reference_plastic = self.state.ding["Fp"].copy() if self.ding_cfg is not None else None
+        self.settling_reason = "pending" if self._baking_start_step is None else "disabled"
+        self.settling_diagnostics = dict(max_speed_m_s=0.0, max_strain_rate_per_s=0.0, max_plastic_rate_per_s=0.0)
         self.node_m = np.zeros(self.node_count, R)
         self.node_p = np.zeros((self.node_count, 3), R)
         self.node_v = np.zeros((self.node_count, 3), R)
         self.vertices = self.reference_vertices.copy()
+
+    # -------------------------------------------------------------- settling (mpm.py:57-109)
+    @property
+    def settling_steps(self):
+        if self._baking_start_step is not None and self._baking_start_step > 0:
+            return self._baking_start_step
+        if self.tr is None:
+            return 0
+        return int(math.ceil(self.tr.settling_duration_s / self.dt - 1e-10))
+
+    @property
+    def IsSettling(self):
+        return self.tr is not None and self.tr.settling_duration_s > 0 and self._baking_start_step is None
+
+    @property
+    def baking_steps(self):
+        return 0 if self.IsSettling else max(0, self.step_count - self.settling_steps)
+
+    @property
+    def baking_time(self):
+        return self.baking_steps * self.dt
+
+    def update_settling(self):
+        """physics/mpm.py:77-109 after an accepted resting substep.
+
+        The tolerance comparisons take the float32 metrics against the configuration's floats in
+        double precision (NumPy 1.x promotion; under NumPy 2 the reference compares in float32,
+        which differs only for a metric equal to a tolerance rounded up to float32)."""
+        t = self.tr
+        dt = self.dt
+        if not t.settling_auto:
+            if self.step_count >= self.settling_steps:
+                self._baking_start_step = self.step_count
+                self.settling_reason = "fixed_duration"
+            return
+        values = np.zeros(3, np.float32)
+        elapsed = (self.step_count - self.settling_reference_step) * dt
+        self.k.SettlingMetrics(self.n, self.state.v, self.state.C, self.settling_reference_plastic, self.state.ding["Fp"],
+                               values, elapsed)
+        if elapsed + 1e-12 >= min(.01, t.settling_hold_duration_s):
+            self.settling_plastic_rate = float(values[2])
+            self.settling_reference_plastic = self.state.ding["Fp"].copy()
+            self.settling_reference_step = self.step_count
+        values[2] = self.settling_plastic_rate
+        self.settling_diagnostics = dict(max_speed_m_s=float(values[0]), max_strain_rate_per_s=float(values[1]),
+                                         max_plastic_rate_per_s=float(values[2]))
+        eligible = (self.step_count - 1) * dt + 1e-12 >= t.settling_min_duration_s
+        small = float(values[0]) <= t.settling_speed_tolerance and float(max(values[1:])) <= t.settling_rate_tolerance
+        self.settling_quiet_steps = self.settling_quiet_steps + 1 if eligible and small else 0
+        if self.settling_quiet_steps * dt + 1e-12 >= t.settling_hold_duration_s:
+            self._baking_start_step = self.step_count
+            self.settling_reason = "quasi_static"
+        elif self.step_count >= self.settling_steps:
+            self.settling_reason = "timeout"
+            raise RuntimeError("Settling did not reach quasi-static tolerances before settling_duration_s; "
+                               "baking was not started. Increase the maximum duration or review tolerances/material.")
 
     def Masses(self, state=None):
         s = self.state if state is None else state
@@ -477,8 +541,9 @@ class OracleSimulator:
         R = self.R
         inp = self.state
         prep = inp
+        was_settling = self.IsSettling
         if self.tr is not None:
-            prep, self.work = self.transport_advance(inp)
+            prep, self.work = self.transport_advance(inp, settling=was_settling)
         out = OracleState(np.zeros_like(inp.x), np.zeros_like(inp.v), np.zeros_like(inp.F), np.zeros_like(inp.C))
         if self.ding_cfg is not None:
             out.ding = {kk: vv.copy() for kk, vv in inp.ding.items()}  # ping-pong buffer contents are overwritten below
@@ -509,6 +574,8 @@ class OracleSimulator:
         self.prepared = prep
         self.state = out
         self.step_count += 1
+        if was_settling:
+            self.update_settling()
 
     def Advance(self, steps):
         for _ in range(steps):

@@ -49,7 +49,7 @@ def test_struct_layouts_match_the_header(tmp_path):
     from baking_amd import _native
     structs = {"bk_params": _native.BkParams, "bk_dough": _native.BkDough, "bk_tuning": _native.BkTuning,
                "bk_state": _native.BkState, "bk_scene": _native.BkScene, "bk_grid": _native.BkGrid,
-               "bk_adjoint": _native.BkAdjoint}
+               "bk_adjoint": _native.BkAdjoint, "bk_settling": _native.BkSettling}
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
