@@ -85,6 +85,11 @@ class BkSettling(ctypes.Structure):
 
 SETTLING_REASONS = ("pending", "fixed_duration", "quasi_static", "timeout")
 
+# out[] slots of bk_check_state
+CHECK_FIELDS = ("nonfinite_state", "minimum_j", "maximum_j", "outside_stencil", "maximum_speed", "gas_wave", "water_wave_sq",
+                "nonfinite_ding", "maximum_residual", "nonfinite_vertices", "inverted_tets", "particle_volume", "total_mass",
+                "grid_mass", "kinetic_energy")
+
 
 # Exported symbols and their argument types; the ABI test checks this list
 # against the header and the library.
@@ -123,6 +128,8 @@ SIGNATURES = {
     "bk_profile_stage_count": ([], c_int32),
     "bk_profile_stage_name": ([I], c_char_p),
     "bk_measure": ([POINTER(BkScene), POINTER(BkState), I, I, P, P], c_int),
+    "bk_check_scratch_size": ([I, L, I, I], c_int64),
+    "bk_check_state": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), P, P, L, P, I, P, I, P, P, P], c_int),
     "bk_grid_untile": ([P, P, I, I, P], c_int),
     "bk_grid_untile_f64": ([P, P, I, P], c_int),
     "bk_ParticleToGrid": ([P, P, P, P, P, P, P, P, P, F, I, F, P, I, P], c_int),

@@ -35,7 +35,6 @@ import torch
 
 from .. import _native
 from ..geometry.mesh_tracking import ParticleMeshBinding
-from ..geometry.mesh_validation import ComputeSignedVolumes
 from .grid import GridState
 from .materials.ding import DingMaterial, DingState
 from .materials.elastic import ElasticMaterial
@@ -218,6 +217,7 @@ class MpmSimulator:
         self._refresh_grid_struct()
         self._settle_record = None
         self._settle_reference = None
+        self._check_buffers = None
         self._params = self._make_params()
 
     def _refresh_grid_struct(self):
@@ -579,66 +579,83 @@ class MpmSimulator:
             self.mesh_step_count = self.step_count
 
     def CheckState(self):
-        """Host-side health checks and run statistics (physics/mpm.py:243-308)."""
+        """Health checks and run statistics (physics/mpm.py:243-308), reduced on the GPU in one pass
+        (``bk_check_state``): the host reads 15 numbers, not the particle state.  Errors are raised in the
+        reference's order with its texts; the mesh is reconstructed into ``vertex_positions`` only when every
+        check passes, as the reference's ``UpdateMesh`` call comes last."""
         flags = self._read_flags()
         self._sync_api_flags(flags)
         if flags[_FLAG_J]:
             raise RuntimeError("nonpositive/nonfinite J; reduce dt or inspect loading and sampling")
         if flags[_FLAG_STENCIL]:
             raise RuntimeError("particle stencil left the grid; enlarge the domain or reduce dt")
-        s = self.current_state
-        x = s.particle_positions.cpu().numpy()
-        v = s.particle_velocities.cpu().numpy()
-        F = s.deformation_gradients.cpu().numpy()
-        C = s.affine_velocities.cpu().numpy()
-        if not all(np.isfinite(a).all() for a in (x, v, F, C)):
+        c = self._device_check()
+        if c["nonfinite_state"]:
             raise RuntimeError("nonfinite particle state; simulation stopped")
-        J = np.linalg.det(F)
-        if J.min() <= 0.0:
+        if c["minimum_j"] <= 0.0:
             raise RuntimeError("inverted particle deformation; reduce dt or loading")
-        sim = self.simulation_config
-        g = (x - np.array(sim.grid_origin)) / sim.grid_spacing
-        if np.any(g < 0.5) or np.any(g >= sim.grid_resolution - 1.5):
+        if c["outside_stencil"]:
             raise RuntimeError("particle outside valid grid stencil")
-        masses = self.Masses().cpu().numpy().astype(np.float64)
-        vmax = float(np.linalg.norm(v, axis=1).max())
+        sim = self.simulation_config
+        vmax = c["maximum_speed"]
         wave = self.material.material_config.wave_speed
         if isinstance(self.material, DingMaterial) and self.material.ding_config.gas_enabled:
-            wave = max(wave, self.material.GasWaveSpeed(s))
-        if self.transport is not None and self.transport.config.water_enabled:
-            water = (self.transport.config.water_stress * s.transport_state.saturations.cpu().numpy() * J
-                     / s.material_state.reference_densities.cpu().numpy())
-            wave = float(np.sqrt(wave * wave + water.max()))
+            wave = max(wave, c["gas_wave"])
+        if self.transport is not None and self.transport.config.water_enabled and isinstance(self.material, DingMaterial):
+            wave = float(np.sqrt(wave * wave + c["water_wave_sq"]))
         cfl = (vmax + wave) * sim.time_step / sim.grid_spacing
         if cfl > sim.cfl_limit:
             raise RuntimeError(f"estimated CFL={cfl:.3f} > {sim.cfl_limit}; reduce dt")
         if isinstance(self.material, DingMaterial):
             self.material.RaiseFor(flags[2:5])
-            m = s.material_state
-            if not torch.isfinite(m.elastic_gradients).all() or not torch.isfinite(m.plastic_gradients).all():
+            if c["nonfinite_ding"]:
                 raise RuntimeError("nonfinite Ding internal state")
-            if float(m.return_residuals.max()) > 1e-4:
+            if c["maximum_residual"] > 1e-4:
                 raise RuntimeError("Ding local return mapping did not converge")
-        self.UpdateMesh()
-        verts = self.vertex_positions.cpu().numpy()
-        if not np.isfinite(verts).all():
+        if c["nonfinite_vertices"]:
             raise RuntimeError("nonfinite reconstructed mesh vertices")
+        self.vertex_positions.copy_(self._check_vertices)
+        self.mesh_step_count = self.step_count
         return {
             "step_count": self.step_count,
             "simulation_time": self.step_count * sim.time_step,
             "particle_count": self.particle_data.particle_count,
-            "particle_volume_m3": float(np.dot(self.particle_samples.reference_volumes, J)),
-            "total_mass": float(masses.sum()),
-            "grid_mass": float(self.grid_state.tiled_masses.double().sum().cpu()),
-            "minimum_j": float(J.min()),
-            "maximum_j": float(J.max()),
+            "particle_volume_m3": c["particle_volume"],
+            "total_mass": c["total_mass"],
+            "grid_mass": c["grid_mass"],
+            "minimum_j": c["minimum_j"],
+            "maximum_j": c["maximum_j"],
             "maximum_speed": vmax,
-            "kinetic_energy": float(0.5 * np.sum(masses * np.sum(v.astype(np.float64) ** 2, axis=1))),
+            "kinetic_energy": c["kinetic_energy"],
             "estimated_cfl": cfl,
             "unmapped_mesh_vertices": self.mesh_binding.unmapped_vertex_count,
-            "inverted_mesh_tets": int(np.count_nonzero(ComputeSignedVolumes(verts.astype(np.float64),
-                                                                             self.tet_mesh.tet_indices) <= 0.0)),
+            "inverted_mesh_tets": int(c["inverted_tets"]),
         }
+
+    def _device_check(self):
+        """One bk_check_state pass over the current state, the grid masses and a freshly reconstructed mesh."""
+        lib = _native.load()
+        dev = self.device_name
+        n = self.particle_data.particle_count
+        if getattr(self, "_check_buffers", None) is None:
+            tets = torch.as_tensor(np.ascontiguousarray(self.tet_mesh.tet_indices, dtype=np.int32), device=dev)
+            nodes = self.grid_state.tiled_masses.numel()
+            size = int(lib.bk_check_scratch_size(n, nodes, len(self.vertex_positions), len(tets)))
+            self._check_buffers = dict(tets=tets, scratch=torch.empty(size, dtype=torch.float64, device=dev),
+                                       out=torch.empty(len(_native.CHECK_FIELDS), dtype=torch.float64, device=dev))
+            self._check_vertices = torch.empty_like(self.vertex_positions)
+        b = self._check_buffers
+        self.mesh_binding.Reconstruct(self.current_state, self._check_vertices)
+        sim = self.simulation_config
+        geometry = (ctypes.c_double * 4)(*sim.grid_origin, sim.grid_spacing)
+        state = self._state_struct(self.current_state)
+        masses = self.grid_state.tiled_masses
+        _native.check(lib.bk_check_state(ctypes.byref(self._params), ctypes.byref(self._scene), ctypes.byref(state),
+                                         ctypes.cast(geometry, ctypes.c_void_p), _native.ptr(masses), masses.numel(),
+                                         _native.ptr(self._check_vertices), len(self._check_vertices), _native.ptr(b["tets"]),
+                                         len(b["tets"]), _native.ptr(b["scratch"]), _native.ptr(b["out"]), _native.stream_ptr()),
+                      "bk_check_state")
+        return dict(zip(_native.CHECK_FIELDS, b["out"].cpu().tolist()))
 
     def CaptureSnapshot(self):
         """Host copy of the full state with the reference's NPZ keys (physics/mpm.py:310-334)."""

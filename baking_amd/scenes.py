@@ -427,3 +427,14 @@ def demo_auto_settling(heat_transfer=20.0, gravity=(0.0, 0.0, 0.0)):
                              settling_hold_duration_s=0.0003, settling_speed_tolerance=1e-4, settling_rate_tolerance=0.01)
     spec.steps = 8
     return spec
+
+
+def demo_translation(velocity=(0.2, -0.1, 0.05), gravity=(0.0, 0.0, 0.0), time_step=1e-4):
+    """Neo-Hookean demo cube translating freely (reference tests/test_mpm.py:10-15): CreateDemoMesh(3), 4
+    particles per tet at density 1000, the default 64^3 grid."""
+    from .geometry.mesh_loader import CreateDemoMesh
+    from .geometry.particle_sampling import SampleParticles
+    mesh = CreateDemoMesh(3)
+    sim = SimulationConfig(gravity=tuple(gravity), time_step=time_step, device_name="auto")
+    return SceneSpec("demo_translation", mesh, SampleParticles(mesh, 1000.0, 4), sim, MaterialConfig(), DingConfig(),
+                     TransportConfig(), steps=30, initial_velocity=tuple(velocity))

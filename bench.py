@@ -108,6 +108,31 @@ def cpu_baseline(spec, steps):
                       f"{spec.simulation.grid_resolution}^3 dense grid) from rest, serial oracle (liboracle.so, g++ -O2)"}
 
 
+def check_state_timing(spec, sim, with_host, repeats=5):
+    """Wall milliseconds of MpmSimulator.CheckState (one bk_check_state pass and a 15-double copy), median of
+    `repeats`; with_host also times the reference's host CheckState on the same state (state copies to the host
+    and the numpy checks of the oracle), the computation the device pass replaces."""
+    import torch
+    times = []
+    for _ in range(repeats):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stats = sim.CheckState()
+        times.append(time.perf_counter() - t0)
+    out = {"gpu_ms": float(np.median(times)) * 1e3, "estimated_cfl": stats["estimated_cfl"], "host_ms": None}
+    if with_host:
+        from tests.harness import load_gpu_state, oracle_from_spec
+        orc = oracle_from_spec(spec)
+        host = []
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            load_gpu_state(orc, sim).check_state()
+            host.append(time.perf_counter() - t0)
+        out["host_ms"] = float(np.median(host)) * 1e3
+    return out
+
+
 def measured_traffic(workload, kernel, particles, active_nodes, deferred):
     """HBM bytes of one launch of `kernel`, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
     (per-unit bytes, calibrated against a plain copy) scaled to this launch's units; None if not profiled."""
@@ -617,6 +642,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": None,
     }
+    result["check_state"] = check_state_timing(spec, sim, with_host=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(spec, args.cpu_baseline_steps)
     print(json.dumps(result))

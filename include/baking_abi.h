@@ -291,6 +291,31 @@ const char* bk_profile_stage_name(int32_t stage);
  * 3 sum W (water), 4 sum water losses, 5 sum heat exchanges, 6 sum V0 * gas moles, 7 sum particle mass.
  * NULL state fields contribute 0 (mass falls back to scene.m0); scene.dough_of NULL = one dough. */
 int bk_measure(const bk_scene* scene, const bk_state* state, int32_t n, int32_t n_doughs, double* out, void* stream);
+/* CheckState in one pass on the device (physics/mpm.py:243-308 with DingMaterial.GasWaveSpeed, ding.py:285-292):
+ * out[BK_CHECK_COUNT] (device doubles) after the work on `stream`:
+ *   NONFINITE_STATE       1 if any x, v, F or C component is not finite, else 0
+ *   MIN_J, MAX_J          extremes of det F (float, as numpy's det of the float32 state)
+ *   OUTSIDE_STENCIL       1 if any (x - origin) / dx < 0.5 or >= res - 1.5 (double, origin/dx from grid_origin_dx)
+ *   MAX_SPEED             max |v|
+ *   GAS_WAVE              max sqrt((lam + 2 mu + p (1 + (1 - n0 + n) / n)) J / rho) (Ding with gas; else -inf)
+ *   WATER_WAVE_SQ         max water_stress s J / rho (Ding with water transport; else -inf)
+ *   NONFINITE_DING        1 if any Fe or Fp component is not finite (Ding)
+ *   MAX_RESIDUAL          max return residual (Ding; else -inf)
+ *   NONFINITE_VERTICES    1 if any of the n_vertices vertex positions is not finite
+ *   INVERTED_TETS         tets whose det([v1-v0, v2-v0, v3-v0]) / 6 <= 0 (double)
+ *   VOLUME, MASS          sum V0 det F, sum of the masses (state.masses with transport, else scene.m0), f64
+ *   GRID_MASS             sum of node_masses[0, node_count), f64
+ *   KINETIC_ENERGY        0.5 sum m |v|^2, f64
+ * The sums are combined in a fixed order (the same on every run).  grid_origin_dx = host double[4]: the
+ * configuration's grid origin and spacing.  scratch = bk_check_scratch_size(...) device doubles. */
+enum { BK_CHECK_NONFINITE_STATE = 0, BK_CHECK_MIN_J, BK_CHECK_MAX_J, BK_CHECK_OUTSIDE_STENCIL, BK_CHECK_MAX_SPEED,
+       BK_CHECK_GAS_WAVE, BK_CHECK_WATER_WAVE_SQ, BK_CHECK_NONFINITE_DING, BK_CHECK_MAX_RESIDUAL,
+       BK_CHECK_NONFINITE_VERTICES, BK_CHECK_INVERTED_TETS, BK_CHECK_VOLUME, BK_CHECK_MASS, BK_CHECK_GRID_MASS,
+       BK_CHECK_KINETIC_ENERGY, BK_CHECK_COUNT };
+int64_t bk_check_scratch_size(int32_t n, int64_t node_count, int32_t n_vertices, int32_t n_tets);
+int bk_check_state(const bk_params* P, const bk_scene* scene, const bk_state* state, const double* grid_origin_dx,
+                   const float* node_masses, int64_t node_count, const float* vertices, int32_t n_vertices, const int32_t* tets,
+                   int32_t n_tets, double* scratch, double* out, void* stream);
 /* Convert the tiled grid's mass/velocity/momentum to the reference's dense layout. */
 int bk_grid_untile(const float* tiled, float* dense, int32_t res, int32_t components, void* stream);
 int bk_grid_untile_f64(const double* tiled, double* dense, int32_t res, void* stream);

@@ -440,6 +440,73 @@ class OracleSimulator:
             raise RuntimeError("Settling did not reach quasi-static tolerances before settling_duration_s; "
                                "baking was not started. Increase the maximum duration or review tolerances/material.")
 
+    # -------------------------------------------------------------- CheckState (mpm.py:243-308)
+    def check_state(self):
+        """The reference's CheckState on the oracle state with numpy: raises its errors in its order and returns
+        its statistics.  GasWaveSpeed follows physics/materials/ding.py:285-292."""
+        sim, mat = self.sim, self.mat
+        if self.flags[0]:
+            raise RuntimeError("nonpositive/nonfinite J; reduce dt or inspect loading and sampling")
+        if self.flags[1]:
+            raise RuntimeError("particle stencil left the grid; enlarge the domain or reduce dt")
+        s = self.state
+        x, v, F, C = (np.asarray(a, np.float32) for a in (s.x, s.v, s.F, s.C))
+        if not all(np.isfinite(a).all() for a in (x, v, F, C)):
+            raise RuntimeError("nonfinite particle state; simulation stopped")
+        J = np.linalg.det(F)
+        if J.min() <= 0.0:
+            raise RuntimeError("inverted particle deformation; reduce dt or loading")
+        g = (x - np.array(sim.grid_origin)) / sim.grid_spacing
+        if np.any(g < 0.5) or np.any(g >= sim.grid_resolution - 1.5):
+            raise RuntimeError("particle outside valid grid stencil")
+        masses = np.asarray(self.Masses(), np.float32).astype(np.float64)
+        vmax = float(np.linalg.norm(v, axis=1).max())
+        wave = mat.wave_speed
+        d = s.ding
+        if self.ding_cfg is not None and self.ding_cfg.gas_enabled:
+            n = np.asarray(d["porosity"], np.float32).astype(np.float64)
+            bulk = d["pressure"] * (1 + (1 - self.ding_cfg.porosity + n) / n)
+            modulus = mat.lame_lambda + 2 * mat.shear_modulus
+            Jd = np.linalg.det(F.astype(np.float64))
+            with np.errstate(divide="ignore", invalid="ignore"):
+                gas = float(np.sqrt((modulus + bulk) * Jd / np.asarray(d["density"], np.float32)).max())
+            wave = max(wave, gas)
+        if self.tr is not None and self.tr.water_enabled:
+            water = self.tr.water_stress * s.transport["s"] * J / np.asarray(d["density"], np.float32)
+            wave = float(np.sqrt(wave * wave + water.max()))
+        cfl = (vmax + wave) * sim.time_step / sim.grid_spacing
+        if cfl > sim.cfl_limit:
+            raise RuntimeError(f"estimated CFL={cfl:.3f} > {sim.cfl_limit}; reduce dt")
+        if self.ding_cfg is not None:
+            self.check_ding()
+            if not np.isfinite(d["Fe"]).all() or not np.isfinite(d["Fp"]).all():
+                raise RuntimeError("nonfinite Ding internal state")
+            if float(np.max(d["residual"])) > 1e-4:
+                raise RuntimeError("Ding local return mapping did not converge")
+        verts = np.zeros((len(self.reference_vertices), 3), self.R)
+        self.reconstruct(s, verts)
+        self.vertices = verts
+        verts = verts.astype(np.float32)
+        if not np.isfinite(verts).all():
+            raise RuntimeError("nonfinite reconstructed mesh vertices")
+        corners = verts[self.mesh.tet_indices]
+        signed = np.linalg.det(corners[:, 1:] - corners[:, :1]) / 6.0
+        return {
+            "step_count": self.step_count,
+            "simulation_time": self.step_count * sim.time_step,
+            "particle_count": self.n,
+            "particle_volume_m3": float(np.dot(np.asarray(self.samples.reference_volumes, np.float64), J)),
+            "total_mass": float(masses.sum()),
+            "grid_mass": float(np.asarray(self.node_m, np.float32).sum(dtype=np.float64)),
+            "minimum_j": float(J.min()),
+            "maximum_j": float(J.max()),
+            "maximum_speed": vmax,
+            "kinetic_energy": float(0.5 * np.sum(masses * np.sum(v ** 2, axis=1))),
+            "estimated_cfl": cfl,
+            "unmapped_mesh_vertices": self.unmapped,
+            "inverted_mesh_tets": int(np.count_nonzero(signed <= 0.0)),
+        }
+
     def Masses(self, state=None):
         s = self.state if state is None else state
         return s.transport["masses"] if s.transport is not None else self.m0

@@ -174,3 +174,37 @@ def three_way(spec, steps, rtol=1e-5, factor=4.0, check_every=1, log=print, solv
                 + ", ".join(f"{k} {e_gpu[k]:.1e}/{e_ref[k]:.1e}" for k in sorted(e_gpu) if e_gpu[k] > 1e-7))
         assert not bad, f"{spec.name} step {step}: fields outside tolerance (gpu/o32, o32/o64): {bad}"
     return e_gpu, e_ref
+
+
+_DING_FIELDS = (("Fe", "elastic_gradients"), ("Fp", "plastic_gradients"), ("tau", "kirchhoff_stresses"),
+                ("residual", "return_residuals"), ("porosity", "porosities"), ("T", "temperatures"), ("gelled", "gelled"),
+                ("yield_stress", "yield_stresses"), ("viscosity", "viscosities"), ("moles", "gas_moles"),
+                ("pressure", "gas_pressures"), ("porosity_residual", "porosity_residuals"), ("density", "reference_densities"))
+_TRANSPORT_FIELDS = (("T", "temperatures"), ("W", "water_masses"), ("s", "saturations"), ("losses", "water_losses"),
+                     ("exchanges", "heat_exchanges"), ("masses", "particle_masses"))
+
+
+def oracle_on_gpu_state(spec, sim):
+    """An FP32 oracle holding the GPU simulator's current state (copied to the host), dense grid masses, flags
+    and step count: the reference's host-side computations (CheckState) on exactly the GPU's state."""
+    return load_gpu_state(oracle_from_spec(spec, "f32"), sim)
+
+
+def load_gpu_state(o, sim):
+    """Copy the GPU simulator's state, dense grid masses, flags and step count into the oracle ``o``."""
+    s = sim.current_state
+    host = lambda t: t.detach().cpu().numpy().copy()  # noqa: E731
+    o.state.x, o.state.v = host(s.particle_positions), host(s.particle_velocities)
+    o.state.F, o.state.C = host(s.deformation_gradients), host(s.affine_velocities)
+    if s.material_state is not None:
+        for key, name in _DING_FIELDS:
+            o.state.ding[key] = host(getattr(s.material_state, name))
+    if s.transport_state is not None:
+        for key, name in _TRANSPORT_FIELDS:
+            o.state.transport[key] = host(getattr(s.transport_state, name))
+    o.node_m = host(sim.grid_state.node_masses).reshape(-1)
+    flags = sim._read_flags()
+    o.flags[:] = flags[:2]
+    o.ding_flags[:] = flags[2:5]
+    o.step_count = sim.step_count
+    return o
