@@ -21,8 +21,11 @@ def SampleParticles(tet_mesh, density, particles_per_tet=0, target_volume=None, 
     """Four symmetric samples per tet, one centroid sample, or seeded Dirichlet samples.
 
     ``particles_per_tet=0`` picks max(4, ceil(V_tet / target_volume)) samples per tet.
-    The random stream (``default_rng(random_seed).dirichlet``) is drawn tet by tet
-    exactly as the reference does, so identical inputs give identical particles.
+    The result equals the reference's tet-by-tet loop bit for bit, without a Python
+    loop over tets: the Dirichlet weights of all randomly sampled tets come from one
+    ``default_rng(random_seed).dirichlet`` call (the same stream as the reference's
+    per-tet calls, in tet order), and positions are stacked (k x 4) @ (4 x 3)
+    products per tet, grouped by sample count, as the reference multiplies them.
     """
     if not np.isfinite(density) or density <= 0.0:
         raise ValueError("density must be positive and finite")
@@ -40,19 +43,27 @@ def SampleParticles(tet_mesh, density, particles_per_tet=0, target_volume=None, 
     if counts.sum() > max_particles:
         raise ValueError("sampling exceeds max_particles")
     source_tets = np.repeat(np.arange(len(counts)), counts)
-    positions = np.empty((len(source_tets), 3), dtype=np.float64)
     volumes = np.repeat(tet_mesh.reference_volumes / counts, counts)
-    rng = np.random.default_rng(random_seed)
+    positions = np.empty((len(source_tets), 3), dtype=np.float64)
+    starts = np.concatenate(([0], np.cumsum(counts)[:-1]))
+    corners = np.asarray(tet_mesh.vertex_positions, dtype=np.float64)[tet_mesh.tet_indices]  # (T, 4, 3)
     symmetric = np.full((4, 4), _SYM_OFF)
     np.fill_diagonal(symmetric, _SYM_ON)
-    start = 0
-    for tet, count in enumerate(counts):
-        if count == 4:
-            weights = symmetric
-        elif count == 1:
-            weights = np.full((1, 4), 0.25)
+    random = (counts != 4) & (counts != 1)
+    drawn = None
+    if random.any():
+        drawn = np.random.default_rng(random_seed).dirichlet(np.ones(4), size=int(counts[random].sum()))
+        first_draw = np.concatenate(([0], np.cumsum(counts[random])[:-1]))
+        draw_of = np.zeros(len(counts), dtype=np.int64)
+        draw_of[random] = first_draw
+    for k in np.unique(counts):
+        tets = np.flatnonzero(counts == k)
+        if k == 4:
+            weights = np.broadcast_to(symmetric, (len(tets), 4, 4))
+        elif k == 1:
+            weights = np.full((len(tets), 1, 4), 0.25)
         else:
-            weights = rng.dirichlet(np.ones(4), size=count)
-        positions[start:start + count] = weights @ tet_mesh.vertex_positions[tet_mesh.tet_indices[tet]]
-        start += count
+            weights = drawn[draw_of[tets][:, None] + np.arange(k)[None, :]]
+        rows = starts[tets][:, None] + np.arange(k)[None, :]
+        positions[rows.reshape(-1)] = np.matmul(weights, corners[tets]).reshape(-1, 3)
     return ParticleSamples(positions, volumes, density * volumes, source_tets)

@@ -46,31 +46,41 @@ def BoundaryFaces(tet_mesh, samples):
 
     A face seen once is on the boundary; it is wound outward relative to its
     tet's opposite vertex, and owned by the particle of that tet closest to the
-    face centre.  Face order is first-seen order over tets and local faces.
+    face centre (the first such particle on a tie).  Face order is first-seen
+    order over tets and local faces.  Vectorised over all 4T faces (a sort of
+    the face keys finds the faces seen once), with the reference's arithmetic
+    per face, so the output equals its loop's.
     """
-    seen = {}
-    for tet_index, tet in enumerate(tet_mesh.tet_indices):
-        for opposite in range(4):
-            face = np.delete(tet, opposite).copy()
-            key = tuple(sorted(face))
-            if key in seen:
-                seen[key] = None
-                continue
-            points = tet_mesh.vertex_positions[face]
-            normal = np.cross(points[1] - points[0], points[2] - points[0])
-            if np.dot(normal, tet_mesh.vertex_positions[tet[opposite]] - points[0]) > 0:
-                face[[1, 2]] = face[[2, 1]]
-            seen[key] = (face, tet_index)
-    faces, owners = [], []
-    for entry in seen.values():
-        if entry is None:
-            continue
-        face, tet_index = entry
-        members = np.flatnonzero(samples.source_tets == tet_index)
-        centre = tet_mesh.vertex_positions[face].mean(axis=0)
-        owners.append(members[np.argmin(np.linalg.norm(samples.particle_positions[members] - centre, axis=1))])
-        faces.append(face)
-    return np.asarray(faces, dtype=np.int32).reshape(-1, 3), np.asarray(owners, dtype=np.int32)
+    tets = np.asarray(tet_mesh.tet_indices)
+    verts = np.asarray(tet_mesh.vertex_positions, dtype=np.float64)
+    ntet = len(tets)
+    keep = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])  # np.delete(tet, opposite) for opposite 0..3
+    faces = tets[:, keep].reshape(-1, 3).copy()                      # face 4 t + o
+    opposite = tets.reshape(-1)                                      # the vertex each face leaves out
+    keys = np.sort(faces, axis=1)
+    _, inverse, seen = np.unique(keys, axis=0, return_inverse=True, return_counts=True)
+    boundary = np.flatnonzero(seen[inverse.reshape(-1)] == 1)
+    faces, opposite = faces[boundary], opposite[boundary]
+    owner_tet = boundary // 4
+    pts = verts[faces]
+    normal = np.cross(pts[:, 1] - pts[:, 0], pts[:, 2] - pts[:, 0])
+    outward = np.einsum("ij,ij->i", normal, verts[opposite] - pts[:, 0]) > 0
+    faces[outward] = faces[outward][:, [0, 2, 1]]
+    # owners: nearest particle of the face's tet to the face centre (first on ties)
+    source = np.asarray(samples.source_tets)
+    order = np.argsort(source, kind="stable")
+    first = np.searchsorted(source[order], np.arange(ntet))
+    count = np.bincount(source, minlength=ntet)
+    members_per_face = count[owner_tet]
+    face_of = np.repeat(np.arange(len(faces)), members_per_face)
+    rank = np.arange(len(face_of)) - np.repeat(np.cumsum(members_per_face) - members_per_face, members_per_face)
+    member = order[first[owner_tet][face_of] + rank]
+    centre = verts[faces].mean(axis=1)
+    dist = np.linalg.norm(np.asarray(samples.particle_positions)[member] - centre[face_of], axis=1)
+    best = np.lexsort((rank, dist, face_of))
+    head = np.concatenate(([True], face_of[best][1:] != face_of[best][:-1]))
+    owners = member[best][head]
+    return faces.astype(np.int32).reshape(-1, 3), owners.astype(np.int32)
 
 
 def SurfaceSlots(owners, particle_count):

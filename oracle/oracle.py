@@ -245,6 +245,28 @@ def boundary_faces(vertex_positions, tet_indices, source_tets, particle_position
     return np.asarray(faces, dtype=np.int32).reshape(-1, 3), np.asarray(owners, dtype=np.int32)
 
 
+def sample_particles(vertex_positions, tet_indices, reference_volumes, density, particles_per_tet=0, target_volume=None,
+                     random_seed=42):
+    """Tet-by-tet particle seeding (geometry/particle_sampling.py:13-53): symmetric 4-point rule, centroid,
+    or seeded Dirichlet weights drawn per tet; returns (positions, volumes, masses, source tets)."""
+    if particles_per_tet == 0:
+        counts = np.maximum(4.0, np.ceil(reference_volumes / target_volume)).astype(np.int64)
+    else:
+        counts = np.full(len(tet_indices), particles_per_tet, dtype=np.int64)
+    source = np.repeat(np.arange(len(counts)), counts)
+    positions = np.empty((len(source), 3))
+    volumes = np.repeat(reference_volumes / counts, counts)
+    rng = np.random.default_rng(random_seed)
+    sym = np.full((4, 4), (5.0 - np.sqrt(5.0)) / 20.0)
+    np.fill_diagonal(sym, (5.0 + 3.0 * np.sqrt(5.0)) / 20.0)
+    start = 0
+    for t, k in enumerate(counts):
+        w = sym if k == 4 else (np.full((1, 4), 0.25) if k == 1 else rng.dirichlet(np.ones(4), size=k))
+        positions[start:start + k] = w @ vertex_positions[tet_indices[t]]
+        start += k
+    return positions, volumes, density * volumes, source
+
+
 def mesh_binding(vertex_positions, tet_indices, source_tets, particle_positions, reference_volumes):
     """Vertex -> particle CSR with normalised volume weights (geometry/mesh_tracking.py:42-75)."""
     n = len(particle_positions)
