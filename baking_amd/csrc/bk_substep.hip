@@ -12,11 +12,12 @@
 //
 // One substep (reference physics/mpm.py:163-225) runs as:
 //   clear    zero the node blocks the previous substep touched
-//   keys     batch cell key of every particle: (batch block << 6) | cell in block
-//   sort     stable LSD radix sort of (key, particle index), 8-bit digits
-//   tiles    bounds of each non-empty particle block in the sorted order;
-//            activate the 2x2x2 node blocks each reaches; cut blocks into P2G
-//            tiles of at most `tile_chunk` particles
+//   keys     batch cell key of every particle: (batch block << 6) | cell in block,
+//            and its rank among its block's particles (per-block counters)
+//   sort     scan of the block counts (each non-empty block's range and the
+//            block list), scatter by block, counting sort by cell per block
+//   tiles    activate the 2x2x2 node blocks each particle block reaches; cut
+//            blocks into P2G tiles of at most `tile_chunk` particles
 //   mesh     reconstruct the boundary faces' vertices, then their areas per owner slot
 //   pass A   thermal P2G (7 f64 scalars) accumulated per tile in LDS, flushed
 //            with one global f64 atomic per touched node and scalar
@@ -148,22 +149,55 @@ __global__ void k_cell_keys(const float* __restrict__ x, int n, float ox, float 
 // every dough.  Thread 0 also resets the per-substep counters.  Off-grid
 // particles take their dough's first cell (the P2G tiles skip them) and raise
 // the stencil flag.
+// Batch cell key of every particle and its rank among the particles of its
+// batch block (the order in which their appends reached the block's counter).
+// The lanes of a wavefront that share a block -- consecutive particles mostly
+// do -- take their ranks from one atomic.
 __global__ void k_batch_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of,
-                             const float* __restrict__ x, int n, uint32_t* __restrict__ keys, bk_workspace* ws) {
+                             const float* __restrict__ x, int n, uint32_t* __restrict__ keys, int32_t* __restrict__ ranks,
+                             bk_workspace* ws) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p == 0) { ws->counters[0] = 0; ws->counters[1] = 0; ws->counters[2] = 0; ws->counters[3] = 0; ws->counters[6] += 1; }
-  if (p >= n) return;
-  per_dough(dough_of_particle(dough_of, p), [&](int d) {
-    const bk_params& P = doughs[d].params;
-    const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
-    float g[3];
-    grid_position(x + 3 * p, o, P.dx, g);
-    int b[3];
-    stencil_base(g, b);
-    bool ok = valid_stencil(b, P.res);
-    flag_max(dough_flags(ws, d), BK_FLAG_STENCIL, !ok);
-    keys[p] = (uint32_t(doughs[d].block_offset) << 6) + (ok ? tiled_cell_key(b, (P.res + 3) / 4) : 0u);
-  });
+  const bool valid = p < n;
+  uint32_t key = 0u;
+  if (valid) {
+    per_dough(dough_of_particle(dough_of, p), [&](int d) {
+      const bk_params& P = doughs[d].params;
+      const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+      float g[3];
+      grid_position(x + 3 * p, o, P.dx, g);
+      int b[3];
+      stencil_base(g, b);
+      bool ok = valid_stencil(b, P.res);
+      flag_max(dough_flags(ws, d), BK_FLAG_STENCIL, !ok);
+      key = (uint32_t(doughs[d].block_offset) << 6) + (ok ? tiled_cell_key(b, (P.res + 3) / 4) : 0u);
+    });
+  }
+  // group the lanes by block (wave-uniform loop, no memory traffic), then every
+  // group's first lane adds the group's size: all of the wavefront's atomics in
+  // one instruction, so their returns are awaited once
+  const unsigned lane = __lane_id();
+  const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  unsigned long long pending = __ballot(valid);
+  int leader = 0, offset = 0, size = 0;
+  while (pending) {
+    const int first = __ffsll((long long)pending) - 1;
+    const uint32_t blk = __shfl(key >> 6, first, 64);
+    const unsigned long long same = __ballot(valid && (key >> 6) == blk) & pending;
+    if ((same >> lane) & 1ull) {
+      leader = first;
+      offset = __popcll(same & below);
+    }
+    if (int(lane) == first) size = __popcll(same);
+    pending &= ~same;
+  }
+  int base = 0;
+  if (size > 0) base = atomicAdd(&ws->blk_count[key >> 6], size);
+  base = __shfl(base, leader, 64);
+  if (valid) {
+    keys[p] = key;
+    ranks[p] = base + offset;
+  }
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
@@ -299,35 +333,143 @@ static int radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int
   return 0;
 }
 
-// ============================================================================ blocks, tiles and active node blocks
-// Bounds of every non-empty particle block in the sorted order, and the list
-// of those blocks.  Each thread covers kBoundsItems keys (coalesced), so one
-// atomic per workgroup appends its block starts.
-constexpr int kBoundsItems = 8;
-__global__ __launch_bounds__(256) void k_block_bounds(const uint32_t* __restrict__ skeys, int n, bk_workspace* ws) {
-  __shared__ int lds[17];
-  const int base = blockIdx.x * (256 * kBoundsItems) + int(threadIdx.x);
-  int count = 0;
+// ============================================================================ block sort, tiles and active node blocks
+// The particles are ordered by batch block and, inside each block, by cell:
+//   k_block_scan_*   exclusive scan of the per-block counts k_batch_keys left:
+//                    each non-empty block's [start, end) in the sorted order,
+//                    the list of non-empty blocks, and the counts reset to 0
+//   k_block_scatter  particle p to start[block] + rank
+//   k_cell_sort      inside each non-empty block, a counting sort by cell
+// Cells come out contiguous inside their block, which is all the tiles need;
+// the order of a cell's particles follows the atomics, as does the order of
+// the P2G accumulations they feed.  Every pass reads and writes each particle
+// once, against the four 8-bit passes of a radix sort of the whole key.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = 256 * kScanItems;  // block counts per scan workgroup
+
+__device__ __forceinline__ int block_sum(int v, int* lds) {
 #pragma unroll
-  for (int k = 0; k < kBoundsItems; ++k) {
-    const int i = base + k * 256;
-    if (i < n) {
-      const uint32_t blk = skeys[i] >> 6;
-      if (i == 0 || (skeys[i - 1] >> 6) != blk) {
-        ws->blk_start[blk] = i;
-        ++count;
-      }
-      if (i == n - 1 || (skeys[i + 1] >> 6) != blk) ws->blk_end[blk] = i + 1;
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = int(threadIdx.x) >> 6;
+  if (__lane_id() == 0) lds[w] = v;
+  __syncthreads();
+  int t = 0;
+  for (int k = 0; k < int(blockDim.x) >> 6; ++k) t += lds[k];
+  __syncthreads();
+  return t;
+}
+
+// Exclusive scan of v over the workgroup (256 threads); `lds` holds 4 ints.
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds) {
+  const unsigned lane = __lane_id();
+  int inc = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(inc, off, 64);
+    if (int(lane) >= off) inc += t;
+  }
+  const int w = int(threadIdx.x) >> 6;
+  if (lane == 63) lds[w] = inc;
+  __syncthreads();
+  int before = 0;
+  for (int k = 0; k < w; ++k) before += lds[k];
+  __syncthreads();
+  return before + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_block_scan_partials(const int32_t* __restrict__ counts, int nb, int32_t* __restrict__ part) {
+  __shared__ int lds[4];
+  const int base = blockIdx.x * kScanTile + int(threadIdx.x) * kScanItems;
+  int sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i)
+    if (base + i < nb) sum += counts[base + i];
+  sum = block_sum(sum, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = sum;
+}
+
+// One workgroup: exclusive scan of the np partials, in place.
+__global__ __launch_bounds__(256) void k_block_scan_top(int32_t* __restrict__ part, int np) {
+  __shared__ int lds[4];
+  int carry = 0;
+  for (int base = 0; base < np; base += 256) {
+    const int i = base + int(threadIdx.x);
+    const int v = i < np ? part[i] : 0;
+    const int ex = block_exclusive_scan(v, lds);
+    const int total = block_sum(v, lds);
+    if (i < np) part[i] = carry + ex;
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_block_scan_apply(int32_t* __restrict__ counts, int nb, const int32_t* __restrict__ part,
+                                                          bk_workspace* ws) {
+  __shared__ int lds[17];
+  const int base = blockIdx.x * kScanTile + int(threadIdx.x) * kScanItems;
+  int c[kScanItems];
+  int sum = 0, nonempty = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    c[i] = base + i < nb ? counts[base + i] : 0;
+    sum += c[i];
+    nonempty += c[i] > 0;
+  }
+  int offset = part[blockIdx.x] + block_exclusive_scan(sum, lds);
+  int slot = block_reserve(&ws->counters[3], nonempty, lds);
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    if (c[i] > 0) {
+      ws->blk_start[base + i] = offset;
+      ws->blk_end[base + i] = offset + c[i];
+      ws->blocks[slot++] = base + i;
+      counts[base + i] = 0;
+      offset += c[i];
     }
   }
-  int slot = block_reserve(&ws->counters[3], count, lds);
+}
+
+__global__ void k_block_scatter(const uint32_t* __restrict__ keys, const int32_t* __restrict__ ranks, int n,
+                                const bk_workspace* __restrict__ ws, uint32_t* __restrict__ kout, int32_t* __restrict__ vout) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const uint32_t key = keys[p];
+  const int pos = ws->blk_start[key >> 6] + ranks[p];
+  kout[pos] = key;
+  vout[pos] = p;
+}
+
+// One wavefront per non-empty block: cell counts in LDS, their exclusive scan
+// (lane c holds cell c), then every particle takes the next slot of its cell.
+__global__ __launch_bounds__(64) void k_cell_sort(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                                                  const bk_workspace* __restrict__ ws, uint32_t* __restrict__ kout,
+                                                  int32_t* __restrict__ vout) {
+  __shared__ int cursor[64];
+  const int nblk = ws->counters[3];
+  const int lane = int(threadIdx.x);
+  for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
+    const int b = ws->blocks[bi];
+    const int start = ws->blk_start[b], end = ws->blk_end[b];
+    cursor[lane] = 0;
+    __syncthreads();
+    for (int j = start + lane; j < end; j += 64) atomicAdd(&cursor[kin[j] & 63u], 1);
+    __syncthreads();
+    const int count = cursor[lane];
+    int inc = count;
 #pragma unroll
-  for (int k = 0; k < kBoundsItems; ++k) {
-    const int i = base + k * 256;
-    if (i < n) {
-      const uint32_t blk = skeys[i] >> 6;
-      if (i == 0 || (skeys[i - 1] >> 6) != blk) ws->blocks[slot++] = int32_t(blk);
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += t;
     }
+    __syncthreads();
+    cursor[lane] = start + inc - count;
+    __syncthreads();
+    for (int j = start + lane; j < end; j += 64) {
+      const uint32_t key = kin[j];
+      const int slot = atomicAdd(&cursor[key & 63u], 1);
+      kout[slot] = key;
+      vout[slot] = vin[j];
+    }
+    __syncthreads();
   }
 }
 
@@ -1191,19 +1333,27 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     hipLaunchKernelGGL(k_clear_blocks, dim3(cap), dim3(256), 0, st, ws, *g, int(hw->thermal));
     BK_LAUNCH(clear);
   }
-  {  // 2. keys (also resets the per-substep counters)
+  {  // 2. keys and ranks within blocks (also resets the per-substep counters)
     StageTimer tm(pr, st, kStageKeys);
-    hipLaunchKernelGGL(k_batch_keys, dim3(div_up(n, 256)), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, ws);
+    hipLaunchKernelGGL(k_batch_keys, dim3(div_up(n, 256)), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, h->vals_s[1],
+                       ws);
     BK_LAUNCH(keys);
   }
-  {  // 3. stable sort by batch cell key
+  {  // 3. order by block, then by cell inside each block
     StageTimer tm(pr, st, kStageSort);
-    int rc = radix_sort(h, h->keys_raw, nullptr, h->sorted_keys, h->sorted_vals, n, hw->key_bits, st);
-    if (rc) return rc;
+    const int nb = h->total_blocks;
+    const int nscan = div_up(nb, kScanTile);
+    hipLaunchKernelGGL(k_block_scan_partials, dim3(nscan), dim3(256), 0, st, h->blk_count, nb, h->scan_part);
+    hipLaunchKernelGGL(k_block_scan_top, dim3(1), dim3(256), 0, st, h->scan_part, nscan);
+    hipLaunchKernelGGL(k_block_scan_apply, dim3(nscan), dim3(256), 0, st, h->blk_count, nb, h->scan_part, ws);
+    hipLaunchKernelGGL(k_block_scatter, dim3(div_up(n, 256)), dim3(256), 0, st, h->keys_raw, h->vals_s[1], n, ws, h->keys_s[0],
+                       h->vals_s[0]);
+    hipLaunchKernelGGL(k_cell_sort, dim3(clampi(div_up(n, 64), 1, 16384)), dim3(64), 0, st, h->keys_s[0], h->vals_s[0], ws,
+                       h->sorted_keys, h->sorted_vals);
+    BK_LAUNCH(sort);
   }
-  {  // 4. block bounds, tiles, active node blocks
+  {  // 4. tiles, active node blocks
     StageTimer tm(pr, st, kStageTiles);
-    hipLaunchKernelGGL(k_block_bounds, dim3(div_up(n, 256 * kBoundsItems)), dim3(256), 0, st, h->sorted_keys, n, ws);
     hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)n * 8, 256), 1, 1024)), dim3(256), 0, st, D, nd, ws, tu.tile_chunk);
     BK_LAUNCH(tiles);
   }
@@ -1434,6 +1584,8 @@ bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blo
   alloc((void**)&h.hist, size_t(256) * nwg * 4);
   alloc((void**)&h.digit_total, 256 * 4);
   alloc((void**)&h.blk_start, nbk * 4);
+  alloc((void**)&h.blk_count, nbk * 4);
+  alloc((void**)&h.scan_part, size_t(div_up((long long)nbk, kScanTile)) * 4);
   alloc((void**)&h.blk_end, nbk * 4);
   alloc((void**)&h.node_flag, nbk * 4);
   alloc((void**)&h.blocks, nbk * 4);
@@ -1478,7 +1630,7 @@ void bk_workspace_destroy(bk_workspace* ws) {
   }
   if (ws_host(ws, h)) return;
   void* ptrs[] = {h.doughs, h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1],
-                  h.hist, h.digit_total, h.blk_start, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
+                  h.hist, h.digit_total, h.blk_start, h.blk_count, h.scan_part, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
                   h.queue, h.tau, h.vertices, h.vert_mark, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions,
                   h.stats, h.settle, h.settle_part};
   for (void* p : ptrs)

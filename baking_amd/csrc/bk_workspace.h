@@ -37,6 +37,8 @@ struct bk_workspace {
   int32_t* hist = nullptr;
   int32_t* digit_total = nullptr;
   int32_t* blk_start = nullptr;                 // per batch block: first sorted particle
+  int32_t* blk_count = nullptr;                 // per batch block: particles keyed to it (0 between substeps)
+  int32_t* scan_part = nullptr;                 // partial sums of the block-count scan
   int32_t* blk_end = nullptr;
   int32_t* node_flag = nullptr;                 // per batch block: active this substep
   int32_t* blocks = nullptr;                    // non-empty particle blocks

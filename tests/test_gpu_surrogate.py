@@ -188,8 +188,10 @@ def test_packaged_network_gain_on_the_loaf():
     pe, pn = se["evaluations"] / se["return_maps"], sn["evaluations"] / sn["return_maps"]
     print(f"evaluations per return map: exact {pe:.2f}, network {pn:.2f}; return-map stage ms: exact {te:.3f}, network {tn:.3f}")
     print("accepted", sn["accepted"], "of", sn["return_maps"])
-    # the two runs yield in (nearly) the same particles: float atomics make their trajectories differ in the last bits
-    assert se["return_maps"] > 100000 and abs(sn["return_maps"] - se["return_maps"]) <= 1e-5 * se["return_maps"]
+    # the two runs yield in (nearly) the same particles: the order of the float atomics that accumulate the grid
+    # (and of each cell's particles, which follows atomics) makes their trajectories differ in the last bits, and
+    # the particles that sit on the yield threshold then go either way
+    assert se["return_maps"] > 100000 and abs(sn["return_maps"] - se["return_maps"]) <= 1e-4 * se["return_maps"]
     assert sn["accepted"] >= 0.95 * sn["return_maps"]
     assert pn < 0.5 * pe
     assert tn < te
