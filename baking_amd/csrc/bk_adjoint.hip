@@ -930,8 +930,7 @@ __device__ inline double adj_b1_particle(const AdjArgs& a, int p) {
       for (int oy = 0; oy < 3; ++oy)
         for (int oz = 0; oz < 3; ++oz) {
           const int64_t node = tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, a.nb);
-          const double nw = a.g.W[node];
-          const double q = nw > 0.0 ? a.g.dW[node] / nw : 0.0;
+          const double q = a.g.dW[node];  // the forward stores dW / W per node
           mr += wx[ox] * wy[oy] * wz[oz] * q;
           const double val = gTn * a.g.dT[node] + gmr * q;
           gx[0] += dwx[ox] * wy[oy] * wz[oz] * inv_dx * val;
@@ -1035,7 +1034,7 @@ __global__ __launch_bounds__(256) void k_adj_diffuse_a(AdjArgs a, bk_workspace* 
     a.ghr[node] = dt * gdT / Ci;
     a.gwr[node] = dt * gdW;
     a.gC[node] = -gdT * a.g.dT[node] / Ci;
-    a.gW[node] = Wi > 0.0 ? -sM * a.g.dW[node] / (Wi * Wi) : 0.0;
+    a.gW[node] = Wi > 0.0 ? -sM * a.g.dW[node] / Wi : 0.0;  // q = change / W: dq / dW = -q / W
   }
 }
 

@@ -752,9 +752,11 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
         }
       heat_bad = dt * hd > Ci;
       double change = dt * wr;
-      water_bad = g.W[node] + change < 0.0 || dt * wd > 1.0;
+      const double Wi = g.W[node];
+      water_bad = Wi + change < 0.0 || dt * wd > 1.0;
       g.dT[node] = dt * hr / Ci;
-      g.dW[node] = change;
+      // the relative water change dW / W each particle gathers (thermal_water.py:222-223), once per node
+      g.dW[node] = Wi > 0.0 ? change / Wi : 0.0;
     }
     flag_max(dough_flags(ws, bg.d), BK_FLAG_HEAT, heat_bad);
     flag_max(dough_flags(ws, bg.d), BK_FLAG_WATER, water_bad);
@@ -798,8 +800,7 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __
               double wt = double(wx[ox] * wy[oy] * wz[oz]);
               int64_t node = node_base + tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, nb);
               tc += wt * g.dT[node];
-              double nw = g.W[node];
-              if (nw > 0.0) mr += wt * g.dW[node] / nw;
+              mr += wt * g.dW[node];
             }
       }
       double T_old = in.T[p], W_old = in.W[p];

@@ -131,7 +131,8 @@ typedef struct bk_scene {
 typedef struct bk_grid {
   float *m, *p, *v;                          /* node mass, momentum (x3), velocity (x3) */
   double *V, *C, *E, *W, *H, *L, *X;         /* thermal P2G: volume, capacity, energy, water, heat, loss, exchange */
-  double *dT, *dW;                           /* diffusion increments */
+  double *dT, *dW;                           /* diffusion increments: temperature change, and water change divided by
+                                                the node's water (0 where it has none) */
 } bk_grid;
 
 typedef struct bk_workspace bk_workspace;
