@@ -133,8 +133,9 @@ __device__ __forceinline__ int64_t tiled_node(int i, int j, int k, int nb) {
 __device__ __forceinline__ void jacobi_rotate(double A[3][3], double V[3][3], int p, int q) {
   double apq = A[p][q];
   if (apq == 0.0) return;
-  double theta = (A[q][q] - A[p][p]) / (2.0 * apq);
-  double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(1.0 + theta * theta));
+  // t = sgn(theta) / (|theta| + sqrt(1 + theta^2)) with theta = d / e, multiplied through by |e|: one division
+  const double d = A[q][q] - A[p][p], e = 2.0 * apq;
+  double t = (d * e >= 0.0 ? fabs(e) : -fabs(e)) / (fabs(d) + sqrt(d * d + e * e));
   double c = rsqrt(1.0 + t * t);
   double s = t * c;
   int r = 3 - p - q;
@@ -152,7 +153,8 @@ __device__ __forceinline__ void jacobi_rotate(double A[3][3], double V[3][3], in
   }
 }
 
-__device__ inline void svd3(const mat3<double>& F, mat3<double>& U, double sig[3], mat3<double>& Vout) {
+// tol2: the sweeps stop once the squared off-diagonal of A is at most tol2 times its squared diagonal.
+__device__ inline void svd3(const mat3<double>& F, mat3<double>& U, double sig[3], mat3<double>& Vout, double tol2 = 1e-34) {
   double A[3][3], V[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -164,7 +166,7 @@ __device__ inline void svd3(const mat3<double>& F, mat3<double>& U, double sig[3
   for (int sweep = 0; sweep < 10; ++sweep) {
     double off = A[0][1] * A[0][1] + A[0][2] * A[0][2] + A[1][2] * A[1][2];
     double diag = A[0][0] * A[0][0] + A[1][1] * A[1][1] + A[2][2] * A[2][2];
-    if (off <= 1e-34 * diag) break;
+    if (off <= tol2 * diag) break;
     jacobi_rotate(A, V, 0, 1);
     jacobi_rotate(A, V, 0, 2);
     jacobi_rotate(A, V, 1, 2);
@@ -196,8 +198,9 @@ __device__ inline void svd3(const mat3<double>& F, mat3<double>& U, double sig[3
   double n0 = sqrt(B[0][0] * B[0][0] + B[1][0] * B[1][0] + B[2][0] * B[2][0]);
   double u0[3], u1[3], u2[3];
   if (n0 > 0.0) {
+    const double r0 = 1.0 / n0;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) u0[i] = B[i][0] / n0;
+    for (int i = 0; i < 3; ++i) u0[i] = B[i][0] * r0;
   } else {
     u0[0] = 1.0; u0[1] = 0.0; u0[2] = 0.0;
   }
@@ -206,8 +209,9 @@ __device__ inline void svd3(const mat3<double>& F, mat3<double>& U, double sig[3
   for (int i = 0; i < 3; ++i) u1[i] = B[i][1] - d01 * u0[i];
   double n1 = sqrt(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
   if (n1 > 1e-300) {
+    const double r1 = 1.0 / n1;
 #pragma unroll
-    for (int i = 0; i < 3; ++i) u1[i] /= n1;
+    for (int i = 0; i < 3; ++i) u1[i] *= r1;
   } else {
     double e0 = fabs(u0[0]) < 0.9 ? 1.0 : 0.0, e1 = fabs(u0[0]) < 0.9 ? 0.0 : 1.0;
     double d = e0 * u0[0] + e1 * u0[1];
@@ -537,7 +541,9 @@ __device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old
   }
   mat3<double> Ud, Vd;
   double sd[3];
-  svd3(m_cast<double>(trial), Ud, sd, Vd);
+  // converged to |off-diagonal| <= 1e-9 |diagonal| of trial^T trial: the singular values' error is second order
+  // (1e-18 relative) and U, V are within 1e-9, both below the float rounding of what the return map uses
+  svd3(m_cast<double>(trial), Ud, sd, Vd, 1e-18);
   mat3<float> U = m_cast<float>(Ud), V = m_cast<float>(Vd);
   float s[3] = {float(sd[0]), float(sd[1]), float(sd[2])};
   float ret_n = porosity;

@@ -440,7 +440,7 @@ class OracleSimulator:
                 self._baking_start_step = self.step_count
                 self.settling_reason = "fixed_duration"
             return
-        values = np.zeros(3, np.float32)
+        values = np.zeros(3, self.R)  # the kernel's precision: float32 as the reference, float64 for the FP64 run
         elapsed = (self.step_count - self.settling_reference_step) * dt
         self.k.SettlingMetrics(self.n, self.state.v, self.state.C, self.settling_reference_plastic, self.state.ding["Fp"],
                                values, elapsed)

@@ -50,9 +50,10 @@ def _same_decisions(tag, sim, orc):
     assert sim.settling_reference_step == orc.settling_reference_step, tag
     got = np.array(list(sim.settling_diagnostics.values()))
     want = np.array(list(orc.settling_diagnostics.values()))
-    # speed, strain rate, plastic rate; the plastic rate differences near-identity FP32 gradients, so its
-    # rounding noise is absolute (~1e-7 / (window * |Fp|))
-    for name, g, w, atol in zip(sim.settling_diagnostics, got, want, (1e-7, 1e-5, 1e-3)):
+    # speed, strain rate, plastic rate.  Rounding noise is absolute: the strain rate of a rigidly moving dough is
+    # |v| 4/dx times FP32 rounding (~1e-5 1/s here; 1e-14 in FP64), and the plastic rate differences near-identity
+    # FP32 gradients (~1e-7 / (window |Fp|)); both bounds sit 100x below the reference's 0.01 1/s tolerance
+    for name, g, w, atol in zip(sim.settling_diagnostics, got, want, (1e-7, 1e-4, 1e-3)):
         np.testing.assert_allclose(g, w, rtol=1e-3, atol=atol, err_msg=f"{tag}: {name}")
 
 

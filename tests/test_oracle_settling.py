@@ -93,3 +93,16 @@ def test_falling_dough_times_out_and_a_kick_resets_the_count():
     k.state.v[...] = np.array([0.01, 0.0, 0.0], np.float32)
     k.Step()
     assert k.IsSettling and k.settling_quiet_steps == 0
+
+
+def test_falling_dough_metrics_in_fp64_are_free_fall():
+    """In FP64 the falling cube translates rigidly: max speed = g t exactly as far as rounding goes, strain and
+    plastic rates at rounding level (the FP32 run's ~1e-5 1/s strain rate is its rounding noise)."""
+    spec = scenes.demo_auto_settling(gravity=(0.0, -9.81, 0.0))
+    o = oracle_from_spec(spec, "f64")
+    with pytest.raises(RuntimeError, match="baking was not started"):
+        for _ in range(25):
+            o.Step()
+    d = o.settling_diagnostics
+    np.testing.assert_allclose(d["max_speed_m_s"], 9.81 * 20 * spec.simulation.time_step, rtol=1e-9)
+    assert d["max_strain_rate_per_s"] < 1e-9 and d["max_plastic_rate_per_s"] < 1e-9
