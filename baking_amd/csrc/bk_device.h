@@ -651,17 +651,26 @@ __device__ __forceinline__ mat3<float> neo_hookean_kirchhoff(const mat3<float>& 
 }
 
 // ------------------------------------------------------------------ mesh reconstruction (geometry/mesh_tracking.py:13-38)
-// One wavefront per vertex: every lane forms the anchor prediction, lanes
-// stride over the vertex's particle bindings summing weighted deviations from
-// the anchor, and a butterfly reduction combines the lanes.  Call with all 64
-// lanes of the wave active and the same `v`.
-__device__ inline void reconstruct_vertex_wave(int v, int lane, const float* __restrict__ x, const float* __restrict__ F,
-                                               const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
-                                               const int32_t* __restrict__ parts, const float* __restrict__ wts,
-                                               const float* __restrict__ ref_offs, float* __restrict__ out) {
-  int s = offs[v], e = offs[v + 1];
+// One 256-thread workgroup per vertex: every thread forms the anchor
+// prediction, the threads stride over the vertex's particle bindings summing
+// weighted deviations from the anchor, a butterfly combines each wavefront and
+// thread 0 adds the four wavefront sums in order.  A vertex can hold ~1000
+// bindings and each is a dependent pair of loads (particle index, then its x
+// and F), so a thread issues kRecUnroll bindings' loads before accumulating
+// them in order; a slot past e reads binding e-1 (a valid address) with weight
+// 0, so the loads are used unconditionally and cannot be sunk into per-slot
+// branches.  Call with the whole workgroup and the same `v`; `red` holds 12
+// floats of LDS.
+constexpr int kRecThreads = 256;
+constexpr int kRecUnroll = 2;
+__device__ inline void reconstruct_vertex_block(int v, const float* __restrict__ x, const float* __restrict__ F,
+                                                const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
+                                                const int32_t* __restrict__ parts, const float* __restrict__ wts,
+                                                const float* __restrict__ ref_offs, float* __restrict__ out, float* red) {
+  const int t = threadIdx.x;
+  const int s = offs[v], e = offs[v + 1];
   if (s == e) {
-    if (lane < 3) out[3 * v + lane] = ref_vertices[3 * v + lane];
+    if (t < 3) out[3 * v + t] = ref_vertices[3 * v + t];
     return;
   }
   float anchor[3];
@@ -673,14 +682,20 @@ __device__ inline void reconstruct_vertex_wave(int v, int lane, const float* __r
     for (int a = 0; a < 3; ++a) anchor[a] = x[3 * ap + a] + (Fa[a * 3] * r[0] + Fa[a * 3 + 1] * r[1] + Fa[a * 3 + 2] * r[2]);
   }
   float d0 = 0.f, d1 = 0.f, d2 = 0.f;
-  for (int k = s + lane; k < e; k += 64) {
-    int pi = parts[k];
-    const float* Fp = F + 9 * pi;
-    const float* r = ref_offs + 3 * k;
-    float wk = wts[k];
-    d0 += wk * ((x[3 * pi + 0] + (Fp[0] * r[0] + Fp[1] * r[1] + Fp[2] * r[2])) - anchor[0]);
-    d1 += wk * ((x[3 * pi + 1] + (Fp[3] * r[0] + Fp[4] * r[1] + Fp[5] * r[2])) - anchor[1]);
-    d2 += wk * ((x[3 * pi + 2] + (Fp[6] * r[0] + Fp[7] * r[1] + Fp[8] * r[2])) - anchor[2]);
+  for (int k0 = s + t; k0 < e; k0 += kRecThreads * kRecUnroll) {
+    int pi[kRecUnroll];
+#pragma unroll
+    for (int u = 0; u < kRecUnroll; ++u) pi[u] = parts[min(k0 + kRecThreads * u, e - 1)];
+#pragma unroll
+    for (int u = 0; u < kRecUnroll; ++u) {
+      const int k = min(k0 + kRecThreads * u, e - 1);
+      const float* Fp = F + 9 * pi[u];
+      const float* r = ref_offs + 3 * k;
+      const float wk = k0 + kRecThreads * u < e ? wts[k] : 0.0f;
+      d0 += wk * ((x[3 * pi[u] + 0] + (Fp[0] * r[0] + Fp[1] * r[1] + Fp[2] * r[2])) - anchor[0]);
+      d1 += wk * ((x[3 * pi[u] + 1] + (Fp[3] * r[0] + Fp[4] * r[1] + Fp[5] * r[2])) - anchor[1]);
+      d2 += wk * ((x[3 * pi[u] + 2] + (Fp[6] * r[0] + Fp[7] * r[1] + Fp[8] * r[2])) - anchor[2]);
+    }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -688,10 +703,17 @@ __device__ inline void reconstruct_vertex_wave(int v, int lane, const float* __r
     d1 += __shfl_xor(d1, off, 64);
     d2 += __shfl_xor(d2, off, 64);
   }
-  if (lane == 0) {
-    out[3 * v + 0] = anchor[0] + d0;
-    out[3 * v + 1] = anchor[1] + d1;
-    out[3 * v + 2] = anchor[2] + d2;
+  if ((t & 63) == 0) {
+    red[3 * (t >> 6) + 0] = d0;
+    red[3 * (t >> 6) + 1] = d1;
+    red[3 * (t >> 6) + 2] = d2;
+  }
+  __syncthreads();
+  if (t < 3) {
+    float d = red[t];
+#pragma unroll
+    for (int w = 1; w < kRecThreads / 64; ++w) d += red[3 * w + t];
+    out[3 * v + t] = anchor[t] + d;
   }
 }
 

@@ -325,12 +325,12 @@ __global__ void k_Saturations(const double* W, const float* V0, const float* F, 
   s[p] = W[p] / (double(rw) * double(V0[p]) * double(m_det(m_load(F + 9 * p))));
 }
 
-// geometry/mesh_tracking.py:13-38 (one wavefront per vertex, 4 vertices per workgroup)
-__global__ __launch_bounds__(256) void k_ReconstructMeshVertices(const float* x, const float* F, const float* rv, const int* offs,
-                                                                 const int* parts, const float* wts, const float* ro, float* out, int nv) {
-  int v = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (v >= nv) return;
-  reconstruct_vertex_wave(v, threadIdx.x & 63, x, F, rv, offs, parts, wts, ro, out);
+// geometry/mesh_tracking.py:13-38 (one workgroup per vertex)
+__global__ __launch_bounds__(kRecThreads) void k_ReconstructMeshVertices(const float* x, const float* F, const float* rv, const int* offs,
+                                                                         const int* parts, const float* wts, const float* ro, float* out,
+                                                                         int nv) {
+  __shared__ float red[3 * kRecThreads / 64];
+  reconstruct_vertex_block(int(blockIdx.x), x, F, rv, offs, parts, wts, ro, out, red);
 }
 
 __global__ void k_gas_pressure(const double* m, const float* T, const float* J, const float* por, double* out, int n) {
@@ -460,8 +460,8 @@ int bk_Saturations(const double* W, const float* V0, const float* F, double* s, 
 int bk_ReconstructMeshVertices(const float* x, const float* F, const float* rv, const int32_t* offs, const int32_t* parts, const float* wts,
                                const float* ro, float* out, int32_t nv, void* stream) {
   if (nv <= 0) return 0;
-  hipLaunchKernelGGL(k_ReconstructMeshVertices, dim3(blocks_for(nv, 4)), dim3(256), 0, (hipStream_t)stream, x, F, rv, offs, parts, wts,
-                     ro, out, nv);
+  hipLaunchKernelGGL(k_ReconstructMeshVertices, dim3(nv), dim3(kRecThreads), 0, (hipStream_t)stream, x, F, rv, offs, parts, wts, ro,
+                     out, nv);
   return launch_status();
 }
 int bk_gas_pressure(const double* m, const float* T, const float* J, const float* por, double* out, int32_t n, void* stream) {

@@ -542,15 +542,16 @@ __global__ void k_mark_face_vertices(bk_scene sc, bk_workspace* ws, int nslots) 
   }
 }
 
-// geometry/mesh_tracking.py:13-38, one wavefront per marked vertex (4 vertices per 256-thread workgroup).
-__global__ __launch_bounds__(256) void k_reconstruct(const float* __restrict__ x, const float* __restrict__ F,
-                                                     const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
-                                                     const int32_t* __restrict__ parts, const float* __restrict__ wts,
-                                                     const float* __restrict__ ref_offs, float* __restrict__ out, int nv,
-                                                     const bk_workspace* ws) {
-  int v = xcd_block() * 4 + (threadIdx.x >> 6);
+// geometry/mesh_tracking.py:13-38, one workgroup per marked vertex.
+__global__ __launch_bounds__(kRecThreads) void k_reconstruct(const float* __restrict__ x, const float* __restrict__ F,
+                                                             const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
+                                                             const int32_t* __restrict__ parts, const float* __restrict__ wts,
+                                                             const float* __restrict__ ref_offs, float* __restrict__ out, int nv,
+                                                             const bk_workspace* ws) {
+  __shared__ float red[3 * kRecThreads / 64];
+  const int v = xcd_block();
   if (v >= nv || ws->vert_mark[v] != ws->counters[6]) return;
-  reconstruct_vertex_wave(v, threadIdx.x & 63, x, F, ref_vertices, offs, parts, wts, ref_offs, out);
+  reconstruct_vertex_block(v, x, F, ref_vertices, offs, parts, wts, ref_offs, out, red);
 }
 
 // physics/thermal_water.py:77-92, summed per owner over its faces in face order.
@@ -1363,7 +1364,7 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
       StageTimer tm(pr, st, kStageMesh);
       if (hw->n_vertices > 0 && hw->n_slots > 0) {
         hipLaunchKernelGGL(k_mark_face_vertices, dim3(div_up(hw->n_slots, 256)), dim3(256), 0, st, *sc, ws, hw->n_slots);
-        hipLaunchKernelGGL(k_reconstruct, dim3(div_up(hw->n_vertices, 4)), dim3(256), 0, st, in->x, in->F, sc->ref_vertices,
+        hipLaunchKernelGGL(k_reconstruct, dim3(hw->n_vertices), dim3(kRecThreads), 0, st, in->x, in->F, sc->ref_vertices,
                            sc->bind_offsets, sc->bind_particles, sc->bind_weights, sc->bind_ref_offsets, h->vertices,
                            hw->n_vertices, ws);
       }

@@ -123,6 +123,38 @@ def test_solve_porosity_matches_oracle(solver):
     np.testing.assert_allclose(got[body], ref[body], rtol=1e-6)
 
 
+@pytest.mark.parametrize("resolution,samples_per_tet", [(2, 150), (3, 40), (2, 1)])
+def test_mesh_reconstruction_matches_oracle(resolution, samples_per_tet):
+    """geometry/mesh_tracking.py:13-38 on seeded perturbed particles: 300-3600 bindings per vertex (several strides of
+    a 256-thread workgroup, a partial last stride) and 1-24.  The GPU sums each vertex's bindings in another order than
+    the oracle's loop, so it is held to the FP64 oracle at a few FP32 roundings."""
+    from baking_amd.geometry.mesh_loader import CreateDemoMesh
+    from baking_amd.geometry.mesh_tracking import BuildBinding, ReconstructMeshVertices
+    from baking_amd.geometry.particle_sampling import SampleParticles
+    mesh = CreateDemoMesh(resolution)
+    samples = SampleParticles(mesh, 1000.0, samples_per_tet)
+    b = BuildBinding(mesh, samples)
+    rng = np.random.default_rng(resolution * 1000 + samples_per_tet)
+    n, nv = len(samples.particle_positions), len(mesh.vertex_positions)
+    x = (samples.particle_positions + rng.normal(0.0, 1e-3, (n, 3))).astype(np.float32)
+    F = (np.eye(3)[None] + rng.normal(0.0, 0.05, (n, 3, 3))).astype(np.float32)
+    rv = mesh.vertex_positions.astype(np.float32)
+    host = (x, F, rv, b["offsets"], b["particles"], b["weights"], b["reference_offsets"])
+    out = torch.empty(nv, 3, device="cuda")
+    ReconstructMeshVertices(nv, *[torch.as_tensor(np.ascontiguousarray(a), device="cuda") for a in host], out)
+    got = out.cpu().numpy()
+    want = {}
+    for prec in ("f32", "f64"):
+        R = np.float32 if prec == "f32" else np.float64
+        o = np.zeros((nv, 3), R)
+        O.Kernels(prec).ReconstructMeshVertices(nv, *[np.ascontiguousarray(a, R) if a.dtype.kind == "f" else a for a in host], o)
+        want[prec] = o
+    err, oerr = np.abs(got - want["f64"]).max(), np.abs(want["f32"] - want["f64"]).max()
+    print(f"res {resolution} x{samples_per_tet}: |gpu - f64| {err:.2e}, |f32 oracle - f64| {oerr:.2e}", flush=True)
+    # the FP32 oracle is within ~4e-8 of FP64; one dropped binding would move a vertex by ~1e-6
+    np.testing.assert_allclose(got, want["f64"], atol=2e-7)
+
+
 def test_free_fall_symplectic_euler():
     """tests/test_mpm.py:40-53 of the reference on the GPU path."""
     from dataclasses import replace
