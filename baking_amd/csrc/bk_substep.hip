@@ -623,7 +623,60 @@ __device__ __forceinline__ Robin robin_terms(const bk_params& P, bool rest, floa
 
 // In both P2G passes lane c of every wave takes the particles of cell c of
 // the tile's block (tile_cell_ranges), so one LDS atomic instruction touches
-// 64 distinct nodes.
+// 64 distinct nodes.  A cell's particles share one stencil base (the cell is
+// the key's, formed from that base), so a lane can take K of them at a time
+// and add their summed contribution with one LDS atomic per node and field.
+// Wave w takes ranks K*w .. K*w + K-1 of each cell, then every 4K-th.  Pass A
+// (four to seven f64 fields per node, bound by its LDS atomics) takes pairs;
+// pass B2 (four fields, more arithmetic per particle) is fastest one at a
+// time.  (A loop over groups of slots with different bases would let the
+// compiler keep all 27 K weight products live across it: 200+ VGPRs.)
+constexpr int kPackA = 2;
+constexpr int kPackB2 = 1;
+
+// Stencil base and axis weights of one pack slot, and whether it holds a
+// particle of this tile (`in_range`: the slot's rank is inside the cell).  A
+// slot that does not gets zero weights, so no NaN of a far-off particle can
+// reach the sums.
+__device__ __forceinline__ bool pack_member(const float g3[3], bool in_range, int res, const Tile& tg, int b[3], float wx[3],
+                                            float wy[3], float wz[3]) {
+  stencil_base(g3, b);
+  const bool ok = in_range && valid_stencil(b, res) && (b[0] >> 2) == tg.g.bx && (b[1] >> 2) == tg.g.by && (b[2] >> 2) == tg.g.bz;
+  axis_weights(g3[0] - float(b[0]), wx);
+  axis_weights(g3[1] - float(b[1]), wy);
+  axis_weights(g3[2] - float(b[2]), wz);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    wx[k] = ok ? wx[k] : 0.0f;
+    wy[k] = ok ? wy[k] : 0.0f;
+    wz[k] = ok ? wz[k] : 0.0f;
+  }
+  return ok;
+}
+
+// The pending pack slots that share the stencil base of the lowest pending one,
+// and that base relative to the tile's block.  The keys are formed from the
+// same positions and the same base computation, so every pending slot of a
+// cell shares it; one that does not is left out and flagged.
+template <int K>
+__device__ __forceinline__ unsigned pack_group(const int (&base)[K][3], const Tile& tg, unsigned pending, int& li, int& lj, int& lk) {
+  int rb[3] = {base[K - 1][0], base[K - 1][1], base[K - 1][2]};
+#pragma unroll
+  for (int u = K - 2; u >= 0; --u)
+    if (pending >> u & 1u) {
+      rb[0] = base[u][0];
+      rb[1] = base[u][1];
+      rb[2] = base[u][2];
+    }
+  unsigned sel = 0u;
+#pragma unroll
+  for (int u = 0; u < K; ++u)
+    if ((pending >> u & 1u) && base[u][0] == rb[0] && base[u][1] == rb[1] && base[u][2] == rb[2]) sel |= 1u << u;
+  li = rb[0] - 4 * tg.g.bx;
+  lj = rb[1] - 4 * tg.g.by;
+  lk = rb[2] - 4 * tg.g.bz;
+  return sel;
+}
 
 // ============================================================================ pass A: thermal P2G
 __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_grid g,
@@ -647,43 +700,63 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
     __syncthreads();
     tile_cell_ranges(ws, tg, cell_lo, cell_hi);
     const int hi = cell_hi[lane];
-    for (int j = cell_lo[lane] + w; j < hi; j += 4) {
-      int p = ws->sorted_vals[tg.start + j];
-      float g3[3];
-      grid_position(in.x + 3 * p, o, P.dx, g3);
-      int b[3];
-      stencil_base(g3, b);
-      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.g.bx || (b[1] >> 2) != tg.g.by || (b[2] >> 2) != tg.g.bz) continue;
-      float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
-      float wx[3], wy[3], wz[3];
-      axis_weights(l[0], wx);
-      axis_weights(l[1], wy);
-      axis_weights(l[2], wz);
-      float Jf = m_det(m_load(in.F + 9 * p));
-      double T = in.T[p], W = in.W[p];
-      Robin r = robin_terms(P, rest, heat_param, Jf, sc.V0[p], T, W, particle_surface(sc, ws, p));
-      double cap = sc.capacities[p];
-      const int li = b[0] - 4 * tg.g.bx, lj = b[1] - 4 * tg.g.by, lk = b[2] - 4 * tg.g.bz;
-      // the Robin terms are zero off the surface (no exposed or contact area): most particles skip their adds
-      const bool surface = r.heat != 0.0 || r.loss != 0.0 || r.exch != 0.0;
+    for (int j0 = cell_lo[lane] + kPackA * w; j0 < hi; j0 += 4 * kPackA) {
+      float wx[kPackA][3], wy[kPackA][3], wz[kPackA][3];
+      double f[kPackA][7];
+      int base[kPackA][3];
+      unsigned pending = 0u;
 #pragma unroll
-      for (int ox = 0; ox < 3; ++ox)
+      for (int u = 0; u < kPackA; ++u) {
+        const int j = j0 + u;
+        const int p = ws->sorted_vals[tg.start + min(j, hi - 1)];
+        float g3[3];
+        grid_position(in.x + 3 * p, o, P.dx, g3);
+        const bool ok = pack_member(g3, j < hi, P.res, tg, base[u], wx[u], wy[u], wz[u]);
+        const float Jf = m_det(m_load(in.F + 9 * p));
+        const double T = in.T[p], W = in.W[p];
+        const Robin r = robin_terms(P, rest, heat_param, Jf, sc.V0[p], T, W, particle_surface(sc, ws, p));
+        const double cap = sc.capacities[p];
+        const double v[7] = {r.vol, cap, cap * T, W, r.heat, r.loss, r.exch};
 #pragma unroll
-        for (int oy = 0; oy < 3; ++oy)
+        for (int a = 0; a < 7; ++a) f[u][a] = ok ? v[a] : 0.0;
+        if (ok) pending |= 1u << u;
+      }
+      int li, lj, lk;
+      const unsigned sel = pack_group(base, tg, pending, li, lj, lk);
+      flag_max(dough_flags(ws, tg.d), BK_FLAG_STENCIL, sel != pending);  // a slot off its cell's base: a key out of date
+      if (sel) {
+        double fs[kPackA][7];
+        bool surface = false;  // the Robin terms are zero off the surface (no exposed or contact area)
 #pragma unroll
-          for (int oz = 0; oz < 3; ++oz) {
-            double wt = double(wx[ox] * wy[oy] * wz[oz]);
-            int ri = region_index(li + ox, lj + oy, lk + oz);
-            atomicAdd(&acc[0][ri], wt * r.vol);
-            atomicAdd(&acc[1][ri], wt * cap);
-            atomicAdd(&acc[2][ri], wt * cap * T);
-            atomicAdd(&acc[3][ri], wt * W);
-            if (surface) {
-              atomicAdd(&acc[4][ri], wt * r.heat);
-              atomicAdd(&acc[5][ri], wt * r.loss);
-              atomicAdd(&acc[6][ri], wt * r.exch);
+        for (int u = 0; u < kPackA; ++u) {
+#pragma unroll
+          for (int a = 0; a < 7; ++a) fs[u][a] = (sel >> u & 1u) ? f[u][a] : 0.0;
+          surface = surface || fs[u][4] != 0.0 || fs[u][5] != 0.0 || fs[u][6] != 0.0;
+        }
+#pragma unroll
+        for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+          for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+              double wt[kPackA];
+#pragma unroll
+              for (int u = 0; u < kPackA; ++u) wt[u] = double(wx[u][ox] * wy[u][oy] * wz[u][oz]);
+              const int ri = region_index(li + ox, lj + oy, lk + oz);
+              auto sum = [&](int a) {
+                double t = wt[0] * fs[0][a];
+#pragma unroll
+                for (int u = 1; u < kPackA; ++u) t += wt[u] * fs[u][a];
+                return t;
+              };
+#pragma unroll
+              for (int a = 0; a < 4; ++a) atomicAdd(&acc[a][ri], sum(a));
+              if (surface) {
+#pragma unroll
+                for (int a = 4; a < 7; ++a) atomicAdd(&acc[a][ri], sum(a));
+              }
             }
-          }
+      }
     }
     __syncthreads();
     for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
@@ -889,42 +962,58 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(const bk_dough* __rest
     __syncthreads();
     tile_cell_ranges(ws, tg, cell_lo, cell_hi);
     const int hi = cell_hi[lane];
-    for (int j = cell_lo[lane] + w; j < hi; j += 4) {
-      int p = ws->sorted_vals[tg.start + j];
-      float g3[3];
-      grid_position(in.x + 3 * p, o, P.dx, g3);
-      int b[3];
-      stencil_base(g3, b);
-      if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.g.bx || (b[1] >> 2) != tg.g.by || (b[2] >> 2) != tg.g.bz) continue;
-      float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
-      float wx[3], wy[3], wz[3];
-      axis_weights(l[0], wx);
-      axis_weights(l[1], wy);
-      axis_weights(l[2], wz);
-      float mass = masses[p];
-      float sc_tau = P.dt * sc.V0[p] * 4.0f / (P.dx * P.dx);
-      mat3<float> Cm = m_load(in.C + 9 * p);
-      mat3<float> tau = m_load(ws->tau + 9 * p);
-      mat3<float> A;
+    for (int j0 = cell_lo[lane] + kPackB2 * w; j0 < hi; j0 += 4 * kPackB2) {
+      float wx[kPackB2][3], wy[kPackB2][3], wz[kPackB2][3], l[kPackB2][3];
+      float mass[kPackB2], mv[kPackB2][3];
+      mat3<float> A[kPackB2];
+      int base[kPackB2][3];
+      unsigned pending = 0u;
 #pragma unroll
-      for (int i = 0; i < 9; ++i) A.a[i] = mass * Cm.a[i] - sc_tau * tau.a[i];
-      float mv[3] = {mass * in.v[3 * p + 0], mass * in.v[3 * p + 1], mass * in.v[3 * p + 2]};
-      const int li = b[0] - 4 * tg.g.bx, lj = b[1] - 4 * tg.g.by, lk = b[2] - 4 * tg.g.bz;
+      for (int u = 0; u < kPackB2; ++u) {
+        const int j = j0 + u;
+        const int p = ws->sorted_vals[tg.start + min(j, hi - 1)];
+        float g3[3];
+        grid_position(in.x + 3 * p, o, P.dx, g3);
+        const bool ok = pack_member(g3, j < hi, P.res, tg, base[u], wx[u], wy[u], wz[u]);
 #pragma unroll
-      for (int ox = 0; ox < 3; ++ox)
+        for (int a = 0; a < 3; ++a) l[u][a] = ok ? g3[a] - float(base[u][a]) : 0.0f;  // finite offsets for an empty slot
+        const float m = masses[p];
+        const float sc_tau = P.dt * sc.V0[p] * 4.0f / (P.dx * P.dx);
+        const mat3<float> Cm = m_load(in.C + 9 * p);
+        const mat3<float> tau = m_load(ws->tau + 9 * p);
 #pragma unroll
-        for (int oy = 0; oy < 3; ++oy)
+        for (int i = 0; i < 9; ++i) A[u].a[i] = ok ? m * Cm.a[i] - sc_tau * tau.a[i] : 0.0f;
+        mass[u] = ok ? m : 0.0f;
 #pragma unroll
-          for (int oz = 0; oz < 3; ++oz) {
-            float wt = wx[ox] * wy[oy] * wz[oz];
-            float dd[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
-            int ri = region_index(li + ox, lj + oy, lk + oz);
-            atomicAdd(&acc[0][ri], double(wt * mass));
+        for (int a = 0; a < 3; ++a) mv[u][a] = ok ? m * in.v[3 * p + a] : 0.0f;
+        if (ok) pending |= 1u << u;
+      }
+      int li, lj, lk;
+      const unsigned sel = pack_group(base, tg, pending, li, lj, lk);
+      flag_max(dough_flags(ws, tg.d), BK_FLAG_STENCIL, sel != pending);  // a slot off its cell's base: a key out of date
+      if (sel) {
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-              atomicAdd(&acc[1 + a][ri],
-                        double(wt * (mv[a] + (A.a[a * 3 + 0] * dd[0] + A.a[a * 3 + 1] * dd[1] + A.a[a * 3 + 2] * dd[2]))));
-          }
+        for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+          for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+              const int ri = region_index(li + ox, lj + oy, lk + oz);
+              double sm = 0.0, sp[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+              for (int u = 0; u < kPackB2; ++u) {
+                const float wt = (sel >> u & 1u) ? wx[u][ox] * wy[u][oy] * wz[u][oz] : 0.0f;
+                const float dd[3] = {(float(ox) - l[u][0]) * P.dx, (float(oy) - l[u][1]) * P.dx, (float(oz) - l[u][2]) * P.dx};
+                sm += double(wt * mass[u]);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                  sp[a] += double(wt * (mv[u][a] + (A[u].a[a * 3 + 0] * dd[0] + A[u].a[a * 3 + 1] * dd[1] + A[u].a[a * 3 + 2] * dd[2])));
+              }
+              atomicAdd(&acc[0][ri], sm);
+#pragma unroll
+              for (int a = 0; a < 3; ++a) atomicAdd(&acc[1 + a][ri], sp[a]);
+            }
+      }
     }
     __syncthreads();
     for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
