@@ -388,20 +388,6 @@ __global__ __launch_bounds__(256) void k_block_scan_partials(const int32_t* __re
   if (threadIdx.x == 0) part[blockIdx.x] = sum;
 }
 
-// One workgroup: exclusive scan of the np partials, in place.
-__global__ __launch_bounds__(256) void k_block_scan_top(int32_t* __restrict__ part, int np) {
-  __shared__ int lds[4];
-  int carry = 0;
-  for (int base = 0; base < np; base += 256) {
-    const int i = base + int(threadIdx.x);
-    const int v = i < np ? part[i] : 0;
-    const int ex = block_exclusive_scan(v, lds);
-    const int total = block_sum(v, lds);
-    if (i < np) part[i] = carry + ex;
-    carry += total;
-  }
-}
-
 __global__ __launch_bounds__(256) void k_block_scan_apply(int32_t* __restrict__ counts, int nb, const int32_t* __restrict__ part,
                                                           bk_workspace* ws) {
   __shared__ int lds[17];
@@ -414,7 +400,11 @@ __global__ __launch_bounds__(256) void k_block_scan_apply(int32_t* __restrict__ 
     sum += c[i];
     nonempty += c[i] > 0;
   }
-  int offset = part[blockIdx.x] + block_exclusive_scan(sum, lds);
+  // the workgroups before this one: a few hundred partials, summed here instead of by a separate scan kernel
+  int before = 0;
+  for (int k = int(threadIdx.x); k < int(blockIdx.x); k += 256) before += part[k];
+  before = block_sum(before, lds);
+  int offset = before + block_exclusive_scan(sum, lds);
   int slot = block_reserve(&ws->counters[3], nonempty, lds);
 #pragma unroll
   for (int i = 0; i < kScanItems; ++i) {
@@ -1435,7 +1425,6 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     const int nb = h->total_blocks;
     const int nscan = div_up(nb, kScanTile);
     hipLaunchKernelGGL(k_block_scan_partials, dim3(nscan), dim3(256), 0, st, h->blk_count, nb, h->scan_part);
-    hipLaunchKernelGGL(k_block_scan_top, dim3(1), dim3(256), 0, st, h->scan_part, nscan);
     hipLaunchKernelGGL(k_block_scan_apply, dim3(nscan), dim3(256), 0, st, h->blk_count, nb, h->scan_part, ws);
     hipLaunchKernelGGL(k_block_scatter, dim3(div_up(n, 256)), dim3(256), 0, st, h->keys_raw, h->vals_s[1], n, ws, h->keys_s[0],
                        h->vals_s[0]);
