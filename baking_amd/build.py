@@ -19,8 +19,10 @@ ARCH = os.environ.get("BAKING_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops off: packed FP32 forms are kept out of every kernel.
+# -amdgpu-mfma-vgpr-form: matrix-core results land in ordinary VGPRs, not in
+# accumulation registers that the network kernel would copy out one by one.
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-I", str(HERE.parent / "include")]
+          "-mllvm", "-amdgpu-mfma-vgpr-form", "-I", str(HERE.parent / "include")]
 
 
 def _stale(src: Path, obj: Path) -> bool:
