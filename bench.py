@@ -5,8 +5,12 @@ Workloads (synthetic seeded inputs, Ding + gas + heat/water transport, FP32
 state with the reference's FP64 transport; one "step" is one substep of every
 particle):
 
-* ``loaf_256`` (default): the 0.30 x 0.15 x 0.12 loaf on a 256^3 grid, ~0.72M
-  particles, one simulator.
+* ``batched_48`` (default, the headline): see below.
+* ``loaf_256``: the 0.30 x 0.15 x 0.12 loaf on a 256^3 grid, ~0.72M
+  particles, one simulator, with the packaged return-map network attached.
+* ``rest_loaf``: ``loaf_256`` resting before baking under the automatic
+  settling controller.
+* ``sphere_small``: the small sphere scene, one simulator.
 * ``adjoint_loaf``: gradients (``baking_amd.adjoint.rollout``) through 64
   substeps of ``loaf_256`` (Ding + pore gas + heat/water transport): forward
   with checkpoints every 16 substeps, then the backward (segment recompute,
