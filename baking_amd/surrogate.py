@@ -275,9 +275,14 @@ class ReturnMapSurrogate:
 # ---------------------------------------------------------------- simulation hooks
 def collect_samples(sim, steps, capacity, stride=1, log=None):
     """Run ``steps`` substeps of an MpmSimulator or BatchSimulator and return the rows its exact coupled
-    return maps emitted (torch (rows, 12) float32), every ``stride``-th substep, at most ``capacity`` rows."""
+    return maps emitted (torch (rows, 12) float32), every ``stride``-th substep, at most ``capacity`` rows.
+
+    The device flags are read after every substep.  Collection stops at the first substep that raises
+    one, such as the gas-acoustic CFL bound, and that substep's rows are dropped, so every row comes
+    from a state the simulator's own checks accept."""
     import torch
     lib = _native.load()
+    read_flags = getattr(sim, "_read_flags", None) or sim.ReadFlags
     buf = torch.empty(capacity * SAMPLE_ROW, dtype=torch.float32, device="cuda")
     stats = (ctypes.c_int64 * 5)()
     emitted = 0
@@ -287,13 +292,20 @@ def collect_samples(sim, steps, capacity, stride=1, log=None):
             _native.check(lib.bk_workspace_set_samples(sim._workspace, _native.ptr(buf[emitted * SAMPLE_ROW:]), capacity - emitted),
                           "bk_workspace_set_samples")
         sim._launch_substeps(1)
+        before = emitted
         if on:
             _native.check(lib.bk_workspace_stats(sim._workspace, ctypes.cast(stats, ctypes.c_void_p), 0, _native.stream_ptr()),
                           "bk_workspace_stats")
             emitted = min(capacity, emitted + int(stats[4]))
             _native.check(lib.bk_workspace_set_samples(sim._workspace, None, 0), "bk_workspace_set_samples")
+        flags = np.asarray(read_flags())
+        if flags.any():
+            emitted = before
             if log:
-                log(f"substep {step}: {emitted} rows")
+                log(f"substep {step}: device flags {flags.reshape(-1).tolist()} raised; collection stops with {emitted} rows")
+            break
+        if on and log:
+            log(f"substep {step}: {emitted} rows")
     torch.cuda.synchronize()
     return buf[:emitted * SAMPLE_ROW].view(emitted, SAMPLE_ROW).clone()
 

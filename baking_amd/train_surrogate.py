@@ -1,11 +1,13 @@
 """Train the return-map surrogate on rows the exact return map emits during the loaf_256 run.
 
-    python -m baking_amd.train_surrogate [--rows 4000000] [--steps 2000] [--stride 100] [--epochs 8]
+    python -m baking_amd.train_surrogate [--rows 4000000] [--steps 2000] [--stride 50] [--epochs 20]
 
 Runs the 256^3 loaf (0.72M particles) for ``--steps`` substeps with the exact
 coupled return map, emitting (inputs, exact outputs) rows for every yielding
 particle on every ``--stride``-th substep, up to ``--rows`` rows plus a
-held-out tail.  Rows are appended in substep order, so the tail comes from
+held-out tail.  Collection stops at the first substep that raises a device
+flag (the loaf reaches its gas-acoustic CFL bound at substep 806), so every
+row comes from a state the simulator accepts.  Rows are appended in substep order, so the tail comes from
 the latest emitting substeps and serves as a hold-out later in the bake than
 anything trained on.  Trains with the library's FP32 kernels and Adam on the
 GPU and writes the model (``--out``) and its held-out error table
@@ -26,7 +28,7 @@ def main(argv=None):
     ap.add_argument("--rows", type=int, default=4_000_000, help="training rows (cfg: 4M)")
     ap.add_argument("--holdout-rows", type=int, default=300_000, help="rows of the latest emitting substeps kept out")
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--stride", type=int, default=100)
+    ap.add_argument("--stride", type=int, default=50)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=2e-3)

@@ -75,7 +75,7 @@ MLP_MACS = 7 * 64 + 3 * 64 * 64 + 64 * 4  # multiply-accumulates of the surrogat
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed substeps (default: 1000 batched_48, 2000 loaf_256, 1000 sphere_small)")
+    ap.add_argument("--steps", type=int, default=None, help="timed substeps (default: 1000 batched_48, 750 loaf_256, 1000 sphere_small)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="batched_48",
                     choices=["batched_48", "loaf_256", "sphere_small", "adjoint_loaf", "rest_loaf"])
@@ -552,7 +552,8 @@ def main():
     from baking_amd import surrogate
     from baking_amd.measure import measure
     spec = build_scene(args.workload)
-    steps = args.steps or (2000 if spec.name == "loaf_256" else 1000)
+    # loaf_256: the scene's 750 substeps; by substep 806 its gas-acoustic CFL bound is reached (2000 are not runnable)
+    steps = args.steps or (spec.steps if spec.name == "loaf_256" else 1000)
     sim = build_simulator(spec)
     sim.use_graphs = not args.no_graph
     n = sim.particle_data.particle_count
