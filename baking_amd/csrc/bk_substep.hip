@@ -818,9 +818,14 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
       double change = dt * wr;
       const double Wi = g.W[node];
       water_bad = Wi + change < 0.0 || dt * wd > 1.0;
-      g.dT[node] = dt * hr / Ci;
+      const double dT = dt * hr / Ci;
       // the relative water change dW / W each particle gathers (thermal_water.py:222-223), once per node
-      g.dW[node] = Wi > 0.0 ? change / Wi : 0.0;
+      const double dW = Wi > 0.0 ? change / Wi : 0.0;
+      g.dT[node] = dT;
+      g.dW[node] = dW;
+      ws->dTW[node] = make_double2(dT, dW);
+    } else {
+      ws->dTW[node] = make_double2(0.0, 0.0);  // as the cleared dT and dW of a node without heat capacity
     }
     flag_max(dough_flags(ws, bg.d), BK_FLAG_HEAT, heat_bad);
     flag_max(dough_flags(ws, bg.d), BK_FLAG_WATER, water_bad);
@@ -863,8 +868,9 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __
             for (int oz = 0; oz < 3; ++oz) {
               double wt = double(wx[ox] * wy[oy] * wz[oz]);
               int64_t node = node_base + tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, nb);
-              tc += wt * g.dT[node];
-              mr += wt * g.dW[node];
+              const double2 q = ws->dTW[node];  // {dT, dW} in one 16-byte load
+              tc += wt * q.x;
+              mr += wt * q.y;
             }
       }
       double T_old = in.T[p], W_old = in.W[p];
@@ -1560,6 +1566,10 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
       BK_HIP(hipMemcpy(ws, &hw->dev, sizeof(bk_workspace), hipMemcpyHostToDevice));
     }
   }
+  if (thermal && !hw->dev.dTW) {  // 16 B per node of the workspace's grid
+    BK_HIP(hipMalloc(&hw->dev.dTW, size_t(h.total_blocks) * 64 * sizeof(double2)));
+    BK_HIP(hipMemcpy(ws, &hw->dev, sizeof(bk_workspace), hipMemcpyHostToDevice));
+  }
   hw->doughs.assign(table, table + nd);
   hw->n = int(n);
   hw->n_vertices = int(nv);
@@ -1712,7 +1722,7 @@ void bk_workspace_destroy(bk_workspace* ws) {
   void* ptrs[] = {h.doughs, h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1],
                   h.hist, h.digit_total, h.blk_start, h.blk_count, h.scan_part, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
                   h.queue, h.tau, h.vertices, h.vert_mark, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions,
-                  h.stats, h.settle, h.settle_part};
+                  h.stats, h.settle, h.settle_part, h.dTW};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   (void)hipFree(ws);

@@ -38,6 +38,9 @@ struct bk_workspace {
   int32_t* digit_total = nullptr;
   int32_t* blk_start = nullptr;                 // per batch block: first sorted particle
   int32_t* blk_count = nullptr;                 // per batch block: particles keyed to it (0 between substeps)
+  double2* dTW = nullptr;                       // per node {dT, dW} of the thermal grid, interleaved for pass B1's 27-node
+                                                // gather (written by the diffusion next to the bk_grid arrays; allocated
+                                                // with the first thermal dough table)
   int32_t* scan_part = nullptr;                 // partial sums of the block-count scan
   int32_t* blk_end = nullptr;
   int32_t* node_flag = nullptr;                 // per batch block: active this substep
