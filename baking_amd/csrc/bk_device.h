@@ -651,16 +651,19 @@ __device__ __forceinline__ mat3<float> neo_hookean_kirchhoff(const mat3<float>& 
 }
 
 // ------------------------------------------------------------------ mesh reconstruction (geometry/mesh_tracking.py:13-38)
-// One 256-thread workgroup per vertex: every thread forms the anchor
-// prediction, the threads stride over the vertex's particle bindings summing
-// weighted deviations from the anchor, a butterfly combines each wavefront and
-// thread 0 adds the four wavefront sums in order.  A vertex can hold ~1000
-// bindings and each is a dependent pair of loads (particle index, then its x
-// and F), so a thread issues kRecUnroll bindings' loads before accumulating
-// them in order; a slot past e reads binding e-1 (a valid address) with weight
-// 0, so the loads are used unconditionally and cannot be sunk into per-slot
-// branches.  Call with the whole workgroup and the same `v`; `red` holds 12
-// floats of LDS.
+// One 256-thread workgroup per vertex; the threads stride over the vertex's
+// particle bindings.  The reference sums weighted deviations of the
+// predictions x + F r from an anchor (the first binding's prediction) to keep
+// float error small at high valence.  Here each thread sums deviations from
+// its own first prediction a_t, with its weight total W_t, and adds
+// W_t (a_t - A) once the anchor A (thread 0's a_t) is known: every summed term
+// is still a small deviation, and no load waits for the anchor's dependent
+// loads.  A vertex can hold ~1000 bindings and each is a dependent pair of
+// loads (particle index, then its x and F), so a thread issues kRecUnroll
+// bindings' loads before accumulating them in order; a slot past e reads
+// binding e-1 (a valid address) with weight 0, so the loads are used
+// unconditionally and cannot be sunk into per-slot branches.  Call with the
+// whole workgroup and the same `v`; `red` holds 16 floats of LDS.
 constexpr int kRecThreads = 256;
 constexpr int kRecUnroll = 2;
 __device__ inline void reconstruct_vertex_block(int v, const float* __restrict__ x, const float* __restrict__ F,
@@ -673,15 +676,9 @@ __device__ inline void reconstruct_vertex_block(int v, const float* __restrict__
     if (t < 3) out[3 * v + t] = ref_vertices[3 * v + t];
     return;
   }
-  float anchor[3];
-  {
-    int ap = parts[s];
-    const float* Fa = F + 9 * ap;
-    const float* r = ref_offs + 3 * s;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) anchor[a] = x[3 * ap + a] + (Fa[a * 3] * r[0] + Fa[a * 3 + 1] * r[1] + Fa[a * 3 + 2] * r[2]);
-  }
-  float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+  float a0 = 0.f, a1 = 0.f, a2 = 0.f;  // this thread's first prediction
+  float d0 = 0.f, d1 = 0.f, d2 = 0.f, wsum = 0.f;
+  bool first = true;
   for (int k0 = s + t; k0 < e; k0 += kRecThreads * kRecUnroll) {
     int pi[kRecUnroll];
 #pragma unroll
@@ -692,10 +689,32 @@ __device__ inline void reconstruct_vertex_block(int v, const float* __restrict__
       const float* Fp = F + 9 * pi[u];
       const float* r = ref_offs + 3 * k;
       const float wk = k0 + kRecThreads * u < e ? wts[k] : 0.0f;
-      d0 += wk * ((x[3 * pi[u] + 0] + (Fp[0] * r[0] + Fp[1] * r[1] + Fp[2] * r[2])) - anchor[0]);
-      d1 += wk * ((x[3 * pi[u] + 1] + (Fp[3] * r[0] + Fp[4] * r[1] + Fp[5] * r[2])) - anchor[1]);
-      d2 += wk * ((x[3 * pi[u] + 2] + (Fp[6] * r[0] + Fp[7] * r[1] + Fp[8] * r[2])) - anchor[2]);
+      const float p0 = x[3 * pi[u] + 0] + (Fp[0] * r[0] + Fp[1] * r[1] + Fp[2] * r[2]);
+      const float p1 = x[3 * pi[u] + 1] + (Fp[3] * r[0] + Fp[4] * r[1] + Fp[5] * r[2]);
+      const float p2 = x[3 * pi[u] + 2] + (Fp[6] * r[0] + Fp[7] * r[1] + Fp[8] * r[2]);
+      if (u == 0 && first) {  // the loop runs only while k0 < e: slot 0 of the first pass is a binding
+        a0 = p0;
+        a1 = p1;
+        a2 = p2;
+        first = false;
+      }
+      d0 += wk * (p0 - a0);
+      d1 += wk * (p1 - a1);
+      d2 += wk * (p2 - a2);
+      wsum += wk;
     }
+  }
+  if (t == 0) {  // thread 0's first binding is binding s: the reference's anchor
+    red[12] = a0;
+    red[13] = a1;
+    red[14] = a2;
+  }
+  __syncthreads();
+  const float A0 = red[12], A1 = red[13], A2 = red[14];
+  if (wsum != 0.0f) {
+    d0 += wsum * (a0 - A0);
+    d1 += wsum * (a1 - A1);
+    d2 += wsum * (a2 - A2);
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -713,7 +732,7 @@ __device__ inline void reconstruct_vertex_block(int v, const float* __restrict__
     float d = red[t];
 #pragma unroll
     for (int w = 1; w < kRecThreads / 64; ++w) d += red[3 * w + t];
-    out[3 * v + t] = anchor[t] + d;
+    out[3 * v + t] = red[12 + t] + d;
   }
 }
 

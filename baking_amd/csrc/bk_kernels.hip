@@ -329,7 +329,7 @@ __global__ void k_Saturations(const double* W, const float* V0, const float* F, 
 __global__ __launch_bounds__(kRecThreads) void k_ReconstructMeshVertices(const float* x, const float* F, const float* rv, const int* offs,
                                                                          const int* parts, const float* wts, const float* ro, float* out,
                                                                          int nv) {
-  __shared__ float red[3 * kRecThreads / 64];
+  __shared__ float red[16];
   reconstruct_vertex_block(int(blockIdx.x), x, F, rv, offs, parts, wts, ro, out, red);
 }
 

@@ -538,7 +538,7 @@ __global__ __launch_bounds__(kRecThreads) void k_reconstruct(const float* __rest
                                                              const int32_t* __restrict__ parts, const float* __restrict__ wts,
                                                              const float* __restrict__ ref_offs, float* __restrict__ out, int nv,
                                                              const bk_workspace* ws) {
-  __shared__ float red[3 * kRecThreads / 64];
+  __shared__ float red[16];
   const int v = xcd_block();
   if (v >= nv || ws->vert_mark[v] != ws->counters[6]) return;
   reconstruct_vertex_block(v, x, F, ref_vertices, offs, parts, wts, ref_offs, out, red);
