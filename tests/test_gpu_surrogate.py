@@ -5,6 +5,7 @@
 * ``bk_mlp_infer`` (BF16 MFMA, FP32 accumulation) against the oracle's BF16
   restatement, for row counts that are not multiples of a wavefront;
 * training lowers the held-out error;
+* sample collection keeps no row of a substep after which a device flag is up;
 * a yielding gas dough run with a network trained on its own emitted rows
   gives the exact run's fields (the guided solve converges to the same root)
   with fewer evaluations of the porosity equation.
@@ -107,6 +108,27 @@ def test_training_lowers_held_out_error():
 
 def _yielding_gas_dough():
     return scenes.strain_rate_field(scenes.demo_gas(subdivisions=2), np.diag([1.5, -0.75, 0.2]))
+
+
+def test_sample_collection_stops_at_the_first_flag():
+    """collect_samples keeps no row of a substep after which a device flag is up, and stops there.  Two copies of
+    the yielding gas dough advance 20 substeps; on one the gas-acoustic CFL flag is then raised from the host
+    (device flags are sticky).  The same three substeps emit rows on the other copy and none on this one."""
+    spec = _yielding_gas_dough()
+    runs = {}
+    for flagged in (False, True):
+        sim = scenes.build_simulator(spec)
+        sim.Advance(20)
+        if flagged:
+            flags = list(sim._read_flags())
+            flags[4] = 1  # BK_FLAG_GAS_CFL
+            sim._write_flags(flags)
+        log = []
+        runs[flagged] = (surrogate.collect_samples(sim, 3, capacity=200000, log=log.append), log)
+        print(flagged, runs[flagged][0].shape[0], log, flush=True)
+    assert runs[False][0].shape[0] > 0 and len(runs[False][1]) == 3
+    rows, log = runs[True]
+    assert rows.shape[0] == 0 and len(log) == 1 and "raised" in log[0]
 
 
 def test_guided_return_map_matches_exact_with_fewer_evaluations():
