@@ -467,10 +467,22 @@ __global__ __launch_bounds__(64) void k_cell_sort(const uint32_t* __restrict__ k
 // node blocks the block's stencils reach; the first also cuts the block into
 // tiles of at most `chunk` particles.  The loop's trip count is uniform per
 // workgroup, so every thread reaches the per-workgroup appends.
-__global__ __launch_bounds__(256) void k_activate(const bk_dough* __restrict__ doughs, int nd, bk_workspace* ws, int chunk) {
+__device__ __forceinline__ void mark_face_vertices(const bk_scene& sc, bk_workspace* ws, int s);
+// Workgroups [0, gridDim.x - mark_blocks) build the tile and active-block
+// lists; the last mark_blocks workgroups mark the boundary faces' vertices
+// (k_mark_face_vertices' work, one launch fewer).  The split is per workgroup,
+// so the list workgroups' collectives see whole workgroups.
+__global__ __launch_bounds__(256) void k_activate(const bk_dough* __restrict__ doughs, int nd, bk_workspace* ws, int chunk,
+                                                  bk_scene sc, int nslots, int mark_blocks) {
   __shared__ int lds[17];
+  const int list_blocks = int(gridDim.x) - mark_blocks;
+  if (int(blockIdx.x) >= list_blocks) {
+    const int s = (int(blockIdx.x) - list_blocks) * int(blockDim.x) + int(threadIdx.x);
+    if (s < nslots) mark_face_vertices(sc, ws, s);
+    return;
+  }
   const int total = ws->counters[3] * 8;
-  for (int base = blockIdx.x * blockDim.x; base < total; base += gridDim.x * blockDim.x) {
+  for (int base = blockIdx.x * blockDim.x; base < total; base += list_blocks * blockDim.x) {
     const int t = base + int(threadIdx.x);
     int b = 0, d = 0, start = 0, end = 0, ntile = 0, nbk = 0;
     bool fresh = false;
@@ -519,10 +531,9 @@ __global__ void k_clear_blocks(bk_workspace* ws, bk_grid g, int thermal) {
 // ============================================================================ mesh + surface areas
 // Only the boundary faces' areas enter the substep (the Robin terms), so only
 // their vertices are reconstructed: each slot marks its faces' vertices with
-// the substep count, and the reconstruction skips the vertices left unmarked.
-__global__ void k_mark_face_vertices(bk_scene sc, bk_workspace* ws, int nslots) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nslots) return;
+// the substep count (in the last workgroups of k_activate), and the
+// reconstruction skips the vertices left unmarked.
+__device__ __forceinline__ void mark_face_vertices(const bk_scene& sc, bk_workspace* ws, int s) {
   const int32_t stamp = ws->counters[6];
   for (int k = sc.slot_face_offsets[s]; k < sc.slot_face_offsets[s + 1]; ++k) {
     const int32_t* f = sc.faces + 3 * sc.slot_faces[k];
@@ -1440,14 +1451,17 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
   }
   {  // 4. tiles, active node blocks
     StageTimer tm(pr, st, kStageTiles);
-    hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)n * 8, 256), 1, 1024)), dim3(256), 0, st, D, nd, ws, tu.tile_chunk);
+    // with the thermal transport, the last workgroups mark the boundary faces' vertices for the mesh stage
+    const int mark_slots = (hw->thermal && hw->n_vertices > 0 && hw->n_slots > 0) ? hw->n_slots : 0;
+    const int mark_blocks = div_up(mark_slots, 256);
+    hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)n * 8, 256), 1, 1024) + mark_blocks), dim3(256), 0, st, D, nd, ws,
+                       tu.tile_chunk, *sc, mark_slots, mark_blocks);
     BK_LAUNCH(tiles);
   }
   if (hw->thermal) {
     {  // 5. mesh reconstruction and surface areas
       StageTimer tm(pr, st, kStageMesh);
       if (hw->n_vertices > 0 && hw->n_slots > 0) {
-        hipLaunchKernelGGL(k_mark_face_vertices, dim3(div_up(hw->n_slots, 256)), dim3(256), 0, st, *sc, ws, hw->n_slots);
         hipLaunchKernelGGL(k_reconstruct, dim3(hw->n_vertices), dim3(kRecThreads), 0, st, in->x, in->F, sc->ref_vertices,
                            sc->bind_offsets, sc->bind_particles, sc->bind_weights, sc->bind_ref_offsets, h->vertices,
                            hw->n_vertices, ws);
