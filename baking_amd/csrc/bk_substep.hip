@@ -153,11 +153,13 @@ __global__ void k_cell_keys(const float* __restrict__ x, int n, float ox, float 
 // batch block (the order in which their appends reached the block's counter).
 // The lanes of a wavefront that share a block -- consecutive particles mostly
 // do -- take their ranks from one atomic.
-__global__ void k_batch_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of,
-                             const float* __restrict__ x, int n, uint32_t* __restrict__ keys, int32_t* __restrict__ ranks,
-                             bk_workspace* ws) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p == 0) { ws->counters[0] = 0; ws->counters[1] = 0; ws->counters[2] = 0; ws->counters[3] = 0; ws->counters[6] += 1; }
+// Keys and per-block ranks of particles (one per lane, p).  Particle 0 resets
+// the substep's list counters and advances its stamp; the active-block count
+// (counters[1]) is reset by k_block_scan_partials, after the clear that reads it.
+__device__ __forceinline__ void batch_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of,
+                                           const float* __restrict__ x, int n, uint32_t* __restrict__ keys,
+                                           int32_t* __restrict__ ranks, bk_workspace* ws, int p) {
+  if (p == 0) { ws->counters[0] = 0; ws->counters[2] = 0; ws->counters[3] = 0; ws->counters[6] += 1; }
   const bool valid = p < n;
   uint32_t key = 0u;
   if (valid) {
@@ -335,7 +337,7 @@ static int radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int
 
 // ============================================================================ block sort, tiles and active node blocks
 // The particles are ordered by batch block and, inside each block, by cell:
-//   k_block_scan_*   exclusive scan of the per-block counts k_batch_keys left:
+//   k_block_scan_*   exclusive scan of the per-block counts the keys left:
 //                    each non-empty block's [start, end) in the sorted order,
 //                    the list of non-empty blocks, and the counts reset to 0
 //   k_block_scatter  particle p to start[block] + rank
@@ -377,8 +379,10 @@ __device__ __forceinline__ int block_exclusive_scan(int v, int* lds) {
   return before + inc - v;
 }
 
-__global__ __launch_bounds__(256) void k_block_scan_partials(const int32_t* __restrict__ counts, int nb, int32_t* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_block_scan_partials(const int32_t* __restrict__ counts, int nb, int32_t* __restrict__ part,
+                                                             int32_t* __restrict__ counters) {
   __shared__ int lds[4];
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[1] = 0;  // the active-block count, read by the clear before this kernel
   const int base = blockIdx.x * kScanTile + int(threadIdx.x) * kScanItems;
   int sum = 0;
 #pragma unroll
@@ -511,10 +515,11 @@ __global__ __launch_bounds__(256) void k_activate(const bk_dough* __restrict__ d
   }
 }
 
-// Zero every grid array over the previous substep's active blocks (64 nodes each).
-__global__ void k_clear_blocks(bk_workspace* ws, bk_grid g, int thermal) {
+// Zero every grid array over the previous substep's active blocks (64 nodes
+// each), as workgroup b of nb.
+__device__ __forceinline__ void clear_active_blocks(bk_workspace* ws, const bk_grid& g, int thermal, int b, int nb) {
   int nact = ws->counters[1];
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
+  for (int t = b * int(blockDim.x) + int(threadIdx.x); t < nact * 64; t += nb * int(blockDim.x)) {
     int blk = ws->active[t >> 6];
     int64_t node = (int64_t(blk) << 6) | (t & 63);
     g.m[node] = 0.0f;
@@ -526,6 +531,18 @@ __global__ void k_clear_blocks(bk_workspace* ws, bk_grid g, int thermal) {
     }
     if ((t & 63) == 0) ws->node_flag[blk] = 0;
   }
+}
+
+// Stages 1 and 2 in one launch: workgroups [0, clear_wgs) clear the previous
+// substep's blocks, the rest form the keys (which need no cleared grid, and do
+// not touch what the clear reads or writes).
+__global__ void k_clear_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of, const float* __restrict__ x,
+                             int n, uint32_t* __restrict__ keys, int32_t* __restrict__ ranks, bk_workspace* ws, bk_grid g,
+                             int thermal, int clear_wgs) {
+  if (int(blockIdx.x) < clear_wgs)
+    clear_active_blocks(ws, g, thermal, int(blockIdx.x), clear_wgs);
+  else
+    batch_keys(doughs, dough_of, x, n, keys, ranks, ws, (int(blockIdx.x) - clear_wgs) * int(blockDim.x) + int(threadIdx.x));
 }
 
 // ============================================================================ mesh + surface areas
@@ -1426,22 +1443,33 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     auto it = g_prof.find(ws);
     if (it != g_prof.end() && it->second.step < it->second.max_steps) pr = &it->second;
   }
-  {  // 1. clear the previous substep's blocks (counters[1] still holds their count)
-    StageTimer tm(pr, st, kStageClear);
-    hipLaunchKernelGGL(k_clear_blocks, dim3(cap), dim3(256), 0, st, ws, *g, int(hw->thermal));
-    BK_LAUNCH(clear);
-  }
-  {  // 2. keys and ranks within blocks (also resets the per-substep counters)
-    StageTimer tm(pr, st, kStageKeys);
-    hipLaunchKernelGGL(k_batch_keys, dim3(div_up(n, 256)), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, h->vals_s[1],
-                       ws);
-    BK_LAUNCH(keys);
+  // 1. clear the previous substep's blocks (counters[1] still holds their count)
+  // 2. keys and ranks within blocks (also resets the per-substep counters)
+  // One launch; under stage profiling, one launch per stage.
+  const int key_wgs = div_up(n, 256);
+  if (pr) {
+    {
+      StageTimer tm(pr, st, kStageClear);
+      hipLaunchKernelGGL(k_clear_keys, dim3(cap), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, h->vals_s[1], ws, *g,
+                         int(hw->thermal), cap);
+      BK_LAUNCH(clear);
+    }
+    {
+      StageTimer tm(pr, st, kStageKeys);
+      hipLaunchKernelGGL(k_clear_keys, dim3(key_wgs), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, h->vals_s[1], ws,
+                         *g, int(hw->thermal), 0);
+      BK_LAUNCH(keys);
+    }
+  } else {
+    hipLaunchKernelGGL(k_clear_keys, dim3(cap + key_wgs), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, h->vals_s[1], ws,
+                       *g, int(hw->thermal), cap);
+    BK_LAUNCH(clear_keys);
   }
   {  // 3. order by block, then by cell inside each block
     StageTimer tm(pr, st, kStageSort);
     const int nb = h->total_blocks;
     const int nscan = div_up(nb, kScanTile);
-    hipLaunchKernelGGL(k_block_scan_partials, dim3(nscan), dim3(256), 0, st, h->blk_count, nb, h->scan_part);
+    hipLaunchKernelGGL(k_block_scan_partials, dim3(nscan), dim3(256), 0, st, h->blk_count, nb, h->scan_part, h->counters);
     hipLaunchKernelGGL(k_block_scan_apply, dim3(nscan), dim3(256), 0, st, h->blk_count, nb, h->scan_part, ws);
     hipLaunchKernelGGL(k_block_scatter, dim3(div_up(n, 256)), dim3(256), 0, st, h->keys_raw, h->vals_s[1], n, ws, h->keys_s[0],
                        h->vals_s[0]);
