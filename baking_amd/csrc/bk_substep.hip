@@ -340,7 +340,7 @@ static int radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int
 //   k_block_scan_*   exclusive scan of the per-block counts the keys left:
 //                    each non-empty block's [start, end) in the sorted order,
 //                    the list of non-empty blocks, and the counts reset to 0
-//   k_block_scatter  particle p to start[block] + rank
+//   block_scatter    particle p to start[block] + rank (in k_activate's launch)
 //   k_cell_sort      inside each non-empty block, a counting sort by cell
 // Cells come out contiguous inside their block, which is all the tiles need;
 // the order of a cell's particles follows the atomics, as does the order of
@@ -422,9 +422,10 @@ __global__ __launch_bounds__(256) void k_block_scan_apply(int32_t* __restrict__ 
   }
 }
 
-__global__ void k_block_scatter(const uint32_t* __restrict__ keys, const int32_t* __restrict__ ranks, int n,
-                                const bk_workspace* __restrict__ ws, uint32_t* __restrict__ kout, int32_t* __restrict__ vout) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+// Particle p to its block's start + its rank (run in k_activate's last workgroups).
+__device__ __forceinline__ void block_scatter(const uint32_t* __restrict__ keys, const int32_t* __restrict__ ranks, int n,
+                                              const bk_workspace* __restrict__ ws, uint32_t* __restrict__ kout,
+                                              int32_t* __restrict__ vout, int p) {
   if (p >= n) return;
   const uint32_t key = keys[p];
   const int pos = ws->blk_start[key >> 6] + ranks[p];
@@ -472,14 +473,23 @@ __global__ __launch_bounds__(64) void k_cell_sort(const uint32_t* __restrict__ k
 // tiles of at most `chunk` particles.  The loop's trip count is uniform per
 // workgroup, so every thread reaches the per-workgroup appends.
 __device__ __forceinline__ void mark_face_vertices(const bk_scene& sc, bk_workspace* ws, int s);
-// Workgroups [0, gridDim.x - mark_blocks) build the tile and active-block
-// lists; the last mark_blocks workgroups mark the boundary faces' vertices
-// (k_mark_face_vertices' work, one launch fewer).  The split is per workgroup,
-// so the list workgroups' collectives see whole workgroups.
+// Three independent jobs that all follow the block-count scan, in one launch:
+// workgroups [0, L) build the tile and active-block lists, the next
+// mark_blocks mark the boundary faces' vertices for the mesh stage, and the
+// last scatter_blocks place each particle at its block's start + rank
+// (L = gridDim.x - mark_blocks - scatter_blocks; L = 0 leaves the lists to
+// another launch).  The split is per workgroup, so the list workgroups'
+// collectives see whole workgroups.
 __global__ __launch_bounds__(256) void k_activate(const bk_dough* __restrict__ doughs, int nd, bk_workspace* ws, int chunk,
-                                                  bk_scene sc, int nslots, int mark_blocks) {
+                                                  bk_scene sc, int nslots, int mark_blocks, const uint32_t* __restrict__ keys,
+                                                  const int32_t* __restrict__ ranks, int n, uint32_t* __restrict__ kout,
+                                                  int32_t* __restrict__ vout, int scatter_blocks) {
   __shared__ int lds[17];
-  const int list_blocks = int(gridDim.x) - mark_blocks;
+  const int list_blocks = int(gridDim.x) - mark_blocks - scatter_blocks;
+  if (int(blockIdx.x) >= list_blocks + mark_blocks) {
+    block_scatter(keys, ranks, n, ws, kout, vout, (int(blockIdx.x) - list_blocks - mark_blocks) * int(blockDim.x) + int(threadIdx.x));
+    return;
+  }
   if (int(blockIdx.x) >= list_blocks) {
     const int s = (int(blockIdx.x) - list_blocks) * int(blockDim.x) + int(threadIdx.x);
     if (s < nslots) mark_face_vertices(sc, ws, s);
@@ -1471,19 +1481,28 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     const int nscan = div_up(nb, kScanTile);
     hipLaunchKernelGGL(k_block_scan_partials, dim3(nscan), dim3(256), 0, st, h->blk_count, nb, h->scan_part, h->counters);
     hipLaunchKernelGGL(k_block_scan_apply, dim3(nscan), dim3(256), 0, st, h->blk_count, nb, h->scan_part, ws);
-    hipLaunchKernelGGL(k_block_scatter, dim3(div_up(n, 256)), dim3(256), 0, st, h->keys_raw, h->vals_s[1], n, ws, h->keys_s[0],
-                       h->vals_s[0]);
+    // the scatter, with the tile lists and the face-vertex marks in the same launch (under stage profiling, the
+    // lists and marks get a launch of their own in stage 4)
+    const int list_wgs = clampi(div_up((long long)n * 8, 256), 1, 1024);
+    const int mark_slots = (hw->thermal && hw->n_vertices > 0 && hw->n_slots > 0) ? hw->n_slots : 0;
+    const int mark_wgs = div_up(mark_slots, 256);
+    const int scatter_wgs = div_up(n, 256);
+    if (pr)
+      hipLaunchKernelGGL(k_activate, dim3(scatter_wgs), dim3(256), 0, st, D, nd, ws, tu.tile_chunk, *sc, 0, 0, h->keys_raw,
+                         h->vals_s[1], n, h->keys_s[0], h->vals_s[0], scatter_wgs);
+    else
+      hipLaunchKernelGGL(k_activate, dim3(list_wgs + mark_wgs + scatter_wgs), dim3(256), 0, st, D, nd, ws, tu.tile_chunk, *sc,
+                         mark_slots, mark_wgs, h->keys_raw, h->vals_s[1], n, h->keys_s[0], h->vals_s[0], scatter_wgs);
     hipLaunchKernelGGL(k_cell_sort, dim3(clampi(div_up(n, 64), 1, 16384)), dim3(64), 0, st, h->keys_s[0], h->vals_s[0], ws,
                        h->sorted_keys, h->sorted_vals);
     BK_LAUNCH(sort);
   }
-  {  // 4. tiles, active node blocks
+  if (pr) {  // 4. tiles, active node blocks (launched with the scatter outside stage profiling)
     StageTimer tm(pr, st, kStageTiles);
-    // with the thermal transport, the last workgroups mark the boundary faces' vertices for the mesh stage
     const int mark_slots = (hw->thermal && hw->n_vertices > 0 && hw->n_slots > 0) ? hw->n_slots : 0;
-    const int mark_blocks = div_up(mark_slots, 256);
-    hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)n * 8, 256), 1, 1024) + mark_blocks), dim3(256), 0, st, D, nd, ws,
-                       tu.tile_chunk, *sc, mark_slots, mark_blocks);
+    const int mark_wgs = div_up(mark_slots, 256);
+    hipLaunchKernelGGL(k_activate, dim3(clampi(div_up((long long)n * 8, 256), 1, 1024) + mark_wgs), dim3(256), 0, st, D, nd, ws,
+                       tu.tile_chunk, *sc, mark_slots, mark_wgs, h->keys_raw, h->vals_s[1], n, h->keys_s[0], h->vals_s[0], 0);
     BK_LAUNCH(tiles);
   }
   if (hw->thermal) {
