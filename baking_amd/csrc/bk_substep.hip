@@ -341,7 +341,7 @@ static int radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int
 //                    each non-empty block's [start, end) in the sorted order,
 //                    the list of non-empty blocks, and the counts reset to 0
 //   block_scatter    particle p to start[block] + rank (in k_activate's launch)
-//   k_cell_sort      inside each non-empty block, a counting sort by cell
+//   cell sort        inside each non-empty block, a counting sort by cell (k_sort_reconstruct)
 // Cells come out contiguous inside their block, which is all the tiles need;
 // the order of a cell's particles follows the atomics, as does the order of
 // the P2G accumulations they feed.  Every pass reads and writes each particle
@@ -435,19 +435,30 @@ __device__ __forceinline__ void block_scatter(const uint32_t* __restrict__ keys,
 
 // One wavefront per non-empty block: cell counts in LDS, their exclusive scan
 // (lane c holds cell c), then every particle takes the next slot of its cell.
-__global__ __launch_bounds__(64) void k_cell_sort(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
-                                                  const bk_workspace* __restrict__ ws, uint32_t* __restrict__ kout,
-                                                  int32_t* __restrict__ vout) {
-  __shared__ int cursor[64];
+// Wavefront-scope barrier: the lanes' LDS accesses before it are visible to
+// every lane after it (the wavefront's own LDS region, no other wavefront).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inside each non-empty block, a counting sort by cell: wavefront w of nw
+// takes blocks w, w + nw, ...; cell counts in its 64-int LDS cursor, their
+// exclusive scan (lane c holds cell c), then every particle takes the next
+// slot of its cell.
+__device__ __forceinline__ void cell_sort_wave(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                                               const bk_workspace* __restrict__ ws, uint32_t* __restrict__ kout,
+                                               int32_t* __restrict__ vout, int* cursor, int w, int nw) {
   const int nblk = ws->counters[3];
-  const int lane = int(threadIdx.x);
-  for (int bi = blockIdx.x; bi < nblk; bi += gridDim.x) {
+  const int lane = int(__lane_id());
+  for (int bi = w; bi < nblk; bi += nw) {
     const int b = ws->blocks[bi];
     const int start = ws->blk_start[b], end = ws->blk_end[b];
     cursor[lane] = 0;
-    __syncthreads();
+    wave_sync();
     for (int j = start + lane; j < end; j += 64) atomicAdd(&cursor[kin[j] & 63u], 1);
-    __syncthreads();
+    wave_sync();
     const int count = cursor[lane];
     int inc = count;
 #pragma unroll
@@ -455,16 +466,16 @@ __global__ __launch_bounds__(64) void k_cell_sort(const uint32_t* __restrict__ k
       const int t = __shfl_up(inc, off, 64);
       if (lane >= off) inc += t;
     }
-    __syncthreads();
+    wave_sync();
     cursor[lane] = start + inc - count;
-    __syncthreads();
+    wave_sync();
     for (int j = start + lane; j < end; j += 64) {
       const uint32_t key = kin[j];
       const int slot = atomicAdd(&cursor[key & 63u], 1);
       kout[slot] = key;
       vout[slot] = vin[j];
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
@@ -570,14 +581,27 @@ __device__ __forceinline__ void mark_face_vertices(const bk_scene& sc, bk_worksp
   }
 }
 
-// geometry/mesh_tracking.py:13-38, one workgroup per marked vertex.
-__global__ __launch_bounds__(kRecThreads) void k_reconstruct(const float* __restrict__ x, const float* __restrict__ F,
-                                                             const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
-                                                             const int32_t* __restrict__ parts, const float* __restrict__ wts,
-                                                             const float* __restrict__ ref_offs, float* __restrict__ out, int nv,
-                                                             const bk_workspace* ws) {
+// Two independent jobs in one launch: workgroups [0, sort_wgs) run the cell
+// sort (four wavefronts each, one block per wavefront at a time), the next nv
+// reconstruct one marked vertex each (geometry/mesh_tracking.py:13-38).
+// sort_wgs is a multiple of 8, so vertex workgroups keep the XCD-contiguous
+// order of xcd_block() within their range.
+__global__ __launch_bounds__(256) void k_sort_reconstruct(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
+                                                          bk_workspace* ws, uint32_t* __restrict__ kout, int32_t* __restrict__ vout,
+                                                          int sort_wgs, const float* __restrict__ x, const float* __restrict__ F,
+                                                          const float* __restrict__ ref_vertices, const int32_t* __restrict__ offs,
+                                                          const int32_t* __restrict__ parts, const float* __restrict__ wts,
+                                                          const float* __restrict__ ref_offs, float* __restrict__ out, int nv) {
+  __shared__ int cursor[4][64];
   __shared__ float red[16];
-  const int v = xcd_block();
+  const int bx = int(blockIdx.x);
+  if (bx < sort_wgs) {
+    cell_sort_wave(kin, vin, ws, kout, vout, cursor[threadIdx.x >> 6], bx * 4 + int(threadIdx.x >> 6), sort_wgs * 4);
+    return;
+  }
+  const int r = bx - sort_wgs;  // XCD r % kXcds, as bx is
+  const int xc = r % kXcds, i = r / kXcds, per = nv / kXcds, rem = nv % kXcds;
+  const int v = xc * per + (xc < rem ? xc : rem) + i;
   if (v >= nv || ws->vert_mark[v] != ws->counters[6]) return;
   reconstruct_vertex_block(v, x, F, ref_vertices, offs, parts, wts, ref_offs, out, red);
 }
@@ -1493,8 +1517,12 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     else
       hipLaunchKernelGGL(k_activate, dim3(list_wgs + mark_wgs + scatter_wgs), dim3(256), 0, st, D, nd, ws, tu.tile_chunk, *sc,
                          mark_slots, mark_wgs, h->keys_raw, h->vals_s[1], n, h->keys_s[0], h->vals_s[0], scatter_wgs);
-    hipLaunchKernelGGL(k_cell_sort, dim3(clampi(div_up(n, 64), 1, 16384)), dim3(64), 0, st, h->keys_s[0], h->vals_s[0], ws,
-                       h->sorted_keys, h->sorted_vals);
+    // the cell sort, with the mesh reconstruction in the same launch outside stage profiling (its vertex marks are set)
+    const int sort_wgs = div_up(div_up(clampi(div_up(n, 64), 1, 16384), 4), kXcds) * kXcds;
+    const int rec_nv = (!pr && hw->thermal && hw->n_vertices > 0 && hw->n_slots > 0) ? hw->n_vertices : 0;
+    hipLaunchKernelGGL(k_sort_reconstruct, dim3(sort_wgs + rec_nv), dim3(256), 0, st, h->keys_s[0], h->vals_s[0], ws, h->sorted_keys,
+                       h->sorted_vals, sort_wgs, in->x, in->F, sc->ref_vertices, sc->bind_offsets, sc->bind_particles,
+                       sc->bind_weights, sc->bind_ref_offsets, h->vertices, rec_nv);
     BK_LAUNCH(sort);
   }
   if (pr) {  // 4. tiles, active node blocks (launched with the scatter outside stage profiling)
@@ -1509,9 +1537,10 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     {  // 5. mesh reconstruction and surface areas
       StageTimer tm(pr, st, kStageMesh);
       if (hw->n_vertices > 0 && hw->n_slots > 0) {
-        hipLaunchKernelGGL(k_reconstruct, dim3(hw->n_vertices), dim3(kRecThreads), 0, st, in->x, in->F, sc->ref_vertices,
-                           sc->bind_offsets, sc->bind_particles, sc->bind_weights, sc->bind_ref_offsets, h->vertices,
-                           hw->n_vertices, ws);
+        if (pr)  // outside stage profiling the reconstruction ran with the cell sort
+          hipLaunchKernelGGL(k_sort_reconstruct, dim3(hw->n_vertices), dim3(256), 0, st, h->keys_s[0], h->vals_s[0], ws,
+                             h->sorted_keys, h->sorted_vals, 0, in->x, in->F, sc->ref_vertices, sc->bind_offsets,
+                             sc->bind_particles, sc->bind_weights, sc->bind_ref_offsets, h->vertices, hw->n_vertices);
       }
       if (hw->n_slots > 0)
         hipLaunchKernelGGL(k_slot_areas, dim3(div_up(hw->n_slots, 256)), dim3(256), 0, st, D, *sc, h->vertices, ws, hw->n_slots);
