@@ -110,6 +110,7 @@ SIGNATURES = {
     "bk_workspace_flags": ([P], c_void_p),
     "bk_workspace_vertices": ([P], c_void_p),
     "bk_workspace_counts": ([P, P, P], c_int),
+    "bk_workspace_sort_order": ([P, P, P, P, I, P], c_int),
     "bk_workspace_read_flags": ([P, P, P], c_int),
     "bk_workspace_write_flags": ([P, P, P], c_int),
     "bk_workspace_read_flags_all": ([P, P, I, P], c_int),

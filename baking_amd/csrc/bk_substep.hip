@@ -71,6 +71,7 @@ int check_hip(hipError_t e, const char* where) {
 constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per workgroup
+static_assert(kSortTile == kRadixTile, "the radix sort's scratch sizing in bk_workspace.h");
 constexpr int kParticleThreads = 256;                 // launch bound of the per-particle kernels
 
 inline int div_up(long long a, long long b) { return int((a + b - 1) / b); }
@@ -335,6 +336,12 @@ static int radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int
   return 0;
 }
 
+// radix_sort for the other sources of the library (declared in bk_workspace.h).
+int bk_internal_radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int32_t* vals_in, uint32_t* keys_out,
+                           int32_t* vals_out, int n, int key_bits, hipStream_t st) {
+  return radix_sort(ws, keys_in, vals_in, keys_out, vals_out, n, key_bits, st);
+}
+
 // ============================================================================ block sort, tiles and active node blocks
 // The particles are ordered by batch block and, inside each block, by cell:
 //   k_block_scan_*   exclusive scan of the per-block counts the keys left:
@@ -342,9 +349,10 @@ static int radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int
 //                    the list of non-empty blocks, and the counts reset to 0
 //   block_scatter    particle p to start[block] + rank (in k_activate's launch)
 //   cell sort        inside each non-empty block, a counting sort by cell (k_sort_reconstruct)
-// Cells come out contiguous inside their block, which is all the tiles need;
-// the order of a cell's particles follows the atomics, as does the order of
-// the P2G accumulations they feed.  Every pass reads and writes each particle
+// The result is the permutation of a stable sort of the keys (a cell's
+// particles in index order), whatever order the atomics of the keys and the
+// scatter left them in; the order of the P2G accumulations the tiles feed
+// still follows their atomics.  Every pass reads and writes each particle
 // once, against the four 8-bit passes of a radix sort of the whole key.
 constexpr int kScanItems = 8;
 constexpr int kScanTile = 256 * kScanItems;  // block counts per scan workgroup
@@ -443,18 +451,26 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Inside each non-empty block, a counting sort by cell: wavefront w of nw
-// takes blocks w, w + nw, ...; cell counts in its 64-int LDS cursor, their
-// exclusive scan (lane c holds cell c), then every particle takes the next
-// slot of its cell.
+// Inside each non-empty block, a stable sort by cell: wavefront w of nw takes
+// blocks w, w + nw, ...  Cell counts go to its 64-int LDS cursor and are
+// scanned (lane c holds cell c); the block's particles are bucketed by cell in
+// LDS as words cell << kIndexBits | particle, and each then takes the slot of
+// its cell's first particle plus its rank by particle index within the cell.
+// The result is the order of a stable sort of the keys -- (block, cell), then
+// particle index -- whatever order the block scatter's atomics left the block
+// in.  A block of more than kCellSortCap particles ranks each particle against
+// its block's input segment in global memory instead (same order, slower).
+constexpr int kCellSortCap = 1024;
+constexpr uint32_t kIndexBits = 26;  // particle indices below 2^26 (checked at workspace creation)
+
 __device__ __forceinline__ void cell_sort_wave(const uint32_t* __restrict__ kin, const int32_t* __restrict__ vin,
                                                const bk_workspace* __restrict__ ws, uint32_t* __restrict__ kout,
-                                               int32_t* __restrict__ vout, int* cursor, int w, int nw) {
+                                               int32_t* __restrict__ vout, int* cursor, int* cstart, uint32_t* buf, int w, int nw) {
   const int nblk = ws->counters[3];
   const int lane = int(__lane_id());
   for (int bi = w; bi < nblk; bi += nw) {
     const int b = ws->blocks[bi];
-    const int start = ws->blk_start[b], end = ws->blk_end[b];
+    const int start = ws->blk_start[b], end = ws->blk_end[b], size = end - start;
     cursor[lane] = 0;
     wave_sync();
     for (int j = start + lane; j < end; j += 64) atomicAdd(&cursor[kin[j] & 63u], 1);
@@ -467,13 +483,34 @@ __device__ __forceinline__ void cell_sort_wave(const uint32_t* __restrict__ kin,
       if (lane >= off) inc += t;
     }
     wave_sync();
-    cursor[lane] = start + inc - count;
+    cstart[lane] = inc - count;
+    cursor[lane] = inc - count;
     wave_sync();
-    for (int j = start + lane; j < end; j += 64) {
-      const uint32_t key = kin[j];
-      const int slot = atomicAdd(&cursor[key & 63u], 1);
-      kout[slot] = key;
-      vout[slot] = vin[j];
+    const uint32_t block_key = uint32_t(b) << 6;
+    if (size <= kCellSortCap) {
+      for (int j = start + lane; j < end; j += 64) {
+        const uint32_t c = kin[j] & 63u;
+        buf[atomicAdd(&cursor[c], 1)] = (c << kIndexBits) | uint32_t(vin[j]);
+      }
+      wave_sync();  // cursor[c] now ends cell c's bucket
+      for (int j = lane; j < size; j += 64) {
+        const uint32_t word = buf[j];
+        const uint32_t c = word >> kIndexBits;
+        const int lo = cstart[c], hi = cursor[c];
+        int rank = 0;
+        for (int k = lo; k < hi; ++k) rank += buf[k] < word;
+        kout[start + lo + rank] = block_key | c;
+        vout[start + lo + rank] = int32_t(word & ((1u << kIndexBits) - 1u));
+      }
+    } else {
+      for (int j = start + lane; j < end; j += 64) {
+        const uint32_t c = kin[j] & 63u;
+        const int32_t v = vin[j];
+        int rank = 0;
+        for (int k = start; k < end; ++k) rank += (kin[k] & 63u) == c && vin[k] < v;
+        kout[start + cstart[c] + rank] = block_key | c;
+        vout[start + cstart[c] + rank] = v;
+      }
     }
     wave_sync();
   }
@@ -593,10 +630,13 @@ __global__ __launch_bounds__(256) void k_sort_reconstruct(const uint32_t* __rest
                                                           const int32_t* __restrict__ parts, const float* __restrict__ wts,
                                                           const float* __restrict__ ref_offs, float* __restrict__ out, int nv) {
   __shared__ int cursor[4][64];
+  __shared__ int cstart[4][64];
+  __shared__ uint32_t buf[4][kCellSortCap];
   __shared__ float red[16];
   const int bx = int(blockIdx.x);
   if (bx < sort_wgs) {
-    cell_sort_wave(kin, vin, ws, kout, vout, cursor[threadIdx.x >> 6], bx * 4 + int(threadIdx.x >> 6), sort_wgs * 4);
+    const int wv = int(threadIdx.x >> 6);
+    cell_sort_wave(kin, vin, ws, kout, vout, cursor[wv], cstart[wv], buf[wv], bx * 4 + wv, sort_wgs * 4);
     return;
   }
   const int r = bx - sort_wgs;  // XCD r % kXcds, as bx is
@@ -1731,8 +1771,8 @@ int bk_device_info(char* name, int name_len, int* cu_count, int* xcd_count) {
 
 bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blocks, int32_t max_doughs, int32_t n_vertices,
                                         int32_t n_slots) {
-  if (max_particles < 1 || total_blocks < 1 || max_doughs < 1 || n_vertices < 0 || n_slots < 0 ||
-      (long long)total_blocks * 64 > (1ll << 32)) {
+  if (max_particles < 1 || max_particles > (1 << kIndexBits) || total_blocks < 1 || max_doughs < 1 || n_vertices < 0 ||
+      n_slots < 0 || (long long)total_blocks * 64 > (1ll << 32)) {
     fail("bk_workspace_create: bad sizes");
     return nullptr;
   }
@@ -1925,6 +1965,17 @@ int bk_workspace_stats(bk_workspace* ws, int64_t* out, int32_t reset, void* stre
     BK_HIP(hipMemsetAsync(hw->dev.stats, 0, sizeof(s), st));
     BK_HIP(hipStreamSynchronize(st));
   }
+  return 0;
+}
+
+int bk_workspace_sort_order(bk_workspace* ws, uint32_t* keys, uint32_t* sorted_keys, int32_t* order, int32_t n, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw || n < 0 || n > hw->dev.max_n) return fail("bk_workspace_sort_order: bad argument");
+  if (n == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (keys) BK_HIP(hipMemcpyAsync(keys, hw->dev.keys_raw, size_t(n) * 4, hipMemcpyDeviceToDevice, st));
+  if (sorted_keys) BK_HIP(hipMemcpyAsync(sorted_keys, hw->dev.sorted_keys, size_t(n) * 4, hipMemcpyDeviceToDevice, st));
+  if (order) BK_HIP(hipMemcpyAsync(order, hw->dev.sorted_vals, size_t(n) * 4, hipMemcpyDeviceToDevice, st));
   return 0;
 }
 

@@ -76,6 +76,15 @@ __device__ __forceinline__ bool resting(const bk_params& P, const bk_workspace* 
   return P.settling == 1 || (P.settling == 2 && ws->settle[d].phase != 0);
 }
 
+// Stable LSD radix sort of (key, value) pairs on the low key_bits bits (values
+// null: 0..n-1), with its scratch (keys_s, vals_s, hist, digit_total) taken from
+// `ws`, a host-side view whose scratch fields point to device memory of at least
+// n keys and values each, 256 * ceil(n / kRadixTile) histogram entries and 256
+// digit totals (bk_substep.hip).
+constexpr int kRadixTile = 2048;
+int bk_internal_radix_sort(const bk_workspace* ws, const uint32_t* keys_in, const int32_t* vals_in, uint32_t* keys_out,
+                           int32_t* vals_out, int n, int key_bits, hipStream_t st);
+
 // Enqueue the settling controller after a substep with input plastic
 // gradients Fp_in and output (v, C, Fp) (bk_settling.hip).
 int bk_internal_settling_launch(bk_workspace* ws, const bk_dough* doughs, int n_doughs, int max_dough_n, const float* v,

@@ -161,6 +161,11 @@ float* bk_workspace_vertices(bk_workspace* ws);         /* device float[V*3]: th
                                                            vertices its boundary faces use (the others are not updated) */
 /* Copies {tiles, active node blocks, deferred return maps} of the last substep; synchronises. */
 int bk_workspace_counts(bk_workspace* ws, int32_t* out3, void* stream);
+/* Device-to-device copies (any may be null) of the last substep's cell keys in particle order (dough's first block
+ * << 6 | its 4x4x4-block-major cell key, the key of physics/transfer.py:16-33 on the tiled grid), the keys in sorted
+ * order and the sort permutation (sorted position -> particle): a stable sort of the keys. n <= the workspace's
+ * particle capacity; enqueued on `stream`. */
+int bk_workspace_sort_order(bk_workspace* ws, uint32_t* keys, uint32_t* sorted_keys, int32_t* order, int32_t n, void* stream);
 /* Blocking copy of the flag block to host int32[8] after the work already on `stream`; and the reverse. */
 int bk_workspace_read_flags(bk_workspace* ws, int32_t* host8, void* stream);
 int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* stream);

@@ -75,26 +75,113 @@ def oracle_fields(sim):
     return out
 
 
-def field_scales(ref, dx):
-    """Magnitude floors for fields whose exact value can be ~0 while its terms are not.
+def field_scales(ref, orc):
+    """Physical magnitude floors: each field is compared at max(max|field|, floor), never at less.
 
-    C = (4/dx^2) sum_i w_i v_i (x_i - x_p)^T: its terms scale like 4 max|v| / dx, so a
-    rigidly moving body (C ~ 0) is compared at that scale.
+    ``ref`` holds the fields of the oracle simulator ``orc`` (its configs and step count set the floors).
+    A field whose exact value is ~0 while its terms are not (a freely falling body's stress, a rigid body's
+    affine velocity) would otherwise be compared at the size of its own rounding noise.  Floors:
+
+    * v, node_v: |g| t, the speed gravity gives in the elapsed time t.  node_v errors are weighted by the node's
+      mass over the largest node mass (``field_errors``' weights): p/m on a fringe node that holds 1e-9 of a
+      particle's mass divides rounding noise by that mass, and G2P gives such a node a proportionally small
+      weight, so it is its momentum error, not its velocity error, that reaches the particles;
+    * C = (4/dx^2) sum_i w_i v_i (x_i - x_p)^T: its terms scale like 4 |v| / dx;
+    * tau: the yield stress of the Ding dough (the stress at which it flows), or 1e-3 mu for neo-Hookean
+      (the stress of a strain of 1e-3), and the weight rho |g| L of a column of height L = 2 dx;
+    * pressure: the ambient pore pressure; moles: their largest value;
+    * residual: 1, the unit term of the return map's equation gamma + 1 - q/(1+d gamma)^2 - p/(1+m gamma)^2;
+    * porosity_residual, porosity: the initial porosity n0 (both are in porosity units);
+    * losses, exchanges: 1e-6 of the water the dough holds and of its heat content C T0 (both start at 0).
     """
+    sim, mat = orc.sim, orc.mat
+    t = max(orc.step_count, 1) * float(sim.time_step)
+    g = float(np.linalg.norm(np.asarray(sim.gravity, dtype=np.float64)))
     vmax = float(np.abs(np.asarray(ref["v"], dtype=np.float64)).max(initial=0.0))
-    return {"C": 4.0 * vmax / dx}
+    vscale = max(vmax, g * t)
+    floors = {"v": vscale, "node_v": vscale, "C": 4.0 * vscale / float(sim.grid_spacing)}
+    stress = float(orc.ding_cfg.yield_stress) if orc.ding_cfg is not None else 1e-3 * float(mat.shear_modulus)
+    floors["tau"] = max(stress, float(mat.density) * g * 2.0 * float(sim.grid_spacing))
+    if orc.ding_cfg is not None:
+        c = orc.ding_cfg
+        floors["pressure"] = float(c.gas_ambient_pressure_pa)
+        floors["porosity"] = floors["porosity_residual"] = max(float(c.porosity), 1e-3)
+        floors["residual"] = 1.0
+        floors["moles"] = float(np.abs(np.asarray(ref["moles"], dtype=np.float64)).max(initial=0.0))
+    if orc.tr is not None:
+        floors["losses"] = 1e-6 * float(np.abs(np.asarray(ref["W"], dtype=np.float64)).sum())
+        floors["exchanges"] = 1e-6 * float(np.sum(orc.capacities)) * float(orc.tr.initial_temperature_k)
+    return floors
 
 
-def field_errors(a, b, names=None, floors=None):
-    """Scale-relative max error per field: max|a - b| / max(max|b|, floor, tiny)."""
+def field_errors(a, b, names=None, floors=None, weights=None):
+    """Scale-relative max error per field: max_i w_i |a_i - b_i| / max(max|b|, floor, tiny), w_i = 1 unless
+    ``weights`` gives a per-row weight for the field (node_v: node mass / largest node mass of ``b``)."""
     errs = {}
     floors = floors or {}
+    if weights is None and "node_v" in b and "node_m" in b:
+        m = np.asarray(b["node_m"], dtype=np.float64).reshape(-1)
+        weights = {"node_v": m / max(m.max(initial=0.0), 1e-300)}
+    weights = weights or {}
     for k in (names or sorted(set(a) & set(b))):
-        x = np.asarray(a[k], dtype=np.float64).reshape(-1)
-        y = np.asarray(b[k], dtype=np.float64).reshape(-1)
+        x = np.asarray(a[k], dtype=np.float64)
+        y = np.asarray(b[k], dtype=np.float64)
         scale = max(np.abs(y).max(initial=0.0), floors.get(k, 0.0), 1e-30)
-        errs[k] = float(np.abs(x - y).max(initial=0.0) / scale)
+        diff = np.abs(x - y).reshape(len(y), -1) if y.ndim else np.abs(x - y).reshape(1, 1)
+        if k in weights:
+            diff = diff * weights[k].reshape(-1, 1)
+        errs[k] = float(diff.max(initial=0.0) / scale)
     return errs
+
+
+ROUNDING_CAP = 1e-2
+
+
+def judge_fields(e_gpu, e_ref, rtol=1e-5, factor=4.0, cap=ROUNDING_CAP):
+    """Per field: 'rtol' when err(gpu, o32) <= rtol; 'fp32' when it is within factor x err(o32, o64) (the
+    reference's own FP32 rounding) and that allowance is at most ``cap``; otherwise 'FAIL'.  So a field passes
+    either at 1e-5 of its physical scale or at a few FP32 roundings, and never at 1e-2 relative error or more.
+    (The FP32 rounding of the reference algorithm itself reaches ~1e-3 of the physical scale in the stiff
+    small sphere: 5 ulp of strain at bulk modulus 1.4 MPa is ~1 Pa of stress against the 811 Pa yield
+    stress, and its acoustic waves carry that into the velocities.)"""
+    out = {}
+    for k, e in e_gpu.items():
+        if e <= rtol:
+            out[k] = "rtol"
+        elif e <= min(factor * e_ref[k], cap):
+            out[k] = "fp32"
+        else:
+            out[k] = "FAIL"
+    return out
+
+
+def check_fields(tag, got, o32, o64, log=print, rtol=1e-5, factor=4.0, cap=ROUNDING_CAP):
+    """Fields ``got`` against the FP32 oracle simulator ``o32`` under ``judge_fields``, with the FP64 oracle
+    ``o64`` at the same substep giving the FP32 rounding allowance.  Logs each field's error, the oracle's own
+    FP32/FP64 gap and the branch it passed by; returns (errors, gaps)."""
+    r32, r64 = oracle_fields(o32), oracle_fields(o64)
+    floors = field_scales(r32, o32)
+    e_gpu = field_errors(got, r32, floors=floors)
+    e_ref = field_errors(r32, r64, floors=floors)
+    verdict = judge_fields(e_gpu, e_ref, rtol, factor, cap)
+    if log:
+        log(f"{tag}: worst {max(e_gpu.values()):.2e}; "
+            + ", ".join(f"{k} {e_gpu[k]:.1e}/{e_ref[k]:.1e} {verdict[k]}" for k in sorted(e_gpu) if e_gpu[k] > 1e-7))
+    bad = {k: (e_gpu[k], e_ref[k]) for k, v in verdict.items() if v == "FAIL"}
+    assert not bad, f"{tag}: fields outside tolerance (gpu/o32, o32/o64): {bad}"
+    return e_gpu, e_ref
+
+
+def check_same(tag, a, b, o32, o64, rtol=1e-5, factor=4.0, cap=ROUNDING_CAP):
+    """Two GPU runs of one scene (batch vs alone, tuned vs untuned, guided vs exact) agree under the same rule,
+    with the FP32/FP64 oracle gap of that scene at that substep as the rounding allowance."""
+    r32, r64 = oracle_fields(o32), oracle_fields(o64)
+    floors = field_scales(r32, o32)
+    e = field_errors(a, b, floors=floors)
+    e_ref = field_errors(r32, r64, floors=floors)
+    verdict = judge_fields(e, e_ref, rtol, factor, cap)
+    bad = {k: (e[k], e_ref[k]) for k, v in verdict.items() if v == "FAIL"}
+    assert not bad, f"{tag}: runs differ beyond tolerance: {bad}"
 
 
 class OraclePoint:
@@ -142,14 +229,15 @@ class OraclePoint:
         return n
 
 
-def three_way(spec, steps, rtol=1e-5, factor=4.0, check_every=1, log=print, solver=None):
+def three_way(spec, steps, rtol=1e-5, factor=4.0, check_every=1, log=print, solver=None, cap=ROUNDING_CAP):
     """Run the GPU path, the FP32 oracle and the FP64 oracle side by side.
 
-    After each checked substep every field must satisfy
-        err(gpu, oracle32) <= max(rtol, factor * err(oracle32, oracle64)),
-    i.e. the GPU agrees with the reference algorithm to 1e-5 (scale-relative),
-    or, where the field is rounding-dominated (stress of a nearly undeformed
-    body), to within a few times the reference's own FP32 rounding error.
+    After each checked substep every field must satisfy (``judge_fields``)
+        err(gpu, oracle32) <= rtol                                   ("rtol"), or
+        err(gpu, oracle32) <= min(factor * err(oracle32, oracle64), cap)   ("fp32"),
+    with errors scale-relative to max(max|field|, physical floor) (``field_scales``): the GPU agrees with
+    the reference algorithm to 1e-5 of the field's physical scale, or, where FP32 rounding of the
+    reference itself is larger, to within a few of those roundings and never worse than ``cap``.
     Returns the last errors (gpu vs oracle32, oracle32 vs oracle64).
     """
     gpu = gpu_from_spec(spec)
@@ -164,15 +252,8 @@ def three_way(spec, steps, rtol=1e-5, factor=4.0, check_every=1, log=print, solv
         o64.Step()
         if step % check_every and step != steps:
             continue
-        r32, r64 = oracle_fields(o32), oracle_fields(o64)
-        floors = field_scales(r32, o32.dx)
-        e_gpu = field_errors(gpu_fields(gpu), r32, floors=floors)
-        e_ref = field_errors(r32, r64, floors=floors)
-        bad = {k: (e_gpu[k], e_ref[k]) for k in e_gpu if e_gpu[k] > max(rtol, factor * e_ref[k])}
-        if log:
-            log(f"{spec.name} step {step}: worst gpu/o32 {max(e_gpu.values()):.2e}; "
-                + ", ".join(f"{k} {e_gpu[k]:.1e}/{e_ref[k]:.1e}" for k in sorted(e_gpu) if e_gpu[k] > 1e-7))
-        assert not bad, f"{spec.name} step {step}: fields outside tolerance (gpu/o32, o32/o64): {bad}"
+        e_gpu, e_ref = check_fields(f"{spec.name} step {step}", gpu_fields(gpu), o32, o64, log=log, rtol=rtol,
+                                    factor=factor, cap=cap)
     return e_gpu, e_ref
 
 
@@ -191,9 +272,16 @@ def oracle_on_gpu_state(spec, sim):
 
 
 def load_gpu_state(o, sim):
-    """Copy the GPU simulator's state, dense grid masses, flags and step count into the oracle ``o``."""
+    """Copy the GPU simulator's state, dense grid masses, flags and step count into the oracle ``o``.
+    float32 fields become the oracle's working precision (float64 for the FP64 oracle); float64 and int
+    fields are kept as they are."""
     s = sim.current_state
-    host = lambda t: t.detach().cpu().numpy().copy()  # noqa: E731
+    R = o.R
+
+    def host(t):
+        a = t.detach().cpu().numpy()
+        return np.ascontiguousarray(a.astype(R) if a.dtype == np.float32 else a.copy())
+
     o.state.x, o.state.v = host(s.particle_positions), host(s.particle_velocities)
     o.state.F, o.state.C = host(s.deformation_gradients), host(s.affine_velocities)
     if s.material_state is not None:

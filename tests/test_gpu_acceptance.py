@@ -1,0 +1,102 @@
+"""Whole-run acceptance of the plain method on the small sphere scene (the 8.8k-particle 64^3 workload).
+
+The GPU runs 1000 substeps (0.1 s of baking: the dough falls 4.9 cm and heats by ~5 mK on average).
+Against the committed oracle run (``tests/golden/sphere_small_1000.npz``, FP32 and FP64 oracles):
+
+* every 10th substep, particle volume sum V0 det F, capacity-weighted mean temperature, water mass and gas
+  content sum V0 N (reference experiments/recording.py:72,97-109, reduced on the GPU by ``bk_measure``) agree
+  with the FP32 oracle within 1e-4 relative; their increments over the run agree within 4x the FP32/FP64
+  gap of the increments or 1e-3 of the increment;
+* at substeps 1, 251, 501, 751 and 1000, one substep from the GPU's own state is checked field by field
+  against the FP32 and FP64 oracles started from that state (the per-substep criterion of the parity tests);
+* the fields of the fixture's 512 sampled particles after 1000 substeps are printed next to the FP32/FP64
+  oracle gap (FP32 rounding accumulates in the stiff dough's stress; positions, velocities and the transport
+  and gas fields are asserted under the parity rule).
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from tests.harness import ROUNDING_CAP, check_fields, gpu_fields, load_gpu_state, oracle_from_spec
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("needs a ROCm GPU", allow_module_level=True)
+
+from baking_amd import scenes  # noqa: E402
+from baking_amd.measure import measure  # noqa: E402
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "sphere_small_1000.npz"
+MAIN = ("particle_volume_m3", "temperature_mean_k", "water_mass_kg", "gas_total_mol")
+CHECKS = (0, 250, 500, 750, 999)
+
+
+def test_sphere_small_1000_substeps():
+    data = np.load(GOLDEN)
+    keys = list(data["keys"])
+    steps, every = int(data["steps"]), int(data["every"])
+    spec = scenes.sphere_small()
+    sim = scenes.build_simulator(spec)
+
+    def row():
+        m = measure(sim)
+        return [float(m[k][0]) for k in keys]
+
+    rows = [row()]
+    marks = sorted(set(CHECKS) | set(range(every, steps + 1, every)))
+    s = 0
+    while s < steps:
+        if s in CHECKS:
+            o32 = load_gpu_state(oracle_from_spec(spec, "f32"), sim)
+            o64 = load_gpu_state(oracle_from_spec(spec, "f64"), sim)
+            sim.Step()
+            o32.Step()
+            o64.Step()
+            check_fields(f"sphere_small substep {s + 1} from the GPU's state", gpu_fields(sim), o32, o64,
+                         log=lambda m: print(m, flush=True))
+            s += 1
+        else:
+            nxt = next(c for c in marks if c > s)
+            sim.Advance(nxt - s, check=False)
+            s = nxt
+        if s % every == 0:
+            rows.append(row())
+        if s % 100 == 0:
+            print(f"substep {s}", flush=True)
+    stats = sim.CheckState()
+    assert stats["step_count"] == steps
+    got = np.asarray(rows)
+    a32, a64 = data["aggregates_f32"], data["aggregates_f64"]
+    assert got.shape == a32.shape
+    for name in MAIN:
+        k = keys.index(name)
+        rel = np.abs(got[:, k] - a32[:, k]) / np.abs(a32[:, k])
+        rel64 = np.abs(got[:, k] - a64[:, k]) / np.abs(a64[:, k])
+        ref_rel = abs(a32[-1, k] - a64[-1, k]) / abs(a64[-1, k])
+        inc, inc32, inc64 = got[-1, k] - got[0, k], a32[-1, k] - a32[0, k], a64[-1, k] - a64[0, k]
+        inc_err, inc_gap = abs(inc - inc32), abs(inc32 - inc64)
+        print(f"{name}: start {a64[0, k]:.9e}  end GPU {got[-1, k]:.9e}  FP32 {a32[-1, k]:.9e}  FP64 {a64[-1, k]:.9e}; "
+              f"|GPU-FP32|/FP32 max over the run {rel.max():.2e}, at the end {rel[-1]:.2e}; |GPU-FP64|/FP64 {rel64[-1]:.2e}; "
+              f"|FP32-FP64|/FP64 {ref_rel:.2e}; increment GPU {inc:.6e} FP32 {inc32:.6e} FP64 {inc64:.6e} "
+              f"(GPU-FP32 {inc_err / abs(inc64):.2e}, FP32-FP64 {inc_gap / abs(inc64):.2e} of the increment)", flush=True)
+        assert rel.max() <= 1e-4, name
+        assert inc_err <= max(4.0 * inc_gap, 1e-3 * abs(inc64)), name
+
+    idx = data["sample"]
+    fields = {k: v[idx] for k, v in gpu_fields(sim).items()}
+    report = []
+    for name in ("x", "v", "F", "C", "Fe", "Fp", "tau", "porosity", "moles", "pressure", "T", "W", "s", "losses",
+                 "exchanges", "masses", "yield_stress", "viscosity", "density", "gelled"):
+        f32 = np.asarray(data[f"f32_{name}"], np.float64)
+        f64 = np.asarray(data[f"f64_{name}"], np.float64)
+        scale = max(np.abs(f64).max(), 1e-30)
+        e = np.abs(np.asarray(fields[name], np.float64) - f32).max() / scale
+        e_ref = np.abs(f32 - f64).max() / scale
+        report.append(f"{name} {e:.1e}/{e_ref:.1e}")
+        if name in ("x", "v", "F", "porosity", "moles", "pressure", "T", "W", "s", "masses", "density", "gelled"):
+            assert e <= max(1e-5, min(4.0 * e_ref, ROUNDING_CAP)), (name, e, e_ref)
+    print("sampled particles after 1000 substeps, GPU vs FP32 / FP32 vs FP64 (relative to max|field|): "
+          + ", ".join(report), flush=True)

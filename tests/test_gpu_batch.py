@@ -10,7 +10,7 @@ Error flags stay with their dough.
 import numpy as np
 import pytest
 
-from tests.harness import field_errors, field_scales, gpu_fields, oracle_fields, oracle_from_spec
+from tests.harness import check_fields, check_same, gpu_fields, oracle_from_spec
 
 pytestmark = pytest.mark.gpu
 
@@ -21,22 +21,9 @@ if not torch.cuda.is_available():
 from baking_amd import scenes  # noqa: E402
 from baking_amd.physics.batch import BatchSimulator  # noqa: E402
 
-RTOL = 1.0e-5
-FACTOR = 4.0
-
 
 def _small_batch(count=4, seed=5):
     return scenes.batched_doughs(count=count, seed=seed, half_extent=(0.02, 0.035), max_particles=6000)
-
-
-def _check(tag, got, r32, r64, dx, log):
-    floors = field_scales(r32, dx)
-    e_gpu = field_errors(got, r32, floors=floors)
-    e_ref = field_errors(r32, r64, floors=floors)
-    bad = {k: (e_gpu[k], e_ref[k]) for k in e_gpu if e_gpu[k] > max(RTOL, FACTOR * e_ref[k])}
-    log(f"{tag}: worst {max(e_gpu.values()):.2e}")
-    assert not bad, f"{tag}: fields outside tolerance (gpu/o32, o32/o64): {bad}"
-    return e_ref
 
 
 @pytest.mark.parametrize("use_graphs", [True, False])
@@ -55,13 +42,10 @@ def test_batch_matches_each_dough_alone(use_graphs):
         for _ in range(steps):
             o32.Step()
             o64.Step()
-        r32, r64 = oracle_fields(o32), oracle_fields(o64)
         got = gpu_fields(batch.doughs[d])
-        e_ref = _check(f"{spec.name} batch vs oracle", got, r32, r64, o32.dx, print)
-        _check(f"{spec.name} alone vs oracle", gpu_fields(alone), r32, r64, o32.dx, print)
-        e_ba = field_errors(got, gpu_fields(alone), floors=field_scales(r32, o32.dx))
-        bad = {k: (e_ba[k], e_ref[k]) for k in e_ba if e_ba[k] > max(RTOL, FACTOR * e_ref[k])}
-        assert not bad, f"{spec.name}: batch differs from the dough alone: {bad}"
+        check_fields(f"{spec.name} batch vs oracle", got, o32, o64)
+        check_fields(f"{spec.name} alone vs oracle", gpu_fields(alone), o32, o64)
+        check_same(f"{spec.name}: batch vs the dough alone", got, gpu_fields(alone), o32, o64)
         assert batch.doughs[d].step_count == steps
     assert batch.ReadFlags().max() == 0
 
@@ -78,12 +62,7 @@ def test_batch_tuning_does_not_change_results():
         for _ in range(5):
             o32.Step()
             o64.Step()
-        r32, r64 = oracle_fields(o32), oracle_fields(o64)
-        floors = field_scales(r32, o32.dx)
-        e_ref = field_errors(r32, r64, floors=floors)
-        e = field_errors(runs[1][d], runs[0][d], floors=floors)
-        bad = {k: (e[k], e_ref[k]) for k in e if e[k] > max(RTOL, FACTOR * e_ref[k])}
-        assert not bad, f"{spec.name}: tuning changed results: {bad}"
+        check_same(f"{spec.name}: tuning changed results", runs[1][d], runs[0][d], o32, o64)
 
 
 def test_batch_flags_stay_with_their_dough():

@@ -7,7 +7,7 @@ and numpy's stable argsort; per-substep particle and grid fields agree within
 import numpy as np
 import pytest
 
-from tests.harness import O, three_way
+from tests.harness import O, check_fields, gpu_fields, oracle_from_spec, three_way
 
 pytestmark = pytest.mark.gpu
 
@@ -226,3 +226,101 @@ def test_far_off_and_nonfinite_particles_are_flagged_not_indexed():
     assert flags.cpu().numpy()[1] == 1
     assert keys.cpu().numpy()[:3].tolist() == [0, 0, 0]
     del ctypes
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+@pytest.mark.parametrize("side", [-1, 1])
+def test_cube_stops_at_each_wall(axis, side):
+    """The fused substep's wall projection: an elastic cube fired at 1 m/s into each of the six walls (reference
+    physics/grid.py:32-37, tests/test_sphere.py:44-47).  Every occupied node of the wall's layers moves inward or
+    tangentially, the cube rebounds without leaving the valid stencil region, and every field agrees with the
+    FP32/FP64 oracles after each 300 substeps."""
+    from tests.test_oracle_walls import band_violation, wall_scene
+    spec = wall_scene(axis, side)
+    sim_cfg = spec.simulation
+    res, bc = sim_cfg.grid_resolution, sim_cfg.boundary_cells
+    gpu = scenes.build_simulator(spec)
+    o32, o64 = oracle_from_spec(spec, "f32"), oracle_from_spec(spec, "f64")
+    mass = o32.m0.astype(np.float64)
+    p0 = float(np.dot(mass, o32.state.v[:, axis])) * side
+    for step in range(1, spec.steps + 1):
+        gpu.Step()
+        o32.Step()
+        o64.Step()
+        if step % 300 == 0:
+            check_fields(f"{spec.name} substep {step}", gpu_fields(gpu), o32, o64)
+            nm = gpu.grid_state.node_masses.cpu().numpy()
+            nv = gpu.grid_state.node_velocities.cpu().numpy()
+            assert band_violation(nm, nv, res, bc, axis, side) == 0.0
+    v = gpu.current_state.particle_velocities.cpu().numpy().astype(np.float64)
+    assert float(np.dot(mass, v[:, axis])) * side < -0.1 * p0
+    assert gpu.CheckState()["minimum_j"] > 0
+
+
+def _substep_order(sim, n):
+    """The last substep's keys (particle order), sorted keys and permutation, read from its workspace."""
+    lib = _native.load()
+    keys = torch.empty(n, dtype=torch.int32, device="cuda")
+    sorted_keys = torch.empty_like(keys)
+    order = torch.empty_like(keys)
+    _native.check(lib.bk_workspace_sort_order(sim._workspace, _native.ptr(keys), _native.ptr(sorted_keys), _native.ptr(order), n,
+                                              _native.stream_ptr()), "bk_workspace_sort_order")
+    torch.cuda.synchronize()
+    return (keys.cpu().numpy().view(np.uint32), sorted_keys.cpu().numpy().view(np.uint32), order.cpu().numpy())
+
+
+def _expected_keys(spec, x, block_offset=0):
+    sim = spec.simulation
+    keys, flags = O.cell_keys(x, sim.grid_origin, sim.grid_spacing, sim.grid_resolution, tiled=True)
+    assert flags[1] == 0
+    return keys + np.uint32(block_offset << 6)
+
+
+def _check_order(keys, sorted_keys, order, want_keys):
+    np.testing.assert_array_equal(keys, want_keys)
+    np.testing.assert_array_equal(order, np.argsort(want_keys, kind="stable"))
+    np.testing.assert_array_equal(sorted_keys, want_keys[order])
+
+
+def test_substep_sort_is_the_stable_sort_of_the_keys():
+    """The fused substep's particle order (reference key physics/transfer.py:16-33 on the tiled grid) equals
+    numpy's stable argsort of the oracle's keys, bit for bit, on the small sphere after 1 and 5 substeps."""
+    spec = scenes.sphere_small()
+    sim = scenes.build_simulator(spec)
+    n = sim.particle_data.particle_count
+    for steps in (1, 4):
+        x = sim.current_state.particle_positions.cpu().numpy().copy()
+        sim._launch_substeps(1)
+        _check_order(*_substep_order(sim, n), _expected_keys(spec, x))
+        if steps > 1:
+            sim._launch_substeps(steps - 1)
+
+
+def test_batched_substep_sort_is_the_stable_sort_of_the_keys():
+    from baking_amd.physics.batch import BatchSimulator
+    specs = scenes.batched_doughs(count=3, seed=8, half_extent=(0.02, 0.035), max_particles=6000)
+    batch = BatchSimulator([scenes.build_simulator(s) for s in specs])
+    batch.Advance(2)
+    xs = [d.current_state.particle_positions.cpu().numpy().copy() for d in batch.doughs]
+    batch.Advance(1)
+    want = np.concatenate([_expected_keys(s, x, int(batch.block_offsets[d])) for d, (s, x) in enumerate(zip(specs, xs))])
+    _check_order(*_substep_order(batch, batch.particle_count), want)
+
+
+def test_crowded_block_sort_is_the_stable_sort_of_the_keys():
+    """More than 1024 particles in one 4x4x4 block (the sort's LDS capacity): the global-memory ranking path."""
+    from baking_amd.config import DingConfig, MaterialConfig, SimulationConfig, TransportConfig
+    from baking_amd.geometry.mesh_loader import VoxelTetMesh
+    from baking_amd.geometry.mesh_validation import ValidateTetMesh
+    from baking_amd.geometry.particle_sampling import SampleParticles
+    verts, tets = VoxelTetMesh((1, 1, 1), 0.06, (0.52, 0.52, 0.52))
+    mesh = ValidateTetMesh(verts, tets)
+    spec = scenes.SceneSpec("crowded", mesh, SampleParticles(mesh, 1000.0, 400), SimulationConfig(gravity=(0.0, -9.81, 0.0)),
+                            MaterialConfig(), DingConfig(), TransportConfig(), steps=2)
+    sim = scenes.build_simulator(spec)
+    n = sim.particle_data.particle_count
+    x = sim.current_state.particle_positions.cpu().numpy().copy()
+    sim._launch_substeps(1)
+    want = _expected_keys(spec, x)
+    assert np.bincount(want >> 6).max() > 1024
+    _check_order(*_substep_order(sim, n), want)

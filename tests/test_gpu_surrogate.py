@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import mlp as M
-from tests.harness import field_errors, field_scales, gpu_fields, oracle_fields, oracle_from_spec
+from tests.harness import check_fields, check_same, gpu_fields, oracle_from_spec
 
 pytestmark = pytest.mark.gpu
 
@@ -162,16 +162,9 @@ def test_guided_return_map_matches_exact_with_fewer_evaluations():
     for _ in range(steps):
         o32.Step()
         o64.Step()
-    r32, r64 = oracle_fields(o32), oracle_fields(o64)
-    floors = field_scales(r32, o32.dx)
-    e_ref = field_errors(r32, r64, floors=floors)
     for name in ("exact", "guided"):
-        e = field_errors(runs[name][0], r32, floors=floors)
-        bad = {k: (e[k], e_ref[k]) for k in e if e[k] > max(1e-5, 4.0 * e_ref[k])}
-        assert not bad, f"{name} run outside tolerance against the oracle: {bad}"
-    e = field_errors(runs["guided"][0], runs["exact"][0], floors=floors)
-    bad = {k: (e[k], e_ref[k]) for k in e if e[k] > max(1e-5, 4.0 * e_ref[k])}
-    assert not bad, f"guided run differs from the exact run: {bad}"
+        check_fields(f"{name} run against the oracle", runs[name][0], o32, o64)
+    check_same("guided run against the exact run", runs["guided"][0], runs["exact"][0], o32, o64)
 
 
 def test_packaged_network_gain_on_the_loaf():
