@@ -21,7 +21,7 @@ Gradients with respect to the initial positions, velocities, affine
 velocities and deformation gradients, the Ding elastic gradients,
 porosities, Kirchhoff stresses and gas moles, the temperatures, water
 masses and loss/exchange accumulators, and the Ding viscosity, gas
-production and heat-transfer parameters.  Resting substeps before baking
+production and heat-transfer parameters and the oven temperature.  Resting substeps before baking
 (``physics/mpm.py:57-109``) are differentiated with the phase the forward's
 settling controller gave them; the stopping decision itself has no
 derivative, as in the reference (``physics/quasistatic.py:1-6``).  Not
@@ -34,6 +34,7 @@ covered: batched doughs.
 from __future__ import annotations
 
 import ctypes
+from dataclasses import replace
 
 import torch
 
@@ -133,7 +134,7 @@ def _run_forward(sim, count):
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, sim, steps, every, fields, visc, prod, heat, *inputs):
+    def forward(ctx, sim, steps, every, fields, visc, prod, heat, oven, *inputs):
         lib = _native.load()
         state = state_field_tensors(sim.current_state)
         for f, t in zip(fields, inputs):
@@ -145,6 +146,11 @@ class _Rollout(torch.autograd.Function):
             sim.material.production_parameter.copy_(prod.detach().reshape(1).to(torch.float32))
         if _transport(sim):
             sim.transport.heat_transfer_parameter.copy_(heat.detach().reshape(1).to(torch.float32))
+            if oven.numel():
+                # the oven temperature is a plain kernel argument (physics/thermal_water.py:100, :325), read from
+                # the live transport config like the reference's
+                sim.transport.config = replace(sim.transport.config, oven_temperature_k=float(oven.detach().reshape(-1)[0]))
+                sim._params = sim._make_params()
         flags_before = sim._read_flags()
         first, resting = sim.step_count, sim.IsSettling
         checkpoints = []
@@ -164,7 +170,7 @@ class _Rollout(torch.autograd.Function):
             ctx.rest = 0
         ctx.shapes = [t.shape for t in inputs]
         ctx.dtypes = [t.dtype for t in inputs]
-        ctx.params = [(visc.shape, visc.dtype), (prod.shape, prod.dtype), (heat.shape, heat.dtype)]
+        ctx.params = [(visc.shape, visc.dtype), (prod.shape, prod.dtype), (heat.shape, heat.dtype), (oven.shape, oven.dtype)]
         ctx.lib = lib
         final = state_field_tensors(sim.current_state)
         return tuple(final[f].clone() for f in fields)
@@ -184,7 +190,7 @@ class _Rollout(torch.autograd.Function):
         if _transport(sim):
             for f, t in _binding_reverse(sim).items():
                 setattr(bk_adj, f, t.data_ptr())
-        param = torch.zeros(3, dtype=torch.float64, device=dev)
+        param = torch.zeros(4, dtype=torch.float64, device=dev)
         scratch = torch.zeros(int(lib.bk_adjoint_scratch_size(ctypes.byref(sim._params))), dtype=torch.float64, device=dev)
         # the simulator is left where the forward ended
         keep = (sim.current_state, sim.next_state, sim.step_count, _clone_state(sim.current_state), sim._read_flags(),
@@ -226,7 +232,7 @@ class _Rollout(torch.autograd.Function):
 
 
 def rollout(sim, steps, inputs=None, viscosity_parameter=None, production_parameter=None, heat_transfer_parameter=None,
-            checkpoint_every=16):
+            oven_temperature=None, checkpoint_every=16):
     """Run ``steps`` substeps of ``sim`` from ``inputs`` and return the final state as differentiable tensors.
 
     ``inputs`` maps state fields (``x, v, C, F``; for the Ding model also ``Fe, porosity, tau``, ``moles``
@@ -234,7 +240,8 @@ def rollout(sim, steps, inputs=None, viscosity_parameter=None, production_parame
     N x 3, N x 3 x 3, ...); fields left out start from the simulator's current state.
     ``viscosity_parameter`` (Ding model), ``production_parameter`` (pore gas) and
     ``heat_transfer_parameter`` (transport), one-element tensors, replace the material's and the
-    transport's parameters.  Returns {field: tensor} for every field; the simulator is left at the final
+    transport's parameters; ``oven_temperature`` (transport), a one-element tensor, replaces the transport
+    config's ``oven_temperature_k`` (the tray keeps its own temperature).  Returns {field: tensor} for every field; the simulator is left at the final
     state, as after ``Advance(steps)``.
     """
     if steps < 1:
@@ -275,5 +282,8 @@ def rollout(sim, steps, inputs=None, viscosity_parameter=None, production_parame
         if heat_transfer_parameter is not None:
             raise ValueError("rollout: heat_transfer_parameter needs transport")
         heat = torch.zeros(0, device=dev)
-    outs = _Rollout.apply(sim, int(steps), int(checkpoint_every), fields, visc, prod, heat, *tensors)
+    if oven_temperature is not None and not _transport(sim):
+        raise ValueError("rollout: oven_temperature needs transport")
+    oven = oven_temperature if oven_temperature is not None else torch.zeros(0, device=dev)
+    outs = _Rollout.apply(sim, int(steps), int(checkpoint_every), fields, visc, prod, heat, oven, *tensors)
     return dict(zip(fields, outs))

@@ -470,7 +470,8 @@ struct AdjArgs {
   double* gm;  // grid adjoints (tiled nodes): mass, momentum (3), velocity (3)
   double* gp;
   double* gv;
-  double* param_grad;  // [0] viscosity parameter, [1] gas production parameter, [2] heat-transfer parameter
+  double* param_grad;  // [0] viscosity parameter, [1] gas production parameter, [2] heat-transfer parameter,
+                       // [3] oven temperature
   int n, nb;
   // transport (when on): forward scene and workspace data
   const double* capacities;
@@ -1103,9 +1104,10 @@ __global__ __launch_bounds__(256) void k_adj_diffuse_b(AdjArgs a, bk_workspace* 
 // Particle p: the thermal P2G's adjoint (thermal_water.py:95-137) with the
 // Robin terms' (108-120) -- also those of B1's loss and exchange accumulators,
 // which use the same inputs.  Adds to adj.T / adj.W / adj.x / adj.F, sets the
-// adjoints of p's exposed and contact areas, returns the heat-transfer
-// parameter's gradient.
-__device__ inline double adj_pass_a_particle(const AdjArgs& a, int p) {
+// adjoints of p's exposed and contact areas, returns the gradients of the
+// heat-transfer parameter h and of the oven temperature (heat = h S_exp
+// (T_oven - T) + ...: d heat / d T_oven = h S_exp).
+__device__ inline double2 adj_pass_a_particle(const AdjArgs& a, int p) {
   const bk_params& P = a.P;
   const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
   const double T = a.in.T[p], W = a.in.W[p], cap = a.capacities[p];
@@ -1160,6 +1162,7 @@ __device__ inline double adj_pass_a_particle(const AdjArgs& a, int p) {
   const double gSexp = h * (double(P.T_oven) - T) * gheat + beta * T * sat * gloss;
   const double gScon = double(P.tray_h) * (double(P.T_tray) - T) * gheat;
   const double gh = Sexp * (double(P.T_oven) - T) * gheat;
+  const double goven = h * Sexp * gheat;
   gW += gsat / (rho_w * vol);
   gvol -= gsat * sat / vol;
   a.adj.T[p] += gT;
@@ -1177,15 +1180,20 @@ __device__ inline double adj_pass_a_particle(const AdjArgs& a, int p) {
     a.gSexp[slot] = gSexp;
     a.gScon[slot] = gScon;
   }
-  return P.settling ? 0.0 : gh;
+  return P.settling ? make_double2(0.0, 0.0) : make_double2(gh, goven);
 }
 
 __global__ __launch_bounds__(128) void k_adj_pass_a(AdjArgs a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  double gh = p < a.n ? adj_pass_a_particle(a, p) : 0.0;
+  const double2 g = p < a.n ? adj_pass_a_particle(a, p) : make_double2(0.0, 0.0);
+  double gh = g.x, go = g.y;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) gh += __shfl_xor(gh, off, 64);
+  for (int off = 32; off > 0; off >>= 1) {
+    gh += __shfl_xor(gh, off, 64);
+    go += __shfl_xor(go, off, 64);
+  }
   if (__lane_id() == 0 && gh != 0.0) unsafeAtomicAdd(a.param_grad + 2, gh);
+  if (__lane_id() == 0 && go != 0.0) unsafeAtomicAdd(a.param_grad + 3, go);
 }
 
 // Per owner slot: the boundary faces' areas (thermal_water.py:77-92) in reverse,

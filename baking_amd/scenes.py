@@ -151,6 +151,52 @@ def sphere_small(seed=0, mesh_subdivisions=6):
                                 parameters="examples/oven_baking.txt material/ding/transport; n0 = 0.05, T0 = 298.15 K"))
 
 
+def sphere_gradient(target_seed=3):
+    """The differentiable workload: the small sphere for 256 substeps, checkpointed every 16.
+
+    Loss L = (y_c - y*)^2 + (s_mean - s*)^2 of the final state, with y_c the mass-weighted centroid height
+    (mass = dry mass + water) and s_mean the mean saturation W / (rho_w V0 det F); the targets are the
+    initial centroid height and saturation moved by U(-0.01, 0.01) each from ``default_rng(target_seed)``.
+    The parameters are each particle's initial porosity theta (its gas content following as
+    p0 theta / (R T0)) and the oven temperature.  ``notes`` holds the targets and sizes."""
+    spec = sphere_small()
+    spec.name = "sphere_gradient"
+    spec.steps = 256
+    rng = np.random.default_rng(target_seed)
+    x = spec.samples.particle_positions
+    t = spec.transport
+    V0 = np.asarray(spec.samples.reference_volumes, np.float64)
+    dry = (spec.material.density - t.water_density * t.initial_saturation) * V0
+    m = dry + t.water_density * t.initial_saturation * V0
+    y0 = float(np.dot(m, x[:, 1]) / m.sum())
+    spec.notes.update(checkpoint_every=16, centroid_target=y0 + float(rng.uniform(-0.01, 0.01)),
+                      saturation_target=float(t.initial_saturation) + float(rng.uniform(-0.01, 0.01)))
+    return spec
+
+
+def gradient_loss(spec, positions, water, gradients, xp=np):
+    """L of ``sphere_gradient`` from final positions (N x 3), water masses (N) and deformation gradients
+    (N x 3 x 3); ``xp`` is numpy or torch (then differentiable; float64 inside)."""
+    t = spec.transport
+    V0 = np.asarray(spec.samples.reference_volumes, np.float64)
+    dry_np = (spec.material.density - t.water_density * t.initial_saturation) * V0
+    if xp is np:
+        W = np.asarray(water, np.float64)
+        y = np.asarray(positions, np.float64)[:, 1]
+        J = np.linalg.det(np.asarray(gradients, np.float64))
+        dry, vol = dry_np, V0
+    else:
+        W = water.double()
+        y = positions[:, 1].double()
+        J = xp.linalg.det(gradients.double())
+        dry = xp.as_tensor(dry_np, device=W.device)
+        vol = xp.as_tensor(V0, device=W.device)
+    m = dry + W
+    yc = (m * y).sum() / m.sum()
+    sat = (W / (float(t.water_density) * vol * J)).mean()
+    return (yc - spec.notes["centroid_target"]) ** 2 + (sat - spec.notes["saturation_target"]) ** 2
+
+
 def gaussian_random_field(shape, spacing, correlation_length, rng):
     """Unit-variance periodic Gaussian random field: white noise filtered by exp(-k^2 l^2 / 4) in Fourier space."""
     noise = rng.standard_normal(shape)

@@ -430,7 +430,9 @@ typedef struct bk_adjoint {
  * Out: adj = the adjoint of `in`; param_grad[0] += the gradient of the Ding
  * viscosity parameter (scene.visc_param), [1] += the gas production
  * parameter's (scene.prod_param), [2] += the heat-transfer parameter's
- * (scene.heat_param).  scratch: bk_adjoint_scratch_size(params) doubles, all
+ * (scene.heat_param), [3] += the oven temperature's (params.T_oven, the
+ * oven_temperature of physics/thermal_water.py:100, passed at :325); param_grad
+ * holds 4 doubles.  scratch: bk_adjoint_scratch_size(params) doubles, all
  * zero before the first call; each call leaves it ready for the next. */
 int64_t bk_adjoint_scratch_size(const bk_params* params);
 int bk_adjoint_substep(const bk_params* params, const bk_scene* scene, const bk_state* in, const bk_state* out, const bk_grid* grid,

@@ -82,10 +82,13 @@ def field_scales(ref, orc):
     A field whose exact value is ~0 while its terms are not (a freely falling body's stress, a rigid body's
     affine velocity) would otherwise be compared at the size of its own rounding noise.  Floors:
 
-    * v, node_v: |g| t, the speed gravity gives in the elapsed time t.  node_v errors are weighted by the node's
-      mass over the largest node mass (``field_errors``' weights): p/m on a fringe node that holds 1e-9 of a
-      particle's mass divides rounding noise by that mass, and G2P gives such a node a proportionally small
-      weight, so it is its momentum error, not its velocity error, that reaches the particles;
+    * v, node_v: |g| t, the speed gravity gives in the elapsed time t, and tau_floor / (rho c), the particle
+      speed of an elastic wave carrying the stress floor below (c = sqrt((lambda + 2 mu) / rho)): a velocity
+      error well below it implies a strain-rate, and so a stress, error well below the stress floor.  node_v
+      errors are weighted by the node's mass over the largest node mass (``field_errors``' weights): p/m on a
+      fringe node that holds 1e-9 of a particle's mass divides rounding noise by that mass, and G2P gives such
+      a node a proportionally small weight, so it is its momentum error, not its velocity error, that reaches
+      the particles;
     * C = (4/dx^2) sum_i w_i v_i (x_i - x_p)^T: its terms scale like 4 |v| / dx;
     * tau: the yield stress of the Ding dough (the stress at which it flows), or 1e-3 mu for neo-Hookean
       (the stress of a strain of 1e-3), and the weight rho |g| L of a column of height L = 2 dx;
@@ -97,11 +100,12 @@ def field_scales(ref, orc):
     sim, mat = orc.sim, orc.mat
     t = max(orc.step_count, 1) * float(sim.time_step)
     g = float(np.linalg.norm(np.asarray(sim.gravity, dtype=np.float64)))
-    vmax = float(np.abs(np.asarray(ref["v"], dtype=np.float64)).max(initial=0.0))
-    vscale = max(vmax, g * t)
-    floors = {"v": vscale, "node_v": vscale, "C": 4.0 * vscale / float(sim.grid_spacing)}
     stress = float(orc.ding_cfg.yield_stress) if orc.ding_cfg is not None else 1e-3 * float(mat.shear_modulus)
-    floors["tau"] = max(stress, float(mat.density) * g * 2.0 * float(sim.grid_spacing))
+    stress = max(stress, float(mat.density) * g * 2.0 * float(sim.grid_spacing))
+    acoustic = stress / (float(mat.density) * float(mat.wave_speed))
+    vmax = float(np.abs(np.asarray(ref["v"], dtype=np.float64)).max(initial=0.0))
+    vscale = max(vmax, g * t, acoustic)
+    floors = {"v": vscale, "node_v": vscale, "C": 4.0 * vscale / float(sim.grid_spacing), "tau": stress}
     if orc.ding_cfg is not None:
         c = orc.ding_cfg
         floors["pressure"] = float(c.gas_ambient_pressure_pa)

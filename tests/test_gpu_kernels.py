@@ -299,6 +299,10 @@ def p2g_grid(scene):
 
 
 def test_UpdateGrid(scene, p2g_grid):
+    """The GPU contracts p/m + dt g into one fused multiply-add, the oracle (built with contraction off) rounds
+    twice, so nodes where p/m nearly cancels dt g differ in the last bits of the small sum: held to the FP64
+    rule like every other output.  The wall projection's zeroed components must agree exactly where the
+    FP64 free velocity is clearly outward."""
     sim = scene["spec"].simulation
     nm, npm = p2g_grid
     G = len(nm)
@@ -306,8 +310,14 @@ def test_UpdateGrid(scene, p2g_grid):
         # bc = 20 puts every wall band through the dough, so the projection acts on occupied nodes of all six walls
         res = _run(G_.UpdateGrid, "UpdateGrid", G, [nm, npm, np.zeros((G, 3), np.float32), res_, bc,
                                                    V3(sim.gravity), float(sim.time_step)], outs=(2,), dim64=True)
-        g, a32 = res[2][0], res[2][1]
-        np.testing.assert_array_equal(g, a32, err_msg=f"UpdateGrid bc={bc}: FP32 division and projection are exact")
+        _check(f"UpdateGrid bc={bc}", res, ("node velocities",))
+        g, a64 = res[2][0], res[2][2]
+        zeroed = (a64 == 0.0) & (nm[:, None] > 0)
+        m = nm.astype(np.float64)[:, None]
+        free = np.where(m > 0, npm.astype(np.float64) / np.where(m > 0, m, 1.0), 0.0) + float(sim.time_step) * np.asarray(sim.gravity)
+        clear = zeroed & (np.abs(free) > 1e-6 * np.abs(free).max())
+        assert bc == int(sim.boundary_cells) or clear.sum() > 0  # the deep bands cut through the dough
+        assert np.all(g[clear] == 0.0), f"bc={bc}: a clearly outward component was not removed"
 
 
 def test_GridToParticle(scene, p2g_grid):

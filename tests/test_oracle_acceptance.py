@@ -83,3 +83,33 @@ def test_sphere_fixture_is_the_oracle_run():
     a32, a64 = data["aggregates_f32"], data["aggregates_f64"]
     main = [gen.KEYS.index(k) for k in ("particle_volume_m3", "temperature_mean_k", "water_mass_kg", "gas_total_mol")]
     np.testing.assert_allclose(a32[:, main], a64[:, main], rtol=1e-6)
+
+
+def test_gradient_fixture_loss_is_the_oracle_run():
+    """The FP64 loss recorded in ``tests/golden/sphere_gradient_fd.npz`` re-derived (one 256-substep run), and the
+    loss function itself: numpy and torch forms agree, and it vanishes at its targets."""
+    import sys
+    import torch
+    from baking_amd import scenes
+    sys.path.insert(0, str(GOLDEN))
+    import make_sphere_gradient_fd as gen
+    data = np.load(GOLDEN / "sphere_gradient_fd.npz")
+    np.testing.assert_allclose(gen.loss_of(("base", -1, 0.0, "f64")), float(data["loss_f64"]), rtol=1e-12)
+    spec = scenes.sphere_gradient()
+    rng = np.random.default_rng(0)
+    n = len(spec.samples.particle_positions)
+    x = spec.samples.particle_positions + rng.normal(0, 1e-3, (n, 3))
+    W = np.full(n, spec.transport.water_density * spec.transport.initial_saturation * spec.samples.reference_volumes[0])
+    F = np.eye(3) + rng.normal(0, 1e-2, (n, 3, 3))
+    a = scenes.gradient_loss(spec, x, W, F)
+    b = scenes.gradient_loss(spec, torch.tensor(x), torch.tensor(W), torch.tensor(F), xp=torch)
+    np.testing.assert_allclose(float(b), a, rtol=1e-12)
+    # at the targets: a rigid shift of the centroid onto y* and W scaled onto s* make L = 0
+    t = spec.transport
+    V0 = np.asarray(spec.samples.reference_volumes)
+    dry = (spec.material.density - t.water_density * t.initial_saturation) * V0
+    W1 = t.water_density * spec.notes["saturation_target"] * V0
+    m = dry + W1
+    x1 = spec.samples.particle_positions.copy()
+    x1[:, 1] += spec.notes["centroid_target"] - np.dot(m, x1[:, 1]) / m.sum()
+    assert scenes.gradient_loss(spec, x1, W1, np.tile(np.eye(3), (n, 1, 1))) < 1e-28
