@@ -117,6 +117,9 @@ SIGNATURES = {
     "bk_workspace_write_flags_all": ([P, P, I, P], c_int),
     "bk_cell_keys": ([P, I, P, F, I, I, P, P, P], c_int),
     "bk_sort_pairs": ([P, P, P, P, P, I, I, P], c_int),
+    "bk_sort_pairs_noalloc": ([P, P, P, P, I, I, P], c_int),
+    "bk_sort_pairs_noalloc_capacity": ([], c_int32),
+    "bk_noalloc_last_error": ([], c_char_p),
     "bk_substep": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P, P], c_int),
     "bk_substep_graph_create": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P],
                                 c_void_p),

@@ -259,6 +259,16 @@ int bk_cell_keys(const float* x, int32_t n, const float* origin3_host, float dx,
  * = NULL means 0..n-1.  Outputs go to keys_out/values_out. */
 int bk_sort_pairs(bk_workspace* ws, const uint32_t* keys_in, const int32_t* values_in, uint32_t* keys_out, int32_t* values_out,
                   int32_t n, int32_t key_bits, void* stream);
+/* No counterpart in the reference.  bk_sort_pairs without a workspace, for callers that cannot allocate (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc): the same sort and the same results, with its
+ * scratch in a static device array (16.5 MiB per GPU, held until the process ends) instead of a workspace.  It
+ * calls no allocating routine.  At most bk_sort_pairs_noalloc_capacity() = 2^20 pairs; more return -1 with the
+ * reason in bk_noalloc_last_error().  Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (concurrent streams or threads) race and must be serialised by the caller. */
+int bk_sort_pairs_noalloc(const uint32_t* keys_in, const int32_t* values_in, uint32_t* keys_out, int32_t* values_out, int32_t n,
+                          int32_t key_bits, void* stream);
+int32_t bk_sort_pairs_noalloc_capacity(void);
+const char* bk_noalloc_last_error(void);
 
 /* ---------------------------------------------------------------- fused substep
  * One MpmSimulator.Step (physics/mpm.py:163-225) in forward mode: transport
