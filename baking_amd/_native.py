@@ -115,6 +115,8 @@ SIGNATURES = {
     "bk_workspace_write_flags": ([P, P, P], c_int),
     "bk_workspace_read_flags_all": ([P, P, I, P], c_int),
     "bk_workspace_write_flags_all": ([P, P, I, P], c_int),
+    "bk_workspace_set_freezing": ([P, I], c_int),
+    "bk_workspace_freeze_modes": ([P, P, I, I, P], c_int),
     "bk_cell_keys": ([P, I, P, F, I, I, P, P, P], c_int),
     "bk_sort_pairs": ([P, P, P, P, P, I, I, P], c_int),
     "bk_sort_pairs_noalloc": ([P, P, P, P, I, I, P], c_int),

@@ -157,14 +157,21 @@ __global__ void k_cell_keys(const float* __restrict__ x, int n, float ox, float 
 // Keys and per-block ranks of particles (one per lane, p).  Particle 0 resets
 // the substep's list counters and advances its stamp; the active-block count
 // (counters[1]) is reset by k_block_scan_partials, after the clear that reads it.
+constexpr uint32_t kFrozenKey = 0xFFFFFFFFu;  // the key of a particle of a stopped dough: not sorted, no tile
+
 __device__ __forceinline__ void batch_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of,
                                            const float* __restrict__ x, int n, uint32_t* __restrict__ keys,
                                            int32_t* __restrict__ ranks, bk_workspace* ws, int p) {
   if (p == 0) { ws->counters[0] = 0; ws->counters[2] = 0; ws->counters[3] = 0; ws->counters[6] += 1; }
   const bool valid = p < n;
+  bool live = valid;  // a stopped dough's particles get no key, so no tile, node block or grid work
   uint32_t key = 0u;
   if (valid) {
     per_dough(dough_of_particle(dough_of, p), [&](int d) {
+      if (frozen_mode(ws, d)) {
+        live = false;
+        return;
+      }
       const bk_params& P = doughs[d].params;
       const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
       float g[3];
@@ -181,12 +188,12 @@ __device__ __forceinline__ void batch_keys(const bk_dough* __restrict__ doughs, 
   // one instruction, so their returns are awaited once
   const unsigned lane = __lane_id();
   const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  unsigned long long pending = __ballot(valid);
+  unsigned long long pending = __ballot(live);
   int leader = 0, offset = 0, size = 0;
   while (pending) {
     const int first = __ffsll((long long)pending) - 1;
     const uint32_t blk = __shfl(key >> 6, first, 64);
-    const unsigned long long same = __ballot(valid && (key >> 6) == blk) & pending;
+    const unsigned long long same = __ballot(live && (key >> 6) == blk) & pending;
     if ((same >> lane) & 1ull) {
       leader = first;
       offset = __popcll(same & below);
@@ -198,7 +205,7 @@ __device__ __forceinline__ void batch_keys(const bk_dough* __restrict__ doughs, 
   if (size > 0) base = atomicAdd(&ws->blk_count[key >> 6], size);
   base = __shfl(base, leader, 64);
   if (valid) {
-    keys[p] = key;
+    keys[p] = live ? key : kFrozenKey;
     ranks[p] = base + offset;
   }
 }
@@ -436,6 +443,7 @@ __device__ __forceinline__ void block_scatter(const uint32_t* __restrict__ keys,
                                               int32_t* __restrict__ vout, int p) {
   if (p >= n) return;
   const uint32_t key = keys[p];
+  if (key == kFrozenKey) return;
   const int pos = ws->blk_start[key >> 6] + ranks[p];
   kout[pos] = key;
   vout[pos] = p;
@@ -940,6 +948,7 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __
   int p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   per_dough(dough_of_particle(sc.dough_of, p), [&](int d) {
+    if (frozen_mode(ws, d)) return;  // a stopped dough: pass C keeps or copies its state
     const bk_params& P = doughs[d].params;
     const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
     const int nb = (P.res + 3) / 4;
@@ -1300,6 +1309,52 @@ __global__ __launch_bounds__(256) void k_pass_c_deferred(const bk_dough* __restr
   if (threadIdx.x < 4 && totals[threadIdx.x]) atomicAdd(&ws->stats[threadIdx.x], totals[threadIdx.x]);
 }
 
+// ============================================================================ stopped doughs
+// Every state field of particle p from `in` to `out` (the fields the state has).
+__device__ __forceinline__ void copy_particle_state(const bk_state& in, const bk_state& out, int p) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) { out.x[3 * p + a] = in.x[3 * p + a]; out.v[3 * p + a] = in.v[3 * p + a]; }
+#pragma unroll
+  for (int a = 0; a < 9; ++a) { out.F[9 * p + a] = in.F[9 * p + a]; out.C[9 * p + a] = in.C[9 * p + a]; }
+  if (in.Fe) {
+#pragma unroll
+    for (int a = 0; a < 9; ++a) {
+      out.Fe[9 * p + a] = in.Fe[9 * p + a];
+      out.Fp[9 * p + a] = in.Fp[9 * p + a];
+      out.tau[9 * p + a] = in.tau[9 * p + a];
+    }
+    out.residual[p] = in.residual[p];
+    out.porosity[p] = in.porosity[p];
+    out.T_mat[p] = in.T_mat[p];
+    out.gelled[p] = in.gelled[p];
+    out.yield_stress[p] = in.yield_stress[p];
+    out.viscosity[p] = in.viscosity[p];
+    out.porosity_residual[p] = in.porosity_residual[p];
+    out.density[p] = in.density[p];
+  }
+  if (in.moles) { out.moles[p] = in.moles[p]; out.pressure[p] = in.pressure[p]; }
+  if (in.T) {
+    out.T[p] = in.T[p]; out.W[p] = in.W[p]; out.s[p] = in.s[p];
+    out.losses[p] = in.losses[p]; out.exchanges[p] = in.exchanges[p]; out.masses[p] = in.masses[p];
+  }
+}
+
+// At the start of a substep, each dough's freeze mode from its flags (raised by earlier substeps) and its mode
+// of the previous substep: no flag 0; the first substep with one 1; later substeps 2; and the substeps run
+// since it stopped, this one included.
+__global__ void k_freeze(bk_workspace* ws, int nd) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= nd) return;
+  const int32_t* fl = ws->flags + d * BK_FLAG_COUNT;
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < BK_FLAG_COUNT; ++k) any |= fl[k] != 0;
+  const int old = ws->freeze[d];
+  const int mode = !any ? 0 : (old == 0 ? 1 : 2);
+  ws->freeze[d] = mode;
+  ws->freeze_age[d] = mode ? ws->freeze_age[d] + 1 : 0;
+}
+
 // ============================================================================ pass C: G2P + Ding + gas refresh + saturation
 __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in,
                                                              bk_state out, bk_grid g, bk_workspace* ws, int n) {
@@ -1309,6 +1364,12 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __r
   float feat[kMlpRow];
   if (p < n) {
     per_dough(dough_of_particle(sc.dough_of, p), [&](int d) {
+      if (const int mode = frozen_mode(ws, d)) {
+        // a stopped dough: its output buffer already holds its last accepted state on the first substep
+        // after the flag (mode 1: nothing is written); from then on the state is copied through (mode 2)
+        if (mode == 2) copy_particle_state(in, out, p);
+        return;
+      }
       const bk_params& P = doughs[d].params;
       const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
       const int nb = (P.res + 3) / 4;
@@ -1516,6 +1577,11 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     auto it = g_prof.find(ws);
     if (it != g_prof.end() && it->second.step < it->second.max_steps) pr = &it->second;
+  }
+  // 0. with freezing on, each dough's freeze mode for this substep from the flags earlier substeps raised
+  if (h->freezing) {
+    hipLaunchKernelGGL(k_freeze, dim3(div_up(nd, 64)), dim3(64), 0, st, ws, nd);
+    BK_LAUNCH(freeze);
   }
   // 1. clear the previous substep's blocks (counters[1] still holds their count)
   // 2. keys and ranks within blocks (also resets the per-substep counters)
@@ -1772,7 +1838,7 @@ int bk_device_info(char* name, int name_len, int* cu_count, int* xcd_count) {
 bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blocks, int32_t max_doughs, int32_t n_vertices,
                                         int32_t n_slots) {
   if (max_particles < 1 || max_particles > (1 << kIndexBits) || total_blocks < 1 || max_doughs < 1 || n_vertices < 0 ||
-      n_slots < 0 || (long long)total_blocks * 64 > (1ll << 32)) {
+      n_slots < 0 || (long long)total_blocks * 64 >= (1ll << 32)) {
     fail("bk_workspace_create: bad sizes");
     return nullptr;
   }
@@ -1823,6 +1889,8 @@ bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blo
   alloc((void**)&h.predictions, n * kMlpOut * 4);
   alloc((void**)&h.stats, 4 * sizeof(unsigned long long));
   alloc((void**)&h.settle, sizeof(bk_settling) * size_t(max_doughs));
+  alloc((void**)&h.freeze, size_t(max_doughs) * 4);
+  alloc((void**)&h.freeze_age, size_t(max_doughs) * 4);
   if (!ok || hipMemcpy(ws, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess) {
     fail("bk_workspace_create: allocation failed");
     return nullptr;
@@ -1852,7 +1920,7 @@ void bk_workspace_destroy(bk_workspace* ws) {
   void* ptrs[] = {h.doughs, h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1],
                   h.hist, h.digit_total, h.blk_start, h.blk_count, h.scan_part, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
                   h.queue, h.tau, h.vertices, h.vert_mark, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions,
-                  h.stats, h.settle, h.settle_part, h.dTW};
+                  h.stats, h.settle, h.settle_part, h.dTW, h.freeze, h.freeze_age};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   (void)hipFree(ws);
@@ -2006,10 +2074,52 @@ int bk_workspace_write_flags_all(bk_workspace* ws, const int32_t* host, int32_t 
   HostWs* hw = host_view(ws);
   if (!hw || !host || n_doughs < 1 || n_doughs > hw->dev.max_doughs) return fail("bk_workspace_write_flags: bad argument");
   BK_HIP(hipMemcpyAsync(hw->dev.flags, host, size_t(BK_FLAG_COUNT) * n_doughs * 4, hipMemcpyHostToDevice, (hipStream_t)stream));
+  // a dough whose flags are written clear runs again; one written with a flag keeps its current state from
+  // the next substep on (mode 1: the next k_freeze makes it 2, copying its state through)
+  std::vector<int32_t> modes(static_cast<size_t>(n_doughs), 0), ages(static_cast<size_t>(n_doughs), 0);
+  for (int d = 0; d < n_doughs; ++d)
+    for (int k = 0; k < BK_FLAG_COUNT; ++k)
+      if (host[d * BK_FLAG_COUNT + k]) modes[d] = 1;
+  BK_HIP(hipMemcpyAsync(hw->dev.freeze, modes.data(), size_t(n_doughs) * 4, hipMemcpyHostToDevice, (hipStream_t)stream));
+  BK_HIP(hipMemcpyAsync(hw->dev.freeze_age, ages.data(), size_t(n_doughs) * 4, hipMemcpyHostToDevice, (hipStream_t)stream));
   BK_HIP(hipStreamSynchronize((hipStream_t)stream));
   return 0;
 }
-int bk_workspace_read_flags(bk_workspace* ws, int32_t* host8, void* stream) { return bk_workspace_read_flags_all(ws, host8, 1, stream); }
+
+int bk_workspace_set_freezing(bk_workspace* ws, int32_t on) {
+  HostWs* hw = host_view(ws);
+  if (!hw || (on != 0 && on != 1)) return fail("bk_workspace_set_freezing: bad argument");
+  int rc = set_device_field(ws, offsetof(bk_workspace, freezing), &on, sizeof(on));
+  if (rc) return rc;
+  hw->dev.freezing = on;
+  return 0;
+}
+
+int bk_workspace_freeze_modes(bk_workspace* ws, int32_t* host, int32_t n_doughs, int32_t write, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !host || n_doughs < 1 || n_doughs > hw->dev.max_doughs) return fail("bk_workspace_freeze_modes: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<int32_t> modes(static_cast<size_t>(n_doughs)), ages(static_cast<size_t>(n_doughs));
+  if (write) {
+    for (int d = 0; d < n_doughs; ++d) {
+      modes[d] = host[2 * d];
+      ages[d] = host[2 * d + 1];
+      if (modes[d] < 0 || modes[d] > 2 || ages[d] < 0) return fail("bk_workspace_freeze_modes: modes are 0, 1 or 2, ages >= 0");
+    }
+    BK_HIP(hipMemcpyAsync(hw->dev.freeze, modes.data(), size_t(n_doughs) * 4, hipMemcpyHostToDevice, st));
+    BK_HIP(hipMemcpyAsync(hw->dev.freeze_age, ages.data(), size_t(n_doughs) * 4, hipMemcpyHostToDevice, st));
+    BK_HIP(hipStreamSynchronize(st));
+  } else {
+    BK_HIP(hipMemcpyAsync(modes.data(), hw->dev.freeze, size_t(n_doughs) * 4, hipMemcpyDeviceToHost, st));
+    BK_HIP(hipMemcpyAsync(ages.data(), hw->dev.freeze_age, size_t(n_doughs) * 4, hipMemcpyDeviceToHost, st));
+    BK_HIP(hipStreamSynchronize(st));
+    for (int d = 0; d < n_doughs; ++d) {
+      host[2 * d] = modes[d];
+      host[2 * d + 1] = ages[d];
+    }
+  }
+  return 0;
+}int bk_workspace_read_flags(bk_workspace* ws, int32_t* host8, void* stream) { return bk_workspace_read_flags_all(ws, host8, 1, stream); }
 int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* stream) {
   return bk_workspace_write_flags_all(ws, host8, 1, stream);
 }

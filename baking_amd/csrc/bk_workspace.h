@@ -67,7 +67,15 @@ struct bk_workspace {
   float* settle_ref = nullptr;                  // caller's [2][max_n][9] settling reference plastic gradients, or null
   float* settle_part = nullptr;                 // settling metric maxima per dough and run of 256 particles
   int64_t settle_part_cap = 0;                  // floats in settle_part
+  int32_t* freeze = nullptr;                    // per dough, this substep: 0 runs, 1 first substep after a flag (writes
+                                                // nothing: the output buffer keeps the last accepted state), 2 later
+                                                // substeps (copies its state through); set by k_freeze at substep start
+  int32_t* freeze_age = nullptr;                // per dough: substeps run since it stopped (0 while it runs)
+  int32_t freezing = 0;                         // 1: doughs with a flag raised stop (batches); 0: they run on
 };
+
+// This substep's freeze mode of dough d (0 when freezing is off): see bk_workspace::freeze.
+__device__ __forceinline__ int frozen_mode(const bk_workspace* ws, int d) { return ws->freezing ? ws->freeze[d] : 0; }
 
 // Whether dough d rests this substep (physics/thermal_water.py:296-355 with
 // settling=True): params.settling 1 always, 2 while its controller record is in

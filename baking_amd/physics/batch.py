@@ -29,7 +29,7 @@ import numpy as np
 import torch
 
 from .. import _native
-from .mpm import state_field_slots
+from .mpm import _state_tensors, state_field_slots
 
 _THERMAL_GRID = ("V", "C", "E", "W", "H", "L", "X", "dT", "dW")
 _FLAG_NAMES = ("J", "stencil", "porosity", "coupled", "gas_cfl", "heat", "water")
@@ -76,6 +76,8 @@ class BatchSimulator:
             _native.check(-2, "bk_workspace_create_batch")
         self._workspace = ctypes.c_void_p(ws)
         _native.check(lib.bk_workspace_set_doughs(self._workspace, table, nd), "bk_workspace_set_doughs")
+        # a dough that raises a flag stops on the device at its last accepted state; the others run on
+        _native.check(lib.bk_workspace_set_freezing(self._workspace, 1), "bk_workspace_set_freezing")
         if tuning is not None:
             self.SetTuning(**tuning)
         self._flags_host = torch.zeros(nd * 8, dtype=torch.int32).pin_memory()
@@ -341,10 +343,11 @@ class BatchSimulator:
     def Advance(self, step_count, check=True, stream=None):
         """Run ``step_count`` substeps of every dough, then read the flags once.
 
-        Flags are per dough and sticky.  With ``check``, a flag raised during
-        the call makes this raise ``RuntimeError`` naming the doughs and their
-        flags; the state is then the one after all ``step_count`` substeps (a
-        lone ``MpmSimulator.Advance`` instead replays to stop at the substep).
+        Flags are per dough and sticky.  A dough that raises a flag stops on the device from the next
+        substep on (no keys, tiles, grid or material work for it) and keeps the state it had before the
+        substep that raised the flag -- the reference rejects that substep (physics/mpm.py:205-207) -- with
+        its ``step_count`` at its last accepted substep; the other doughs run on.  With ``check``, a flag
+        raised during the call makes this raise ``RuntimeError`` naming the doughs and their flags.
         Returns the (doughs, 8) flag array.
         """
         if step_count < 1:
@@ -358,21 +361,57 @@ class BatchSimulator:
             if remaining == 0:
                 return flags
 
+    def FreezeModes(self):
+        """(doughs, 2) int array: each dough's stop mode after the last substep (0 running, 1 stopped in that
+        substep, 2 stopped earlier) and the substeps run since it stopped."""
+        lib = _native.load()
+        host = (ctypes.c_int32 * (2 * self.dough_count))()
+        _native.check(lib.bk_workspace_freeze_modes(self._workspace, ctypes.cast(host, ctypes.c_void_p), self.dough_count, 0,
+                                                    _native.stream_ptr()), "bk_workspace_freeze_modes")
+        return np.frombuffer(host, dtype=np.int32).reshape(self.dough_count, 2).copy()
+
+    def _stop_flagged(self, before, flags, step_count):
+        """After a chunk of ``step_count`` substeps: a dough stopped before the chunk stays at its step count; one
+        that raised its first flag in the chunk takes the state and step count of its last accepted substep."""
+        for d in np.flatnonzero(before.any(axis=1)):
+            self.doughs[d].step_count -= step_count
+        new = flags.any(axis=1) & ~before.any(axis=1)
+        if not new.any():
+            return
+        modes = self.FreezeModes()
+        restore = False
+        for d in np.flatnonzero(new):
+            s = self.doughs[d]
+            mode, age = int(modes[d, 0]), int(modes[d, 1])
+            if mode == 0:
+                # the flag came in the chunk's last substep: its input is the other buffer
+                for dst, src in zip(_state_tensors(s.current_state), _state_tensors(s.next_state)):
+                    dst.copy_(src)
+                modes[d] = (1, 0)
+                restore = True
+            s.step_count -= age + 1  # the substeps after the rejected one and the rejected one itself
+        if restore:
+            lib = _native.load()
+            host = (ctypes.c_int32 * (2 * self.dough_count))(*modes.reshape(-1).tolist())
+            _native.check(lib.bk_workspace_freeze_modes(self._workspace, ctypes.cast(host, ctypes.c_void_p), self.dough_count, 1,
+                                                        _native.stream_ptr()), "bk_workspace_freeze_modes")
+
     def _advance_chunk(self, step_count, check, stream):
         resting = [d for d, s in enumerate(self.doughs) if s.IsSettling]
-        before = self.ReadFlags() if check else None
+        before = self.ReadFlags()
         for _ in range(step_count):
             self._launch(stream)
         self.step_count += step_count
         self._sync_doughs(step_count)
         flags = self.ReadFlags()
+        self._stop_flagged(before, flags, step_count)
         for d, s in enumerate(self.doughs):
             s._sync_api_flags(flags[d].tolist())
         if check:
             new = (flags != 0) & (before == 0)
             if new.any():
                 bad = {int(d): [_FLAG_NAMES[i] for i in np.flatnonzero(new[d, :7])] for d in np.flatnonzero(new.any(axis=1))}
-                raise RuntimeError(f"batched substep flags raised in doughs {bad}")
+                raise RuntimeError(f"batched substep flags raised in doughs {bad}; each stopped at its last accepted substep")
         if resting:
             self._pull_settling(resting)
         return flags

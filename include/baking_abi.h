@@ -172,6 +172,18 @@ int bk_workspace_write_flags(bk_workspace* ws, const int32_t* host8, void* strea
 /* The same for the first n_doughs flag blocks (int32[8 * n_doughs], dough-major). */
 int bk_workspace_read_flags_all(bk_workspace* ws, int32_t* host, int32_t n_doughs, void* stream);
 int bk_workspace_write_flags_all(bk_workspace* ws, const int32_t* host, int32_t n_doughs, void* stream);
+/* Stopped doughs (no reference counterpart; the reference stops the whole run at an error).  With freezing on
+ * (off by default; batches turn it on), a dough with any flag raised stops at the start of the next substep:
+ * its particles get no key (no tile, grid or material work) and its state is the one before the substep that
+ * raised the flag -- on the first stopped substep nothing is written, so the output buffer keeps the last
+ * accepted state, and later substeps copy the state through.  bk_workspace_freeze_modes reads (write = 0) or
+ * writes host int32[2 * n_doughs] pairs per dough: the mode of the last substep (0 ran, 1 first stopped
+ * substep, 2 copying) and its age, the substeps run since it stopped.  A dough with a flag and mode 0 raised
+ * it in the last substep run: its accepted state is in the other buffer (the caller restores it and writes
+ * mode 1).  Writing flags also sets each written dough's mode (0 clear, 1 flagged) and age 0.  Both calls
+ * synchronise `stream`. */
+int bk_workspace_set_freezing(bk_workspace* ws, int32_t on);
+int bk_workspace_freeze_modes(bk_workspace* ws, int32_t* host, int32_t n_doughs, int32_t write, void* stream);
 
 /* ---------------------------------------------------------------- settling controller
  * The rest before baking (physics/mpm.py:57-109 settling_steps/IsSettling/UpdateSettling with
