@@ -97,19 +97,70 @@ def build_scene(name):
     return scenes.loaf_256() if name == "loaf_256" else scenes.sphere_small()
 
 
-def cpu_baseline(spec, steps):
-    """The serial C++ oracle (CPU restatement of the reference kernels) timed on this host."""
+def host_cpu():
+    """The host this runs on: CPU model (lscpu), nproc, and the threads this process may use (its share of the
+    box: at most 16)."""
+    import os
+    import subprocess
+    model = None
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, check=False).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], capture_output=True, text=True, check=False).stdout.strip())
+    except (OSError, ValueError):
+        nproc = os.cpu_count()
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    return {"lscpu_model": model, "nproc": nproc, "threads_used": threads}
+
+
+def oracle_builds():
+    """The CPU baseline's oracle builds, compiled here for this host: serial -O3 -march=native and OpenMP.  Falls back
+    to the checker build (serial -O2) when the compiler is unavailable."""
+    from oracle import oracle as O
+    try:
+        return {"serial": O.build_timing(False), "openmp": O.build_timing(True),
+                "flags": "g++ -O3 -march=native -ffp-contract=off (OpenMP build: -fopenmp, atomic scatters)"}
+    except Exception as error:  # noqa: BLE001 -- report the fallback instead of failing the bench
+        return {"serial": None, "openmp": None, "flags": f"checker build g++ -O2 (timing build failed: {error})"}
+
+
+def _time_oracle(specs, steps, library, threads):
+    """particle-substeps/s of the oracle on `specs` (each: 1 untimed substep, then `steps` timed), and seconds."""
+    from oracle import oracle as O
     from tests.harness import oracle_from_spec
-    sim = oracle_from_spec(spec)
-    sim.Step()  # first substep outside the timing (allocations, page faults)
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    if library is not None:
+        O.lib(library).oracle_set_threads(threads)
+    total_p, total_s = 0, 0.0
+    for spec in specs:
+        sim = oracle_from_spec(spec, library=library)
         sim.Step()
-    elapsed = time.perf_counter() - t0
-    n = sim.n
-    return {"value": n * steps / elapsed, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} substeps (after 1 untimed) of the full {spec.name} scene ({n} particles, "
-                      f"{spec.simulation.grid_resolution}^3 dense grid) from rest, serial oracle (liboracle.so, g++ -O2)"}
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.Step()
+        total_s += time.perf_counter() - t0
+        total_p += sim.n
+        del sim
+    return total_p * steps / total_s, total_s
+
+
+def cpu_baseline(spec, steps, label=None):
+    """The oracle (CPU restatement of the reference kernels) on `steps` substeps of `spec`, timed on this host with
+    the serial -O3 build and the OpenMP build on every thread of this process's share."""
+    host, builds = host_cpu(), oracle_builds()
+    serial, t_serial = _time_oracle([spec], steps, builds["serial"], 1)
+    out = {"value": serial, "unit": "particle-substeps/s", "cores": 1, "kind": "port", "build": builds["flags"],
+           "host": host, "serial": {"value": serial, "cores": 1, "seconds": t_serial}}
+    if builds["openmp"] is not None:
+        par, t_par = _time_oracle([spec], steps, builds["openmp"], host["threads_used"])
+        out["openmp"] = {"value": par, "cores": host["threads_used"], "seconds": t_par}
+        out["value"], out["cores"] = par, host["threads_used"]
+    out["sample"] = (label or f"{steps} substeps (after 1 untimed) of the full {spec.name} scene") + \
+        f"; value = the {'OpenMP' if builds['openmp'] is not None else 'serial'} build"
+    return out
 
 
 def check_state_timing(spec, sim, with_host, repeats=5):
@@ -339,21 +390,30 @@ def run_batched(args):
 
 
 def cpu_baseline_batched(specs, steps, every=4):
-    """The serial C++ oracle on every `every`-th dough of the batch (a bounded sample of the workload)."""
-    from tests.harness import oracle_from_spec
-    total_p, total_s = 0, 0.0
-    for spec in specs[::every]:
-        sim = oracle_from_spec(spec)
-        sim.Step()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            sim.Step()
-        total_s += time.perf_counter() - t0
-        total_p += sim.n
-        del sim
-    return {"value": total_p * steps / total_s, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} substeps (after 1 untimed) of every {every}th dough of the batch ({len(specs[::every])} doughs, "
-                      f"{total_p} particles, each on its own dense grid), serial oracle (liboracle.so, g++ -O2)"}
+    """The oracle on every `every`-th dough of the batch (a bounded sample of the workload), serial -O3 and OpenMP
+    builds compiled for this host; plus the small sphere scene (8.8k particles) for its full 1000 substeps."""
+    from baking_amd import scenes
+    host, builds = host_cpu(), oracle_builds()
+    sample = specs[::every]
+    serial, t_serial = _time_oracle(sample, steps, builds["serial"], 1)
+    n = sum(len(s.samples.particle_positions) for s in sample)
+    out = {"value": serial, "unit": "particle-substeps/s", "cores": 1, "kind": "port", "build": builds["flags"], "host": host,
+           "serial": {"value": serial, "cores": 1, "seconds": t_serial}}
+    if builds["openmp"] is not None:
+        par, t_par = _time_oracle(sample, steps, builds["openmp"], host["threads_used"])
+        out["openmp"] = {"value": par, "cores": host["threads_used"], "seconds": t_par}
+        out["value"], out["cores"] = par, host["threads_used"]
+    out["sample"] = (f"{steps} substeps (after 1 untimed) of every {every}th dough of the batch ({len(sample)} doughs, {n} "
+                     f"particles, each on its own dense grid); value = the {'OpenMP' if builds['openmp'] is not None else 'serial'} build")
+    sphere = scenes.sphere_small()
+    full = {"substeps": 1000, "particles": len(sphere.samples.particle_positions)}
+    v, t = _time_oracle([sphere], 1000, builds["serial"], 1)
+    full["serial"] = {"value": v, "seconds": t, "cores": 1}
+    if builds["openmp"] is not None:
+        v, t = _time_oracle([sphere], 1000, builds["openmp"], host["threads_used"])
+        full["openmp"] = {"value": v, "seconds": t, "cores": host["threads_used"]}
+    out["sphere_small_1000"] = full
+    return out
 
 
 def copy_probe():

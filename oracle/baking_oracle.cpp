@@ -26,7 +26,29 @@
 #include <cstring>
 #include <algorithm>
 
+// The timing build of bench.py's CPU baseline (-DORACLE_OPENMP -fopenmp) splits each kernel's loop over the
+// host's threads and makes the scatters' adds and the flag raises atomic; the checker build (no define) is
+// the serial restatement described above, in particle order.
+#ifdef ORACLE_OPENMP
+#include <omp.h>
+#define ORACLE_FOR _Pragma("omp parallel for schedule(static, 256)")
+#define ORACLE_ATOMIC _Pragma("omp atomic")
+#else
+#define ORACLE_FOR
+#define ORACLE_ATOMIC
+#endif
+
 namespace {
+
+// flags[k] = max(flags[k], 1): the reference's wp.atomic_max(flags, k, 1)
+inline void raise_flag(int* flags, int k) {
+#ifdef ORACLE_OPENMP
+#pragma omp atomic write
+  flags[k] = 1;
+#else
+  flags[k] = std::max(flags[k], 1);
+#endif
+}
 
 // ---------------------------------------------------------------- small linear algebra
 template <typename T> struct M3 { T a[9]; };
@@ -310,12 +332,13 @@ template <typename R>
 static void ParticleToGrid(int n, const R* x, const R* v, const R* C, const R* tau, const R* m, const R* V0, R* node_m,
                            R* node_p, const R* origin, R dx, int res, R dt, int* flags) {
   // physics/transfer.py:36-70
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     R g[3];
     grid_position(x + 3 * p, origin, dx, g);
     int b[3];
     stencil_base(g, b);
-    if (!valid_stencil(b, res)) { flags[1] = std::max(flags[1], 1); continue; }
+    if (!valid_stencil(b, res)) { raise_flag(flags, 1); continue; }
     R l[3] = {g[0] - R(b[0]), g[1] - R(b[1]), g[2] - R(b[2])};
     R mass = m[p];
     R s = dt * V0[p] * R(4.0) / (dx * dx);
@@ -328,8 +351,10 @@ static void ParticleToGrid(int n, const R* x, const R* v, const R* C, const R* t
           int64_t node = flatten_node(b[0] + ox, b[1] + oy, b[2] + oz, res);
           V3<R> d = {{(R(ox) - l[0]) * dx, (R(oy) - l[1]) * dx, (R(oz) - l[2]) * dx}};
           V3<R> Ad = mulv(A, d);
-          node_m[node] += w * mass;
-          for (int a = 0; a < 3; ++a) node_p[3 * node + a] += w * (mass * vel.a[a] + Ad.a[a]);
+          ORACLE_ATOMIC node_m[node] += w * mass;
+          for (int a = 0; a < 3; ++a) {
+            ORACLE_ATOMIC node_p[3 * node + a] += w * (mass * vel.a[a] + Ad.a[a]);
+          }
         }
   }
 }
@@ -337,6 +362,7 @@ static void ParticleToGrid(int n, const R* x, const R* v, const R* C, const R* t
 template <typename R>
 static void UpdateGrid(int64_t count, const R* node_m, const R* node_p, R* node_v, int res, int bc, const R* gravity, R dt) {
   // physics/grid.py:20-38
+  ORACLE_FOR
   for (int64_t i = 0; i < count; ++i) {
     if (!(node_m[i] > R(0.0))) continue;
     R vel[3];
@@ -354,12 +380,13 @@ template <typename R>
 static void GridToParticle(int n, const R* x_in, const R* F_in, const R* node_v, R* x_out, R* v_out, R* C_out, R* F_out,
                            const R* origin, R dx, int res, R dt, int* flags) {
   // physics/transfer.py:73-109
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     R g[3];
     grid_position(x_in + 3 * p, origin, dx, g);
     int b[3];
     stencil_base(g, b);
-    if (!valid_stencil(b, res)) { flags[1] = std::max(flags[1], 1); continue; }
+    if (!valid_stencil(b, res)) { raise_flag(flags, 1); continue; }
     R l[3] = {g[0] - R(b[0]), g[1] - R(b[1]), g[2] - R(b[2])};
     V3<R> nv = {{0, 0, 0}};
     M3<R> nc = zero_m<R>();
@@ -385,10 +412,11 @@ static void GridToParticle(int n, const R* x_in, const R* F_in, const R* node_v,
 template <typename R>
 static void ComputeElasticStress(int n, const R* F, R* tau, R mu, R lam, int* flags) {
   // physics/materials/elastic.py:4-26
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     M3<R> f = load_m(F + 9 * p);
     R J = det(f);
-    if (!std::isfinite(J) || J <= R(0.0)) { flags[0] = std::max(flags[0], 1); store_m(tau + 9 * p, zero_m<R>()); continue; }
+    if (!std::isfinite(J) || J <= R(0.0)) { raise_flag(flags, 0); store_m(tau + 9 * p, zero_m<R>()); continue; }
     M3<R> t = add(scale(mu, sub(mul(f, transpose(f)), ident<R>())), scale(lam * std::log(J), ident<R>()));
     store_m(tau + 9 * p, t);
   }
@@ -400,9 +428,10 @@ static void UpdateDing(int n, const R* old_Fe, const R* old_n, const int* old_ge
                        const R* visc_param, int* flags, R* por_res, int coupled, R mu, R lam, R y0, R n0, int temp_on,
                        const R* knots, const R* yf, const R* vf, R dt, const R* local_T) {
   // physics/materials/ding.py:140-221
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     R porosity = old_n[p];
-    if (!std::isfinite(porosity) || porosity < R(0.0) || porosity >= R(1.0)) { flags[0] = std::max(flags[0], 1); continue; }
+    if (!std::isfinite(porosity) || porosity < R(0.0) || porosity >= R(1.0)) { raise_flag(flags, 0); continue; }
     int gel = old_gelled[p];
     R T = local_T[p];
     int reached = int(T >= knots[3]);
@@ -437,7 +466,7 @@ static void UpdateDing(int n, const R* old_Fe, const R* old_n, const int* old_ge
       ret_n = solve_porosity<R>(q, pv, df, ms, ex, opv, n0);
       double pred = porosity_prediction<R>(double(ret_n), q, pv, df, ms, ex, opv, n0);
       cres = R(std::fabs(double(ret_n) - pred));
-      if (cres > R(1.0e-6) || ret_n <= R(0.0) || ret_n >= R(1.0)) flags[1] = std::max(flags[1], 1);
+      if (cres > R(1.0e-6) || ret_n <= R(0.0) || ret_n >= R(1.0)) raise_flag(flags, 1);
     }
     M3<R> Fe, tau;
     R resid;
@@ -452,7 +481,7 @@ static void UpdateDing(int n, const R* old_Fe, const R* old_n, const int* old_ge
     R nn = det(Fp) - R(1.0) + n0;
     if (coupled) nn = ret_n;
     if (n0 == R(0.0) && porosity == R(0.0)) nn = R(0.0);
-    if (!std::isfinite(nn) || nn < R(0.0) || nn >= R(1.0)) flags[0] = std::max(flags[0], 1);
+    if (!std::isfinite(nn) || nn < R(0.0) || nn >= R(1.0)) raise_flag(flags, 0);
     store_m(new_Fe + 9 * p, Fe);
     store_m(new_Fp + 9 * p, Fp);
     store_m(new_tau + 9 * p, tau);
@@ -470,21 +499,23 @@ template <typename R>
 static void CombineGasStress(int n, const R* solid, const R* F, const R* por, const double* moles, const R* T, R* total,
                              int* flags, R p_amb, R n0, const R* density, R emod, R dt, R dx, R cfl) {
   // physics/materials/gas.py:16-35
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     R J = det(load_m(F + 9 * p));
     R nv = por[p];
-    if (nv <= R(0.0) || nv >= R(1.0) || !std::isfinite(nv) || J <= R(0.0)) { flags[0] = std::max(flags[0], 1); continue; }
+    if (nv <= R(0.0) || nv >= R(1.0) || !std::isfinite(nv) || J <= R(0.0)) { raise_flag(flags, 0); continue; }
     R pr = R(gas_pressure<R>(moles[p], T[p], J, nv));
     store_m(total + 9 * p, add(load_m(solid + 9 * p), scale(J * (p_amb - pr), ident<R>())));
     R bulk = pr * (R(1.0) + (R(1.0) - n0 + nv) / nv);
     R wave = std::sqrt((emod + bulk) * J / density[p]);
-    if (!std::isfinite(pr) || wave * dt / dx > cfl) flags[2] = std::max(flags[2], 1);
+    if (!std::isfinite(pr) || wave * dt / dx > cfl) raise_flag(flags, 2);
   }
 }
 
 template <typename R>
 static void RefreshGas(int n, const double* old_moles, double* new_moles, const R* F, const R* por, double* pres, const R* T) {
   // physics/materials/gas.py:38-45
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     new_moles[p] = old_moles[p];
     if (por[p] > R(0.0) && por[p] < R(1.0)) pres[p] = gas_pressure<R>(old_moles[p], T[p], det(load_m(F + 9 * p)), por[p]);
@@ -495,6 +526,7 @@ template <typename R>
 static void SurfaceAreas(int nf, const R* verts, const int* faces, const int* owners, double* exposed, double* contact, R floor_h,
                          R cdist) {
   // physics/thermal_water.py:77-92
+  ORACLE_FOR
   for (int f = 0; f < nf; ++f) {
     double P[3][3];
     for (int k = 0; k < 3; ++k)
@@ -505,8 +537,11 @@ static void SurfaceAreas(int nf, const R* verts, const int* faces, const int* ow
     double len = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
     double area = 0.5 * len;
     double cy = (P[0][1] + P[1][1] + P[2][1]) / 3.0;
-    if (cy <= double(floor_h + cdist) && nrm[1] < -0.2 * len) contact[owners[f]] += area;
-    else exposed[owners[f]] += area;
+    if (cy <= double(floor_h + cdist) && nrm[1] < -0.2 * len) {
+      ORACLE_ATOMIC contact[owners[f]] += area;
+    } else {
+      ORACLE_ATOMIC exposed[owners[f]] += area;
+    }
   }
 }
 
@@ -516,6 +551,7 @@ static void ScalarToGrid(int n, const R* x, const R* F, const R* V0, const doubl
                          R loss_coef, R rho_w, R latent, int water_on, double* nV, double* nC, double* nE, double* nW, double* nH,
                          double* nL, double* nX, double* p_heat, double* p_loss, const R* origin, R dx, int res) {
   // physics/thermal_water.py:95-137
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     double vol = double(V0[p]) * double(det(load_m(F + 9 * p)));
     double t = T[p];
@@ -539,13 +575,13 @@ static void ScalarToGrid(int n, const R* x, const R* F, const R* V0, const doubl
         for (int oz = 0; oz < 3; ++oz) {
           double w = double(quadratic_weight(l[0], ox) * quadratic_weight(l[1], oy) * quadratic_weight(l[2], oz));
           int64_t node = flatten_node(b[0] + ox, b[1] + oy, b[2] + oz, res);
-          nV[node] += w * vol;
-          nC[node] += w * cap[p];
-          nE[node] += w * cap[p] * t;
-          nW[node] += w * W[p];
-          nH[node] += w * heat;
-          nL[node] += w * loss;
-          nX[node] += w * exch;
+          ORACLE_ATOMIC nV[node] += w * vol;
+          ORACLE_ATOMIC nC[node] += w * cap[p];
+          ORACLE_ATOMIC nE[node] += w * cap[p] * t;
+          ORACLE_ATOMIC nW[node] += w * W[p];
+          ORACLE_ATOMIC nH[node] += w * heat;
+          ORACLE_ATOMIC nL[node] += w * loss;
+          ORACLE_ATOMIC nX[node] += w * exch;
         }
   }
 }
@@ -553,6 +589,7 @@ static void ScalarToGrid(int n, const R* x, const R* F, const R* V0, const doubl
 static void GridFields(int64_t count, const double* V, const double* C, const double* E, const double* W, double* T, double* S,
                        double rho_w) {
   // physics/thermal_water.py:140-147
+  ORACLE_FOR
   for (int64_t i = 0; i < count; ++i)
     if (C[i] > 0.0) { T[i] = E[i] / C[i]; S[i] = W[i] / (rho_w * V[i]); }
 }
@@ -562,6 +599,7 @@ static void DiffuseGrid(int64_t count, const double* V, const double* C, const d
                         const double* bh, const double* bl, const double* bx, double* dT, double* dW, int* errors, R kappa, R diff,
                         R rho_w, int water_on, R dx, int res, R dt) {
   // physics/thermal_water.py:150-190
+  ORACLE_FOR
   for (int64_t i = 0; i < count; ++i) {
     if (C[i] <= 0.0) continue;
     int64_t c[3] = {i / (int64_t(res) * res), (i / res) % res, i % res};
@@ -585,9 +623,9 @@ static void DiffuseGrid(int64_t count, const double* V, const double* C, const d
           }
         }
       }
-    if (double(dt) * hd > C[i]) errors[0] = std::max(errors[0], 1);
+    if (double(dt) * hd > C[i]) raise_flag(errors, 0);
     double change = double(dt) * wr;
-    if (W[i] + change < 0.0 || double(dt) * wd > 1.0) errors[1] = std::max(errors[1], 1);
+    if (W[i] + change < 0.0 || double(dt) * wd > 1.0) raise_flag(errors, 1);
     dT[i] = double(dt) * hr / C[i];
     dW[i] = change;
   }
@@ -599,6 +637,7 @@ static void ScalarToParticles(int n, const R* x, const double* T_old, const doub
                               const double* p_heat, const double* p_loss, double* T_new, R* T_mat, double* W_new, double* L_new,
                               double* X_new, R* m_new, R* rho_ref, double* s_new, R rho_w, const R* origin, R dx, int res, R dt) {
   // physics/thermal_water.py:193-229
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     R g[3];
     grid_position(x + 3 * p, origin, dx, g);
@@ -633,6 +672,7 @@ template <typename R>
 static void LocalGas(int n, const double* old_moles, const double* T_old, const double* T_new, double* new_moles, double* pres,
                      const R* F, const R* por, const R* param, R density, R peak, R width, R dt, int produce) {
   // physics/thermal_water.py:232-245
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     double moles = old_moles[p];
     if (produce) {
@@ -647,6 +687,7 @@ static void LocalGas(int n, const double* old_moles, const double* T_old, const 
 template <typename R>
 static void WaterStress(int n, const R* F, const double* sat, const R* old_s, R* new_s, R s0, R stiff) {
   // physics/thermal_water.py:248-253
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     R J = det(load_m(F + 9 * p));
     R k = J * stiff * (R(sat[p]) - s0);
@@ -657,6 +698,7 @@ static void WaterStress(int n, const R* F, const double* sat, const R* old_s, R*
 template <typename R>
 static void Saturations(int n, const double* W, const R* V0, const R* F, double* sat, R rho_w) {
   // physics/thermal_water.py:256-260
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) sat[p] = W[p] / (double(rho_w) * double(V0[p]) * double(det(load_m(F + 9 * p))));
 }
 
@@ -688,6 +730,7 @@ template <typename R>
 static void ReconstructMeshVertices(int nv, const R* x, const R* F, const R* ref_verts, const int* offs, const int* parts,
                                     const R* weights, const R* ref_offs, R* out) {
   // geometry/mesh_tracking.py:13-38
+  ORACLE_FOR
   for (int v = 0; v < nv; ++v) {
     int s = offs[v], e = offs[v + 1];
     if (s == e) { for (int a = 0; a < 3; ++a) out[3 * v + a] = ref_verts[3 * v + a]; continue; }
@@ -709,12 +752,13 @@ static void CellKeys(int n, const R* x, const R* origin, R dx, int res, int tile
   // Cell key of each particle: flatten_node(stencil_base((x - origin) / dx)) (physics/transfer.py:16-33),
   // or, with tiled != 0, the same cell numbered block-major over 4x4x4 cell blocks.
   int nb = (res + 3) / 4;
+  ORACLE_FOR
   for (int p = 0; p < n; ++p) {
     R g[3];
     grid_position(x + 3 * p, origin, dx, g);
     int b[3];
     stencil_base(g, b);
-    if (!valid_stencil(b, res)) { flags[1] = std::max(flags[1], 1); keys[p] = 0xffffffffu; continue; }
+    if (!valid_stencil(b, res)) { raise_flag(flags, 1); keys[p] = 0xffffffffu; continue; }
     if (!tiled) keys[p] = uint32_t(flatten_node(b[0], b[1], b[2], res));
     else {
       uint32_t blk = (uint32_t(b[0] >> 2) * nb + uint32_t(b[1] >> 2)) * nb + uint32_t(b[2] >> 2);
@@ -729,6 +773,22 @@ static void CellKeys(int n, const R* x, const R* origin, R dx, int res, int tile
   extern "C" void oracle_##name##_f64 sig_d { name<double> args; }
 
 extern "C" {
+
+// Threads of the timing build's loops (1 in the checker build).
+void oracle_set_threads(int threads) {
+#ifdef ORACLE_OPENMP
+  omp_set_num_threads(threads < 1 ? 1 : threads);
+#else
+  (void)threads;
+#endif
+}
+int oracle_threads(void) {
+#ifdef ORACLE_OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 
 void oracle_svd3(const double* F, double* U, double* sigma, double* V) {
   M3<double> f = load_m(F), u, v;

@@ -36,6 +36,7 @@ import numpy as np
 
 _HERE = Path(__file__).resolve().parent
 _LIB_PATH = _HERE / "liboracle.so"
+_TIMING_DIR = _HERE / "_timing"
 GAS_CONSTANT = 8.31446261815324  # physics/materials/gas.py:8
 
 _lib = None
@@ -55,27 +56,59 @@ def build(force=False):
     return _LIB_PATH
 
 
-def lib():
+def build_timing(openmp, march="native"):
+    """Compile a timing build of the CPU baseline into ``oracle/_timing/``: the same source at -O3 -march=<march>,
+    contraction still off -- serial (``liboracle_o3.so``, the checker's loops) or with OpenMP over each kernel's
+    loop and atomic scatters (``liboracle_omp.so``).  Built on the host that runs it (the machine code is specific
+    to that CPU); returns its path."""
+    import subprocess
+    src = _HERE / "baking_oracle.cpp"
+    _TIMING_DIR.mkdir(exist_ok=True)
+    path = _TIMING_DIR / ("liboracle_omp.so" if openmp else "liboracle_o3.so")
+    tmp = path.with_suffix(".so.tmp")
+    cmd = ["g++", "-O3", f"-march={march}", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math"]
+    cmd += ["-fopenmp", "-DORACLE_OPENMP"] if openmp else []
+    subprocess.run(cmd + [str(src), "-o", str(tmp)], check=True)
+    os.replace(tmp, path)
+    return path
+
+
+_libs = {}
+
+
+def _configure(handle):
+    handle.oracle_porosity_coefficient.restype = ctypes.c_float
+    handle.oracle_porosity_coefficient.argtypes = [ctypes.c_float]
+    handle.oracle_solve_overstress.restype = ctypes.c_float
+    handle.oracle_solve_overstress.argtypes = [ctypes.c_float] * 4
+    handle.oracle_overstress64.restype = ctypes.c_double
+    handle.oracle_overstress64.argtypes = [ctypes.c_double] * 4
+    handle.oracle_solve_porosity.restype = ctypes.c_float
+    handle.oracle_solve_porosity.argtypes = [ctypes.c_float] * 5 + [ctypes.c_double, ctypes.c_float]
+    handle.oracle_porosity_prediction.restype = ctypes.c_double
+    handle.oracle_porosity_prediction.argtypes = [ctypes.c_double] + [ctypes.c_float] * 5 + [ctypes.c_double, ctypes.c_float]
+    handle.oracle_gas_pressure.restype = ctypes.c_double
+    handle.oracle_gas_pressure.argtypes = [ctypes.c_double, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    handle.oracle_temperature_factor.restype = ctypes.c_float
+    handle.oracle_temperature_factor.argtypes = [ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
+    handle.oracle_set_threads.argtypes = [ctypes.c_int]
+    handle.oracle_threads.restype = ctypes.c_int
+    return handle
+
+
+def lib(path=None):
+    """The checker library (serial, ``liboracle.so``), or the library at ``path`` (the timing build)."""
     global _lib
-    if _lib is None:
-        if not _LIB_PATH.exists():
-            build()
-        _lib = ctypes.CDLL(str(_LIB_PATH))
-        _lib.oracle_porosity_coefficient.restype = ctypes.c_float
-        _lib.oracle_porosity_coefficient.argtypes = [ctypes.c_float]
-        _lib.oracle_solve_overstress.restype = ctypes.c_float
-        _lib.oracle_solve_overstress.argtypes = [ctypes.c_float] * 4
-        _lib.oracle_overstress64.restype = ctypes.c_double
-        _lib.oracle_overstress64.argtypes = [ctypes.c_double] * 4
-        _lib.oracle_solve_porosity.restype = ctypes.c_float
-        _lib.oracle_solve_porosity.argtypes = [ctypes.c_float] * 5 + [ctypes.c_double, ctypes.c_float]
-        _lib.oracle_porosity_prediction.restype = ctypes.c_double
-        _lib.oracle_porosity_prediction.argtypes = [ctypes.c_double] + [ctypes.c_float] * 5 + [ctypes.c_double, ctypes.c_float]
-        _lib.oracle_gas_pressure.restype = ctypes.c_double
-        _lib.oracle_gas_pressure.argtypes = [ctypes.c_double, ctypes.c_float, ctypes.c_float, ctypes.c_float]
-        _lib.oracle_temperature_factor.restype = ctypes.c_float
-        _lib.oracle_temperature_factor.argtypes = [ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p]
-    return _lib
+    if path is None:
+        if _lib is None:
+            if not _LIB_PATH.exists():
+                build()
+            _lib = _configure(ctypes.CDLL(str(_LIB_PATH)))
+        return _lib
+    key = str(Path(path).resolve())
+    if key not in _libs:
+        _libs[key] = _configure(ctypes.CDLL(key))
+    return _libs[key]
 
 
 # ------------------------------------------------------------------ argument marshalling
@@ -89,12 +122,12 @@ def _p(a):
 class Kernels:
     """Typed wrappers around the serial C++ kernels for one precision."""
 
-    def __init__(self, precision="f32"):
+    def __init__(self, precision="f32", library=None):
         assert precision in ("f32", "f64")
         self.precision = precision
         self.R = np.float32 if precision == "f32" else np.float64
         self._c = ctypes.c_float if precision == "f32" else ctypes.c_double
-        self._lib = lib()
+        self._lib = lib(library)
 
     def _call(self, name, *args):
         symbol = f"oracle_{name}_{self.precision}"
@@ -309,8 +342,8 @@ class OracleSimulator:
     """
 
     def __init__(self, tet_mesh, particle_samples, simulation_config, material_config, initial_velocity=(0.0, 0.0, 0.0),
-                 ding_config=None, transport_config=None, precision="f32"):
-        self.k = Kernels(precision)
+                 ding_config=None, transport_config=None, precision="f32", library=None):
+        self.k = Kernels(precision, library)
         R = self.k.R
         self.R = R
         self.sim = simulation_config
@@ -562,7 +595,7 @@ class OracleSimulator:
                        t.oven_temperature_k, t.tray_temperature_k, 0.0 if settling else t.tray_heat_transfer,
                        0.0 if settling else t.surface_water_loss, t.water_density, t.latent_heat, int(t.water_enabled),
                        w["V"], w["C"], w["E"], w["W"], w["H"], w["L"], w["X"], p_heat, p_loss, self.origin, self.dx, self.res)
-        lib().oracle_GridFields(ctypes.c_int64(G), _p(w["V"]), _p(w["C"]), _p(w["E"]), _p(w["W"]), _p(w["T"]), _p(w["S"]),
+        self.k._lib.oracle_GridFields(ctypes.c_int64(G), _p(w["V"]), _p(w["C"]), _p(w["E"]), _p(w["W"]), _p(w["T"]), _p(w["S"]),
                                 ctypes.c_double(float(R(t.water_density))))
         k.DiffuseGrid(i64(G), w["V"], w["C"], w["T"], w["S"], w["W"], w["H"], w["L"], w["X"], w["dT"], w["dW"], self.transport_flags,
                       0.0 if settling else t.conductivity, 0.0 if settling else t.water_diffusivity, t.water_density,
