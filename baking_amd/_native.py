@@ -30,7 +30,7 @@ class BkParams(ctypes.Structure):
         ("latent", c_float), ("kappa", c_float), ("diffusivity", c_float), ("s0", c_float), ("water_stress", c_float),
         ("contact_distance", c_float),
         ("n", c_int32), ("n_vertices", c_int32), ("n_faces", c_int32), ("n_slots", c_int32),
-        ("return_solver", c_int32), ("surrogate", c_int32),
+        ("return_solver", c_int32), ("surrogate", c_int32), ("sticky_floor", c_int32),
     ]
 
 

@@ -1159,6 +1159,7 @@ __global__ __launch_bounds__(256) void k_grid_update(const bk_dough* __restrict_
       if (c[a] < P.boundary_cells && vel[a] < 0.0f) vel[a] = 0.0f;
       if (c[a] >= P.res - P.boundary_cells && vel[a] > 0.0f) vel[a] = 0.0f;
     }
+    if (P.sticky_floor && c[1] < P.boundary_cells) vel[0] = vel[1] = vel[2] = 0.0f;
 #pragma unroll
     for (int a = 0; a < 3; ++a) g.v[3 * node + a] = vel[a];
   }

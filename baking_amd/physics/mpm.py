@@ -279,6 +279,7 @@ class MpmSimulator:
         P.n = self.particle_data.particle_count
         P.return_solver = {"bisection": 0, "bracketed": 1}[self.return_solver]
         P.surrogate = int(self._surrogate_on)
+        P.sticky_floor = int(getattr(self, "_sticky_floor", False))
         return P
 
     # ------------------------------------------------------------------ settling controller (physics/mpm.py:77-109)
@@ -406,6 +407,13 @@ class MpmSimulator:
 
     def SetSurrogate(self, on):
         self._surrogate_on = bool(on)
+        self._params = self._make_params()
+
+    def SetStickyFloor(self, on=True):
+        """Floor option without a reference counterpart: every node of the floor's boundary_cells layers gets
+        velocity 0 after the reference's frictionless unilateral wall projection (physics/grid.py:32-37), so dough
+        on the tray does not slide.  Off by default (the reference's walls)."""
+        self._sticky_floor = bool(on)
         self._params = self._make_params()
 
     def SetReturnSolver(self, name):

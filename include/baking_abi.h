@@ -72,6 +72,10 @@ typedef struct bk_params {
   /* 1: start each coupled-porosity solve from the return-map network's prediction (needs a network set
    * with bk_workspace_set_surrogate; no reference counterpart), 0: from the old porosity */
   int32_t surrogate;
+  /* 1: sticky floor -- after the reference's frictionless unilateral walls (physics/grid.py:32-37), every node
+   * of the floor's boundary_cells layers (j < boundary_cells) gets velocity 0 (no reference counterpart; 0 keeps
+   * the reference's walls) */
+  int32_t sticky_floor;
 } bk_params;
 
 /* One dough of a batch (no reference counterpart: the reference runs one

@@ -342,8 +342,10 @@ class OracleSimulator:
     """
 
     def __init__(self, tet_mesh, particle_samples, simulation_config, material_config, initial_velocity=(0.0, 0.0, 0.0),
-                 ding_config=None, transport_config=None, precision="f32", library=None):
+                 ding_config=None, transport_config=None, precision="f32", library=None, sticky_floor=False):
         self.k = Kernels(precision, library)
+        # no reference counterpart: after UpdateGrid's walls, velocity 0 on every node of the floor's layers
+        self.sticky_floor = bool(sticky_floor)
         R = self.k.R
         self.R = R
         self.sim = simulation_config
@@ -684,6 +686,8 @@ class OracleSimulator:
                          self.dx, self.res, self.dt, self.flags)
         k.UpdateGrid(i64(self.node_count), self.node_m, self.node_p, self.node_v, self.res, self.sim.boundary_cells, self.gravity,
                      self.dt)
+        if self.sticky_floor:
+            self.node_v.reshape(self.res, self.res, self.res, 3)[:, :self.sim.boundary_cells] = 0.0
         k.GridToParticle(n, prep.x, prep.F, self.node_v, out.x, out.v, out.C, out.F, self.origin, self.dx, self.res, self.dt,
                          self.flags)
         if self.ding_cfg is not None:
@@ -713,16 +717,59 @@ class OracleSimulator:
 
     # -------------------------------------------------------------- aggregates (experiments/recording.py:46-112)
     def Measure(self):
+        """The reference recorder's measurement of the current state (experiments/recording.py:46-112): mesh
+        extents and volume after UpdateMesh, particle volume sum V0 det F, Ding porosity/temperature/gelation
+        statistics, gas content and pressures, capacity-weighted and surface/core temperatures, water and
+        energy balances."""
         s = self.state
-        J = np.linalg.det(s.F.astype(np.float64))
+        verts = np.asarray(self.UpdateMesh(), dtype=np.float64)
+        F = np.asarray(s.F, dtype=np.float64)
+        if not np.isfinite(verts).all() or not np.isfinite(F).all():
+            raise ValueError("Cannot record nonfinite simulation state")
+        corners = verts[self.mesh.tet_indices]
+        signed = np.linalg.det(corners[:, 1:] - corners[:, :1]) / 6.0
+        J = np.linalg.det(F)
+        extent = np.ptp(verts, axis=0)
         V0 = np.asarray(self.samples.reference_volumes, dtype=np.float64)
-        out = dict(particle_volume_m3=float(np.dot(V0, J)))
+        out = {}
+        if s.ding is not None:
+            d = s.ding
+            por = np.asarray(d["porosity"], np.float32)
+            out.update(temperature_k=float(np.asarray(d["T"], np.float32).mean()), porosity_min=float(por.min()),
+                       porosity_mean=float(por.mean(dtype=np.float64)), porosity_max=float(por.max()),
+                       yield_stress_pa=float(np.asarray(d["yield_stress"], np.float32)[0]),
+                       viscosity_pa3_s=float(np.asarray(d["viscosity"], np.float32)[0]),
+                       gelled_fraction=float(np.asarray(d["gelled"]).mean()))
+            if self.ding_cfg.gas_enabled:
+                pr = d["pressure"]
+                out.update(gas_total_mol=float(np.dot(V0, d["moles"])), gas_pressure_min_pa=float(pr.min()),
+                           gas_pressure_max_pa=float(pr.max()), gas_pressure_mean_pa=float(pr.mean()),
+                           pore_volume_m3=float(np.dot(V0, J * por)))
         if s.transport is not None:
+            t = s.transport
+            T, W, sat = t["T"], t["W"], t["s"]
             cap = self.capacities
-            out.update(temperature_mean_k=float(np.average(s.transport["T"], weights=cap)),
-                       water_mass_kg=float(s.transport["W"].sum()), water_lost_kg=float(s.transport["losses"].sum()),
-                       heat_exchange_j=float(s.transport["exchanges"].sum()),
-                       thermal_energy_j=float(np.dot(cap, s.transport["T"])))
-        if s.ding is not None and self.ding_cfg.gas_enabled:
-            out["gas_total_mol"] = float(np.dot(V0, s.ding["moles"]))
+            surface = np.zeros(self.n, dtype=bool)
+            surface[self.owners] = True
+            pos = np.asarray(self.samples.particle_positions)
+            radius = np.linalg.norm(pos - np.mean(self.mesh.vertex_positions, axis=0), axis=1)
+            core = radius < 0.4 * radius.max()
+            if not core.any():
+                core = radius == radius.min()
+            initial_water = self.tr.water_density * self.tr.initial_saturation * V0.sum()
+            energy = float(np.dot(cap, T))
+            exchange = float(t["exchanges"].sum())
+            losses = float(t["losses"].sum())
+            out.update(temperature_min_k=float(T.min()), temperature_max_k=float(T.max()),
+                       temperature_mean_k=float(np.average(T, weights=cap)), temperature_surface_k=float(T[surface].mean()),
+                       temperature_core_k=float(T[core].mean()), water_mass_kg=float(W.sum()), water_lost_kg=losses,
+                       water_balance_error_kg=float(W.sum() + losses - initial_water), saturation_min=float(sat.min()),
+                       saturation_max=float(sat.max()), saturation_mean=float(sat.mean()), thermal_energy_j=energy,
+                       heat_exchange_j=exchange,
+                       energy_balance_error_j=float(energy - self.tr.initial_temperature_k * cap.sum() - exchange))
+        out.update(baking_time_s=self.baking_time, phase="settling" if self.IsSettling else "baking",
+                   step=self.step_count, time_s=self.step_count * self.dt, length_x_m=float(extent[0]),
+                   width_z_m=float(extent[2]), height_y_m=float(extent[1]), particle_volume_m3=float(np.dot(V0, J)),
+                   mesh_volume_m3=float(np.abs(signed).sum()), inverted_tets=int(np.count_nonzero(signed <= 0)),
+                   invalid_particles=int(np.count_nonzero(J <= 0)))
         return out

@@ -17,10 +17,10 @@ if str(ROOT) not in sys.path:
 from oracle import oracle as O  # noqa: E402
 
 
-def oracle_from_spec(spec, precision="f32", library=None):
+def oracle_from_spec(spec, precision="f32", library=None, sticky_floor=False):
     """The oracle simulator of a scene (``library``: a timing build's path; default the checker build)."""
     sim = O.OracleSimulator(spec.tet_mesh, spec.samples, spec.simulation, spec.material, spec.initial_velocity, spec.ding,
-                            spec.transport, precision=precision, library=library)
+                            spec.transport, precision=precision, library=library, sticky_floor=sticky_floor)
     st = sim.state
     if spec.water_masses is not None:
         st.transport["W"][...] = spec.water_masses

@@ -100,3 +100,71 @@ def test_sphere_small_1000_substeps():
             assert e <= max(1e-5, min(4.0 * e_ref, ROUNDING_CAP)), (name, e, e_ref)
     print("sampled particles after 1000 substeps, GPU vs FP32 / FP32 vs FP64 (relative to max|field|): "
           + ", ".join(report), flush=True)
+
+
+# The reference's snapshot keys (physics/mpm.py:310-334; DingState physics/materials/ding.py:33-48, TransportState
+# physics/thermal_water.py:41-48) and the array types its Warp arrays give them.
+SNAPSHOT_KEYS = ("particle_positions", "particle_velocities", "deformation_gradients", "affine_velocities", "particle_masses",
+                 "reference_volumes", "vertices", "tetrahedra", "reference_vertices", "source_tets", "simulation_time",
+                 "baking_time_s", "settling_steps")
+DING_KEYS = ("elastic_gradients", "plastic_gradients", "kirchhoff_stresses", "return_residuals", "porosities", "temperatures",
+             "gelled", "yield_stresses", "viscosities", "gas_moles", "gas_pressures", "porosity_residuals", "reference_densities")
+TRANSPORT_KEYS = ("temperatures", "water_masses", "saturations", "water_losses", "heat_exchanges", "particle_masses")
+FLOAT64 = {"ding_gas_moles", "ding_gas_pressures", "transport_temperatures", "transport_water_masses", "transport_saturations",
+           "transport_water_losses", "transport_heat_exchanges", "reference_vertices"}
+
+
+def _compare_measure(tag, got, want):
+    assert set(got) == set(want), (set(got) ^ set(want))
+    for k, w in want.items():
+        g = got[k]
+        if isinstance(w, str):
+            assert g == w, (tag, k)
+            continue
+        # float32 means (numpy pairwise vs the device's order) and the mesh (vertices reconstructed on the GPU in
+        # another summation order) to float32 rounding; sums, extremes and counts to double rounding
+        rtol = 1e-5 if k in ("temperature_k", "length_x_m", "width_z_m", "height_y_m", "mesh_volume_m3") else 1e-9
+        if k.endswith("balance_error_kg") or k.endswith("balance_error_j"):
+            ref = abs(want["water_mass_kg"] if k.endswith("_kg") else want["thermal_energy_j"])
+            assert abs(g - w) <= 1e-12 * ref, (tag, k, g, w)
+        else:
+            assert abs(g - w) <= rtol * max(abs(w), 1e-300), (tag, k, g, w)
+    print(f"{tag}: {len(want)} measurement keys agree with the oracle's", flush=True)
+
+
+def test_measure_and_snapshot_match_the_oracle(tmp_path):
+    """The reference recorder's measurement (experiments/recording.py:46-112) reduced on the GPU (baking_amd.measure.Measure)
+    against the oracle's restatement on the GPU's own state, for the sphere after 20 substeps and for two doughs of a
+    batch; CaptureSnapshot/SaveSnapshot carry the reference's keys and array types with the state's values."""
+    from baking_amd.measure import Measure
+    from baking_amd.physics.batch import BatchSimulator
+    spec = scenes.sphere_small()
+    sim = scenes.build_simulator(spec)
+    sim.Advance(20, check=False)
+    o = load_gpu_state(oracle_from_spec(spec, "f32"), sim)
+    _compare_measure("sphere_small", Measure(sim), o.Measure())
+    snap = sim.CaptureSnapshot()
+    want_keys = set(SNAPSHOT_KEYS) | {"ding_" + k for k in DING_KEYS} | {"transport_" + k for k in TRANSPORT_KEYS}
+    assert set(snap) == want_keys, set(snap) ^ want_keys
+    fields = gpu_fields(sim)
+    pairs = dict(particle_positions="x", particle_velocities="v", deformation_gradients="F", affine_velocities="C",
+                 ding_elastic_gradients="Fe", ding_plastic_gradients="Fp", ding_kirchhoff_stresses="tau",
+                 ding_porosities="porosity", ding_gas_moles="moles", transport_temperatures="T", transport_water_masses="W")
+    for key, field in pairs.items():
+        np.testing.assert_array_equal(snap[key], fields[field], err_msg=key)
+    for key, value in snap.items():
+        if isinstance(value, np.ndarray) and value.dtype.kind == "f":
+            assert value.dtype == (np.float64 if key in FLOAT64 else np.float32), key
+    assert snap["ding_gelled"].dtype == np.int32 and snap["tetrahedra"].shape[1] == 4
+    assert snap["simulation_time"] == pytest.approx(20 * spec.simulation.time_step)
+    path = tmp_path / "snap.npz"
+    sim.SaveSnapshot(path)
+    with np.load(path) as back:
+        assert set(back.files) == want_keys
+        np.testing.assert_array_equal(back["ding_porosities"], snap["ding_porosities"])
+    specs = scenes.batched_doughs(count=2, seed=10, half_extent=(0.02, 0.035), max_particles=6000)
+    batch = BatchSimulator([scenes.build_simulator(s) for s in specs])
+    batch.Advance(6)
+    for d, s in enumerate(specs):
+        _compare_measure(f"{s.name} in a batch", Measure(batch.doughs[d]), load_gpu_state(oracle_from_spec(s, "f32"),
+                                                                                        batch.doughs[d]).Measure())

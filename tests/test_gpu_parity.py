@@ -324,3 +324,33 @@ def test_crowded_block_sort_is_the_stable_sort_of_the_keys():
     want = _expected_keys(spec, x)
     assert np.bincount(want >> 6).max() > 1024
     _check_order(*_substep_order(sim, n), want)
+
+
+@pytest.mark.parametrize("sticky", [True, False])
+def test_sticky_floor_on_the_gpu(sticky):
+    """The fused substep's floor option (MpmSimulator.SetStickyFloor; no reference counterpart) against the oracle
+    with the same option, and its closed form: particles whose whole stencil lies in the floor layers keep their
+    position and deformation gradient exactly, every occupied floor node has velocity 0.  Off, the reference's
+    frictionless floor lets them slide."""
+    from tests.test_oracle_walls import sliding_cube
+    spec = sliding_cube(sticky)
+    gpu = scenes.build_simulator(spec)
+    gpu.SetStickyFloor(sticky)
+    o32, o64 = oracle_from_spec(spec, "f32", sticky_floor=sticky), oracle_from_spec(spec, "f64", sticky_floor=sticky)
+    sim = spec.simulation
+    res, bc, dx = sim.grid_resolution, sim.boundary_cells, sim.grid_spacing
+    x0 = gpu.current_state.particle_positions.cpu().numpy().copy()
+    deep = (x0[:, 1] - sim.grid_origin[1]) / dx < 1.5
+    for step in range(1, spec.steps + 1):
+        gpu.Step()
+        o32.Step()
+        o64.Step()
+        if step % 10 == 0:
+            check_fields(f"{spec.name} substep {step}", gpu_fields(gpu), o32, o64)
+            if sticky:
+                m = gpu.grid_state.node_masses.cpu().numpy().reshape((res,) * 3)
+                v = gpu.grid_state.node_velocities.cpu().numpy().reshape((res,) * 3 + (3,))
+                assert np.all(v[:, :bc][m[:, :bc] > 0] == 0.0)
+                np.testing.assert_array_equal(gpu.current_state.particle_positions.cpu().numpy()[deep], x0[deep])
+    if not sticky:
+        assert np.all(np.abs(gpu.current_state.particle_velocities.cpu().numpy()[deep, 0] - 0.5) < 0.05)

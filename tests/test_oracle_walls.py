@@ -114,3 +114,46 @@ def test_cube_stops_at_each_wall(axis, side):
     assert depth >= 1.0, depth  # still well inside the valid stencil region (0.5 from the edge)
     p_end = float(np.dot(mass, o.state.v[:, axis])) * side
     assert p_end < -0.1 * p0, (p_end, p0)  # stopped and rebounding from the wall
+
+
+def sliding_cube(sticky):
+    """A 0.1 x 0.05 x 0.1 m elastic cube whose bottom lies in the floor layers (y = 0.6 dx), sliding along x at
+    0.5 m/s under gravity on the default 40^3 grid."""
+    from baking_amd import scenes
+    from baking_amd.config import DingConfig, MaterialConfig, SimulationConfig, TransportConfig
+    from baking_amd.geometry.mesh_loader import VoxelTetMesh
+    from baking_amd.geometry.mesh_validation import ValidateTetMesh
+    from baking_amd.geometry.particle_sampling import SampleParticles
+    sim = SimulationConfig(device_name="auto")
+    verts, tets = VoxelTetMesh((4, 2, 4), 0.025, (0.4, 0.6 * sim.grid_spacing, 0.4))
+    mesh = ValidateTetMesh(verts, tets)
+    spec = scenes.SceneSpec("sticky_floor" if sticky else "frictionless_floor", mesh, SampleParticles(mesh, 1000.0, 4), sim,
+                            MaterialConfig(), DingConfig(), TransportConfig(), steps=50, initial_velocity=(0.5, 0.0, 0.0))
+    return spec
+
+
+@pytest.mark.parametrize("sticky", [True, False])
+def test_sticky_floor_closed_form(sticky):
+    """Sticky floor (no reference counterpart): every node of the floor's layers has velocity 0, so a particle whose
+    whole stencil lies in those layers (grid y < 1.5) keeps its position and deformation gradient exactly; with the
+    reference's frictionless floor the same particles keep sliding at about 0.5 m/s."""
+    spec = sliding_cube(sticky)
+    o = oracle_from_spec(spec, "f32", sticky_floor=sticky)
+    sim = spec.simulation
+    res, bc, dx = sim.grid_resolution, sim.boundary_cells, sim.grid_spacing
+    x0 = o.state.x.copy()
+    deep = (x0[:, 1] - sim.grid_origin[1]) / dx < 1.5
+    assert deep.sum() >= 8
+    for step in range(1, spec.steps + 1):
+        o.Step()
+        m = o.node_m.reshape((res,) * 3)
+        v = o.node_v.reshape((res,) * 3 + (3,))
+        floor_nodes = m[:, :bc] > 0
+        assert floor_nodes.any()
+        if sticky:
+            assert np.all(v[:, :bc][floor_nodes] == 0.0), step
+            np.testing.assert_array_equal(o.state.x[deep], x0[deep])
+            np.testing.assert_array_equal(o.state.F[deep], np.tile(np.eye(3, dtype=np.float32), (int(deep.sum()), 1, 1)))
+    if not sticky:
+        assert np.all(np.abs(o.state.v[deep, 0] - 0.5) < 0.05)
+        assert np.all(o.state.x[deep, 0] > x0[deep, 0] + 0.8 * 0.5 * spec.steps * sim.time_step)
