@@ -433,6 +433,30 @@ __device__ inline bool solve_porosity_guided(float q, float p, float df, float m
   return false;
 }
 
+// The network's porosity standing in for the coupled solve (params.surrogate 2): one evaluation of the
+// porosity equation f(n) = n - prediction(n) at the network's value (inner overstress warm-started from its
+// return), one Newton correction.  Accepted when the correction is at most kStandinStep: the corrected value
+// is then within |f''/(2 f')| step^2 (about 1e-9 here) of the root the exact solve finds, about one float
+// ulp of the porosity.  The porosity residual |root - prediction(root)| of the float result follows from the
+// same linearisation, f(root) = f(x) + f'(x) (root - x) with f(x) of order step^2.  Returns false when the
+// check fails (the caller then solves exactly).
+constexpr double kStandinStep = 2.0e-5;
+__device__ inline bool porosity_standin(float q, float p, float df, float ms, float ex, double pv, float n0, double n_net,
+                                        double gamma_start, float& root, double& residual, int& evals) {
+  if (!(n_net > 0.0 && n_net < 1.0 - 1.0e-9)) return false;
+  double gamma = gamma_start > 0.0 ? gamma_start : 0.0, slope = 1.0;
+  const double f = porosity_gap(n_net, q, p, df, ms, ex, pv, n0, gamma, slope);
+  ++evals;
+  if (!(slope > 0.0) || !isfinite(f)) return false;
+  const double step = f / slope;
+  if (!(fabs(step) <= kStandinStep)) return false;
+  const double x = n_net - step;
+  if (!(x > 0.0 && x < 1.0 - 2.0e-9)) return false;
+  root = float(x);
+  residual = fabs(slope * (double(root) - x));
+  return true;
+}
+
 // Result of the Ding return map for one particle.
 struct DingOut {
   mat3<float> Fe, Fp, tau;
@@ -497,12 +521,14 @@ enum DingStatus { kDingInvalid = 0, kDingDone = 1, kDingDeferred = 2 };
 
 // Optional extras of ding_update for the coupled (gas) return map.
 struct DingAux {
-  // in: solve the coupled porosity from a predicted start first (see solve_porosity_guided)
+  // in: solve the coupled porosity from a predicted start first (see solve_porosity_guided); with `standin`,
+  // accept the prediction after one corrected evaluation when it passes porosity_standin's check
   bool guided = false;
+  bool standin = false;
   double n_start = 0.0, gamma_start = 0.0;
   float dev_target = 0.0f;  // predicted |deviatoric returned Kirchhoff stress| (0: start the overstress at 0)
   // out
-  bool accepted = false;  // the guided solve converged
+  bool accepted = false;  // the guided solve converged (standin: the prediction was accepted)
   int evals = 0;          // evaluations of the porosity equation
   float eps[3];           // principal trial log-strains, descending
   int order[3];           // eps[k] is the principal value order[k] of the SVD
@@ -583,10 +609,16 @@ __device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old
       bool inside = trial_n > 0.0 && trial_n < 1.0 && double(q) + pressure_coefficient64(trial_n) * double(pv) <= 1.0;
       if (!inside) return kDingDeferred;
     }
-    bool solved = false;
-    if (aux && aux->guided) {
+    bool solved = false, stood_in = false;
+    double standin_residual = 0.0;
+    if (aux && aux->standin) {
+      stood_in = porosity_standin(q, pv, df, ms, ex, opv, P.n0, aux->n_start, aux->gamma_start, ret_n, standin_residual,
+                                  aux->evals);
+      aux->accepted = solved = stood_in;
+    }
+    if (!solved && aux && aux->guided) {
       solved = solve_porosity_guided(q, pv, df, ms, ex, opv, P.n0, aux->n_start, aux->gamma_start, ret_n, aux->evals);
-      aux->accepted = solved;
+      if (!aux->standin) aux->accepted = solved;
     }
     if (!solved) {
       if (P.bracketed) {
@@ -596,9 +628,13 @@ __device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old
         if (aux) aux->evals += 64;
       }
     }
-    double pred = porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
-    if (aux) ++aux->evals;
-    cres = float(fabs(double(ret_n) - pred));
+    if (stood_in) {
+      cres = float(standin_residual);
+    } else {
+      double pred = porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
+      if (aux) ++aux->evals;
+      cres = float(fabs(double(ret_n) - pred));
+    }
     if (cres > 1.0e-6f || ret_n <= 0.0f || ret_n >= 1.0f) o.flag1 = 1;
   }
   mat3<float> Fe, tau;

@@ -1262,6 +1262,7 @@ __global__ __launch_bounds__(256) void k_pass_c_deferred(const bk_dough* __restr
           // deviator onto the predicted one (|dev tau_ret| = |dev tau_trial| / (1 + df gamma))
           const float* pr = ws->predictions + size_t(i) * kMlpOut;
           aux.guided = true;
+          aux.standin = P.surrogate == 2;
           aux.n_start = double(in.porosity[p]) + double(pr[3]);
           const float m = (pr[0] + pr[1] + pr[2]) / 3.0f;
           aux.dev_target = 2.0f * P.mu *

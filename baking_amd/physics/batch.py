@@ -254,9 +254,13 @@ class BatchSimulator:
         self._scene = scene
 
     def SetSurrogate(self, using):
-        """Per-dough switch of the guided coupled-porosity solve (see baking_amd.surrogate.attach)."""
+        """Per-dough network mode of the coupled-porosity solve: 0 none, 1 guided, 2 stand-in (see
+        baking_amd.surrogate.attach)."""
         for d, on in enumerate(using):
-            self.doughs[d]._surrogate_on = bool(on)
+            mode = int(on)
+            if mode not in (0, 1, 2):
+                raise ValueError("surrogate mode must be 0, 1 or 2")
+            self.doughs[d]._surrogate_on = mode
         self._refresh_table()
 
     # ------------------------------------------------------------------ tuning

@@ -403,10 +403,15 @@ class MpmSimulator:
     return_solver = "bracketed"
 
     # Start coupled-porosity solves from the return-map network (baking_amd.surrogate.attach sets it)
-    _surrogate_on = False
+    _surrogate_on = 0
 
     def SetSurrogate(self, on):
-        self._surrogate_on = bool(on)
+        """0/False: no network; 1/True: the network guides the coupled solves (exact results); 2: it stands in for
+        them when its one-step correction passes (see baking_amd.surrogate.attach)."""
+        mode = int(on)
+        if mode not in (0, 1, 2):
+            raise ValueError("surrogate mode must be 0, 1 or 2")
+        self._surrogate_on = mode
         self._params = self._make_params()
 
     def SetStickyFloor(self, on=True):

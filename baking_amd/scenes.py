@@ -53,6 +53,16 @@ class SceneSpec:
     notes: dict = field(default_factory=dict)
 
 
+def block_major(positions, dx, origin=(0.0, 0.0, 0.0)):
+    """The particles reordered 4x4x4-block-major: by the block of the grid cell they lie in (row-major over blocks),
+    then by cell inside the block, keeping their order within a cell -- the order the substep's sort groups
+    them in, so each P2G tile's particles are contiguous in memory."""
+    c = np.floor((np.asarray(positions) - np.asarray(origin)) / dx).astype(np.int64)
+    b, l = c >> 2, c & 3
+    keys = np.stack([b[:, 0], b[:, 1], b[:, 2], l[:, 0], l[:, 1], l[:, 2]])
+    return np.lexsort(keys[::-1])
+
+
 def jittered_cells(inside, lo, hi, dx, seed):
     """8 jittered particles per cell (2x2x2) for cells overlapping [lo, hi]; keep those with inside(x)."""
     c0 = np.floor(np.asarray(lo) / dx).astype(int)
@@ -114,15 +124,22 @@ def make_samples(tet_mesh, positions, dx, density):
     return ParticleSamples(positions, volumes, density * volumes, source)
 
 
+# Ding yield stress of the scaled workloads (the loaf and the batched doughs): twice the example's 810.82 Pa.  At
+# the example's value the loaf's weight (rho g h ~ 1.7 kPa) compacts its base until the pore gas stiffens past
+# the gas-acoustic CFL bound at substep ~806 of dt = 5e-5, whatever the gas production or the modulus; at 1600 Pa
+# the base still yields (porosity down to ~0.017) and all 2000 substeps stay below CFL 0.31.
+SCALED_YIELD_STRESS = 1600.0
+
+
 def _oven_material(young=1.25e5, poisson=0.485):
     return MaterialConfig(density=1150.0, young_modulus=young, poisson_ratio=poisson, model="ding2019")
 
 
-def _oven_ding(porosity, p0):
-    return DingConfig(yield_stress=810.82, viscosity=1.5e6, porosity=porosity, temperature_enabled=True,
+def _oven_ding(porosity, p0, gas_production=2.04e-5, yield_stress=810.82, viscosity=1.5e6):
+    return DingConfig(yield_stress=yield_stress, viscosity=viscosity, porosity=porosity, temperature_enabled=True,
                       temperature_knots_k=(278.15, 300.15, 328.15, 353.15), yield_factors=(1.5, 1.0, 0.5, 3.0),
                       viscosity_factors=(1.5, 1.0, 0.5, 5.0), gas_enabled=True, gas_initial_pressure_pa=p0,
-                      gas_ambient_pressure_pa=p0, gas_production_rate=2.04e-5, gas_reaction_temperature_k=318.15,
+                      gas_ambient_pressure_pa=p0, gas_production_rate=gas_production, gas_reaction_temperature_k=318.15,
                       gas_reaction_width_k=15.42)
 
 
@@ -221,16 +238,16 @@ def trilinear(field, origin, spacing, points):
     return out
 
 
-def loaf_256(seed=0, field_seed=1, voxel=0.015, p0=1000.0):
+def loaf_256(seed=0, field_seed=1, voxel=0.015, p0=1000.0, young=1.25e5, gas_production=2.04e-5, steps=2000,
+             yield_stress=SCALED_YIELD_STRESS, viscosity=1.5e6):
     """The scaled plain scene: 0.30 x 0.15 x 0.12 loaf on a 256^3 grid, ~0.72M particles.
 
     The pore-gas pressure p0 = ambient = 1000 Pa keeps the gas-acoustic CFL
     below 0.4 at dx = 1/256, dt = 5e-5 (atmospheric pore gas would trip it on
-    the first substep); Poisson's ratio 0.45 keeps the elastic CFL below 0.4.
-    With the example's gas production (2.04e-5 mol/(kg s), peak at 318 K) the
-    heated surface's pore pressure rises about twentyfold by substep 806, where
-    the gas wave speed (31.3 m/s) reaches the gas-acoustic bound and the
-    simulator raises its CFL error; ``steps`` stays below that.
+    the first substep); Poisson's ratio 0.45 keeps the elastic CFL below 0.4;
+    the yield stress SCALED_YIELD_STRESS keeps the base's compaction, and with
+    it the pore gas's stiffness, below the gas-acoustic bound for all 2000
+    substeps (the example's 810.82 Pa trips it at substep ~806).
     """
     res, dx = 256, 1.0 / 256.0
     size = np.array([0.30, 0.15, 0.12])
@@ -242,7 +259,7 @@ def loaf_256(seed=0, field_seed=1, voxel=0.015, p0=1000.0):
     verts, tets = VoxelTetMesh(shape, voxel, lo)
     verts = lo + (verts - lo) * (size / (shape * voxel))
     mesh = ValidateTetMesh(verts, tets)
-    material = _oven_material(poisson=0.45)
+    material = _oven_material(young=young, poisson=0.45)
     samples = make_samples(mesh, pos, dx, material.density)
     rng = np.random.default_rng(field_seed)
     margin = 0.06
@@ -256,9 +273,11 @@ def loaf_256(seed=0, field_seed=1, voxel=0.015, p0=1000.0):
     moles = p0 * porosity / (gas_constant * T0)
     sim = SimulationConfig(grid_resolution=res, grid_spacing=dx, grid_origin=(0.0, 0.0, 0.0), time_step=5.0e-5,
                            gravity=(0.0, -9.8, 0.0), boundary_cells=3, device_name="auto")
-    return SceneSpec("loaf_256", mesh, samples, sim, material, _oven_ding(0.05, p0), _oven_transport(T0, 473.15), steps=750,
-                     water_masses=water, porosities=porosity.astype(np.float32), gas_moles=moles,
-                     notes=dict(seed=seed, field_seed=field_seed, p0=p0, poisson=0.45, voxel=voxel))
+    return SceneSpec("loaf_256", mesh, samples, sim, material, _oven_ding(0.05, p0, gas_production, yield_stress, viscosity),
+                     _oven_transport(T0, 473.15),
+                     steps=steps, water_masses=water, porosities=porosity.astype(np.float32), gas_moles=moles,
+                     notes=dict(seed=seed, field_seed=field_seed, p0=p0, poisson=0.45, voxel=voxel, young=young,
+                                gas_production=gas_production, yield_stress=yield_stress, viscosity=viscosity))
 
 
 def dough_shapes(count=48, seed=2, half_extent=(0.05, 0.15), max_particles=60000):
@@ -324,7 +343,8 @@ def dough_scene(shape, axes, res, seed, name="dough", p0=1000.0, porosity_range=
     T0 = 298.15
     sim = SimulationConfig(grid_resolution=res, grid_spacing=dx, grid_origin=(0.0, 0.0, 0.0), time_step=dt,
                            gravity=(0.0, -9.8, 0.0), boundary_cells=3, device_name="auto")
-    return SceneSpec(name, mesh, samples, sim, material, _oven_ding(0.05, p0), _oven_transport(T0, 473.15), steps=1000,
+    return SceneSpec(name, mesh, samples, sim, material, _oven_ding(0.05, p0, yield_stress=SCALED_YIELD_STRESS),
+                     _oven_transport(T0, 473.15), steps=1000,
                      porosities=porosity.astype(np.float32), gas_moles=p0 * porosity / (gas_constant * T0),
                      notes=dict(shape=shape, axes=axes.tolist(), res=res, seed=seed, p0=p0))
 

@@ -314,16 +314,22 @@ def _simulators(sim):
     return getattr(sim, "doughs", None) or [sim]
 
 
-def attach(sim, model, strict=True, same_material=False):
-    """Start the coupled solves of ``sim`` (MpmSimulator or BatchSimulator) from ``model``'s predictions.
+MODES = {"guided": 1, "standin": 2}
 
-    Every dough with a coupled (gas) Ding return map uses it (``strict``: raise
-    if a dough has none).  The network only supplies the starting point of
-    each solve, whose accepted roots are the exact ones, so a dough whose
-    material parameters or time step differ from the training run's may use
-    it too (it may just need more Newton steps); ``same_material`` restricts it
-    to doughs whose parameters match.  Returns the number of doughs using it.
+
+def attach(sim, model, strict=True, same_material=False, mode="standin"):
+    """Use ``model`` in the coupled solves of ``sim`` (MpmSimulator or BatchSimulator).
+
+    ``mode`` "standin": the network's returned porosity, corrected by one Newton step of the porosity equation,
+    is the result when that step is at most 2e-5 (then within about a float ulp of the exact root); otherwise
+    the solve runs exactly, starting from the prediction.  "guided": every solve starts from the prediction and
+    converges to the exact root, so results equal the exact return map's.
+    Every dough with a coupled (gas) Ding return map uses it (``strict``: raise if a dough has none).
+    ``same_material`` restricts it to doughs whose material parameters match the training run's.  Returns the
+    number of doughs using it.
     """
+    if mode not in MODES:
+        raise ValueError(f"mode must be one of {sorted(MODES)}")
     lib = _native.load()
     trained = model.meta.get("signature")
     using = []
@@ -333,16 +339,16 @@ def attach(sim, model, strict=True, same_material=False):
             raise ValueError("the dough has no coupled (gas) return map to start from the surrogate")
         if ok and same_material and trained is not None:
             ok = material_signature(s) == trained
-        using.append(ok)
+        using.append(MODES[mode] if ok else 0)
     sim._surrogate_blob = model.blob_tensor("cuda")
     _native.check(lib.bk_workspace_set_surrogate(sim._workspace, _native.ptr(sim._surrogate_blob)), "bk_workspace_set_surrogate")
     _set_surrogate_flags(sim, using)
-    return sum(using)
+    return sum(1 for u in using if u)
 
 
 def detach(sim):
     lib = _native.load()
-    _set_surrogate_flags(sim, [False] * len(_simulators(sim)))
+    _set_surrogate_flags(sim, [0] * len(_simulators(sim)))
     _native.check(lib.bk_workspace_set_surrogate(sim._workspace, None), "bk_workspace_set_surrogate")
     sim._surrogate_blob = None
 
@@ -351,7 +357,7 @@ def _set_surrogate_flags(sim, using):
     if hasattr(sim, "doughs"):
         sim.SetSurrogate(using)
     else:
-        sim.SetSurrogate(bool(using[0]))
+        sim.SetSurrogate(using[0])
 
 
 def stats(sim, reset=False):

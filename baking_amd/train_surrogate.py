@@ -6,8 +6,7 @@ Runs the 256^3 loaf (0.72M particles) for ``--steps`` substeps with the exact
 coupled return map, emitting (inputs, exact outputs) rows for every yielding
 particle on every ``--stride``-th substep, up to ``--rows`` rows plus a
 held-out tail.  Collection stops at the first substep that raises a device
-flag (the loaf reaches its gas-acoustic CFL bound at substep 806), so every
-row comes from a state the simulator accepts.  Rows are appended in substep order, so the tail comes from
+flag, so every row comes from a state the simulator accepts.  Rows are appended in substep order, so the tail comes from
 the latest emitting substeps and serves as a hold-out later in the bake than
 anything trained on.  Trains with the library's FP32 kernels and Adam on the
 GPU and writes the model (``--out``) and its held-out error table

@@ -75,7 +75,7 @@ MLP_MACS = 7 * 64 + 3 * 64 * 64 + 64 * 4  # multiply-accumulates of the surrogat
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed substeps (default: 1000 batched_48, 750 loaf_256, 1000 sphere_small)")
+    ap.add_argument("--steps", type=int, default=None, help="timed substeps (default: 1000 batched_48, 2000 loaf_256, 1000 sphere_small)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="batched_48",
                     choices=["batched_48", "loaf_256", "sphere_small", "adjoint_loaf", "rest_loaf"])
@@ -301,6 +301,23 @@ def run_batched(args):
         stop.synchronize()
         plain_ms += start.elapsed_time(stop)
     plain_rate = n * args.plain_steps / (plain_ms * 1e-3)
+    # the same doughs alone, each substep one HIP graph replay
+    plain_graph_ms = 0.0
+    for s in sims:
+        s.use_graphs = True
+        for _ in range(2):
+            s._launch_substep()
+            s._accept()
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(args.plain_steps):
+            s._launch_substep()
+            s._accept()
+        stop.record(stream)
+        stop.synchronize()
+        plain_graph_ms += start.elapsed_time(stop)
+        s._destroy_graphs()
+    plain_graph_rate = n * args.plain_steps / (plain_graph_ms * 1e-3)
 
     batch = BatchSimulator(sims)
     model = None
@@ -323,10 +340,18 @@ def run_batched(args):
     flags_exact = batch.ReadFlags()
     result_method = "batched (exact return map)"
     value_ms, stats_used, aggr_used = exact_ms, exact_stats, exact_aggr
-    accuracy = None
+    accuracy = guided = None
     if model is not None:
+        # the network guiding every solve to the exact root, then standing in for it (the measured method)
         batch._restore_after_tuning(snapshot)
-        surrogate.attach(batch, model)
+        surrogate.attach(batch, model, mode="guided")
+        guided_ms = _timed(batch, steps, stream)
+        g_aggr, g_stats = measure(batch), surrogate.stats(batch, reset=True)
+        guided = {"value": n * steps / (guided_ms * 1e-3), "ms_per_step": guided_ms / steps,
+                  "evaluations_per_map": g_stats["evaluations"] / max(g_stats["return_maps"], 1),
+                  "accuracy_vs_exact": _relative_errors(exact_aggr, g_aggr, start_aggr)}
+        batch._restore_after_tuning(snapshot)
+        surrogate.attach(batch, model, mode="standin")
         value_ms = _timed(batch, steps, stream)
         aggr_used, stats_used = measure(batch), surrogate.stats(batch, reset=True)
         accuracy = _relative_errors(exact_aggr, aggr_used, start_aggr)
@@ -377,9 +402,14 @@ def run_batched(args):
                                     "used": [[int(d), int(k)] for d, k in zip(*np.nonzero(flags))]}},
         "plain": {"value": plain_rate, "unit": "particle-substeps/s", "steps_per_dough": args.plain_steps,
                   "method": "each dough alone, single-dough substep launched kernel by kernel (no graph), exact return map",
-                  "ms_total": plain_ms},
+                  "ms_total": plain_ms,
+                  "graph": {"value": plain_graph_rate, "ms_total": plain_graph_ms,
+                            "method": "each dough alone, single-dough substep as one HIP graph replay, exact return map"}},
         "speedup_vs_plain": value / plain_rate,
+        "speedup_vs_plain_graph": value / plain_graph_rate,
         "batched_exact": {"value": exact_rate, "ms_per_step": exact_ms / steps},
+        "surrogate_gain_vs_batched_exact": value / exact_rate,
+        "surrogate_guided": guided,
         "accuracy_vs_exact": accuracy,
         "roofline": roof,
         "cpu_baseline": None,
@@ -612,7 +642,7 @@ def main():
     from baking_amd import surrogate
     from baking_amd.measure import measure
     spec = build_scene(args.workload)
-    # loaf_256: the scene's 750 substeps; by substep 806 its gas-acoustic CFL bound is reached (2000 are not runnable)
+    # loaf_256: the scene's 2000 substeps
     steps = args.steps or (spec.steps if spec.name == "loaf_256" else 1000)
     sim = build_simulator(spec)
     sim.use_graphs = not args.no_graph

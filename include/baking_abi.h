@@ -69,8 +69,11 @@ typedef struct bk_params {
   /* coupled-porosity solve: 0 = the reference's 64-step bisection (porous_return.py:43-61),
    * 1 = a safeguarded Newton solve of the same equation inside the bisection's bracket */
   int32_t return_solver;
-  /* 1: start each coupled-porosity solve from the return-map network's prediction (needs a network set
-   * with bk_workspace_set_surrogate; no reference counterpart), 0: from the old porosity */
+  /* the return-map network (set with bk_workspace_set_surrogate; no reference counterpart) in the coupled
+   * porosity solve: 0 not used (the solve starts from the old porosity); 1 guided: every solve starts from the
+   * network's prediction and converges to the exact root; 2 stand-in: the prediction, corrected by one Newton
+   * step of the porosity equation, is the result when that step is at most 2e-5 (within about a float ulp of
+   * the root), else the guided and then the exact solve run */
   int32_t surrogate;
   /* 1: sticky floor -- after the reference's frictionless unilateral walls (physics/grid.py:32-37), every node
    * of the floor's boundary_cells layers (j < boundary_cells) gets velocity 0 (no reference counterpart; 0 keeps

@@ -329,28 +329,33 @@ def test_crowded_block_sort_is_the_stable_sort_of_the_keys():
 @pytest.mark.parametrize("sticky", [True, False])
 def test_sticky_floor_on_the_gpu(sticky):
     """The fused substep's floor option (MpmSimulator.SetStickyFloor; no reference counterpart) against the oracle
-    with the same option, and its closed form: particles whose whole stencil lies in the floor layers keep their
-    position and deformation gradient exactly, every occupied floor node has velocity 0.  Off, the reference's
-    frictionless floor lets them slide."""
+    with the same option: a cube sliding at 0.5 m/s with its bottom just inside the simulator's initial domain
+    (2.04 dx up, so its lowest particles reach the floor layers).  Sticky, every occupied floor-layer node has
+    velocity 0 and the bottom particles are braked (below 0.2 m/s within 50 substeps); off, the reference's
+    frictionless floor lets them slide at 0.5 m/s."""
     from tests.test_oracle_walls import sliding_cube
-    spec = sliding_cube(sticky)
+    spec = sliding_cube(sticky, bottom_cells=2.04)
     gpu = scenes.build_simulator(spec)
     gpu.SetStickyFloor(sticky)
     o32, o64 = oracle_from_spec(spec, "f32", sticky_floor=sticky), oracle_from_spec(spec, "f64", sticky_floor=sticky)
     sim = spec.simulation
     res, bc, dx = sim.grid_resolution, sim.boundary_cells, sim.grid_spacing
-    x0 = gpu.current_state.particle_positions.cpu().numpy().copy()
-    deep = (x0[:, 1] - sim.grid_origin[1]) / dx < 1.5
+    low = (gpu.current_state.particle_positions.cpu().numpy()[:, 1] - sim.grid_origin[1]) / dx < 2.5
+    assert low.sum() > 0
     for step in range(1, spec.steps + 1):
         gpu.Step()
         o32.Step()
         o64.Step()
         if step % 10 == 0:
             check_fields(f"{spec.name} substep {step}", gpu_fields(gpu), o32, o64)
+            m = gpu.grid_state.node_masses.cpu().numpy().reshape((res,) * 3)
+            v = gpu.grid_state.node_velocities.cpu().numpy().reshape((res,) * 3 + (3,))
+            occupied = m[:, :bc] > 0
+            assert occupied.any()
             if sticky:
-                m = gpu.grid_state.node_masses.cpu().numpy().reshape((res,) * 3)
-                v = gpu.grid_state.node_velocities.cpu().numpy().reshape((res,) * 3 + (3,))
-                assert np.all(v[:, :bc][m[:, :bc] > 0] == 0.0)
-                np.testing.assert_array_equal(gpu.current_state.particle_positions.cpu().numpy()[deep], x0[deep])
-    if not sticky:
-        assert np.all(np.abs(gpu.current_state.particle_velocities.cpu().numpy()[deep, 0] - 0.5) < 0.05)
+                assert np.all(v[:, :bc][occupied] == 0.0)
+    vx = gpu.current_state.particle_velocities.cpu().numpy()[low, 0]
+    if sticky:
+        assert vx.max() < 0.2
+    else:
+        assert np.all(np.abs(vx - 0.5) < 0.05)

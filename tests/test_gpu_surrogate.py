@@ -8,7 +8,9 @@
 * sample collection keeps no row of a substep after which a device flag is up;
 * a yielding gas dough run with a network trained on its own emitted rows
   gives the exact run's fields (the guided solve converges to the same root)
-  with fewer evaluations of the porosity equation.
+  with fewer evaluations of the porosity equation; standing in (the
+  prediction kept after one Newton correction) meets the parity rule
+  against the oracle with fewer evaluations than the exact method.
 """
 import numpy as np
 import pytest
@@ -142,34 +144,36 @@ def test_guided_return_map_matches_exact_with_fewer_evaluations():
     print("held-out:", {k: v["mean_abs"] for k, v in report.items() if isinstance(v, dict)})
 
     runs = {}
-    for name in ("exact", "guided"):
+    for name in ("exact", "guided", "standin"):
         sim = scenes.build_simulator(spec)
-        if name == "guided":
-            assert surrogate.attach(sim, model, strict=False) == 1
+        if name != "exact":
+            assert surrogate.attach(sim, model, strict=False, mode=name) == 1
         surrogate.stats(sim, reset=True)
         sim.Advance(steps)
         runs[name] = (gpu_fields(sim), surrogate.stats(sim))
-    st_e, st_g = runs["exact"][1], runs["guided"][1]
-    print("exact stats", st_e, "guided stats", st_g)
+    st_e, st_g, st_s = runs["exact"][1], runs["guided"][1], runs["standin"][1]
+    print("exact stats", st_e, "guided stats", st_g, "stand-in stats", st_s)
     assert st_g["return_maps"] == st_e["return_maps"] > 0
     assert st_g["accepted"] >= 0.8 * st_g["return_maps"]
-    per_exact = st_e["evaluations"] / st_e["return_maps"]
-    per_guided = st_g["evaluations"] / st_g["return_maps"]
-    print(f"porosity-equation evaluations per return map: exact {per_exact:.2f}, guided {per_guided:.2f}")
-    assert per_guided < per_exact
+    per = {k: v["evaluations"] / v["return_maps"] for k, v in (("exact", st_e), ("guided", st_g), ("standin", st_s))}
+    print("porosity-equation evaluations per return map: " + ", ".join(f"{k} {v:.2f}" for k, v in per.items()))
+    # a stand-in prediction is kept only when one Newton correction moves it by at most 2e-5; the rest fall back to
+    # the exact solve, so with this small network stand-in need not beat guided, only the exact method
+    assert per["guided"] < per["exact"] and per["standin"] < per["exact"]
 
     o32, o64 = oracle_from_spec(spec, "f32"), oracle_from_spec(spec, "f64")
     for _ in range(steps):
         o32.Step()
         o64.Step()
-    for name in ("exact", "guided"):
+    for name in ("exact", "guided", "standin"):
         check_fields(f"{name} run against the oracle", runs[name][0], o32, o64)
     check_same("guided run against the exact run", runs["guided"][0], runs["exact"][0], o32, o64)
 
 
 def test_packaged_network_gain_on_the_loaf():
-    """The packaged network (trained on the 256^3 loaf) against the method without it, from one state:
-    fewer porosity-equation evaluations and a faster return-map stage, same aggregates."""
+    """The packaged network (trained on the 256^3 loaf) against the method without it, from one state: guided,
+    fewer porosity-equation evaluations and the exact run's aggregates; standing in, fewer still, a faster
+    return-map stage, and aggregates within 1e-6 (1e-3 of their increments) of the exact run's."""
     import ctypes
     from baking_amd.measure import measure
     model = surrogate.ReturnMapSurrogate.load()
@@ -179,10 +183,12 @@ def test_packaged_network_gain_on_the_loaf():
     start = measure(sim)
     lib = _native.load()
     results = {}
-    for name in ("exact", "network"):
+    for name in ("exact", "guided", "standin"):
         sim._restore_after_tuning(snap)
-        if name == "network":
-            assert surrogate.attach(sim, model) == 1
+        if name == "exact":
+            surrogate.detach(sim)
+        else:
+            assert surrogate.attach(sim, model, mode=name) == 1
         sim._launch_substeps(2)  # capture both graphs
         sim._restore_after_tuning(snap)
         surrogate.stats(sim, reset=True)
@@ -199,21 +205,24 @@ def test_packaged_network_gain_on_the_loaf():
                                          ctypes.cast(ctypes.pointer(rec), ctypes.c_void_p)), "bk_profile_end")
         stages = {lib.bk_profile_stage_name(k).decode(): ms[k] / rec.value for k in range(nst)}
         results[name] = (aggr, st, stages["surrogate"] + stages["pass_c_deferred"])
-    (ae, se, te), (an, sn, tn) = results["exact"], results["network"]
-    pe, pn = se["evaluations"] / se["return_maps"], sn["evaluations"] / sn["return_maps"]
-    print(f"evaluations per return map: exact {pe:.2f}, network {pn:.2f}; return-map stage ms: exact {te:.3f}, network {tn:.3f}")
-    print("accepted", sn["accepted"], "of", sn["return_maps"])
-    # the two runs yield in (nearly) the same particles: the order of the float atomics that accumulate the grid
-    # (and of each cell's particles, which follows atomics) makes their trajectories differ in the last bits, and
-    # the particles that sit on the yield threshold then go either way
-    assert se["return_maps"] > 100000 and abs(sn["return_maps"] - se["return_maps"]) <= 1e-4 * se["return_maps"]
-    assert sn["accepted"] >= 0.95 * sn["return_maps"]
-    assert pn < 0.5 * pe
-    assert tn < te
-    for key in ("particle_volume_m3", "temperature_mean_k", "water_mass_kg", "gas_total_mol"):
-        rel = abs(an[key][0] - ae[key][0]) / abs(ae[key][0])
-        inc = abs(ae[key][0] - start[key][0])
-        print(key, rel, abs(an[key][0] - ae[key][0]) / inc if inc else 0.0)
-        assert rel <= 1e-6
-        if inc > 0:
-            assert abs(an[key][0] - ae[key][0]) <= 1e-3 * inc
+    ae, se, te = results["exact"]
+    pe = se["evaluations"] / se["return_maps"]
+    for name in ("guided", "standin"):
+        an, sn, tn = results[name]
+        pn = sn["evaluations"] / sn["return_maps"]
+        print(f"{name}: evaluations per return map exact {pe:.2f}, {name} {pn:.2f}; return-map stage ms exact {te:.3f}, "
+              f"{name} {tn:.3f}; accepted {sn['accepted']} of {sn['return_maps']}", flush=True)
+        # the runs yield in (nearly) the same particles: the order of the float atomics that accumulate the grid
+        # makes their trajectories differ in the last bits, and particles on the yield threshold go either way
+        assert se["return_maps"] > 100000 and abs(sn["return_maps"] - se["return_maps"]) <= 1e-4 * se["return_maps"]
+        assert sn["accepted"] >= 0.95 * sn["return_maps"]
+        assert pn < 0.5 * pe
+        assert tn < te
+        for key in ("particle_volume_m3", "temperature_mean_k", "water_mass_kg", "gas_total_mol"):
+            rel = abs(an[key][0] - ae[key][0]) / abs(ae[key][0])
+            inc = abs(ae[key][0] - start[key][0])
+            print(name, key, rel, abs(an[key][0] - ae[key][0]) / inc if inc else 0.0, flush=True)
+            assert rel <= 1e-6
+            if inc > 0:
+                assert abs(an[key][0] - ae[key][0]) <= 1e-3 * inc
+    assert results["standin"][1]["evaluations"] < results["guided"][1]["evaluations"]

@@ -116,16 +116,16 @@ def test_cube_stops_at_each_wall(axis, side):
     assert p_end < -0.1 * p0, (p_end, p0)  # stopped and rebounding from the wall
 
 
-def sliding_cube(sticky):
-    """A 0.1 x 0.05 x 0.1 m elastic cube whose bottom lies in the floor layers (y = 0.6 dx), sliding along x at
-    0.5 m/s under gravity on the default 40^3 grid."""
+def sliding_cube(sticky, bottom_cells=0.6):
+    """A 0.1 x 0.05 x 0.1 m elastic cube with its bottom at y = bottom_cells dx (inside the floor layers at 0.6), sliding
+    along x at 0.5 m/s under gravity on the default 40^3 grid."""
     from baking_amd import scenes
     from baking_amd.config import DingConfig, MaterialConfig, SimulationConfig, TransportConfig
     from baking_amd.geometry.mesh_loader import VoxelTetMesh
     from baking_amd.geometry.mesh_validation import ValidateTetMesh
     from baking_amd.geometry.particle_sampling import SampleParticles
     sim = SimulationConfig(device_name="auto")
-    verts, tets = VoxelTetMesh((4, 2, 4), 0.025, (0.4, 0.6 * sim.grid_spacing, 0.4))
+    verts, tets = VoxelTetMesh((4, 2, 4), 0.025, (0.4, bottom_cells * sim.grid_spacing, 0.4))
     mesh = ValidateTetMesh(verts, tets)
     spec = scenes.SceneSpec("sticky_floor" if sticky else "frictionless_floor", mesh, SampleParticles(mesh, 1000.0, 4), sim,
                             MaterialConfig(), DingConfig(), TransportConfig(), steps=50, initial_velocity=(0.5, 0.0, 0.0))
