@@ -123,6 +123,24 @@ __device__ __forceinline__ int64_t flatten_node(int i, int j, int k, int res) { 
 __device__ __forceinline__ int64_t tiled_node(int i, int j, int k, int nb) {
   return ((int64_t((i >> 2) * nb + (j >> 2)) * nb + (k >> 2)) << 6) | ((i & 3) << 4) | ((j & 3) << 2) | (k & 3);
 }
+// The tiled number is a sum of one term per axis (block part times its stride plus the position inside the block),
+// so the 27 nodes of a stencil at base b are x[o0] + y[o1] + z[o2], each term formed once per axis, in 32 bits
+// (a dough's res^3 nodes stay below 2^30).  scale multiplies every term (3: float index into an xyz array).
+struct StencilTerms {
+  uint32_t x[3], y[3], z[3];
+};
+__device__ __forceinline__ StencilTerms stencil_terms(const int b[3], int nb, uint32_t scale = 1u) {
+  StencilTerms t;
+  const uint32_t sx = uint32_t(nb) * uint32_t(nb) * 64u, sy = uint32_t(nb) * 64u;
+#pragma unroll
+  for (int o = 0; o < 3; ++o) {
+    const uint32_t i = uint32_t(b[0] + o), j = uint32_t(b[1] + o), k = uint32_t(b[2] + o);
+    t.x[o] = scale * ((i >> 2) * sx + ((i & 3u) << 4));
+    t.y[o] = scale * ((j >> 2) * sy + ((j & 3u) << 2));
+    t.z[o] = scale * (((k >> 2) << 6) + (k & 3u));
+  }
+  return t;
+}
 
 // ------------------------------------------------------------------ SVD (double)
 // Symmetric cyclic Jacobi on A = F^T F (Golub & Van Loan, Matrix Computations,

@@ -72,6 +72,7 @@ constexpr int kSortThreads = 256;
 constexpr int kSortItems = 8;
 constexpr int kSortTile = kSortThreads * kSortItems;  // keys per workgroup
 static_assert(kSortTile == kRadixTile, "the radix sort's scratch sizing in bk_workspace.h");
+constexpr int kMaxDoughRes = 1024;                    // 3 * res^3 stays below 2^32 (stencil_terms)
 constexpr int kParticleThreads = 256;                 // launch bound of the per-particle kernels
 
 inline int div_up(long long a, long long b) { return int((a + b - 1) / b); }
@@ -943,8 +944,9 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
 }
 
 // ============================================================================ pass B1: scalar G2P, gas, stresses
-__global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in,
-                                                              bk_state out, bk_grid g, bk_workspace* ws, int n) {
+// Five wavefronts per SIMD: the 27-node gather's terms fit in 88 VGPRs without spilling (128 unconstrained, four).
+__global__ __launch_bounds__(kParticleThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_pass_b1(
+    const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_state out, bk_grid g, bk_workspace* ws, int n) {
   int p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= n) return;
   per_dough(dough_of_particle(sc.dough_of, p), [&](int d) {
@@ -971,6 +973,8 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __
         axis_weights(l[0], wx);
         axis_weights(l[1], wy);
         axis_weights(l[2], wz);
+        const double2* __restrict__ tw = ws->dTW + node_base;
+        const StencilTerms st = stencil_terms(b, nb);
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox)
 #pragma unroll
@@ -978,8 +982,7 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_b1(const bk_dough* __
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
               double wt = double(wx[ox] * wy[oy] * wz[oz]);
-              int64_t node = node_base + tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, nb);
-              const double2 q = ws->dTW[node];  // {dT, dW} in one 16-byte load
+              const double2 q = tw[st.x[ox] + st.y[oy] + st.z[oz]];  // {dT, dW} in one 16-byte load
               tc += wt * q.x;
               mr += wt * q.y;
             }
@@ -1358,8 +1361,11 @@ __global__ void k_freeze(bk_workspace* ws, int nd) {
 }
 
 // ============================================================================ pass C: G2P + Ding + gas refresh + saturation
-__global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in,
-                                                             bk_state out, bk_grid g, bk_workspace* ws, int n) {
+// Four wavefronts per SIMD (at most 128 VGPRs): left to itself the compiler takes 134 and the SIMD holds three,
+// too few to cover the node gather and the fp64 SVD's dependent divide and square-root chains; five (96 VGPRs)
+// spill to scratch and run slower than three.
+__global__ __launch_bounds__(kParticleThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_pass_c(
+    const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_state out, bk_grid g, bk_workspace* ws, int n) {
   __shared__ int lds[17];
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   bool deferred = false;
@@ -1392,6 +1398,8 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __r
         float nv[3] = {0.f, 0.f, 0.f};
         mat3<float> Cn = m_zero<float>();
         const float s4 = 4.0f / (P.dx * P.dx);
+        const float* __restrict__ gv = g.v + 3 * node_base;
+        const StencilTerms st = stencil_terms(b, nb, 3u);
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox)
 #pragma unroll
@@ -1399,8 +1407,8 @@ __global__ __launch_bounds__(kParticleThreads) void k_pass_c(const bk_dough* __r
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
               float wt = wx[ox] * wy[oy] * wz[oz];
-              int64_t node = node_base + tiled_node(b[0] + ox, b[1] + oy, b[2] + oz, nb);
-              float vi[3] = {g.v[3 * node + 0], g.v[3 * node + 1], g.v[3 * node + 2]};
+              const float* vp = gv + (st.x[ox] + st.y[oy] + st.z[oz]);
+              float vi[3] = {vp[0], vp[1], vp[2]};
               float dd[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
               float sw = s4 * wt;
 #pragma unroll
@@ -1731,6 +1739,8 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
     if (D.params.settling < 0 || D.params.settling > 2) return fail("bk_workspace_set_doughs: params.settling must be 0, 1 or 2");
     if (D.params.res < 4 || D.params.n < 1 || !(D.params.dx > 0.0f))
       return fail("bk_workspace_set_doughs: every dough needs res >= 4, n >= 1 and dx > 0");
+    // the G2P gathers form a dough's float index 3 * node in 32 bits
+    if (D.params.res > kMaxDoughRes) return fail("bk_workspace_set_doughs: a dough's grid resolution is at most 1024");
     if (D.particle_offset != n || D.block_offset != blocks)
       return fail("bk_workspace_set_doughs: doughs must be packed in order (particle_offset, block_offset)");
     long long nb = (D.params.res + 3) / 4;
