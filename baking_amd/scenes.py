@@ -53,16 +53,6 @@ class SceneSpec:
     notes: dict = field(default_factory=dict)
 
 
-def block_major(positions, dx, origin=(0.0, 0.0, 0.0)):
-    """The particles reordered 4x4x4-block-major: by the block of the grid cell they lie in (row-major over blocks),
-    then by cell inside the block, keeping their order within a cell -- the order the substep's sort groups
-    them in, so each P2G tile's particles are contiguous in memory."""
-    c = np.floor((np.asarray(positions) - np.asarray(origin)) / dx).astype(np.int64)
-    b, l = c >> 2, c & 3
-    keys = np.stack([b[:, 0], b[:, 1], b[:, 2], l[:, 0], l[:, 1], l[:, 2]])
-    return np.lexsort(keys[::-1])
-
-
 def jittered_cells(inside, lo, hi, dx, seed):
     """8 jittered particles per cell (2x2x2) for cells overlapping [lo, hi]; keep those with inside(x)."""
     c0 = np.floor(np.asarray(lo) / dx).astype(int)
