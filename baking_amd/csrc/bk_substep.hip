@@ -878,16 +878,49 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
 
 // ============================================================================ diffusion (thermal_water.py:140-190, GridFields fused)
 // 64 consecutive threads own one active node block, so the block (and its
-// dough) is wave-uniform.
+// dough) is wave-uniform.  The wavefront stages {C, V, E, W} of its block's 64
+// nodes and of the 96 nodes just outside its six faces in LDS (coalesced
+// loads of the block, one load per halo node), then every lane takes its six
+// neighbours from LDS.  Face f = 2 * axis + side holds the 16 nodes beyond
+// that side, indexed by the two other axes' positions in the block (lower
+// axis first); a node outside the grid is staged as zeros and never read.
+constexpr int kDiffHalo = 96;
+__device__ __forceinline__ int halo_slot(int ax, int side, const int lc[3]) {
+  const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
+  return (2 * ax + side) * 16 + lc[a1] * 4 + lc[a2];
+}
+
 __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ doughs, int nd, bk_grid g, bk_workspace* ws) {
+  __shared__ double4 stage[4][64 + kDiffHalo];
+  const int wv = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+  double4* const st = stage[wv];
   const int nact = ws->counters[1];
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
     const int blk = __builtin_amdgcn_readfirstlane(ws->active[t >> 6]);
     const BlockGeom bg = block_geom(doughs, dough_of_block(doughs, nd, blk), blk);
     const bk_params& P = doughs[bg.d].params;
-    const int loc = t & 63;
+    const int loc = lane;
     const int64_t node = (int64_t(blk) << 6) | loc;
-    double Ci = g.C[node];
+    const int lc[3] = {loc >> 4, (loc >> 2) & 3, loc & 3};
+    const int c[3] = {4 * bg.bx + lc[0], 4 * bg.by + lc[1], 4 * bg.bz + lc[2]};
+    const double Ci = g.C[node];
+    st[loc] = make_double4(Ci, g.V[node], g.E[node], g.W[node]);
+    for (int h = lane; h < kDiffHalo; h += 64) {
+      const int f = h >> 4, ax = f >> 1, side = f & 1;
+      const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
+      int q[3];
+      q[ax] = side ? 4 : -1;
+      q[a1] = (h >> 2) & 3;
+      q[a2] = h & 3;
+      const int o[3] = {4 * bg.bx + q[0], 4 * bg.by + q[1], 4 * bg.bz + q[2]};
+      double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (o[ax] >= 0 && o[ax] < P.res) {
+        const int64_t other = bg.node_base + tiled_node(o[0], o[1], o[2], bg.nb);
+        v = make_double4(g.C[other], g.V[other], g.E[other], g.W[other]);
+      }
+      st[64 + h] = v;
+    }
+    wave_sync();
     bool heat_bad = false, water_bad = false;
     if (Ci > 0.0) {
       const double rho_w = double(P.rho_w);
@@ -896,29 +929,30 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
       const bool rest = resting(P, ws, bg.d);
       const double hk = rest ? 0.0 : double(P.kappa * P.dx);
       const double wk = rest ? 0.0 : double(P.diffusivity * P.rho_w * P.dx);
-      int c[3] = {4 * bg.bx + (loc >> 4), 4 * bg.by + ((loc >> 2) & 3), 4 * bg.bz + (loc & 3)};
-      double Vi = g.V[node];
-      double Ti = g.E[node] / Ci;
-      double Si = g.W[node] / (rho_w * Vi);
+      const double4 own = st[loc];
+      double Vi = own.y;
+      double Ti = own.z / Ci;
+      double Si = own.w / (rho_w * Vi);
       double hr = g.H[node], wr = -g.L[node], hd = g.X[node], wd = 0.0;
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax)
 #pragma unroll
         for (int side = 0; side < 2; ++side) {
-          int o[3] = {c[0], c[1], c[2]};
-          o[ax] += 2 * side - 1;
-          if (o[ax] < 0 || o[ax] >= P.res) continue;
-          int64_t other = bg.node_base + tiled_node(o[0], o[1], o[2], bg.nb);
-          double Cj = g.C[other];
+          const int step = 2 * side - 1;
+          if (c[ax] + step < 0 || c[ax] + step >= P.res) continue;
+          int ln[3] = {lc[0], lc[1], lc[2]};
+          ln[ax] += step;
+          const double4 nb4 = (ln[ax] >= 0 && ln[ax] < 4) ? st[(ln[0] << 4) | (ln[1] << 2) | ln[2]] : st[64 + halo_slot(ax, side, lc)];
+          double Cj = nb4.x;
           if (Cj > 0.0) {
-            double Vj = g.V[other];
-            double Tj = g.E[other] / Cj;
+            double Vj = nb4.y;
+            double Tj = nb4.z / Cj;
             double fill = 2.0 * Vi * Vj / (Vi + Vj) / dx3;
             double hc = hk * fill;
             hr += hc * (Tj - Ti);
             hd += hc;
             if (P.water_enabled) {
-              double Sj = g.W[other] / (rho_w * Vj);
+              double Sj = nb4.w / (rho_w * Vj);
               double mob = wk * fill * 0.5 * (Ti + Tj) * 0.5 * (Si + Sj);
               wr += mob * (Sj - Si);
               wd += mob / (rho_w * Vi);
@@ -927,7 +961,7 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
         }
       heat_bad = dt * hd > Ci;
       double change = dt * wr;
-      const double Wi = g.W[node];
+      const double Wi = own.w;
       water_bad = Wi + change < 0.0 || dt * wd > 1.0;
       const double dT = dt * hr / Ci;
       // the relative water change dW / W each particle gathers (thermal_water.py:222-223), once per node
@@ -940,6 +974,7 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
     }
     flag_max(dough_flags(ws, bg.d), BK_FLAG_HEAT, heat_bad);
     flag_max(dough_flags(ws, bg.d), BK_FLAG_WATER, water_bad);
+    wave_sync();  // every lane has read the stage before the next block overwrites it
   }
 }
 

@@ -17,16 +17,26 @@
 //   float  bias[4][64]      hidden layers 1..4
 //   float  bias_out[16]     outputs 0..3, then zeros
 //   uint16 w1[4][64][4]     bf16 tiles of W1, layer-1 input padded to 16
-//   uint16 w2..w4[16][64][4]
-//   uint16 w5[4][64][4]     W5 padded to 16 output units
-// Each tile is one 16x16 A operand of v_mfma_f32_16x16x16_bf16 in lane order:
-// lane l, element i holds W[16*mu + l%16][16*kt + 4*(l/16) + i] (zero outside
-// the matrix), tile index mu*KT + kt.
+//   uint16 w2..w4[8][64][8]
+//   uint16 w5[2][64][8]     W5 padded to 16 output units
+// w1's tiles are 16x16 A operands of v_mfma_f32_16x16x16_bf16 in lane order:
+// lane l, element i holds W1[16*mu + l%16][4*(l/16) + i] (zero outside the
+// matrix), tile index mu.  The other layers use v_mfma_f32_16x16x32_bf16: tile
+// (mu, kc) is a 16x32 A operand whose lane l, element j holds
+// W[16*mu + l%16][unit(kc, l/16, j)] with
+//   unit(kc, g, j) = 32*kc + 4*g + j          (j < 4)
+//                    32*kc + 16 + 4*g + j - 4  (j >= 4),
+// tile index mu*2 + kc.  The instruction pairs A's element j of lane group g
+// with B's element j of lane group g (one K slot); the sum over K does not
+// depend on which unit a slot stands for, so the slots follow the units a lane
+// already holds (below).
 //
 // A layer computes H^T = W * X^T with the units as rows (A = weights) and the
-// particles as columns (B = the previous layer), so the 16x16 output tile a
-// lane holds -- D[4*(l/16) + i][l%16] -- is exactly the B operand of the next
-// layer: activations never leave registers between layers.
+// particles as columns (B = the previous layer).  The 16x16 output tile mu a
+// lane holds -- D[4*(l/16) + i][l%16], units 16*mu + 4*(l/16) + i -- is the
+// B operand of 16x16x16 as it is, and tiles 2*kc and 2*kc + 1 side by side
+// are the B operand of chunk kc of 16x16x32 in the unit order above:
+// activations never leave registers between layers.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -57,12 +67,24 @@ constexpr int kBlobBias = kBlobOutScale + 16;              // float[4][64]
 constexpr int kBlobBiasOut = kBlobBias + 4 * 64 * 4;       // float[16]
 constexpr int kBlobW1 = kBlobBiasOut + 16 * 4;             // 4 tiles
 constexpr int kBlobTile = 64 * 8;                          // bytes per 16x16 bf16 tile
-constexpr int kBlobW2 = kBlobW1 + 4 * kBlobTile;           // 16 tiles each
-constexpr int kBlobW5 = kBlobW2 + 3 * 16 * kBlobTile;      // 4 tiles
-constexpr int kBlobBytes = kBlobW5 + 4 * kBlobTile;        // 29856
+constexpr int kBlobTile32 = 64 * 16;                       // bytes per 16x32 bf16 tile
+constexpr int kBlobW2 = kBlobW1 + 4 * kBlobTile;           // 8 tiles each
+constexpr int kBlobW5 = kBlobW2 + 3 * 8 * kBlobTile32;     // 2 tiles
+constexpr int kBlobBytes = kBlobW5 + 2 * kBlobTile32;      // 29856
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mfma_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// 16x16x32 BF16 product with FP32 accumulation; operands as bf16 bit patterns.
+__device__ __forceinline__ floatx4 mfma32(const bf16x8 a, const bf16x8 b, const floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(mfma_bf16x8, a), __builtin_bit_cast(mfma_bf16x8, b), c, 0, 0, 0);
+}
+// B operand of chunk kc: output tiles 2*kc and 2*kc + 1 of the previous layer side by side.
+__device__ __forceinline__ bf16x8 chunk(const bf16x4 lo, const bf16x4 hi) {
+  return bf16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
 
 // FP32 -> BF16 bits, round to nearest even (one v_cvt_pk_bf16_f32 on gfx950).
 __device__ __forceinline__ short to_bf16(float x) { return __builtin_bit_cast(short, (__bf16)x); }
@@ -84,7 +106,7 @@ __device__ inline void mlp_wave(const uint8_t* blob, float* xin) {
   const float* bias = reinterpret_cast<const float*>(blob + kBlobBias);
   const float* bias_out = reinterpret_cast<const float*>(blob + kBlobBiasOut);
   const bf16x4* w1 = reinterpret_cast<const bf16x4*>(blob + kBlobW1);
-  const bf16x4* w5 = reinterpret_cast<const bf16x4*>(blob + kBlobW5);
+  const bf16x8* w5 = reinterpret_cast<const bf16x8*>(blob + kBlobW5);
 
   // layer-1 B operands: particle 16*pt + col, inputs 4*grp .. 4*grp+3 (grp >= 2: padding)
   bf16x4 h[4][4];  // [k tile][particle tile]
@@ -119,7 +141,7 @@ __device__ inline void mlp_wave(const uint8_t* blob, float* xin) {
         for (int i = 0; i < 4; ++i) b[i] = to_bf16(silu_fast(acc[mu][pt][i]));
         h[mu][pt] = b;
       }
-    const bf16x4* w = reinterpret_cast<const bf16x4*>(blob + kBlobW2 + layer * 16 * kBlobTile);
+    const bf16x8* w = reinterpret_cast<const bf16x8*>(blob + kBlobW2 + layer * 8 * kBlobTile32);
     const float* bl = bias + 64 * (layer + 1);
 #pragma unroll
     for (int mu = 0; mu < 4; ++mu) {
@@ -129,10 +151,10 @@ __device__ inline void mlp_wave(const uint8_t* blob, float* xin) {
 #pragma unroll
       for (int pt = 0; pt < 4; ++pt) acc[mu][pt] = c;
 #pragma unroll
-      for (int kt = 0; kt < 4; ++kt) {
-        const bf16x4 a = w[(mu * 4 + kt) * 64 + lane];
+      for (int kc = 0; kc < 2; ++kc) {
+        const bf16x8 a = w[(mu * 2 + kc) * 64 + lane];
 #pragma unroll
-        for (int pt = 0; pt < 4; ++pt) acc[mu][pt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h[kt][pt], acc[mu][pt], 0, 0, 0);
+        for (int pt = 0; pt < 4; ++pt) acc[mu][pt] = mfma32(a, chunk(h[2 * kc][pt], h[2 * kc + 1][pt]), acc[mu][pt]);
       }
     }
   }
@@ -153,10 +175,10 @@ __device__ inline void mlp_wave(const uint8_t* blob, float* xin) {
     for (int i = 0; i < 4; ++i) out[pt][i] = bias_out[4 * grp + i];
   }
 #pragma unroll
-  for (int kt = 0; kt < 4; ++kt) {
-    const bf16x4 a = w5[kt * 64 + lane];
+  for (int kc = 0; kc < 2; ++kc) {
+    const bf16x8 a = w5[kc * 64 + lane];
 #pragma unroll
-    for (int pt = 0; pt < 4; ++pt) out[pt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h[kt][pt], out[pt], 0, 0, 0);
+    for (int pt = 0; pt < 4; ++pt) out[pt] = mfma32(a, chunk(h[2 * kc][pt], h[2 * kc + 1][pt]), out[pt]);
   }
   // lanes 0..15 hold output units 0..3 of particles 16*pt + col
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -81,7 +81,7 @@ def bf16_bits(x):
 
 
 def _tiles(W, mus, kts):
-    """Matrix-core A operands of a weight matrix [out][in]: tile (mu, kt), lane l, element i holds
+    """16x16 A operands of v_mfma_f32_16x16x16_bf16 for a weight matrix [out][in]: tile (mu, kt), lane l, element i holds
     W[16 mu + l % 16, 16 kt + 4 (l // 16) + i] (zero outside W)."""
     pad = np.zeros((16 * mus, 16 * kts), np.float32)
     pad[:W.shape[0], :W.shape[1]] = W
@@ -91,6 +91,23 @@ def _tiles(W, mus, kts):
         for kt in range(kts):
             for i in range(4):
                 out[mu, kt, :, i] = pad[16 * mu + lane % 16, 16 * kt + 4 * (lane // 16) + i]
+    return bf16_bits(out.reshape(-1))
+
+
+def _tiles32(W, mus, kcs):
+    """16x32 A operands of v_mfma_f32_16x16x32_bf16 for a weight matrix [out][in]: tile (mu, kc), lane l,
+    element j holds W[16 mu + l % 16, unit] with unit = 32 kc + 4 (l // 16) + j for j < 4 and
+    32 kc + 16 + 4 (l // 16) + j - 4 for j >= 4 (zero outside W): the units the previous layer's output
+    tiles 2 kc and 2 kc + 1 leave in lane l."""
+    pad = np.zeros((16 * mus, 32 * kcs), np.float32)
+    pad[:W.shape[0], :W.shape[1]] = W
+    lane = np.arange(64)
+    out = np.empty((mus, kcs, 64, 8), np.float32)
+    for mu in range(mus):
+        for kc in range(kcs):
+            for j in range(8):
+                unit = 32 * kc + 4 * (lane // 16) + (j if j < 4 else 16 + j - 4)
+                out[mu, kc, :, j] = pad[16 * mu + lane % 16, unit]
     return bf16_bits(out.reshape(-1))
 
 
@@ -104,8 +121,8 @@ def build_blob(theta, in_shift, in_scale, out_shift, out_scale):
     w1 = np.pad(v["W1"], ((0, 0), (0, 8)))  # 64 x 16
     parts = [head.view(np.uint8), bias.view(np.uint8), _tiles(w1, 4, 1).view(np.uint8)]
     for name in ("W2", "W3", "W4"):
-        parts.append(_tiles(v[name], 4, 4).view(np.uint8))
-    parts.append(_tiles(v["W5"], 1, 4).view(np.uint8))
+        parts.append(_tiles32(v[name], 4, 2).view(np.uint8))
+    parts.append(_tiles32(v["W5"], 1, 2).view(np.uint8))
     blob = np.concatenate(parts)
     expected = _native.load().bk_mlp_blob_bytes() if _native.library_path().exists() else blob.size
     if blob.size != expected:

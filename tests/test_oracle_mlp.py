@@ -57,7 +57,8 @@ def test_bf16_rounding_known_values():
 
 
 def test_inference_blob_layout_decodes_to_the_weights():
-    """Decode the product's blob tile by tile (lane l, element i of tile (mu, kt) is W[16mu + l%16][16kt + 4(l//16) + i])."""
+    """Decode the product's blob tile by tile: W1's 16x16 tiles (lane l, element i of tile mu is W1[16mu + l%16][4(l//16) + i]),
+    the other layers' 16x32 tiles (lane l, element j of tile (mu, kc) is W[16mu + l%16][unit32(kc, l//16, j)])."""
     surrogate = pytest.importorskip("baking_amd.surrogate")
     rng = np.random.default_rng(5)
     theta = rng.normal(size=M.COUNT).astype(np.float32)
@@ -75,26 +76,70 @@ def test_inference_blob_layout_decodes_to_the_weights():
     np.testing.assert_array_equal(bias[:256], np.concatenate([p["b1"], p["b2"], p["b3"], p["b4"]]))
     np.testing.assert_array_equal(bias[256:260], p["b5"])
 
-    def decode(offset, mus, kts, rows, cols):
-        t = blob[offset:offset + mus * kts * 512].view(np.uint16).reshape(mus, kts, 64, 4)
-        W = np.zeros((16 * mus, 16 * kts), np.float32)
+    def bits(t):
+        return (t.astype(np.uint32) << 16).view(np.float32)
+
+    def decode16(offset, mus, rows, cols):
+        t = blob[offset:offset + mus * 512].view(np.uint16).reshape(mus, 64, 4)
+        W = np.zeros((16 * mus, 16), np.float32)
         for mu in range(mus):
-            for kt in range(kts):
+            for lane in range(64):
+                for i in range(4):
+                    W[16 * mu + lane % 16, 4 * (lane // 16) + i] = bits(t[mu, lane, i])
+        assert np.all(W[rows:, :] == 0) and np.all(W[:, cols:] == 0)
+        return W[:rows, :cols]
+
+    def decode32(offset, mus, rows, cols):
+        t = blob[offset:offset + mus * 2 * 1024].view(np.uint16).reshape(mus, 2, 64, 8)
+        W = np.zeros((16 * mus, 64), np.float32)
+        for mu in range(mus):
+            for kc in range(2):
                 for lane in range(64):
-                    for i in range(4):
-                        W[16 * mu + lane % 16, 16 * kt + 4 * (lane // 16) + i] = \
-                            (t[mu, kt, lane, i].astype(np.uint32) << 16).view(np.float32)
+                    for j in range(8):
+                        W[16 * mu + lane % 16, unit32(kc, lane // 16, j)] = bits(t[mu, kc, lane, j])
         assert np.all(W[rows:, :] == 0) and np.all(W[:, cols:] == 0)
         return W[:rows, :cols]
 
     off = 96 + 272 * 4
-    np.testing.assert_array_equal(decode(off, 4, 1, 64, 8), M.round_bf16(p["W1"]))
+    np.testing.assert_array_equal(decode16(off, 4, 64, 8), M.round_bf16(p["W1"]))
     off += 4 * 512
     for name in ("W2", "W3", "W4"):
-        np.testing.assert_array_equal(decode(off, 4, 4, 64, 64), M.round_bf16(p[name]))
-        off += 16 * 512
-    np.testing.assert_array_equal(decode(off, 1, 4, 4, 64), M.round_bf16(p["W5"]))
-    assert off + 4 * 512 == blob.size
+        np.testing.assert_array_equal(decode32(off, 4, 64, 64), M.round_bf16(p[name]))
+        off += 8 * 1024
+    np.testing.assert_array_equal(decode32(off, 1, 4, 64), M.round_bf16(p["W5"]))
+    assert off + 2 * 1024 == blob.size
+
+
+def unit32(kc, g, j):
+    """The unit that K slot (lane group g, element j) of chunk kc stands for in the 16x16x32 layers."""
+    return 32 * kc + 4 * g + (j if j < 4 else 16 + j - 4)
+
+
+def test_register_flow_of_the_16x16x32_layers():
+    """The kernel's data flow, emulated lane by lane: output tile mu of a layer leaves lane l units
+    16 mu + 4 (l // 16) + i of particle l % 16; chunk kc's B operand is tiles 2 kc and 2 kc + 1 side by side;
+    v_mfma_f32_16x16x32_bf16 sums A's element j of lane group g against B's element j of the same group
+    (the lane layout the instruction was checked to have on gfx950).  With the blob's tiles as A operands
+    the emulated layer equals W @ a for every layer the 16x16x32 form serves."""
+    surrogate = pytest.importorskip("baking_amd.surrogate")
+    rng = np.random.default_rng(8)
+    W = M.round_bf16(rng.normal(size=(64, 64)).astype(np.float32))
+    a = M.round_bf16(rng.normal(size=(64, 16)).astype(np.float32))  # units x 16 particles
+    tiles = surrogate._tiles32(W, 4, 2).reshape(4, 2, 64, 8)
+    A = (tiles.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+    # previous layer's output tiles as lanes hold them
+    h = np.zeros((4, 64, 4))
+    for mu in range(4):
+        for lane in range(64):
+            h[mu, lane] = a[16 * mu + 4 * (lane // 16) + np.arange(4), lane % 16]
+    D = np.zeros((64, 16))
+    for mu in range(4):
+        for kc in range(2):
+            B = np.concatenate([h[2 * kc], h[2 * kc + 1]], axis=1)  # lane x 8
+            for m in range(16):
+                for n in range(16):
+                    D[16 * mu + m, n] += sum(A[mu, kc, m + 16 * g, j] * B[n + 16 * g, j] for g in range(4) for j in range(8))
+    np.testing.assert_allclose(D, W.astype(np.float64) @ a.astype(np.float64), rtol=1e-12, atol=1e-12)
 
 
 def test_product_bf16_rounding_matches_the_oracle():
