@@ -491,7 +491,7 @@ struct DingParams {
 };
 
 // ding.py:103-137 with the SVD of the trial gradient precomputed.
-__device__ __forceinline__ void ding_return(const mat3<float>& trial, const mat3<float>& U, const float s[3], const mat3<float>& V,
+__device__ __forceinline__ void ding_return(float det_trial, const mat3<float>& U, const float s[3], const mat3<float>& V,
                                             float mu, float lam, float y, float eta, float a, float dt, mat3<float>& Fe,
                                             mat3<float>& stress, float& residual, float* eps_ret = nullptr) {
   float eps[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
@@ -526,7 +526,7 @@ __device__ __forceinline__ void ding_return(const mat3<float>& trial, const mat3
       stress.a[i * 3 + j] =
           U.a[i * 3 + 0] * cs[0] * U.a[j * 3 + 0] + U.a[i * 3 + 1] * cs[1] * U.a[j * 3 + 1] + U.a[i * 3 + 2] * cs[2] * U.a[j * 3 + 2];
     }
-  if (a == 0.0f) Fe = m_scale(powf(m_det(trial) / m_det(Fe), 1.0f / 3.0f), Fe);
+  if (a == 0.0f) Fe = m_scale(powf(det_trial / m_det(Fe), 1.0f / 3.0f), Fe);
   residual = 0.0f;
   if (q2 + p2 > 1.0f) {
     float u = 1.0f + df * g, v = 1.0f + mf * g;
@@ -555,27 +555,60 @@ struct DingAux {
   float dev_norm;         // |deviatoric trial Kirchhoff stress|
 };
 
-// ding.py:140-221 (UpdateDing body for one particle).  Returns kDingInvalid when
-// the incoming porosity is outside [0, 1) (flag 0, outputs untouched).  With
-// `defer` set, a particle whose coupled porosity needs the bisection solve
-// (porous_return.py:50-61) returns kDingDeferred without outputs, so a second
-// pass can run the expensive solves densely; recomputing from the same inputs
-// gives the same result.
-__device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old, float porosity, int gelled_in, const mat3<float>& F_total,
-                                  const mat3<float>& C_new, float T, float visc_param, DingOut& o, bool defer = false,
-                                  DingAux* aux = nullptr) {
-  o.flag0 = 0;
-  o.flag1 = 0;
-  if (!isfinite(porosity) || porosity < 0.0f || porosity >= 1.0f) { o.flag0 = 1; return kDingInvalid; }
-  int gel = gelled_in;
-  float Teff = T;
-  float yv = P.yield_stress;
-  float ev = visc_param;
+// The first half of the Ding update of one particle (ding.py:140-160): the
+// temperature-scaled yield stress and viscosity, the gel state and the SVD of
+// the elastic trial gradient.  Everything the second half needs, so a particle
+// whose coupled solve is deferred to a second pass carries this record there
+// instead of decomposing its trial gradient again.
+struct DingPrep {
+  mat3<float> U, V;
+  float s[3];
+  float det_trial;  // det of the elastic trial gradient (the volume fit of a return at zero porosity coefficient)
+  float Teff;       // temperature the yield and viscosity factors were taken at
+  float yv, ev;     // yield stress and viscosity at Teff
+  int gel;
+};
+// The record as 7 float4 (pass C -> deferred pass queue): U, V, s, det_trial, Teff, yv, ev, gel (bits).
+constexpr int kDingPrepVec = 7;
+__device__ __forceinline__ void store_prep(float4* dst, const DingPrep& r) {
+  const mat3<float>& U = r.U;
+  const mat3<float>& V = r.V;
+  dst[0] = make_float4(U.a[0], U.a[1], U.a[2], U.a[3]);
+  dst[1] = make_float4(U.a[4], U.a[5], U.a[6], U.a[7]);
+  dst[2] = make_float4(U.a[8], V.a[0], V.a[1], V.a[2]);
+  dst[3] = make_float4(V.a[3], V.a[4], V.a[5], V.a[6]);
+  dst[4] = make_float4(V.a[7], V.a[8], r.s[0], r.s[1]);
+  dst[5] = make_float4(r.s[2], r.det_trial, r.Teff, r.yv);
+  dst[6] = make_float4(r.ev, __int_as_float(r.gel), 0.0f, 0.0f);
+}
+__device__ __forceinline__ DingPrep load_prep(const float4* src) {
+  DingPrep r;
+  const float4 a = src[0], b = src[1], c = src[2], d = src[3], e = src[4], f = src[5], g = src[6];
+  const float u[9] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x};
+  const float v[9] = {c.y, c.z, c.w, d.x, d.y, d.z, d.w, e.x, e.y};
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    r.U.a[k] = u[k];
+    r.V.a[k] = v[k];
+  }
+  r.s[0] = e.z; r.s[1] = e.w; r.s[2] = f.x;
+  r.det_trial = f.y; r.Teff = f.z; r.yv = f.w;
+  r.ev = g.x;
+  r.gel = __float_as_int(g.y);
+  return r;
+}
+
+__device__ inline void ding_prepare(const DingParams& P, const mat3<float>& Fe_old, int gelled_in, const mat3<float>& C_new, float T,
+                                    float visc_param, DingPrep& r) {
+  r.gel = gelled_in;
+  r.Teff = T;
+  r.yv = P.yield_stress;
+  r.ev = visc_param;
   if (P.temperature_enabled) {
-    if (T >= P.knots[3]) gel = 1;
-    if (gel) Teff = P.knots[3];
-    yv = P.yield_stress * temperature_factor(Teff, P.knots, P.yf);
-    ev = visc_param * temperature_factor(Teff, P.knots, P.vf);
+    if (T >= P.knots[3]) r.gel = 1;
+    if (r.gel) r.Teff = P.knots[3];
+    r.yv = P.yield_stress * temperature_factor(r.Teff, P.knots, P.yf);
+    r.ev = visc_param * temperature_factor(r.Teff, P.knots, P.vf);
   }
   mat3<float> trial;
   {
@@ -583,82 +616,105 @@ __device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old
     inc.a[0] += 1.0f; inc.a[4] += 1.0f; inc.a[8] += 1.0f;
     trial = m_mul(inc, Fe_old);
   }
+  r.det_trial = m_det(trial);
   mat3<double> Ud, Vd;
   double sd[3];
   // converged to |off-diagonal| <= 1e-9 |diagonal| of trial^T trial: the singular values' error is second order
   // (1e-18 relative) and U, V are within 1e-9, both below the float rounding of what the return map uses
   svd3(m_cast<double>(trial), Ud, sd, Vd, 1e-18);
-  mat3<float> U = m_cast<float>(Ud), V = m_cast<float>(Vd);
-  float s[3] = {float(sd[0]), float(sd[1]), float(sd[2])};
-  float ret_n = porosity;
-  float cres = 0.0f;
+  r.U = m_cast<float>(Ud);
+  r.V = m_cast<float>(Vd);
+  r.s[0] = float(sd[0]);
+  r.s[1] = float(sd[1]);
+  r.s[2] = float(sd[2]);
+}
+
+// The second half (ding.py:160-221) in two steps.  ding_solve: the coupled
+// porosity of the return (porous_return.py:43-61) from the trial's principal
+// stretches s and the yield stress and viscosity; with `defer` set, a particle
+// whose porosity needs the bisection solve returns kDingDeferred, so a second
+// pass can run the expensive solves densely.  ding_complete: the return map,
+// the volume fit and the outputs.
+__device__ inline int ding_solve(const DingParams& P, const float s[3], float yv, float ev, float Teff, float porosity, bool defer,
+                                 DingAux* aux, float& ret_n, float& cres, int& flag1) {
+  ret_n = porosity;
+  cres = 0.0f;
+  if (!P.coupled) return kDingDone;
   const float mu = P.mu, lam = P.lam, dt = P.dt;
-  if (P.coupled) {
-    float eps[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
-    float ts[3];
+  float eps[3] = {logf(s[0]), logf(s[1]), logf(s[2])};
+  float ts[3];
 #pragma unroll
-    for (int i = 0; i < 3; ++i) ts[i] = 2.0f * mu * eps[i] + lam * (eps[0] + eps[1] + eps[2]);
-    float mean = (ts[0] + ts[1] + ts[2]) / 3.0f;
-    float dev[3] = {ts[0] - mean, ts[1] - mean, ts[2] - mean};
-    float q = 1.5f * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]) / (yv * yv);
-    float pv = mean * mean / (yv * yv);
-    float df = 6.0f * mu * dt * yv * yv / ev;
-    float ms = 2.0f / 3.0f * (2.0f * mu + 3.0f * lam) * dt * yv * yv / ev;
-    float ex = 3.0f * mean / (2.0f * mu + 3.0f * lam);
-    double opv = 1.0 - double(P.n0) + double(porosity);
-    if (aux) {
-      // descending order by a 3-element network on (value, index) pairs, kept in registers
-      float e0 = eps[0], e1 = eps[1], e2 = eps[2];
-      int o0 = 0, o1 = 1, o2 = 2;
-      if (e0 < e1) { float t = e0; e0 = e1; e1 = t; int u = o0; o0 = o1; o1 = u; }
-      if (e1 < e2) { float t = e1; e1 = e2; e2 = t; int u = o1; o1 = o2; o2 = u; }
-      if (e0 < e1) { float t = e0; e0 = e1; e1 = t; int u = o0; o0 = o1; o1 = u; }
-      aux->order[0] = o0; aux->order[1] = o1; aux->order[2] = o2;
-      aux->eps[0] = e0; aux->eps[1] = e1; aux->eps[2] = e2;
-      aux->T = Teff;
-      aux->dev_norm = sqrtf(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
-    }
-    if (aux && aux->guided && aux->dev_target > 0.0f && df > 0.0f) {
-      const double g = (double(aux->dev_norm) / double(aux->dev_target) - 1.0) / double(df);
-      aux->gamma_start = g > 0.0 ? g : 0.0;
-    }
-    if (defer) {
-      double trial_n = opv - 1.0 + double(P.n0);
-      bool inside = trial_n > 0.0 && trial_n < 1.0 && double(q) + pressure_coefficient64(trial_n) * double(pv) <= 1.0;
-      if (!inside) return kDingDeferred;
-    }
-    bool solved = false, stood_in = false;
-    double standin_residual = 0.0;
-    if (aux && aux->standin) {
-      stood_in = porosity_standin(q, pv, df, ms, ex, opv, P.n0, aux->n_start, aux->gamma_start, ret_n, standin_residual,
-                                  aux->evals);
-      aux->accepted = solved = stood_in;
-    }
-    if (!solved && aux && aux->guided) {
-      solved = solve_porosity_guided(q, pv, df, ms, ex, opv, P.n0, aux->n_start, aux->gamma_start, ret_n, aux->evals);
-      if (!aux->standin) aux->accepted = solved;
-    }
-    if (!solved) {
-      if (P.bracketed) {
-        ret_n = solve_porosity_bracketed(q, pv, df, ms, ex, opv, P.n0, aux ? &aux->evals : nullptr);
-      } else {
-        ret_n = solve_porosity(q, pv, df, ms, ex, opv, P.n0);
-        if (aux) aux->evals += 64;
-      }
-    }
-    if (stood_in) {
-      cres = float(standin_residual);
-    } else {
-      double pred = porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
-      if (aux) ++aux->evals;
-      cres = float(fabs(double(ret_n) - pred));
-    }
-    if (cres > 1.0e-6f || ret_n <= 0.0f || ret_n >= 1.0f) o.flag1 = 1;
+  for (int i = 0; i < 3; ++i) ts[i] = 2.0f * mu * eps[i] + lam * (eps[0] + eps[1] + eps[2]);
+  float mean = (ts[0] + ts[1] + ts[2]) / 3.0f;
+  float dev[3] = {ts[0] - mean, ts[1] - mean, ts[2] - mean};
+  float q = 1.5f * (dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]) / (yv * yv);
+  float pv = mean * mean / (yv * yv);
+  float df = 6.0f * mu * dt * yv * yv / ev;
+  float ms = 2.0f / 3.0f * (2.0f * mu + 3.0f * lam) * dt * yv * yv / ev;
+  float ex = 3.0f * mean / (2.0f * mu + 3.0f * lam);
+  double opv = 1.0 - double(P.n0) + double(porosity);
+  if (aux) {
+    // descending order by a 3-element network on (value, index) pairs, kept in registers
+    float e0 = eps[0], e1 = eps[1], e2 = eps[2];
+    int o0 = 0, o1 = 1, o2 = 2;
+    if (e0 < e1) { float t = e0; e0 = e1; e1 = t; int u = o0; o0 = o1; o1 = u; }
+    if (e1 < e2) { float t = e1; e1 = e2; e2 = t; int u = o1; o1 = o2; o2 = u; }
+    if (e0 < e1) { float t = e0; e0 = e1; e1 = t; int u = o0; o0 = o1; o1 = u; }
+    aux->order[0] = o0; aux->order[1] = o1; aux->order[2] = o2;
+    aux->eps[0] = e0; aux->eps[1] = e1; aux->eps[2] = e2;
+    aux->T = Teff;
+    aux->dev_norm = sqrtf(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
   }
+  if (aux && aux->guided && aux->dev_target > 0.0f && df > 0.0f) {
+    const double g = (double(aux->dev_norm) / double(aux->dev_target) - 1.0) / double(df);
+    aux->gamma_start = g > 0.0 ? g : 0.0;
+  }
+  if (defer) {
+    double trial_n = opv - 1.0 + double(P.n0);
+    bool inside = trial_n > 0.0 && trial_n < 1.0 && double(q) + pressure_coefficient64(trial_n) * double(pv) <= 1.0;
+    if (!inside) return kDingDeferred;
+  }
+  bool solved = false, stood_in = false;
+  double standin_residual = 0.0;
+  if (aux && aux->standin) {
+    stood_in = porosity_standin(q, pv, df, ms, ex, opv, P.n0, aux->n_start, aux->gamma_start, ret_n, standin_residual,
+                                aux->evals);
+    aux->accepted = solved = stood_in;
+  }
+  if (!solved && aux && aux->guided) {
+    solved = solve_porosity_guided(q, pv, df, ms, ex, opv, P.n0, aux->n_start, aux->gamma_start, ret_n, aux->evals);
+    if (!aux->standin) aux->accepted = solved;
+  }
+  if (!solved) {
+    if (P.bracketed) {
+      ret_n = solve_porosity_bracketed(q, pv, df, ms, ex, opv, P.n0, aux ? &aux->evals : nullptr);
+    } else {
+      ret_n = solve_porosity(q, pv, df, ms, ex, opv, P.n0);
+      if (aux) aux->evals += 64;
+    }
+  }
+  if (stood_in) {
+    cres = float(standin_residual);
+  } else {
+    double pred = porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
+    if (aux) ++aux->evals;
+    cres = float(fabs(double(ret_n) - pred));
+  }
+  if (cres > 1.0e-6f || ret_n <= 0.0f || ret_n >= 1.0f) flag1 = 1;
+  return kDingDone;
+}
+
+__device__ inline void ding_complete(const DingParams& P, const DingPrep& pr, float porosity, float ret_n, float cres,
+                                     const mat3<float>& F_total, float T, DingOut& o, DingAux* aux) {
+  const mat3<float>& U = pr.U;
+  const mat3<float>& V = pr.V;
+  const float* s = pr.s;
+  const float yv = pr.yv, ev = pr.ev;
+  const float mu = P.mu, lam = P.lam, dt = P.dt;
   mat3<float> Fe, tau;
   float resid;
   float eps_ret[3];
-  ding_return(trial, U, s, V, mu, lam, yv, ev, porosity_coefficient(ret_n), dt, Fe, tau, resid, eps_ret);
+  ding_return(pr.det_trial, U, s, V, mu, lam, yv, ev, porosity_coefficient(ret_n), dt, Fe, tau, resid, eps_ret);
   if (aux && P.coupled) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -683,10 +739,36 @@ __device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old
   o.porosity_residual = cres;
   o.porosity = nn;
   o.T = T;
-  o.gelled = gel;
+  o.gelled = pr.gel;
   o.yield_stress = yv;
   o.viscosity = ev;
+}
+
+__device__ inline int ding_finish(const DingParams& P, const DingPrep& pr, float porosity, const mat3<float>& F_total, float T,
+                                  DingOut& o, bool defer = false, DingAux* aux = nullptr, float4* defer_store = nullptr) {
+  o.flag0 = 0;
+  o.flag1 = 0;
+  float ret_n, cres;
+  if (ding_solve(P, pr.s, pr.yv, pr.ev, pr.Teff, porosity, defer, aux, ret_n, cres, o.flag1) == kDingDeferred) {
+    if (defer_store) store_prep(defer_store, pr);
+    return kDingDeferred;
+  }
+  ding_complete(P, pr, porosity, ret_n, cres, F_total, T, o, aux);
   return kDingDone;
+}
+
+// ding.py:140-221 (UpdateDing body for one particle).  Returns kDingInvalid when
+// the incoming porosity is outside [0, 1) (flag 0, outputs untouched).  A
+// deferred particle leaves the first half's record at `defer_store`.
+__device__ inline int ding_update(const DingParams& P, const mat3<float>& Fe_old, float porosity, int gelled_in, const mat3<float>& F_total,
+                                  const mat3<float>& C_new, float T, float visc_param, DingOut& o, bool defer = false,
+                                  DingAux* aux = nullptr, float4* defer_store = nullptr) {
+  o.flag0 = 0;
+  o.flag1 = 0;
+  if (!isfinite(porosity) || porosity < 0.0f || porosity >= 1.0f) { o.flag0 = 1; return kDingInvalid; }
+  DingPrep pr;
+  ding_prepare(P, Fe_old, gelled_in, C_new, T, visc_param, pr);
+  return ding_finish(P, pr, porosity, F_total, T, o, defer, aux, defer_store);
 }
 
 // ------------------------------------------------------------------ gas (physics/materials/gas.py)

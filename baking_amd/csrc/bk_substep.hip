@@ -1206,21 +1206,19 @@ __global__ __launch_bounds__(256) void k_grid_update(const bk_dough* __restrict_
 // ============================================================================ material update after G2P
 // Ding return map, gas refresh and saturation of particle p
 // (ding.py:303-327, gas.py:38-45, thermal_water.py:256-260).
-__device__ inline int material_update(const bk_params& P, float visc_param, const bk_scene& sc, const bk_state& in,
-                                      const bk_state& out, int p, const mat3<float>& F_new, const mat3<float>& Cn, bool defer,
-                                      bool& f0, bool& f1, DingAux* aux = nullptr) {
+__device__ __forceinline__ DingParams ding_params(const bk_params& P) {
   DingParams D;
   D.mu = P.mu; D.lam = P.lam; D.yield_stress = P.yield_stress; D.n0 = P.n0; D.dt = P.dt;
   for (int i = 0; i < 4; ++i) { D.knots[i] = P.knots[i]; D.yf[i] = P.yield_factors[i]; D.vf[i] = P.viscosity_factors[i]; }
   D.temperature_enabled = P.temperature_enabled;
   D.coupled = P.gas_enabled;
   D.bracketed = P.return_solver;
-  float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
-  DingOut r;
-  int status = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, visc_param, r, defer, aux);
-  f0 = r.flag0;
-  f1 = r.flag1;
-  if (status == kDingDeferred) return status;
+  return D;
+}
+
+// The outputs of a finished Ding update, the gas pressure and the saturation.
+__device__ inline void material_outputs(const bk_params& P, const bk_scene& sc, const bk_state& in, const bk_state& out, int p,
+                                        const mat3<float>& F_new, int status, const DingOut& r) {
   if (status == kDingDone) {
     m_store(out.Fe + 9 * p, r.Fe);
     m_store(out.Fp + 9 * p, r.Fp);
@@ -1242,7 +1240,48 @@ __device__ inline int material_update(const bk_params& P, float visc_param, cons
     if (nn > 0.0f && nn < 1.0f) out.pressure[p] = gas_pressure(moles, out.T_mat[p], Jn, nn);
   }
   if (P.transport_enabled) out.s[p] = out.W[p] / (double(P.rho_w) * double(sc.V0[p]) * double(Jn));
+}
+
+// Pass C's update of particle p.  A particle deferred to the second pass
+// (`defer`) leaves the first half's record at prep_out + kDingPrepVec * p.
+__device__ inline int material_update(const bk_params& P, float visc_param, const bk_scene& sc, const bk_state& in,
+                                      const bk_state& out, int p, const mat3<float>& F_new, const mat3<float>& Cn, bool defer,
+                                      bool& f0, bool& f1, DingAux* aux, float4* __restrict__ prep_out) {
+  const DingParams D = ding_params(P);
+  float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
+  DingOut r;
+  int status = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, visc_param, r, defer, aux,
+                           prep_out + kDingPrepVec * int64_t(p));
+  f0 = r.flag0;
+  f1 = r.flag1;
+  if (status == kDingDeferred) return status;
+  material_outputs(P, sc, in, out, p, F_new, status, r);
   return status;
+}
+
+// The deferred pass's update of particle p: the second half from pass C's record, whose stretches, yield
+// stress and viscosity are read for the solve and its rotations only after it.
+__device__ inline int material_finish(const bk_params& P, const bk_scene& sc, const bk_state& in, const bk_state& out, int p,
+                                      const float4* __restrict__ prep, bool& f0, bool& f1, DingAux* aux) {
+  const DingParams D = ding_params(P);
+  const float4* rec = prep + kDingPrepVec * int64_t(p);
+  const float porosity = in.porosity[p];
+  DingOut r;
+  r.flag0 = 0;
+  r.flag1 = 0;
+  float ret_n, cres;
+  {
+    const float4 e = rec[4], f = rec[5], g = rec[6];
+    const float s[3] = {e.z, e.w, f.x};
+    ding_solve(D, s, f.w, g.x, f.z, porosity, false, aux, ret_n, cres, r.flag1);
+  }
+  const float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
+  const mat3<float> F_new = m_load(out.F + 9 * p);
+  ding_complete(D, load_prep(rec), porosity, ret_n, cres, F_new, T_local, r, aux);
+  f0 = r.flag0;
+  f1 = r.flag1;
+  material_outputs(P, sc, in, out, p, F_new, kDingDone, r);
+  return kDingDone;
 }
 
 // Surrogate inputs of particle p from its Ding extras (see bk_surrogate.h).
@@ -1306,8 +1345,7 @@ __global__ __launch_bounds__(256) void k_pass_c_deferred(const bk_dough* __restr
           aux.dev_target = 2.0f * P.mu *
                            sqrtf((pr[0] - m) * (pr[0] - m) + (pr[1] - m) * (pr[1] - m) + (pr[2] - m) * (pr[2] - m));
         }
-        const int status =
-            material_update(P, sc.visc_param[d], sc, in, out, p, m_load(out.F + 9 * p), m_load(out.C + 9 * p), false, f0, f1, &aux);
+        const int status = material_finish(P, sc, in, out, p, ws->prep, f0, f1, &aux);
         flag_max(dough_flags(ws, d), BK_FLAG_POROSITY, f0);
         flag_max(dough_flags(ws, d), BK_FLAG_COUPLED, f1);
         evals += unsigned(aux.evals);
@@ -1466,8 +1504,8 @@ __global__ __launch_bounds__(kParticleThreads) __attribute__((amdgpu_waves_per_e
         m_store(out.F + 9 * p, F_new);
         if (P.model == 1) {
           DingAux aux;
-          deferred = material_update(P, sc.visc_param[d], sc, in, out, p, F_new, Cn, true, f0, f1, P.gas_enabled ? &aux : nullptr) ==
-                     kDingDeferred;
+          deferred = material_update(P, sc.visc_param[d], sc, in, out, p, F_new, Cn, true, f0, f1, P.gas_enabled ? &aux : nullptr,
+                                     ws->prep) == kDingDeferred;
           if (deferred) surrogate_features(P, in, p, aux, feat);
         }
       }
@@ -1927,6 +1965,7 @@ bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blo
   alloc((void**)&h.counters, 64);
   alloc((void**)&h.tau, n * 36);
   alloc((void**)&h.queue, n * 4);
+  alloc((void**)&h.prep, n * kDingPrepVec * sizeof(float4));
   alloc((void**)&h.vertices, size_t(n_vertices) * 12);
   alloc((void**)&h.vert_mark, size_t(n_vertices) * 4);
   alloc((void**)&h.slot_exposed, size_t(n_slots) * 8);
@@ -1966,7 +2005,7 @@ void bk_workspace_destroy(bk_workspace* ws) {
   if (ws_host(ws, h)) return;
   void* ptrs[] = {h.doughs, h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1],
                   h.hist, h.digit_total, h.blk_start, h.blk_count, h.scan_part, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
-                  h.queue, h.tau, h.vertices, h.vert_mark, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions,
+                  h.queue, h.prep, h.tau, h.vertices, h.vert_mark, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions,
                   h.stats, h.settle, h.settle_part, h.dTW, h.freeze, h.freeze_age};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);

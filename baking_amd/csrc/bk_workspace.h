@@ -50,6 +50,8 @@ struct bk_workspace {
   int32_t* counters = nullptr;                  // [0] tiles, [1] active blocks, [2] deferred, [3] particle blocks,
                                                 // [4] emitted samples, [6] substeps run
   int32_t* queue = nullptr;                     // particles whose coupled porosity needs the full solve
+  float4* prep = nullptr;                       // per particle, pass C -> deferred pass: the first half of a deferred
+                                                // particle's Ding update (DingPrep, kDingPrepVec float4)
   float* tau = nullptr;
   float* vertices = nullptr;                    // reconstructed positions of the boundary faces' vertices
   int32_t* vert_mark = nullptr;                 // per vertex: substep count when a boundary face last used it
