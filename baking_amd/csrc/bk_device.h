@@ -319,6 +319,8 @@ __device__ __forceinline__ double overstress64(double q, double p, double df, do
 __device__ __forceinline__ double porosity_prediction(double n, float q, float p, float df, float ms, float ex, double pv, float n0) {
   double a = pressure_coefficient64(n);
   double root = overstress64(double(q), a * double(p), double(df), double(ms) * a);
+  // no overstress: with finite ms * a and ex, den = 1 and pv * exp(ex * 0) is pv exactly
+  if (root == 0.0 && isfinite(double(ms) * a) && isfinite(double(ex))) return pv - 1.0 + double(n0);
   double den = 1.0 + double(ms) * a * root;
   return pv * exp(double(ex) * (1.0 - 1.0 / den)) - 1.0 + double(n0);
 }
@@ -669,14 +671,17 @@ __device__ inline int ding_solve(const DingParams& P, const float s[3], float yv
     const double g = (double(aux->dev_norm) / double(aux->dev_target) - 1.0) / double(df);
     aux->gamma_start = g > 0.0 ? g : 0.0;
   }
+  bool solved = false, stood_in = false;
   if (defer) {
     double trial_n = opv - 1.0 + double(P.n0);
     bool inside = trial_n > 0.0 && trial_n < 1.0 && double(q) + pressure_coefficient64(trial_n) * double(pv) <= 1.0;
     if (!inside) return kDingDeferred;
+    // the solvers' elastic exit, decided by the same test on the same values
+    ret_n = float(trial_n);
+    solved = true;
   }
-  bool solved = false, stood_in = false;
   double standin_residual = 0.0;
-  if (aux && aux->standin) {
+  if (!solved && aux && aux->standin) {
     stood_in = porosity_standin(q, pv, df, ms, ex, opv, P.n0, aux->n_start, aux->gamma_start, ret_n, standin_residual,
                                 aux->evals);
     aux->accepted = solved = stood_in;

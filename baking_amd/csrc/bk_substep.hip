@@ -891,9 +891,13 @@ __device__ __forceinline__ int halo_slot(int ax, int side, const int lc[3]) {
 }
 
 __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ doughs, int nd, bk_grid g, bk_workspace* ws) {
-  __shared__ double4 stage[4][64 + kDiffHalo];
+  // one array per field: a wavefront's 64 consecutive doubles of a field fall in distinct bank pairs
+  __shared__ double stage[4][4][64 + kDiffHalo];
   const int wv = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
-  double4* const st = stage[wv];
+  double* const sC = stage[wv][0];
+  double* const sV = stage[wv][1];
+  double* const sE = stage[wv][2];
+  double* const sW = stage[wv][3];
   const int nact = ws->counters[1];
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nact * 64; t += gridDim.x * blockDim.x) {
     const int blk = __builtin_amdgcn_readfirstlane(ws->active[t >> 6]);
@@ -904,7 +908,10 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
     const int lc[3] = {loc >> 4, (loc >> 2) & 3, loc & 3};
     const int c[3] = {4 * bg.bx + lc[0], 4 * bg.by + lc[1], 4 * bg.bz + lc[2]};
     const double Ci = g.C[node];
-    st[loc] = make_double4(Ci, g.V[node], g.E[node], g.W[node]);
+    sC[loc] = Ci;
+    sV[loc] = g.V[node];
+    sE[loc] = g.E[node];
+    sW[loc] = g.W[node];
     for (int h = lane; h < kDiffHalo; h += 64) {
       const int f = h >> 4, ax = f >> 1, side = f & 1;
       const int a1 = ax == 0 ? 1 : 0, a2 = ax == 2 ? 1 : 2;
@@ -913,12 +920,18 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
       q[a1] = (h >> 2) & 3;
       q[a2] = h & 3;
       const int o[3] = {4 * bg.bx + q[0], 4 * bg.by + q[1], 4 * bg.bz + q[2]};
-      double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+      double c4 = 0.0, v4 = 0.0, e4 = 0.0, w4 = 0.0;
       if (o[ax] >= 0 && o[ax] < P.res) {
         const int64_t other = bg.node_base + tiled_node(o[0], o[1], o[2], bg.nb);
-        v = make_double4(g.C[other], g.V[other], g.E[other], g.W[other]);
+        c4 = g.C[other];
+        v4 = g.V[other];
+        e4 = g.E[other];
+        w4 = g.W[other];
       }
-      st[64 + h] = v;
+      sC[64 + h] = c4;
+      sV[64 + h] = v4;
+      sE[64 + h] = e4;
+      sW[64 + h] = w4;
     }
     wave_sync();
     bool heat_bad = false, water_bad = false;
@@ -929,10 +942,9 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
       const bool rest = resting(P, ws, bg.d);
       const double hk = rest ? 0.0 : double(P.kappa * P.dx);
       const double wk = rest ? 0.0 : double(P.diffusivity * P.rho_w * P.dx);
-      const double4 own = st[loc];
-      double Vi = own.y;
-      double Ti = own.z / Ci;
-      double Si = own.w / (rho_w * Vi);
+      double Vi = sV[loc];
+      double Ti = sE[loc] / Ci;
+      double Si = sW[loc] / (rho_w * Vi);
       double hr = g.H[node], wr = -g.L[node], hd = g.X[node], wd = 0.0;
 #pragma unroll
       for (int ax = 0; ax < 3; ++ax)
@@ -942,17 +954,17 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
           if (c[ax] + step < 0 || c[ax] + step >= P.res) continue;
           int ln[3] = {lc[0], lc[1], lc[2]};
           ln[ax] += step;
-          const double4 nb4 = (ln[ax] >= 0 && ln[ax] < 4) ? st[(ln[0] << 4) | (ln[1] << 2) | ln[2]] : st[64 + halo_slot(ax, side, lc)];
-          double Cj = nb4.x;
+          const int j = (ln[ax] >= 0 && ln[ax] < 4) ? ((ln[0] << 4) | (ln[1] << 2) | ln[2]) : 64 + halo_slot(ax, side, lc);
+          double Cj = sC[j];
           if (Cj > 0.0) {
-            double Vj = nb4.y;
-            double Tj = nb4.z / Cj;
+            double Vj = sV[j];
+            double Tj = sE[j] / Cj;
             double fill = 2.0 * Vi * Vj / (Vi + Vj) / dx3;
             double hc = hk * fill;
             hr += hc * (Tj - Ti);
             hd += hc;
             if (P.water_enabled) {
-              double Sj = nb4.w / (rho_w * Vj);
+              double Sj = sW[j] / (rho_w * Vj);
               double mob = wk * fill * 0.5 * (Ti + Tj) * 0.5 * (Si + Sj);
               wr += mob * (Sj - Si);
               wd += mob / (rho_w * Vi);
@@ -961,7 +973,7 @@ __global__ __launch_bounds__(256) void k_diffuse(const bk_dough* __restrict__ do
         }
       heat_bad = dt * hd > Ci;
       double change = dt * wr;
-      const double Wi = own.w;
+      const double Wi = sW[loc];
       water_bad = Wi + change < 0.0 || dt * wd > 1.0;
       const double dT = dt * hr / Ci;
       // the relative water change dW / W each particle gathers (thermal_water.py:222-223), once per node

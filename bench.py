@@ -41,6 +41,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+FP64_PEAK_TFLOPS = 78.6   # MI355X vector/matrix FP64, published
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32, published
+# VALU issue: 256 CUs x 4 SIMDs, one 64-lane wavefront instruction per SIMD every 4 cycles at 2.4 GHz
+VALU_ISSUE_PEAK_G = 256 * 4 * 2.4 / 4
 
 # Algorithmic bytes of each stage at the reference's precisions: (per particle, per active stencil node).
 # Particle terms count each array the stage must read once and write once; node terms the grid arrays it
@@ -188,19 +192,34 @@ def check_state_timing(spec, sim, with_host, repeats=5):
     return out
 
 
-def measured_traffic(workload, kernel, particles, active_nodes, deferred):
-    """HBM bytes of one launch of `kernel`, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary
-    (per-unit bytes, calibrated against a plain copy) scaled to this launch's units; None if not profiled."""
-    path = os.path.join(ROOT, "profiles", f"{workload}_hbm_traffic.json")
+def counter_entry(workload, kernel, particles, active_nodes, deferred):
+    """The committed rocprofv3 counter figures of `kernel` (profiles/<workload>_counters.json: HBM bytes from the
+    sized read and write requests, VALU instructions and FLOPs, per unit of work) and this launch's units;
+    (None, 0) if the kernel was not profiled."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_counters.json")
     if not os.path.exists(path):
-        return None
+        return None, 0
     with open(path) as stream:
-        per_kernel = json.load(stream)["per_kernel"]
-    entry = per_kernel.get(kernel)
+        entry = json.load(stream)["per_kernel"].get(f"k_{kernel}")
     if entry is None:
-        return None
+        return None, 0
     units = {"particle": particles, "queued particle": deferred, "active stencil node": active_nodes}[entry["unit"]]
-    return entry["hbm_bytes_per_unit"] * units
+    return entry, units
+
+
+def compute_roofline(entry, units, ms):
+    """fp64, fp32 and VALU-issue rates of one launch: the profiled per-unit counts times this launch's units over
+    its live duration.  FLOPs count every lane of each issued instruction (SQ_INSTS_VALU_FLOPS_* x 64)."""
+    if entry is None or ms <= 0:
+        return None
+    sec = ms * 1e-3
+    fp64 = entry["fp64_flops_per_unit"] * units / sec / 1e12
+    fp32 = entry["fp32_flops_per_unit"] * units / sec / 1e12
+    valu = entry["valu_instr_per_unit"] * units / sec / 1e9
+    return {"fp64": {"achieved": fp64, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": fp64 / FP64_PEAK_TFLOPS},
+            "fp32": {"achieved": fp32, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": fp32 / FP32_PEAK_TFLOPS},
+            "valu_issue": {"achieved": valu, "peak": VALU_ISSUE_PEAK_G, "unit": "G wavefront-instructions/s",
+                           "frac": valu / VALU_ISSUE_PEAK_G}}
 
 
 def stage_profile(lib_launch, workspace, steps):
@@ -236,10 +255,12 @@ def roofline(stages, n, active_nodes, deferred, substep_ms, workload):
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9 if dom_ms > 0 else 0.0
     substep_bytes = SUBSTEP_BYTES[0] * n + SUBSTEP_BYTES[1] * active_nodes
     substep_gbs = substep_bytes / (substep_ms * 1e-3) / 1e9
+    entry, units = counter_entry(workload, dominant, n, active_nodes, deferred)
     roof = {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(workload, dominant, n, active_nodes, deferred),
+            "frac": achieved / HBM_PEAK_GBS, "traffic": entry["hbm_bytes_per_unit"] * units if entry else None,
             "algorithmic_bytes": dom_bytes, "avg_ms": dom_ms, "stages_ms": stages,
-            "substep_achieved": substep_gbs, "substep_frac": substep_gbs / HBM_PEAK_GBS}
+            "substep_achieved": substep_gbs, "substep_frac": substep_gbs / HBM_PEAK_GBS,
+            "compute": compute_roofline(entry, units, dom_ms)}
     return roof, dominant, dom_bytes, dom_ms
 
 
