@@ -532,7 +532,7 @@ def run_adjoint(args):
     for f, t in adjoint._binding_reverse(sim).items():
         setattr(bk_adj, f, t.data_ptr())
     scratch = torch.zeros(int(lib.bk_adjoint_scratch_size(ctypes.byref(sim._params))), dtype=torch.float64, device="cuda")
-    param = torch.zeros(3, dtype=torch.float64, device="cuda")
+    param = torch.zeros(4, dtype=torch.float64, device="cuda")  # viscosity, gas production, heat transfer, oven temperature
     st_in, st_out = sim._state_struct(sim.current_state), sim._state_struct(sim.next_state)
     stream = torch.cuda.current_stream()
 
