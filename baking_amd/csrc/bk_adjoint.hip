@@ -632,7 +632,7 @@ __global__ __launch_bounds__(128) void k_adj_g2p(AdjArgs a) {
 __global__ __launch_bounds__(kTileThreads) void k_adj_scatter(const bk_dough* __restrict__ doughs, AdjArgs a, bk_workspace* ws) {
   __shared__ double acc[3][kRegionNodes];
   __shared__ int cell_lo[64], cell_hi[64];
-  const int ntiles = ws->counters[0];
+  const int ntiles = tile_count(ws);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bk_params& P = a.P;
   const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
@@ -962,7 +962,7 @@ __global__ __launch_bounds__(128) void k_adj_b1(AdjArgs a) {
 __global__ __launch_bounds__(kTileThreads) void k_adj_scatter_b1(const bk_dough* __restrict__ doughs, AdjArgs a, bk_workspace* ws) {
   __shared__ double acc[2][kRegionNodes];
   __shared__ int cell_lo[64], cell_hi[64];
-  const int ntiles = ws->counters[0];
+  const int ntiles = tile_count(ws);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bk_params& P = a.P;
   const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};

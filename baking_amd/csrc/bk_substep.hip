@@ -163,7 +163,7 @@ constexpr uint32_t kFrozenKey = 0xFFFFFFFFu;  // the key of a particle of a stop
 __device__ __forceinline__ void batch_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of,
                                            const float* __restrict__ x, int n, uint32_t* __restrict__ keys,
                                            int32_t* __restrict__ ranks, bk_workspace* ws, int p) {
-  if (p == 0) { ws->counters[0] = 0; ws->counters[2] = 0; ws->counters[3] = 0; ws->counters[6] += 1; }
+  if (p == 0) { ws->counters[0] = 0; ws->counters[2] = 0; ws->counters[3] = 0; ws->counters[6] += 1; ws->counters[7] = 0; }
   const bool valid = p < n;
   bool live = valid;  // a stopped dough's particles get no key, so no tile, node block or grid work
   uint32_t key = 0u;
@@ -573,10 +573,14 @@ __global__ __launch_bounds__(256) void k_activate(const bk_dough* __restrict__ d
         fresh = atomicExch(&ws->node_flag[nbk], 1) == 0;
       }
     }
-    const int slot = block_reserve(&ws->counters[0], ntile, lds);
+    int nfull = 0;  // tiles of at least kFullTile particles: every one but the last, and the last when it is
+    for (int k = 0; k < ntile; ++k) nfull += min(chunk, end - start - k * chunk) >= kFullTile;
+    int front = block_reserve(&ws->counters[0], nfull, lds);
+    int back = block_reserve(&ws->counters[7], ntile - nfull, lds);
+    const int last = ws->max_n + ws->total_blocks - 1;
     for (int k = 0; k < ntile; ++k) {
-      const int s = start + k * chunk;
-      ws->tiles[slot + k] = make_int4(b, d, s, min(chunk, end - s));
+      const int s = start + k * chunk, c = min(chunk, end - s);
+      ws->tiles[c >= kFullTile ? front++ : last - back++] = make_int4(b, d, s, c);
     }
     block_append(&ws->counters[1], ws->active, fresh, nbk, lds);
   }
@@ -784,7 +788,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
                                                          bk_workspace* ws) {
   __shared__ double acc[7][kRegionNodes];
   __shared__ int cell_lo[64], cell_hi[64];
-  const int ntiles = ws->counters[0];
+  const int ntiles = tile_count(ws);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tg = tile_at(doughs, ws, t);
@@ -1104,7 +1108,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(const bk_dough* __rest
   // for this scatter, and each node's sum is rounded to float once, at the flush.
   __shared__ double acc[4][kRegionNodes];
   __shared__ int cell_lo[64], cell_hi[64];
-  const int ntiles = ws->counters[0];
+  const int ntiles = tile_count(ws);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const Tile tg = tile_at(doughs, ws, t);
@@ -2156,8 +2160,12 @@ float* bk_workspace_vertices(bk_workspace* ws) {
 int bk_workspace_counts(bk_workspace* ws, int32_t* out3, void* stream) {
   HostWs* hw = host_view(ws);
   if (!hw || !out3) return fail("bk_workspace_counts: bad argument");
-  BK_HIP(hipMemcpyAsync(out3, hw->dev.counters, 12, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  int32_t c[8];
+  BK_HIP(hipMemcpyAsync(c, hw->dev.counters, sizeof(c), hipMemcpyDeviceToHost, (hipStream_t)stream));
   BK_HIP(hipStreamSynchronize((hipStream_t)stream));
+  out3[0] = c[0] + c[7];  // full and sparse tiles
+  out3[1] = c[1];
+  out3[2] = c[2];
   return 0;
 }
 

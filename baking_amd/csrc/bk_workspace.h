@@ -47,8 +47,8 @@ struct bk_workspace {
   int32_t* blocks = nullptr;                    // non-empty particle blocks
   int4* tiles = nullptr;                        // P2G tiles: (block, dough, first sorted particle, count)
   int32_t* active = nullptr;                    // active node blocks
-  int32_t* counters = nullptr;                  // [0] tiles, [1] active blocks, [2] deferred, [3] particle blocks,
-                                                // [4] emitted samples, [6] substeps run
+  int32_t* counters = nullptr;                  // [0] full tiles, [1] active blocks, [2] deferred, [3] particle blocks,
+                                                // [4] emitted samples, [6] substeps run, [7] sparse tiles
   int32_t* queue = nullptr;                     // particles whose coupled porosity needs the full solve
   float4* prep = nullptr;                       // per particle, pass C -> deferred pass: the first half of a deferred
                                                 // particle's Ding update (DingPrep, kDingPrepVec float4)
@@ -117,13 +117,23 @@ __device__ __forceinline__ BlockGeom block_geom(const bk_dough* __restrict__ D, 
 }
 
 // One P2G tile: up to `chunk` sorted particles of one particle block.  Every
-// field is wave-uniform.
+// field is wave-uniform.  The tile list holds the tiles of at least
+// kFullTile particles from its front (counters[0] of them) and the sparser
+// ones -- blocks on a dough's surface, a split block's remainder -- from its
+// back (counters[7]), so tile index t walks the full tiles first: a tiled pass
+// over more tiles than the GPU holds at once ends on its shortest work.
+constexpr int kFullTile = 256;
+__device__ __forceinline__ int tile_count(const bk_workspace* ws) { return ws->counters[0] + ws->counters[7]; }
+__device__ __forceinline__ int tile_slot(const bk_workspace* ws, int t) {
+  const int front = ws->counters[0];
+  return t < front ? t : ws->max_n + ws->total_blocks - 1 - (t - front);
+}
 struct Tile {
   int d, start, count;
   BlockGeom g;
 };
 __device__ __forceinline__ Tile tile_at(const bk_dough* __restrict__ doughs, const bk_workspace* ws, int t) {
-  const int4 e = ws->tiles[t];
+  const int4 e = ws->tiles[tile_slot(ws, t)];
   Tile T;
   const int blk = __builtin_amdgcn_readfirstlane(e.x);
   T.d = __builtin_amdgcn_readfirstlane(e.y);
