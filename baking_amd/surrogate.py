@@ -42,6 +42,15 @@ LAYERS = (("W1", (WIDTH, ROW)), ("b1", (WIDTH,)), ("W2", (WIDTH, WIDTH)), ("b2",
           ("b3", (WIDTH,)), ("W4", (WIDTH, WIDTH)), ("b4", (WIDTH,)), ("W5", (N_OUT, WIDTH)), ("b5", (N_OUT,)))
 THETA_COUNT = sum(int(np.prod(shape)) for _, shape in LAYERS)
 DEFAULT_MODEL = Path(__file__).resolve().parent / "data" / "return_map_surrogate.npz"
+BATCHED_MODEL = Path(__file__).resolve().parent / "data" / "return_map_surrogate_batched48.npz"
+
+
+def default_model(workload="loaf_256"):
+    """The packaged network of a workload: the loaf's (dt 5e-5), or the batched doughs' (dt 1.25e-5) when it is
+    packaged."""
+    if workload == "batched_48" and BATCHED_MODEL.exists():
+        return BATCHED_MODEL
+    return DEFAULT_MODEL
 
 # the material parameters a trained network is valid for (the map depends on them)
 SIGNATURE_KEYS = ("shear_modulus", "lame_lambda", "yield_stress", "viscosity", "porosity", "time_step",

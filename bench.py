@@ -84,7 +84,7 @@ def parse():
     ap.add_argument("--workload", default="batched_48",
                     choices=["batched_48", "loaf_256", "sphere_small", "adjoint_loaf", "rest_loaf"])
     ap.add_argument("--no-surrogate", action="store_true", help="exact return map only")
-    ap.add_argument("--model", default=None, help="surrogate model file (default: the packaged one)")
+    ap.add_argument("--model", default=None, help="surrogate model file (default: the workload's packaged one)")
     ap.add_argument("--plain-steps", type=int, default=20, help="batched_48: substeps per dough of the plain comparator")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--profile-steps", type=int, default=20)
@@ -343,7 +343,7 @@ def run_batched(args):
     batch = BatchSimulator(sims)
     model = None
     if not args.no_surrogate:
-        model = surrogate.ReturnMapSurrogate.load(args.model or surrogate.DEFAULT_MODEL)
+        model = surrogate.ReturnMapSurrogate.load(args.model or surrogate.default_model("batched_48"))
         surrogate.attach(batch, model)
     tuned = None if args.no_autotune else autotune(batch)
     if model is not None:

@@ -10,7 +10,9 @@
   gives the exact run's fields (the guided solve converges to the same root)
   with fewer evaluations of the porosity equation; standing in (the
   prediction kept after one Newton correction) meets the parity rule
-  against the oracle with fewer evaluations than the exact method.
+  against the oracle with fewer evaluations than the exact method;
+* the packaged networks of the loaf and of the 48 batched doughs on their
+  own workloads against the method without them.
 """
 import numpy as np
 import pytest
@@ -226,3 +228,66 @@ def test_packaged_network_gain_on_the_loaf():
             if inc > 0:
                 assert abs(an[key][0] - ae[key][0]) <= 1e-3 * inc
     assert results["standin"][1]["evaluations"] < results["guided"][1]["evaluations"]
+
+
+def test_packaged_network_gain_on_the_batched_doughs():
+    """The batched workload's packaged network (trained on the 48 doughs at dt 1.25e-5; the loaf's network, trained at
+    dt 5e-5, sees another return map) against the method without it, from one state of the full 48-dough batch:
+    standing in, nearly every prediction accepted, fewer porosity-equation evaluations, a faster return-map stage and
+    aggregates within 1e-6 (1e-3 of their increments) of the exact run's; and fewer evaluations than the loaf's
+    network gives on the same substeps."""
+    import ctypes
+    from baking_amd.measure import measure
+    from baking_amd.physics.batch import BatchSimulator
+    assert surrogate.default_model("batched_48") == surrogate.BATCHED_MODEL and surrogate.BATCHED_MODEL.exists()
+    nets = dict(batched=surrogate.ReturnMapSurrogate.load(surrogate.BATCHED_MODEL), loaf=surrogate.ReturnMapSurrogate.load())
+    batch = BatchSimulator([scenes.build_simulator(s) for s in scenes.batched_doughs()])
+    batch._launch_substeps(200)
+    assert batch.ReadFlags().max() == 0
+    snap = batch._save_for_tuning()
+    start = measure(batch)
+    lib = _native.load()
+    results = {}
+    for name in ("exact", "batched", "loaf"):
+        batch._restore_after_tuning(snap)
+        if name == "exact":
+            surrogate.detach(batch)
+        else:
+            assert surrogate.attach(batch, nets[name], mode="standin") == 48
+        batch._launch_substeps(2)  # capture both graphs
+        batch._restore_after_tuning(snap)
+        surrogate.stats(batch, reset=True)
+        batch._launch_substeps(60)
+        aggr, st = measure(batch), surrogate.stats(batch, reset=True)
+        _native.check(lib.bk_profile_begin(batch._workspace, 5), "bk_profile_begin")
+        for _ in range(5):
+            batch._launch(use_graph=False)
+        nst = lib.bk_profile_stage_count()
+        ms = (ctypes.c_float * nst)()
+        rec = ctypes.c_int32(0)
+        _native.check(lib.bk_profile_end(batch._workspace, ctypes.cast(ms, ctypes.c_void_p),
+                                         ctypes.cast(ctypes.pointer(rec), ctypes.c_void_p)), "bk_profile_end")
+        stages = {lib.bk_profile_stage_name(k).decode(): ms[k] / max(rec.value, 1) for k in range(nst)}
+        results[name] = (aggr, st, stages["surrogate"] + stages["pass_c_deferred"])
+        assert batch.ReadFlags().max() == 0
+    ae, se, te = results["exact"]
+    pe = se["evaluations"] / se["return_maps"]
+    pn = {}
+    for name in ("batched", "loaf"):
+        an, sn, tn = results[name]
+        pn[name] = sn["evaluations"] / sn["return_maps"]
+        print(f"{name} network: evaluations per return map exact {pe:.2f}, with it {pn[name]:.2f}; return-map stage ms "
+              f"exact {te:.3f}, with it {tn:.3f}; accepted {sn['accepted']} of {sn['return_maps']}", flush=True)
+    an, sn, tn = results["batched"]
+    # as on the loaf, the runs yield in nearly the same particles; 48 doughs hold more particles on the threshold
+    assert se["return_maps"] > 1000000 and abs(sn["return_maps"] - se["return_maps"]) <= 1e-2 * se["return_maps"]
+    assert sn["accepted"] >= 0.98 * sn["return_maps"]
+    assert pn["batched"] < 0.25 * pe and pn["batched"] < 0.75 * pn["loaf"]
+    assert tn < te
+    for key in ("particle_volume_m3", "temperature_mean_k", "water_mass_kg", "gas_total_mol"):
+        for d in range(48):
+            rel = abs(an[key][d] - ae[key][d]) / abs(ae[key][d])
+            inc = abs(ae[key][d] - start[key][d])
+            assert rel <= 1e-6, (key, d, rel)
+            if inc > 0:
+                assert abs(an[key][d] - ae[key][d]) <= 1e-3 * inc, (key, d)
