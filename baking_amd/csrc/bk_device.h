@@ -671,14 +671,21 @@ __device__ inline int ding_solve(const DingParams& P, const float s[3], float yv
     const double g = (double(aux->dev_norm) / double(aux->dev_target) - 1.0) / double(df);
     aux->gamma_start = g > 0.0 ? g : 0.0;
   }
-  bool solved = false, stood_in = false;
+  bool solved = false, stood_in = false, clear_inside = false;
   if (defer) {
     double trial_n = opv - 1.0 + double(P.n0);
-    bool inside = trial_n > 0.0 && trial_n < 1.0 && double(q) + pressure_coefficient64(trial_n) * double(pv) <= 1.0;
+    const double a_trial = (trial_n > 0.0 && trial_n < 1.0) ? pressure_coefficient64(trial_n) : 0.0;
+    bool inside = trial_n > 0.0 && trial_n < 1.0 && double(q) + a_trial * double(pv) <= 1.0;
     if (!inside) return kDingDeferred;
     // the solvers' elastic exit, decided by the same test on the same values
     ret_n = float(trial_n);
     solved = true;
+    // The residual below evaluates the prediction at float(trial_n), within 2^-24 of trial_n.  On (0, 0.5] the
+    // pressure coefficient a(n) = 9 / (4 ln(1/n)^2 (3.5 n + 1)) has |d ln a / d ln n| = |2 / ln(1/n) - 3.5 n / (3.5 n + 1)|
+    // < 3, so q + a p moves by less than 2e-7 of a p <= 1: with more margin than 1e-6 below 1 the prediction's
+    // overstress is 0 and it returns pv - 1 + n0 without evaluating a again.
+    clear_inside = trial_n > 1.0e-30 && trial_n <= 0.5 && 1.0 - (double(q) + a_trial * double(pv)) > 1.0e-6 && isfinite(ms) &&
+                   isfinite(ex);
   }
   double standin_residual = 0.0;
   if (!solved && aux && aux->standin) {
@@ -701,7 +708,7 @@ __device__ inline int ding_solve(const DingParams& P, const float s[3], float yv
   if (stood_in) {
     cres = float(standin_residual);
   } else {
-    double pred = porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
+    double pred = clear_inside ? opv - 1.0 + double(P.n0) : porosity_prediction(double(ret_n), q, pv, df, ms, ex, opv, P.n0);
     if (aux) ++aux->evals;
     cres = float(fabs(double(ret_n) - pred));
   }

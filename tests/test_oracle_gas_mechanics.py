@@ -179,3 +179,29 @@ def test_excessive_time_step_rejected():
     """Reference tests/test_mpm.py:71-73."""
     with pytest.raises(ValueError, match="CFL"):
         replace(SimulationConfig(), time_step=1.0).Validate(MaterialConfig())
+
+
+def test_inside_residual_shortcut_finds_no_overstress():
+    """The fused substep skips the porosity prediction of an elastic particle's residual when the trial porosity t is
+    in (1e-30, 0.5] and 1 - (q + a(t) p) > 1e-6 (a the f64 pressure coefficient): the prediction at float(t) then
+    finds no overstress and returns pv - 1 + n0.  Pinned here on the oracle's prediction at float(t), over random
+    states packed against the 1e-6 margin and across the whole range of t, the largest slope of a included."""
+    rng = np.random.default_rng(17)
+    checked = 0
+    for k in range(4000):
+        n0 = np.float32(rng.uniform(0.0, 0.3))
+        t = float(10 ** rng.uniform(-12, np.log10(0.5))) if k % 4 else float(rng.uniform(0.3, 0.5))
+        pv = 1.0 - float(n0) + t  # the old plastic volume whose trial porosity is t
+        a = 9.0 / (4.0 * np.log(1.0 / t) ** 2 * (3.5 * t + 1.0))  # the f64 pressure coefficient (porous_return.py:11-16)
+        # q + a p just inside the margin: q in [0, 1), p = (1 - q - margin) / a with the margin a little above 1e-6
+        q = np.float32(rng.uniform(0.0, 0.999))
+        margin = 1e-6 * (1.0 + 10 ** rng.uniform(-6, 1))
+        p = np.float32(max(0.0, (1.0 - float(q) - margin) / a)) if a > 0 else np.float32(rng.uniform(0, 10))
+        if not (1.0 - (float(q) + a * float(p)) > 1e-6):
+            continue
+        df, ms = np.float32(10 ** rng.uniform(-4, 1)), np.float32(10 ** rng.uniform(-4, 1))
+        ex = np.float32(rng.uniform(-0.1, 0.1))
+        pred = O.porosity_prediction(float(np.float32(t)), q, p, df, ms, ex, pv, n0)
+        assert pred == pv - 1.0 + float(n0), (t, q, p, margin)
+        checked += 1
+    assert checked > 2500
