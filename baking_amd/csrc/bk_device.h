@@ -189,18 +189,28 @@ __device__ inline void svd3(const mat3<double>& F, mat3<double>& U, double sig[3
     jacobi_rotate(A, V, 0, 2);
     jacobi_rotate(A, V, 1, 2);
   }
+  // sort descending (3-element network), swapping the eigenvalues with their columns of V
   double lam[3] = {A[0][0], A[1][1], A[2][2]};
-  int o0 = 0, o1 = 1, o2 = 2;
-  // sort descending (3-element network)
-  if (lam[o0] < lam[o1]) { int t = o0; o0 = o1; o1 = t; }
-  if (lam[o1] < lam[o2]) { int t = o1; o1 = o2; o2 = t; }
-  if (lam[o0] < lam[o1]) { int t = o0; o0 = o1; o1 = t; }
-  int ord[3] = {o0, o1, o2};
   double Vs[3][3];
 #pragma unroll
-  for (int j = 0; j < 3; ++j)
+  for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int i = 0; i < 3; ++i) Vs[i][j] = V[i][ord[j]];
+    for (int j = 0; j < 3; ++j) Vs[i][j] = V[i][j];
+  auto order = [&](int p, int q) {
+    const bool sw = lam[p] < lam[q];
+    const double lp = lam[p], lq = lam[q];
+    lam[p] = sw ? lq : lp;
+    lam[q] = sw ? lp : lq;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const double vp = Vs[i][p], vq = Vs[i][q];
+      Vs[i][p] = sw ? vq : vp;
+      Vs[i][q] = sw ? vp : vq;
+    }
+  };
+  order(0, 1);
+  order(1, 2);
+  order(0, 1);
   double dv = Vs[0][0] * (Vs[1][1] * Vs[2][2] - Vs[1][2] * Vs[2][1]) - Vs[0][1] * (Vs[1][0] * Vs[2][2] - Vs[1][2] * Vs[2][0]) +
               Vs[0][2] * (Vs[1][0] * Vs[2][1] - Vs[1][1] * Vs[2][0]);
   if (dv < 0.0) {

@@ -1484,9 +1484,10 @@ __global__ __launch_bounds__(kParticleThreads) __attribute__((amdgpu_waves_per_e
         axis_weights(l[0], wx);
         axis_weights(l[1], wy);
         axis_weights(l[2], wz);
-        float nv[3] = {0.f, 0.f, 0.f};
-        mat3<float> Cn = m_zero<float>();
-        const float s4 = 4.0f / (P.dx * P.dx);
+        // C = 4/dx^2 sum_i w_i v_i (x_i - x_p) with x_i - x_p = (o_i - l) dx, o_i in {0, 1, 2}^3, formed as
+        // 4/dx (sum_i w_i v_i o_i - v l): the offsets are constants of the unrolled loop, so the first sum
+        // takes one add (o = 1) or fma (o = 2) per velocity component and axis instead of a product per pair
+        float nv[3] = {0.f, 0.f, 0.f}, U[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
         const float* __restrict__ gv = g.v + 3 * node_base;
         const StencilTerms st = stencil_terms(b, nb, 3u);
 #pragma unroll
@@ -1495,18 +1496,26 @@ __global__ __launch_bounds__(kParticleThreads) __attribute__((amdgpu_waves_per_e
           for (int oy = 0; oy < 3; ++oy)
 #pragma unroll
             for (int oz = 0; oz < 3; ++oz) {
-              float wt = wx[ox] * wy[oy] * wz[oz];
+              const float wt = wx[ox] * wy[oy] * wz[oz];
               const float* vp = gv + (st.x[ox] + st.y[oy] + st.z[oz]);
-              float vi[3] = {vp[0], vp[1], vp[2]};
-              float dd[3] = {(float(ox) - l[0]) * P.dx, (float(oy) - l[1]) * P.dx, (float(oz) - l[2]) * P.dx};
-              float sw = s4 * wt;
+              const int o[3] = {ox, oy, oz};
 #pragma unroll
               for (int a = 0; a < 3; ++a) {
-                nv[a] += wt * vi[a];
+                const float wv = wt * vp[a];
+                nv[a] += wv;
 #pragma unroll
-                for (int c = 0; c < 3; ++c) Cn.a[a * 3 + c] += sw * (vi[a] * dd[c]);
+                for (int c = 0; c < 3; ++c) {
+                  if (o[c] == 1) U[a][c] += wv;
+                  if (o[c] == 2) U[a][c] = fmaf(2.0f, wv, U[a][c]);
+                }
               }
             }
+        mat3<float> Cn;
+        const float s1 = 4.0f / P.dx;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) Cn.a[a * 3 + c] = s1 * (U[a][c] - nv[a] * l[c]);
         mat3<float> F_old = m_load(in.F + 9 * p);
         mat3<float> inc = m_scale(P.dt, Cn);
         inc.a[0] += 1.0f; inc.a[4] += 1.0f; inc.a[8] += 1.0f;
