@@ -732,7 +732,7 @@ __device__ __forceinline__ Robin robin_terms(const bk_params& P, bool rest, floa
 // the key's, formed from that base), so a lane can take K of them at a time
 // and add their summed contribution with one LDS atomic per node and field.
 // Wave w takes ranks K*w .. K*w + K-1 of each cell, then every 4K-th.  Pass A
-// (four to seven f64 fields per node, bound by its LDS atomics) takes pairs;
+// (four f64 fields per node, bound by its LDS atomics) takes pairs;
 // pass B2 (four fields, more arithmetic per particle) is fastest one at a
 // time.  (A loop over groups of slots with different bases would let the
 // compiler keep all 27 K weight products live across it: 200+ VGPRs.)
@@ -784,10 +784,53 @@ __device__ __forceinline__ unsigned pack_group(const int (&base)[K][3], const Ti
 }
 
 // ============================================================================ pass A: thermal P2G
+// The Robin heat, water loss and exchange of one face owner (thermal_water.py:108-120) added to the tile's H, L and
+// X sums (thermal_water.py:121-137).  Only a face owner has exposed or contact area: for every other particle the
+// three terms are 0 * finite = 0, so a tile's main loop adds only the volume, capacity, energy and water sums and
+// lists its face owners, whose terms are added afterwards, one owner per lane (owners whose stencils share a node
+// serialise on its LDS address).  A particle off the grid or off the tile's block (a key out of date, flagged by
+// the main loop) adds nothing, as in the main loop.
+constexpr int kSurfCap = 128;  // face owners listed per tile; a tile with more sweeps its particles instead
+__device__ __forceinline__ void owner_to_tile(const bk_params& P, bool rest, float heat_param, const bk_scene& sc,
+                                              const bk_state& in, const bk_workspace* ws, const Tile& tg, int p, int slot,
+                                              double (*rob)[kRegionNodes]) {
+  const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
+  float g3[3];
+  grid_position(in.x + 3 * p, o, P.dx, g3);
+  int b[3];
+  stencil_base(g3, b);
+  if (!valid_stencil(b, P.res) || (b[0] >> 2) != tg.g.bx || (b[1] >> 2) != tg.g.by || (b[2] >> 2) != tg.g.bz) return;
+  const float Jf = m_det(m_load(in.F + 9 * p));
+  const Surface sf{ws->slot_exposed[slot], ws->slot_contact[slot]};
+  const Robin r = robin_terms(P, rest, heat_param, Jf, sc.V0[p], in.T[p], in.W[p], sf);
+  const double f[3] = {r.heat, r.loss, r.exch};
+  float wx[3], wy[3], wz[3];
+  axis_weights(g3[0] - float(b[0]), wx);
+  axis_weights(g3[1] - float(b[1]), wy);
+  axis_weights(g3[2] - float(b[2]), wz);
+  const int li = b[0] - 4 * tg.g.bx, lj = b[1] - 4 * tg.g.by, lk = b[2] - 4 * tg.g.bz;
+#pragma unroll
+  for (int ox = 0; ox < 3; ++ox)
+#pragma unroll
+    for (int oy = 0; oy < 3; ++oy)
+#pragma unroll
+      for (int oz = 0; oz < 3; ++oz) {
+        const double wt = double(wx[ox] * wy[oy] * wz[oz]);
+        const int ri = region_index(li + ox, lj + oy, lk + oz);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const double v = wt * f[a];
+          if (v != 0.0) atomicAdd(&rob[a][ri], v);
+        }
+      }
+}
+
 __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_grid g,
                                                          bk_workspace* ws) {
-  __shared__ double acc[7][kRegionNodes];
+  __shared__ double acc[7][kRegionNodes];  // V, C, E, W from the main loop; H, L, X from the face owners
   __shared__ int cell_lo[64], cell_hi[64];
+  __shared__ int surf_p[kSurfCap], surf_s[kSurfCap];  // listed face owners: particle, slot
+  __shared__ int surf_n;
   const int ntiles = tile_count(ws);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -795,49 +838,46 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
     const bk_params& P = doughs[tg.d].params;
     if (!P.transport_enabled) continue;  // a dough without transport in a batch with some (uniform per tile)
     const float o[3] = {P.origin[0], P.origin[1], P.origin[2]};
-    const bool rest = resting(P, ws, tg.d);
-    const float heat_param = sc.heat_param[tg.d];
     for (int i = threadIdx.x; i < 7 * kRegionNodes; i += kTileThreads) (&acc[0][0])[i] = 0.0;
     if (threadIdx.x < 64) {
       cell_lo[threadIdx.x] = 0;
       cell_hi[threadIdx.x] = 0;
     }
+    if (threadIdx.x == 0) surf_n = 0;
     __syncthreads();
     tile_cell_ranges(ws, tg, cell_lo, cell_hi);
     const int hi = cell_hi[lane];
     for (int j0 = cell_lo[lane] + kPackA * w; j0 < hi; j0 += 4 * kPackA) {
       float wx[kPackA][3], wy[kPackA][3], wz[kPackA][3];
-      double f[kPackA][7];
-      int base[kPackA][3];
+      double f[kPackA][4];
+      int base[kPackA][3], pid[kPackA], slot[kPackA];
       unsigned pending = 0u;
 #pragma unroll
       for (int u = 0; u < kPackA; ++u) {
         const int j = j0 + u;
         const int p = ws->sorted_vals[tg.start + min(j, hi - 1)];
+        pid[u] = p;
+        slot[u] = sc.particle_slot[p];
         float g3[3];
         grid_position(in.x + 3 * p, o, P.dx, g3);
         const bool ok = pack_member(g3, j < hi, P.res, tg, base[u], wx[u], wy[u], wz[u]);
         const float Jf = m_det(m_load(in.F + 9 * p));
         const double T = in.T[p], W = in.W[p];
-        const Robin r = robin_terms(P, rest, heat_param, Jf, sc.V0[p], T, W, particle_surface(sc, ws, p));
         const double cap = sc.capacities[p];
-        const double v[7] = {r.vol, cap, cap * T, W, r.heat, r.loss, r.exch};
+        const double v[4] = {double(sc.V0[p]) * double(Jf), cap, cap * T, W};
 #pragma unroll
-        for (int a = 0; a < 7; ++a) f[u][a] = ok ? v[a] : 0.0;
+        for (int a = 0; a < 4; ++a) f[u][a] = ok ? v[a] : 0.0;
         if (ok) pending |= 1u << u;
       }
       int li, lj, lk;
       const unsigned sel = pack_group(base, tg, pending, li, lj, lk);
       flag_max(dough_flags(ws, tg.d), BK_FLAG_STENCIL, sel != pending);  // a slot off its cell's base: a key out of date
       if (sel) {
-        double fs[kPackA][7];
-        bool surface = false;  // the Robin terms are zero off the surface (no exposed or contact area)
+        double fs[kPackA][4];
 #pragma unroll
-        for (int u = 0; u < kPackA; ++u) {
+        for (int u = 0; u < kPackA; ++u)
 #pragma unroll
-          for (int a = 0; a < 7; ++a) fs[u][a] = (sel >> u & 1u) ? f[u][a] : 0.0;
-          surface = surface || fs[u][4] != 0.0 || fs[u][5] != 0.0 || fs[u][6] != 0.0;
-        }
+          for (int a = 0; a < 4; ++a) fs[u][a] = (sel >> u & 1u) ? f[u][a] : 0.0;
 #pragma unroll
         for (int ox = 0; ox < 3; ++ox)
 #pragma unroll
@@ -856,14 +896,35 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
               };
 #pragma unroll
               for (int a = 0; a < 4; ++a) atomicAdd(&acc[a][ri], sum(a));
-              if (surface) {
-#pragma unroll
-                for (int a = 4; a < 7; ++a) atomicAdd(&acc[a][ri], sum(a));
-              }
             }
       }
+#pragma unroll
+      for (int u = 0; u < kPackA; ++u)
+        if ((sel >> u & 1u) && slot[u] >= 0) {
+          const int i = atomicAdd(&surf_n, 1);
+          if (i < kSurfCap) {
+            surf_p[i] = pid[u];
+            surf_s[i] = slot[u];
+          }
+        }
     }
     __syncthreads();
+    const int ns = surf_n;  // uniform
+    if (ns > 0) {
+      const bool rest = resting(P, ws, tg.d);
+      const float heat_param = sc.heat_param[tg.d];
+      if (ns <= kSurfCap) {
+        for (int i = threadIdx.x; i < ns; i += kTileThreads) owner_to_tile(P, rest, heat_param, sc, in, ws, tg, surf_p[i], surf_s[i], acc + 4);
+      } else {  // more owners than the list holds: every thread takes the tile's owners among its particles
+        for (int j = threadIdx.x; j < tg.count; j += kTileThreads) {
+          const int p = ws->sorted_vals[tg.start + j];
+          const int sl = sc.particle_slot[p];
+          if (sl >= 0) owner_to_tile(P, rest, heat_param, sc, in, ws, tg, p, sl, acc + 4);
+        }
+      }
+      __syncthreads();
+    }
+    const int nf = ns > 0 ? 7 : 4;
     for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
       int li = ri / 36, lj = (ri / 6) % 6, lk = ri % 6;
       int i = 4 * tg.g.bx + li, j = 4 * tg.g.by + lj, k = 4 * tg.g.bz + lk;
@@ -873,7 +934,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
 #pragma unroll
       for (int a = 0; a < 7; ++a) {
         double v = acc[a][ri];
-        if (v != 0.0) unsafeAtomicAdd(dst[a] + node, v);
+        if (a < nf && v != 0.0) unsafeAtomicAdd(dst[a] + node, v);
       }
     }
     __syncthreads();
