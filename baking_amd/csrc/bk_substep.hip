@@ -1169,6 +1169,8 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(const bk_dough* __rest
   // for this scatter, and each node's sum is rounded to float once, at the flush.
   __shared__ double acc[4][kRegionNodes];
   __shared__ int cell_lo[64], cell_hi[64];
+  __shared__ int work[kWorkCap];
+  __shared__ int work_n, work_next;
   const int ntiles = tile_count(ws);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -1183,8 +1185,16 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(const bk_dough* __rest
     }
     __syncthreads();
     tile_cell_ranges(ws, tg, cell_lo, cell_hi);
-    const int hi = cell_hi[lane];
-    for (int j0 = cell_lo[lane] + kPackB2 * w; j0 < hi; j0 += 4 * kPackB2) {
+    for (int q0 = 0; q0 >= 0;) {
+    tile_work_list<kPackB2>(cell_lo, cell_hi, q0, work, &work_n, &work_next);
+    __syncthreads();
+    const int nwork = work_n;
+    q0 = work_next;
+    for (int e0 = 64 * w; e0 < nwork; e0 += 4 * 64) {
+      const bool live = e0 + lane < nwork;  // a lane past the list reads the last entry and takes no particle
+      const int ent = work[min(e0 + lane, nwork - 1)];
+      const int hi = cell_hi[ent & 63];
+      const int j0 = ent >> 6;
       float wx[kPackB2][3], wy[kPackB2][3], wz[kPackB2][3], l[kPackB2][3];
       float mass[kPackB2], mv[kPackB2][3];
       mat3<float> A[kPackB2];
@@ -1196,7 +1206,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(const bk_dough* __rest
         const int p = ws->sorted_vals[tg.start + min(j, hi - 1)];
         float g3[3];
         grid_position(in.x + 3 * p, o, P.dx, g3);
-        const bool ok = pack_member(g3, j < hi, P.res, tg, base[u], wx[u], wy[u], wz[u]);
+        const bool ok = pack_member(g3, live && j < hi, P.res, tg, base[u], wx[u], wy[u], wz[u]);
 #pragma unroll
         for (int a = 0; a < 3; ++a) l[u][a] = ok ? g3[a] - float(base[u][a]) : 0.0f;  // finite offsets for an empty slot
         const float m = masses[p];
@@ -1237,7 +1247,8 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_b2(const bk_dough* __rest
             }
       }
     }
-    __syncthreads();
+    __syncthreads();  // the list is read before the next window overwrites it, and every add is done before the flush
+    }
     for (int ri = threadIdx.x; ri < kRegionNodes; ri += kTileThreads) {
       int li = ri / 36, lj = (ri / 6) % 6, lk = ri % 6;
       int i = 4 * tg.g.bx + li, j = 4 * tg.g.by + lj, k = 4 * tg.g.bz + lk;
@@ -2147,9 +2158,9 @@ int bk_workspace_set_tuning(bk_workspace* ws, const bk_tuning* tuning) {
   HostWs* hw = host_view(ws);
   if (!hw || !tuning) return fail("bk_workspace_set_tuning: bad argument");
   const bk_tuning t = *tuning;
-  if (t.tile_chunk < 1 || t.grid_cap < 1 || t.grid_cap > 65536 || t.particle_block < 64 || t.particle_block > kParticleThreads ||
-      t.particle_block % 64)
-    return fail("bk_workspace_set_tuning: need tile_chunk >= 1, grid_cap in [1, 65536], particle_block in {64, 128, 192, 256}");
+  if (t.tile_chunk < 1 || t.tile_chunk > kMaxTileChunk || t.grid_cap < 1 || t.grid_cap > 65536 || t.particle_block < 64 ||
+      t.particle_block > kParticleThreads || t.particle_block % 64)
+    return fail("bk_workspace_set_tuning: need tile_chunk in [1, 2^20], grid_cap in [1, 65536], particle_block in {64, 128, 192, 256}");
   hw->tuning = t;
   return 0;
 }

@@ -163,3 +163,35 @@ __device__ __forceinline__ void tile_cell_ranges(const bk_workspace* ws, const T
   __syncthreads();
 }
 
+// A tile's work list for K particles per lane: entry (c, q) is the rank group q (ranks K q .. K q + K - 1) of cell
+// c, and the entries run group-major (every cell's group 0, then every cell's group 1, ...), so 64 consecutive
+// entries are 64 distinct cells except where they cross from one group to the next.  Lanes taking consecutive
+// entries stay busy where a tile has empty or short cells, which lane-per-cell leaves idle.  Entry e holds
+// (tile-relative position of the group's first particle) << 6 | c.  Wavefront 0 fills groups q0, q0 + 1, ...
+// while they fit kWorkCap entries and leaves the entry count in *n and the next group in *next (-1: none left).
+// Call with lo / hi from tile_cell_ranges, the whole workgroup, then a barrier.
+constexpr int kWorkCap = 1024;
+constexpr int kMaxTileChunk = 1 << 20;  // tile-relative positions stay below 2^20, so (position << 6 | cell) fits 32 bits
+template <int K>
+__device__ __forceinline__ void tile_work_list(const int* lo, const int* hi, int q0, int* list, int* n, int* next) {
+  if (threadIdx.x >= 64) return;
+  const int c = int(threadIdx.x);
+  const int first = lo[c], len = hi[c] - first;
+  const unsigned long long below = c == 0 ? 0ull : (~0ull >> (64 - c));
+  int e = 0, q = q0;
+  for (;;) {
+    const bool has = len > K * q;
+    const unsigned long long m = __ballot(has);
+    const int cnt = __popcll(m);
+    if (cnt == 0) { q = -1; break; }
+    if (e + cnt > kWorkCap) break;
+    if (has) list[e + __popcll(m & below)] = ((first + K * q) << 6) | c;
+    e += cnt;
+    ++q;
+  }
+  if (c == 0) {
+    *n = e;
+    *next = q;
+  }
+}
+

@@ -93,7 +93,7 @@ typedef struct bk_dough {
 } bk_dough;
 
 /* Launch tuning of a workspace (no reference counterpart):
- *   tile_chunk      most particles per P2G tile (a particle block holding more is split)
+ *   tile_chunk      most particles per P2G tile, at most 2^20 (a particle block holding more is split)
  *   particle_block  workgroup size of the per-particle passes (64, 128, 192 or 256)
  *   grid_cap        workgroups of the tile and node-block passes (grid-stride)      */
 typedef struct bk_tuning {

@@ -307,16 +307,21 @@ def test_batched_substep_sort_is_the_stable_sort_of_the_keys():
     _check_order(*_substep_order(batch, batch.particle_count), want)
 
 
-def test_crowded_block_sort_is_the_stable_sort_of_the_keys():
-    """More than 1024 particles in one 4x4x4 block (the sort's LDS capacity): the global-memory ranking path."""
+def _crowded_spec():
+    """A 0.06 m cube sampled 400 particles per tet: more than 1024 particles in one 4x4x4 block."""
     from baking_amd.config import DingConfig, MaterialConfig, SimulationConfig, TransportConfig
     from baking_amd.geometry.mesh_loader import VoxelTetMesh
     from baking_amd.geometry.mesh_validation import ValidateTetMesh
     from baking_amd.geometry.particle_sampling import SampleParticles
     verts, tets = VoxelTetMesh((1, 1, 1), 0.06, (0.52, 0.52, 0.52))
     mesh = ValidateTetMesh(verts, tets)
-    spec = scenes.SceneSpec("crowded", mesh, SampleParticles(mesh, 1000.0, 400), SimulationConfig(gravity=(0.0, -9.81, 0.0)),
+    return scenes.SceneSpec("crowded", mesh, SampleParticles(mesh, 1000.0, 400), SimulationConfig(gravity=(0.0, -9.81, 0.0)),
                             MaterialConfig(), DingConfig(), TransportConfig(), steps=2)
+
+
+def test_crowded_block_sort_is_the_stable_sort_of_the_keys():
+    """More than 1024 particles in one 4x4x4 block (the sort's LDS capacity): the global-memory ranking path."""
+    spec = _crowded_spec()
     sim = scenes.build_simulator(spec)
     n = sim.particle_data.particle_count
     x = sim.current_state.particle_positions.cpu().numpy().copy()
@@ -324,6 +329,31 @@ def test_crowded_block_sort_is_the_stable_sort_of_the_keys():
     want = _expected_keys(spec, x)
     assert np.bincount(want >> 6).max() > 1024
     _check_order(*_substep_order(sim, n), want)
+
+
+def test_dense_tile_substep_parity():
+    """A 0.06 m cube of 6^3 voxels, 2 particles per tet, of the transport demo's dough (Ding, pore gas, heat and
+    water transport): all 2,592 particles and the 419 owners of its boundary faces fall in one 4x4x4 block.  So one
+    P2G tile holds more than 1024 particles (the mechanical P2G fills and walks its (cell, rank) work list in several
+    windows) and more face owners than the thermal P2G lists (it sweeps the tile for them).  Three substeps against
+    the FP32/FP64 oracle."""
+    from baking_amd.config import DingConfig, MaterialConfig, SimulationConfig, TransportConfig
+    from baking_amd.geometry.mesh_loader import VoxelTetMesh
+    from baking_amd.geometry.mesh_validation import ValidateTetMesh
+    from baking_amd.geometry.particle_sampling import SampleParticles
+    from baking_amd.physics.thermal_water import BoundaryFaces
+    verts, tets = VoxelTetMesh((6, 6, 6), 0.01, (0.52, 0.52, 0.52))
+    mesh = ValidateTetMesh(verts, tets)
+    samples = SampleParticles(mesh, 1000.0, 2)
+    sim = SimulationConfig(time_step=1e-4, gravity=(0.0, -9.81, 0.0), grid_spacing=0.025, grid_resolution=40)
+    ding = DingConfig(gas_enabled=True, porosity=0.2, gas_production_rate=0.001, temperature_enabled=True)
+    spec = scenes.SceneSpec("dense_tile", mesh, samples, sim, MaterialConfig(model="ding2019", young_modulus=10000.0), ding,
+                            TransportConfig(enabled=True, heat_transfer=20.0), steps=3)
+    keys = _expected_keys(spec, samples.particle_positions.astype(np.float32))
+    owners = np.unique(BoundaryFaces(mesh, samples)[1])
+    assert np.bincount(keys >> 6).max() > 1024
+    assert np.bincount(keys[owners] >> 6).max() > 128
+    three_way(spec, 3, rtol=RTOL, check_every=1)
 
 
 @pytest.mark.parametrize("sticky", [True, False])
