@@ -831,6 +831,8 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
   __shared__ int cell_lo[64], cell_hi[64];
   __shared__ int surf_p[kSurfCap], surf_s[kSurfCap];  // listed face owners: particle, slot
   __shared__ int surf_n;
+  __shared__ int work[kWorkCap];
+  __shared__ int work_n, work_next;
   const int ntiles = tile_count(ws);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -846,8 +848,16 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
     if (threadIdx.x == 0) surf_n = 0;
     __syncthreads();
     tile_cell_ranges(ws, tg, cell_lo, cell_hi);
-    const int hi = cell_hi[lane];
-    for (int j0 = cell_lo[lane] + kPackA * w; j0 < hi; j0 += 4 * kPackA) {
+    for (int q0 = 0; q0 >= 0;) {
+    tile_work_list<kPackA>(cell_lo, cell_hi, q0, work, &work_n, &work_next);
+    __syncthreads();
+    const int nwork = work_n;
+    q0 = work_next;
+    for (int e0 = 64 * w; e0 < nwork; e0 += 4 * 64) {
+      const bool live = e0 + lane < nwork;  // a lane past the list reads the last entry and takes no particle
+      const int ent = work[min(e0 + lane, nwork - 1)];
+      const int hi = cell_hi[ent & 63];
+      const int j0 = ent >> 6;
       float wx[kPackA][3], wy[kPackA][3], wz[kPackA][3];
       double f[kPackA][4];
       int base[kPackA][3], pid[kPackA], slot[kPackA];
@@ -860,7 +870,7 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
         slot[u] = sc.particle_slot[p];
         float g3[3];
         grid_position(in.x + 3 * p, o, P.dx, g3);
-        const bool ok = pack_member(g3, j < hi, P.res, tg, base[u], wx[u], wy[u], wz[u]);
+        const bool ok = pack_member(g3, live && j < hi, P.res, tg, base[u], wx[u], wy[u], wz[u]);
         const float Jf = m_det(m_load(in.F + 9 * p));
         const double T = in.T[p], W = in.W[p];
         const double cap = sc.capacities[p];
@@ -908,7 +918,8 @@ __global__ __launch_bounds__(kTileThreads) void k_pass_a(const bk_dough* __restr
           }
         }
     }
-    __syncthreads();
+    __syncthreads();  // the list is read before the next window overwrites it, and every add is done after the last
+    }
     const int ns = surf_n;  // uniform
     if (ns > 0) {
       const bool rest = resting(P, ws, tg.d);
