@@ -56,7 +56,7 @@ class BkDough(ctypes.Structure):
 
 
 class BkTuning(ctypes.Structure):
-    _fields_ = [("tile_chunk", c_int32), ("particle_block", c_int32), ("grid_cap", c_int32)]
+    _fields_ = [("tile_chunk", c_int32), ("particle_block", c_int32), ("grid_cap", c_int32), ("deferred_wgs", c_int32)]
 
 
 GRID_FIELDS = ["m", "p", "v", "V", "C", "E", "W", "H", "L", "X", "dT", "dW"]
@@ -128,6 +128,11 @@ SIGNATURES = {
     "bk_graph_launch": ([P, P], c_int),
     "bk_batch_substep": ([P, POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P], c_int),
     "bk_batch_graph_create": ([P, POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid)], c_void_p),
+    "bk_substep_graph_create_pipelined": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid),
+                                           P], c_void_p),
+    "bk_batch_graph_create_pipelined": ([P, POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid)], c_void_p),
+    "bk_substep_finish": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), P, P], c_int),
+    "bk_batch_finish": ([P, POINTER(BkScene), POINTER(BkState), POINTER(BkState), P], c_int),
     "bk_graph_destroy": ([P], None),
     "bk_profile_begin": ([P, I], c_int),
     "bk_profile_end": ([P, P, P], c_int),

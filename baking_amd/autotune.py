@@ -8,6 +8,9 @@ under candidate ``bk_tuning`` values and keeps the fastest:
   atomics); large tiles keep one workgroup per block.
 * ``particle_block`` -- workgroup size of the per-particle passes.
 * ``grid_cap``       -- workgroups of the grid-stride tile and node-block passes.
+* ``deferred_wgs``   -- workgroups of the deferred return-map pass where it runs beside the next substep in a
+  pipelined run of substeps (0: runs are not pipelined).  A short queue finishes within the next substep's first
+  stages however it is spread; a long one slows them unless it is held to fewer wavefront slots.
 
 Coordinate descent: each parameter in turn is swept with the others held,
 twice over.  Every trial replays the captured HIP graph for ``steps``
@@ -24,6 +27,7 @@ CANDIDATES = {
     "tile_chunk": (32, 64, 128, 256, 512, 4096),
     "particle_block": (64, 128, 256),
     "grid_cap": (512, 1024, 2048, 4096, 8192),
+    "deferred_wgs": (0, 128, 256, 512, 1024),
 }
 
 

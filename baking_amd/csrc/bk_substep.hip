@@ -92,7 +92,7 @@ struct HostWs {
   std::vector<bk_dough> doughs;
   int n = 0, n_vertices = 0, n_slots = 0, key_bits = 6, max_dough_n = 0;
   bool thermal = false, gas = false, surrogate = false, settle = false;
-  bk_tuning tuning{4096, 256, 4096};
+  bk_tuning tuning{4096, 256, 4096, 256};
 };
 static std::mutex g_ws_mutex;
 static std::unordered_map<const bk_workspace*, HostWs> g_ws_cache;
@@ -162,8 +162,14 @@ constexpr uint32_t kFrozenKey = 0xFFFFFFFFu;  // the key of a particle of a stop
 
 __device__ __forceinline__ void batch_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of,
                                            const float* __restrict__ x, int n, uint32_t* __restrict__ keys,
-                                           int32_t* __restrict__ ranks, bk_workspace* ws, int p) {
-  if (p == 0) { ws->counters[0] = 0; ws->counters[2] = 0; ws->counters[3] = 0; ws->counters[6] += 1; ws->counters[7] = 0; }
+                                           int32_t* __restrict__ ranks, bk_workspace* ws, int p, int reset_queue) {
+  if (p == 0) {
+    ws->counters[0] = 0;
+    if (reset_queue) ws->counters[2] = 0;  // else k_join resets it, after the previous substep's deferred pass read it
+    ws->counters[3] = 0;
+    ws->counters[6] += 1;
+    ws->counters[7] = 0;
+  }
   const bool valid = p < n;
   bool live = valid;  // a stopped dough's particles get no key, so no tile, node block or grid work
   uint32_t key = 0u;
@@ -609,11 +615,12 @@ __device__ __forceinline__ void clear_active_blocks(bk_workspace* ws, const bk_g
 // not touch what the clear reads or writes).
 __global__ void k_clear_keys(const bk_dough* __restrict__ doughs, const int32_t* __restrict__ dough_of, const float* __restrict__ x,
                              int n, uint32_t* __restrict__ keys, int32_t* __restrict__ ranks, bk_workspace* ws, bk_grid g,
-                             int thermal, int clear_wgs) {
+                             int thermal, int clear_wgs, int reset_queue) {
   if (int(blockIdx.x) < clear_wgs)
     clear_active_blocks(ws, g, thermal, int(blockIdx.x), clear_wgs);
   else
-    batch_keys(doughs, dough_of, x, n, keys, ranks, ws, (int(blockIdx.x) - clear_wgs) * int(blockDim.x) + int(threadIdx.x));
+    batch_keys(doughs, dough_of, x, n, keys, ranks, ws, (int(blockIdx.x) - clear_wgs) * int(blockDim.x) + int(threadIdx.x),
+               reset_queue);
 }
 
 // ============================================================================ mesh + surface areas
@@ -1411,12 +1418,15 @@ __global__ __launch_bounds__(256) void k_surrogate(bk_workspace* ws) {
 // Second half of pass C: the deferred coupled-porosity return maps, 64 per
 // wavefront with no idle lanes waiting on a neighbour's solve.  The loop's
 // trip count is uniform per workgroup (sample appends are per workgroup).
+// fl: the flag block the pass raises its flags in (ws->flags, or ws->dflags beside the next substep); pending: take
+// the queue only while it is marked unprocessed (counters[5]), so a deferred pass launched after it was taken is a
+// no-op.
 __global__ __launch_bounds__(256) void k_pass_c_deferred(const bk_dough* __restrict__ doughs, bk_scene sc, bk_state in, bk_state out,
-                                                         bk_workspace* ws) {
+                                                         bk_workspace* ws, int32_t* __restrict__ fl, int pending) {
   __shared__ int lds[17];
   __shared__ unsigned long long totals[4];
   if (threadIdx.x < 4) totals[threadIdx.x] = 0ull;
-  const int nq = ws->counters[2];
+  const int nq = (!pending || ws->counters[5]) ? ws->counters[2] : 0;
   const bool emit_on = ws->samples != nullptr;
   const bool have_mlp = ws->mlp != nullptr;
   unsigned evals = 0, accepted = 0, rejected = 0, handled = 0;
@@ -1445,8 +1455,8 @@ __global__ __launch_bounds__(256) void k_pass_c_deferred(const bk_dough* __restr
                            sqrtf((pr[0] - m) * (pr[0] - m) + (pr[1] - m) * (pr[1] - m) + (pr[2] - m) * (pr[2] - m));
         }
         const int status = material_finish(P, sc, in, out, p, ws->prep, f0, f1, &aux);
-        flag_max(dough_flags(ws, d), BK_FLAG_POROSITY, f0);
-        flag_max(dough_flags(ws, d), BK_FLAG_COUPLED, f1);
+        flag_max(fl + d * BK_FLAG_COUNT, BK_FLAG_POROSITY, f0);
+        flag_max(fl + d * BK_FLAG_COUNT, BK_FLAG_COUPLED, f1);
         evals += unsigned(aux.evals);
         if (aux.guided) {
           if (aux.accepted) ++accepted; else ++rejected;
@@ -1530,6 +1540,41 @@ __global__ void k_freeze(bk_workspace* ws, int nd) {
   const int mode = !any ? 0 : (old == 0 ? 1 : 2);
   ws->freeze[d] = mode;
   ws->freeze_age[d] = mode ? ws->freeze_age[d] + 1 : 0;
+}
+
+// The join of a substep whose first stages ran beside the previous substep's deferred pass (after both): that
+// pass raised its flags in dflags.  A dough the substep started unfrozen but that pass flagged stops now, as if
+// k_freeze had seen the flags: mode 1, and its flags are the deferred pass's alone (what the substep's first
+// stages raised for it does not count, since they would not have run for a stopped dough).  Otherwise the flags
+// are merged.  Then the deferred queue is emptied for this substep's pass C and marked unprocessed.
+__global__ void k_join(bk_workspace* ws, int nd) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d == 0) {
+    ws->counters[2] = 0;
+    ws->counters[5] = 1;
+  }
+  if (d >= nd) return;
+  int32_t* fl = ws->flags + d * BK_FLAG_COUNT;
+  int32_t* df = ws->dflags + d * BK_FLAG_COUNT;
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < BK_FLAG_COUNT; ++k) any |= df[k] != 0;
+  if (!any) return;
+  const bool stopping = ws->freezing && ws->freeze[d] == 0;
+#pragma unroll
+  for (int k = 0; k < BK_FLAG_COUNT; ++k) {
+    fl[k] = stopping ? df[k] : max(fl[k], df[k]);
+    df[k] = 0;
+  }
+  if (stopping) {
+    ws->freeze[d] = 1;
+    ws->freeze_age[d] = 1;
+  }
+}
+
+// After the deferred pass that ends a run of such substeps: the queue is processed.
+__global__ void k_pending_done(bk_workspace* ws) {
+  if (threadIdx.x == 0) ws->counters[5] = 0;
 }
 
 // ============================================================================ pass C: G2P + Ding + gas refresh + saturation
@@ -1750,9 +1795,23 @@ struct StageTimer {
   }
 };
 
-// One substep of the uploaded batch, enqueued on `st`.
+// A substep whose first stages run beside the previous substep's deferred pass (captured graphs only): the side
+// stream and the events of the fork and the join.
+struct Pipe {
+  hipStream_t side;
+  hipEvent_t fork, join;
+};
+// Whether a batch's substeps run that way: tuned so (deferred_wgs > 0), a deferred pass exists (pore gas) and no
+// settling controller reads the state after it.
+static bool can_pipeline(const HostWs* hw) { return hw->tuning.deferred_wgs > 0 && hw->gas && !hw->settle; }
+
+// One substep of the uploaded batch, enqueued on `st`.  With `pipe`, the previous substep's deferred pass runs on
+// the side stream beside stages 1-7 (which neither read nor write what it writes: the Ding, gas and saturation
+// fields of the queued particles, its flags), k_join follows both, and this substep's own deferred pass is left for
+// the next substep or for bk_*_finish.
 static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g,
-                         hipStream_t st, bool allow_profile) {
+                         hipStream_t st, bool allow_profile, const Pipe* pipe = nullptr) {
+  const bool pipelined = pipe != nullptr;
   if (hw->doughs.empty()) return fail("bk_batch_substep: no dough table uploaded");
   if (!sc || !in || !out || !g) return fail("bk_batch_substep: null argument");
   if (!sc->dough_of && hw->doughs.size() > 1) return fail("bk_batch_substep: a batch of several doughs needs scene.dough_of");
@@ -1774,6 +1833,20 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     hipLaunchKernelGGL(k_freeze, dim3(div_up(nd, 64)), dim3(64), 0, st, ws, nd);
     BK_LAUNCH(freeze);
   }
+  if (pipelined) BK_HIP(hipEventRecord(pipe->fork, st));
+  // The previous substep's deferred pass on the side stream (its input state is this substep's output buffers and
+  // back), issued after stage 1-2's launch so that the stages keep the launch stream's hardware queue in the graph.
+  // tu.deferred_wgs workgroups at most (the pass strides over its queue): fewer hold fewer of the wavefront slots
+  // the stages beside it need, for longer.
+  auto side_branch = [&]() -> int {
+    if (!pipelined) return 0;
+    BK_HIP(hipStreamWaitEvent(pipe->side, pipe->fork, 0));
+    hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(n, 256), 1, tu.deferred_wgs)), dim3(256), 0, pipe->side, D, *sc, *out,
+                       *in, ws, h->dflags, 1);
+    BK_LAUNCH(pass_c_deferred);
+    BK_HIP(hipEventRecord(pipe->join, pipe->side));
+    return 0;
+  };
   // 1. clear the previous substep's blocks (counters[1] still holds their count)
   // 2. keys and ranks within blocks (also resets the per-substep counters)
   // One launch; under stage profiling, one launch per stage.
@@ -1782,20 +1855,21 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     {
       StageTimer tm(pr, st, kStageClear);
       hipLaunchKernelGGL(k_clear_keys, dim3(cap), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, h->vals_s[1], ws, *g,
-                         int(hw->thermal), cap);
+                         int(hw->thermal), cap, int(!pipelined));
       BK_LAUNCH(clear);
     }
     {
       StageTimer tm(pr, st, kStageKeys);
       hipLaunchKernelGGL(k_clear_keys, dim3(key_wgs), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, h->vals_s[1], ws,
-                         *g, int(hw->thermal), 0);
+                         *g, int(hw->thermal), 0, int(!pipelined));
       BK_LAUNCH(keys);
     }
   } else {
     hipLaunchKernelGGL(k_clear_keys, dim3(cap + key_wgs), dim3(256), 0, st, D, sc->dough_of, in->x, n, h->keys_raw, h->vals_s[1], ws,
-                       *g, int(hw->thermal), cap);
+                       *g, int(hw->thermal), cap, int(!pipelined));
     BK_LAUNCH(clear_keys);
   }
+  if (int rc = side_branch()) return rc;
   {  // 3. order by block, then by cell inside each block
     StageTimer tm(pr, st, kStageSort);
     const int nb = h->total_blocks;
@@ -1854,6 +1928,11 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
       BK_LAUNCH(diffuse);
     }
   }
+  if (pipelined) {
+    BK_HIP(hipStreamWaitEvent(st, pipe->join, 0));
+    hipLaunchKernelGGL(k_join, dim3(div_up(nd, 64)), dim3(64), 0, st, ws, nd);
+    BK_LAUNCH(join);
+  }
   {  // 8. scalar G2P, gas and water stress
     StageTimer tm(pr, st, kStagePassB1);
     hipLaunchKernelGGL(k_pass_b1, dim3(div_up(n, pblock)), dim3(pblock), 0, st, D, *sc, *in, *out, *g, ws, n);
@@ -1879,9 +1958,9 @@ static int batch_substep(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
     hipLaunchKernelGGL(k_surrogate, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, ws);
     BK_LAUNCH(surrogate);
   }
-  if (hw->gas) {  // 12. deferred coupled-porosity return maps
+  if (hw->gas && !pipelined) {  // 12. deferred coupled-porosity return maps
     StageTimer tm(pr, st, kStagePassCDeferred);
-    hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, D, *sc, *in, *out, ws);
+    hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(n, 256), 1, 1024)), dim3(256), 0, st, D, *sc, *in, *out, ws, h->flags, 0);
     BK_LAUNCH(pass_c_deferred);
   }
   if (hw->settle) {  // 13. settling controller (rest metrics and the rest/bake decision)
@@ -1981,20 +2060,34 @@ struct bk_graph {
   std::vector<bk_dough> table;  // the table the graph was captured with
 };
 
-static bk_graph* capture(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g) {
+// pipelined: the substep with its first stages beside the previous substep's deferred pass, when the batch allows
+// it (can_pipeline); else the plain substep.
+static bk_graph* capture(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g,
+                         bool pipelined = false) {
   hipStream_t cap = nullptr;
   if (check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return nullptr;
+  Pipe pipe{nullptr, nullptr, nullptr};
+  const bool pipe_on = pipelined && can_pipeline(hw);
+  int rc = 0;
+  if (pipe_on) {
+    rc = check_hip(hipStreamCreateWithFlags(&pipe.side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&pipe.fork, hipEventDisableTiming), "hipEventCreateWithFlags");
+    if (!rc) rc = check_hip(hipEventCreateWithFlags(&pipe.join, hipEventDisableTiming), "hipEventCreateWithFlags");
+  }
   bk_graph* gr = new bk_graph();
   gr->ws = ws;
   gr->table = hw->doughs;
-  int rc = check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  if (!rc) rc = check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   if (!rc) {
-    int rc_sub = batch_substep(ws, hw, sc, in, out, g, cap, false);
+    int rc_sub = batch_substep(ws, hw, sc, in, out, g, cap, false, pipe_on ? &pipe : nullptr);
     int rc_end = check_hip(hipStreamEndCapture(cap, &gr->graph), "hipStreamEndCapture");
     rc = rc_sub ? rc_sub : rc_end;
   }
   if (!rc) rc = check_hip(hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   (void)hipStreamDestroy(cap);
+  if (pipe.side) (void)hipStreamDestroy(pipe.side);
+  if (pipe.fork) (void)hipEventDestroy(pipe.fork);
+  if (pipe.join) (void)hipEventDestroy(pipe.join);
   if (rc) {
     if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
     if (gr->graph) (void)hipGraphDestroy(gr->graph);
@@ -2079,6 +2172,7 @@ bk_workspace* bk_workspace_create_batch(int32_t max_particles, int32_t total_blo
   alloc((void**)&h.slot_exposed, size_t(n_slots) * 8);
   alloc((void**)&h.slot_contact, size_t(n_slots) * 8);
   alloc((void**)&h.flags, size_t(BK_FLAG_COUNT) * max_doughs * 4);
+  alloc((void**)&h.dflags, size_t(BK_FLAG_COUNT) * max_doughs * 4);
   alloc((void**)&h.features, n * kMlpRow * 4);
   alloc((void**)&h.predictions, n * kMlpOut * 4);
   alloc((void**)&h.stats, 4 * sizeof(unsigned long long));
@@ -2113,7 +2207,7 @@ void bk_workspace_destroy(bk_workspace* ws) {
   if (ws_host(ws, h)) return;
   void* ptrs[] = {h.doughs, h.keys_raw, h.sorted_keys, h.sorted_vals, h.keys_s[0], h.keys_s[1], h.vals_s[0], h.vals_s[1],
                   h.hist, h.digit_total, h.blk_start, h.blk_count, h.scan_part, h.blk_end, h.node_flag, h.blocks, h.tiles, h.active, h.counters,
-                  h.queue, h.prep, h.tau, h.vertices, h.vert_mark, h.slot_exposed, h.slot_contact, h.flags, h.features, h.predictions,
+                  h.queue, h.prep, h.tau, h.vertices, h.vert_mark, h.slot_exposed, h.slot_contact, h.flags, h.dflags, h.features, h.predictions,
                   h.stats, h.settle, h.settle_part, h.dTW, h.freeze, h.freeze_age};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -2170,8 +2264,9 @@ int bk_workspace_set_tuning(bk_workspace* ws, const bk_tuning* tuning) {
   if (!hw || !tuning) return fail("bk_workspace_set_tuning: bad argument");
   const bk_tuning t = *tuning;
   if (t.tile_chunk < 1 || t.tile_chunk > kMaxTileChunk || t.grid_cap < 1 || t.grid_cap > 65536 || t.particle_block < 64 ||
-      t.particle_block > kParticleThreads || t.particle_block % 64)
-    return fail("bk_workspace_set_tuning: need tile_chunk in [1, 2^20], grid_cap in [1, 65536], particle_block in {64, 128, 192, 256}");
+      t.particle_block > kParticleThreads || t.particle_block % 64 || t.deferred_wgs < 0 || t.deferred_wgs > 65536)
+    return fail("bk_workspace_set_tuning: need tile_chunk in [1, 2^20], grid_cap in [1, 65536], particle_block in {64, 128, 192, "
+                "256}, deferred_wgs in [0, 65536]");
   hw->tuning = t;
   return 0;
 }
@@ -2395,6 +2490,33 @@ bk_graph* bk_batch_graph_create(bk_workspace* ws, const bk_scene* scene, const b
   return capture(ws, hw, scene, in, out, grid);
 }
 
+bk_graph* bk_batch_graph_create_pipelined(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out,
+                                          bk_grid* grid) {
+  HostWs* hw = host_view(ws);
+  if (!hw) { fail("bk_batch_graph_create_pipelined: bad workspace"); return nullptr; }
+  return capture(ws, hw, scene, in, out, grid, true);
+}
+
+// The deferred pass a run of pipelined substeps left (a no-op when none is pending), on the last substep's in / out.
+static int finish(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const bk_state* in, bk_state* out, hipStream_t st) {
+  if (!sc || !in || !out) return fail("bk_*_finish: null argument");
+  if (!hw->gas) return 0;
+  const bk_workspace* h = &hw->dev;
+  hipLaunchKernelGGL(k_pass_c_deferred, dim3(clampi(div_up(hw->n, 256), 1, 1024)), dim3(256), 0, st, h->doughs, *sc, *in, *out, ws,
+                     h->flags, 1);
+  BK_LAUNCH(pass_c_deferred);
+  hipLaunchKernelGGL(k_pending_done, dim3(1), dim3(64), 0, st, ws);
+  BK_LAUNCH(pending_done);
+  return 0;
+}
+
+int bk_batch_finish(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw) return fail("bk_batch_finish: bad workspace");
+  if (hw->doughs.empty()) return fail("bk_batch_finish: no dough table uploaded");
+  return finish(ws, hw, scene, in, out, (hipStream_t)stream);
+}
+
 int bk_substep(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g, bk_workspace* ws,
                void* stream) {
   HostWs* hw = host_view(ws);
@@ -2412,6 +2534,24 @@ bk_graph* bk_substep_graph_create(const bk_params* P, const bk_scene* sc, const 
   bk_dough d = single_dough(P);
   if (upload_doughs(ws, hw, &d, 1)) return nullptr;
   return capture(ws, hw, sc, in, out, g);
+}
+
+bk_graph* bk_substep_graph_create_pipelined(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g,
+                                            bk_workspace* ws) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !P) { fail("bk_substep_graph_create_pipelined: bad argument"); return nullptr; }
+  bk_dough d = single_dough(P);
+  if (upload_doughs(ws, hw, &d, 1)) return nullptr;
+  return capture(ws, hw, sc, in, out, g, true);
+}
+
+int bk_substep_finish(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_workspace* ws, void* stream) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !P) return fail("bk_substep_finish: bad argument");
+  bk_dough d = single_dough(P);
+  int rc = upload_doughs(ws, hw, &d, 1);
+  if (rc) return rc;
+  return finish(ws, hw, sc, in, out, (hipStream_t)stream);
 }
 
 int bk_graph_launch(bk_graph* gr, void* stream) {

@@ -48,7 +48,8 @@ struct bk_workspace {
   int4* tiles = nullptr;                        // P2G tiles: (block, dough, first sorted particle, count)
   int32_t* active = nullptr;                    // active node blocks
   int32_t* counters = nullptr;                  // [0] full tiles, [1] active blocks, [2] deferred, [3] particle blocks,
-                                                // [4] emitted samples, [6] substeps run, [7] sparse tiles
+                                                // [4] emitted samples, [5] deferred queue not yet processed,
+                                                // [6] substeps run, [7] sparse tiles
   int32_t* queue = nullptr;                     // particles whose coupled porosity needs the full solve
   float4* prep = nullptr;                       // per particle, pass C -> deferred pass: the first half of a deferred
                                                 // particle's Ding update (DingPrep, kDingPrepVec float4)
@@ -58,6 +59,8 @@ struct bk_workspace {
   double* slot_exposed = nullptr;
   double* slot_contact = nullptr;
   int32_t* flags = nullptr;                     // BK_FLAG_COUNT per dough
+  int32_t* dflags = nullptr;                    // BK_FLAG_COUNT per dough: a deferred pass running beside the next
+                                                // substep raises its flags here; k_join merges them
   float* features = nullptr;                    // surrogate inputs of the queued particles, kMlpRow per queue slot
   float* predictions = nullptr;                 // surrogate outputs, kMlpOut per queue slot
   const uint8_t* mlp = nullptr;                 // surrogate inference blob (caller-owned), or null

@@ -264,21 +264,23 @@ class BatchSimulator:
         self._refresh_table()
 
     # ------------------------------------------------------------------ tuning
-    def SetTuning(self, tile_chunk=None, particle_block=None, grid_cap=None):
+    def SetTuning(self, tile_chunk=None, particle_block=None, grid_cap=None, deferred_wgs=None):
         """Launch shapes of the batch substep (see ``bk_tuning``); captured graphs are dropped."""
         lib = _native.load()
         t = self.Tuning()
-        for name, value in (("tile_chunk", tile_chunk), ("particle_block", particle_block), ("grid_cap", grid_cap)):
+        for name, value in (("tile_chunk", tile_chunk), ("particle_block", particle_block), ("grid_cap", grid_cap),
+                            ("deferred_wgs", deferred_wgs)):
             if value is not None:
                 t[name] = int(value)
-        struct = _native.BkTuning(t["tile_chunk"], t["particle_block"], t["grid_cap"])
+        struct = _native.BkTuning(t["tile_chunk"], t["particle_block"], t["grid_cap"], t["deferred_wgs"])
         _native.check(lib.bk_workspace_set_tuning(self._workspace, ctypes.byref(struct)), "bk_workspace_set_tuning")
         self._destroy_graphs()
 
     def Tuning(self):
         struct = _native.BkTuning()
         _native.check(_native.load().bk_workspace_get_tuning(self._workspace, ctypes.byref(struct)), "bk_workspace_get_tuning")
-        return dict(tile_chunk=struct.tile_chunk, particle_block=struct.particle_block, grid_cap=struct.grid_cap)
+        return dict(tile_chunk=struct.tile_chunk, particle_block=struct.particle_block, grid_cap=struct.grid_cap,
+                    deferred_wgs=struct.deferred_wgs)
 
     # ------------------------------------------------------------------ stepping
     def _state_struct(self, k):
@@ -293,31 +295,34 @@ class BatchSimulator:
             lib.bk_graph_destroy(handle)
         self._graphs = {}
 
-    def _ensure_graphs(self):
-        """Capture both ping-pong directions now (capture launches nothing)."""
-        lib = _native.load()
-        for cur in (0, 1):
-            if cur not in self._graphs:
-                s_in, s_out = self._state_struct(cur), self._state_struct(1 - cur)
-                handle = lib.bk_batch_graph_create(self._workspace, ctypes.byref(self._scene), ctypes.byref(s_in),
-                                                   ctypes.byref(s_out), ctypes.byref(self._grid))
-                if not handle:
-                    _native.check(-2, "bk_batch_graph_create")
-                self._graphs[cur] = ctypes.c_void_p(handle)
+    def _graph(self, cur, pipelined):
+        """The captured substep graph from state ``cur`` to ``1 - cur`` (plain or pipelined), captured on first use."""
+        key = (cur, bool(pipelined))
+        graph = self._graphs.get(key)
+        if graph is None:
+            lib = _native.load()
+            s_in, s_out = self._state_struct(cur), self._state_struct(1 - cur)
+            create = lib.bk_batch_graph_create_pipelined if pipelined else lib.bk_batch_graph_create
+            handle = create(self._workspace, ctypes.byref(self._scene), ctypes.byref(s_in), ctypes.byref(s_out),
+                            ctypes.byref(self._grid))
+            if not handle:
+                _native.check(-2, "bk_batch_graph_create")
+            graph = self._graphs[key] = ctypes.c_void_p(handle)
+        return graph
 
-    def _launch(self, stream=None, use_graph=None):
+    def _ensure_graphs(self):
+        """Capture both ping-pong directions, plain and pipelined, now (capture launches nothing)."""
+        for cur in (0, 1):
+            for pipelined in (False, True):
+                self._graph(cur, pipelined)
+
+    def _launch(self, stream=None, use_graph=None, pipelined=False):
+        """One substep.  ``pipelined`` (graphs only): beside the previous substep's deferred return maps, leaving
+        its own pending (see ``_launch_substeps``)."""
         lib = _native.load()
         use_graph = self.use_graphs if use_graph is None else use_graph
         if use_graph:
-            graph = self._graphs.get(self._cur)
-            if graph is None:
-                s_in, s_out = self._state_struct(self._cur), self._state_struct(1 - self._cur)
-                handle = lib.bk_batch_graph_create(self._workspace, ctypes.byref(self._scene), ctypes.byref(s_in),
-                                                   ctypes.byref(s_out), ctypes.byref(self._grid))
-                if not handle:
-                    _native.check(-2, "bk_batch_graph_create")
-                graph = self._graphs[self._cur] = ctypes.c_void_p(handle)
-            _native.check(lib.bk_graph_launch(graph, _native.stream_ptr(stream)), "bk_graph_launch")
+            _native.check(lib.bk_graph_launch(self._graph(self._cur, pipelined), _native.stream_ptr(stream)), "bk_graph_launch")
         else:
             s_in, s_out = self._state_struct(self._cur), self._state_struct(1 - self._cur)
             _native.check(lib.bk_batch_substep(self._workspace, ctypes.byref(self._scene), ctypes.byref(s_in),
@@ -403,8 +408,7 @@ class BatchSimulator:
     def _advance_chunk(self, step_count, check, stream):
         resting = [d for d, s in enumerate(self.doughs) if s.IsSettling]
         before = self.ReadFlags()
-        for _ in range(step_count):
-            self._launch(stream)
+        self._launch_substeps(step_count, stream)
         self.step_count += step_count
         self._sync_doughs(step_count)
         flags = self.ReadFlags()
@@ -431,8 +435,19 @@ class BatchSimulator:
 
     # ------------------------------------------------------------------ autotuner hooks
     def _launch_substeps(self, count, stream=None):
+        """``count`` substeps.  As HIP graphs, a run of two or more is pipelined (each substep's transport pre-pass
+        and P2G set-up beside the previous substep's deferred return maps; the last one's are finished here), so
+        the state is complete in stream order when this returns either way."""
+        if count < 2 or not self.use_graphs:
+            for _ in range(count):
+                self._launch(stream)
+            return
         for _ in range(count):
-            self._launch(stream)
+            self._launch(stream, pipelined=True)
+        lib = _native.load()
+        s_in, s_out = self._state_struct(1 - self._cur), self._state_struct(self._cur)  # the last substep's
+        _native.check(lib.bk_batch_finish(self._workspace, ctypes.byref(self._scene), ctypes.byref(s_in), ctypes.byref(s_out),
+                                          _native.stream_ptr(stream)), "bk_batch_finish")
 
     def _save_for_tuning(self):
         flags = self.ReadFlags()

@@ -427,26 +427,40 @@ class MpmSimulator:
         self.return_solver = name
         self._params = self._make_params()
 
-    def SetTuning(self, tile_chunk=None, particle_block=None, grid_cap=None):
+    def SetTuning(self, tile_chunk=None, particle_block=None, grid_cap=None, deferred_wgs=None):
         """Launch shapes of the fused substep (``bk_tuning``); captured graphs are dropped."""
         lib = _native.load()
         t = self.Tuning()
-        for name, value in (("tile_chunk", tile_chunk), ("particle_block", particle_block), ("grid_cap", grid_cap)):
+        for name, value in (("tile_chunk", tile_chunk), ("particle_block", particle_block), ("grid_cap", grid_cap),
+                            ("deferred_wgs", deferred_wgs)):
             if value is not None:
                 t[name] = int(value)
-        struct = _native.BkTuning(t["tile_chunk"], t["particle_block"], t["grid_cap"])
+        struct = _native.BkTuning(t["tile_chunk"], t["particle_block"], t["grid_cap"], t["deferred_wgs"])
         _native.check(lib.bk_workspace_set_tuning(self._workspace, ctypes.byref(struct)), "bk_workspace_set_tuning")
         self._destroy_graphs()
 
     def Tuning(self):
         struct = _native.BkTuning()
         _native.check(_native.load().bk_workspace_get_tuning(self._workspace, ctypes.byref(struct)), "bk_workspace_get_tuning")
-        return dict(tile_chunk=struct.tile_chunk, particle_block=struct.particle_block, grid_cap=struct.grid_cap)
+        return dict(tile_chunk=struct.tile_chunk, particle_block=struct.particle_block, grid_cap=struct.grid_cap,
+                    deferred_wgs=struct.deferred_wgs)
 
     def _launch_substeps(self, count, stream=None):
+        """Enqueue ``count`` substeps and accept each.  As HIP graphs, a run of two or more is pipelined (each
+        substep's transport pre-pass and P2G set-up beside the previous substep's deferred return maps; the last
+        one's are finished here), so the state is complete in stream order when this returns either way."""
+        if count < 2 or not self.use_graphs:
+            for _ in range(count):
+                self._launch_substep(stream)
+                self._accept()
+            return
         for _ in range(count):
-            self._launch_substep(stream)
+            self._launch_substep(stream, pipelined=True)
             self._accept()
+        lib = _native.load()
+        s_in, s_out = self._state_struct(self.next_state), self._state_struct(self.current_state)  # the last substep's
+        _native.check(lib.bk_substep_finish(ctypes.byref(self._params), ctypes.byref(self._scene), ctypes.byref(s_in),
+                                            ctypes.byref(s_out), self._workspace, _native.stream_ptr(stream)), "bk_substep_finish")
 
     def _save_for_tuning(self):
         flags = self._read_flags()
@@ -472,20 +486,23 @@ class MpmSimulator:
             lib.bk_graph_destroy(handle)
         self._graphs = {}
 
-    def _launch_substep(self, stream=None, use_graph=None):
-        """Enqueue one substep; replays a captured HIP graph for this ping-pong direction when possible."""
+    def _launch_substep(self, stream=None, use_graph=None, pipelined=False):
+        """Enqueue one substep; replays a captured HIP graph for this ping-pong direction when possible.
+        ``pipelined`` (graphs only): the graph that runs beside the previous substep's deferred return maps and
+        leaves its own pending (see ``_launch_substeps``)."""
         lib = _native.load()
         s_in = self._state_struct(self.current_state)
         s_out = self._state_struct(self.next_state)
         use_graph = self.use_graphs if use_graph is None else use_graph
         if use_graph:
-            key = bytes(s_in) + bytes(s_out) + bytes(self._params)
+            key = bytes(s_in) + bytes(s_out) + bytes(self._params) + bytes([int(pipelined)])
             graph = self._graphs.get(key)
             if graph is None:
-                if len(self._graphs) >= 4:
+                if len(self._graphs) >= 8:
                     self._destroy_graphs()
-                handle = lib.bk_substep_graph_create(ctypes.byref(self._params), ctypes.byref(self._scene), ctypes.byref(s_in),
-                                                     ctypes.byref(s_out), ctypes.byref(self._grid), self._workspace)
+                create = lib.bk_substep_graph_create_pipelined if pipelined else lib.bk_substep_graph_create
+                handle = create(ctypes.byref(self._params), ctypes.byref(self._scene), ctypes.byref(s_in), ctypes.byref(s_out),
+                                ctypes.byref(self._grid), self._workspace)
                 if not handle:
                     _native.check(-2, "bk_substep_graph_create")
                 graph = self._graphs[key] = ctypes.c_void_p(handle)
@@ -567,9 +584,7 @@ class MpmSimulator:
         start_state, start_next, start_steps = self.current_state, self.next_state, self.step_count
         saved = [t.clone() for t in _state_tensors(start_state)]
         saved_settling = self._save_settling() if was_settling else None
-        for _ in range(count):
-            self._launch_substep()
-            self._accept()
+        self._launch_substeps(count)
         flags = self._read_flags()
         if any(flags[i] and not flags_before[i] for i in range(_FLAG_POROSITY, _FLAG_WATER + 1)):
             # Replay from the starting state, checking after every substep, so the

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define BK_ABI_VERSION 2
+#define BK_ABI_VERSION 3
 
 /* Flag block of the fused substep (one int32[8] array):
  *   [0] J nonpositive / nonfinite        (reference MpmSimulator.error_flags[0], physics/mpm.py:245)
@@ -95,11 +95,15 @@ typedef struct bk_dough {
 /* Launch tuning of a workspace (no reference counterpart):
  *   tile_chunk      most particles per P2G tile, at most 2^20 (a particle block holding more is split)
  *   particle_block  workgroup size of the per-particle passes (64, 128, 192 or 256)
- *   grid_cap        workgroups of the tile and node-block passes (grid-stride)      */
+ *   grid_cap        workgroups of the tile and node-block passes (grid-stride)
+ *   deferred_wgs    workgroups of the deferred return-map pass where it runs beside
+ *                   the next substep (pipelined graphs; it strides over its queue);
+ *                   0: pipelined graphs are the plain substep                      */
 typedef struct bk_tuning {
   int32_t tile_chunk;
   int32_t particle_block;
   int32_t grid_cap;
+  int32_t deferred_wgs;
 } bk_tuning;
 
 /* Per-particle state of one step (physics/state.py ParticleState, ding.py DingState, thermal_water.py TransportState). */
@@ -312,6 +316,24 @@ void bk_graph_destroy(bk_graph* graph);
  * are raised in the dough's own block.  bk_substep is this with a table of one. */
 int bk_batch_substep(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out, bk_grid* grid, void* stream);
 bk_graph* bk_batch_graph_create(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out, bk_grid* grid);
+/* Pipelined substeps (no reference counterpart; a schedule of the same
+ * substeps): a graph whose transport pre-pass and P2G set-up (sort, mesh, thermal
+ * P2G, diffusion) run beside the PREVIOUS substep's deferred return maps, which
+ * write only Ding, gas and saturation fields of the particles they finish.  The
+ * substep's own deferred return maps are left pending, so a run of graph
+ * launches on alternating ping-pong directions must end with bk_*_finish on the
+ * last substep's in / out (a no-op when nothing is pending) before its state,
+ * flags or statistics are read.  Flags of a pending deferred pass are merged at
+ * the next substep's join; a dough it flags stops there exactly as at the start
+ * of that substep.  Without pore gas, or with the settling controller, the
+ * graph is the plain substep. */
+bk_graph* bk_substep_graph_create_pipelined(const bk_params* params, const bk_scene* scene, const bk_state* in, bk_state* out,
+                                            bk_grid* grid, bk_workspace* ws);
+bk_graph* bk_batch_graph_create_pipelined(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out,
+                                          bk_grid* grid);
+int bk_substep_finish(const bk_params* params, const bk_scene* scene, const bk_state* in, bk_state* out, bk_workspace* ws,
+                      void* stream);
+int bk_batch_finish(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out, void* stream);
 /* Per-stage timing of bk_substep with HIP events on its stream: begin() arms
  * recording for the next max_steps substeps; end() waits for them and returns
  * the summed milliseconds per stage (bk_profile_stage_count() entries, named

@@ -40,7 +40,7 @@ def test_ctypes_binding_covers_the_header():
         pytest.skip("libbaking_hip.so not built")
     lib = _native.load()  # loads on a machine without a GPU: no device call happens
     assert set(_native.SIGNATURES) == set(declared_functions())
-    assert lib.bk_abi_version() == 2
+    assert lib.bk_abi_version() == 3
 
 
 def test_struct_layouts_match_the_header(tmp_path):

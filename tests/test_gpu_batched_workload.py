@@ -9,12 +9,13 @@
 * A dough that raises a flag stops on the device at its last accepted state (the substep that raised the
   flag is rejected, as physics/mpm.py:205-207 rejects it) with its step count there, gets no keys (so no
   tiles or grid work) while stopped, and the other doughs of the batch run on as without it -- whether the
-  flag comes mid-chunk or in a chunk's last substep.
+  flag comes mid-chunk or in a chunk's last substep, from the key kernel or from a deferred return-map pass.
+* Pipelined chunks of substeps agree with substeps run one at a time.
 """
 import numpy as np
 import pytest
 
-from tests.harness import check_fields, gpu_fields, oracle_from_spec
+from tests.harness import check_fields, check_same, gpu_fields, oracle_from_spec
 
 pytestmark = pytest.mark.gpu
 
@@ -61,21 +62,34 @@ def _small_specs():
     return scenes.batched_doughs(count=3, seed=9, half_extent=(0.02, 0.035), max_particles=6000)
 
 
+def _plant(batch, kind):
+    """Plant a flag-raising particle 0 in dough 1: far off its grid (the key kernel raises the stencil flag,
+    index 1) or with elastic gradient 0.5 I (its coupled porosity has no interior root: the deferred return-map
+    pass raises the coupled-porosity flag, index 3, as the oracle does for the same state)."""
+    if kind == "off_grid":
+        batch.doughs[1].current_state.particle_positions[0, 0] = 10.0
+        return 1
+    fe = batch.doughs[1].current_state.material_state.elastic_gradients
+    fe[0] = 0.5 * torch.eye(3, dtype=fe.dtype, device=fe.device)
+    return 3
+
+
+@pytest.mark.parametrize("kind", ["off_grid", "compressed"])
 @pytest.mark.parametrize("run_on", [0, 4])
-def test_flagged_dough_stops_at_its_last_accepted_state(run_on):
-    """Dough 1 gets a particle far off its grid after 2 substeps: the next substep raises its stencil flag,
-    is rejected, and dough 1 stays at the planted state and step 2 through the remaining substeps (run_on
-    more in the same chunk, or none: then the flag comes in the chunk's last substep)."""
+def test_flagged_dough_stops_at_its_last_accepted_state(run_on, kind):
+    """Dough 1 gets a flag-raising particle after 2 substeps: the next substep raises its flag, is rejected,
+    and dough 1 stays at the planted state and step 2 through the remaining substeps (run_on more in the same
+    chunk, or none: then the flag comes in the chunk's last substep).  A chunk of several substeps runs
+    pipelined, so the compressed particle's flag comes from a deferred pass that ran beside the next substep."""
     specs = _small_specs()
     batch = BatchSimulator([scenes.build_simulator(s) for s in specs])
     batch.Advance(2)
-    x = batch.doughs[1].current_state.particle_positions
-    x[0, 0] = 10.0
+    index = _plant(batch, kind)
     planted = gpu_fields(batch.doughs[1])
     with pytest.raises(RuntimeError, match="doughs"):
         batch.Advance(1 + run_on)
     flags = batch.ReadFlags()
-    assert flags[1, 1] == 1 and flags[0].max() == 0 and flags[2].max() == 0
+    assert flags[1, index] == 1 and flags[1].sum() == 1 and flags[0].max() == 0 and flags[2].max() == 0
     stopped = gpu_fields(batch.doughs[1])
     for k in planted:
         if k not in ("node_m", "node_v"):
@@ -100,6 +114,32 @@ def test_flagged_dough_stops_at_its_last_accepted_state(run_on):
         check_fields(f"{specs[d].name} beside a stopped dough", gpu_fields(batch.doughs[d]), o32, o64)
     # clearing the flags lets the dough run again from where it stopped
     batch.ClearFlags()
-    batch.doughs[1].current_state.particle_positions[0, 0] = float(specs[1].samples.particle_positions[0, 0])
+    if kind == "off_grid":
+        batch.doughs[1].current_state.particle_positions[0, 0] = float(specs[1].samples.particle_positions[0, 0])
+    else:
+        batch.doughs[1].current_state.material_state.elastic_gradients[0] = torch.eye(3, device="cuda")
     batch.Advance(1)
     assert batch.ReadFlags().max() == 0 and batch.doughs[1].step_count == 3
+
+
+def test_pipelined_substeps_match_single_ones():
+    """A chunk of substeps runs pipelined (each substep's transport pre-pass and P2G set-up beside the previous
+    substep's deferred return maps, the last one's finished at the end); substeps one at a time run plain.  From
+    the same start the two agree field by field under the parity rule, and both agree with the oracle."""
+    specs = _small_specs()
+    runs = []
+    for chunk in (6, 1):
+        batch = BatchSimulator([scenes.build_simulator(s) for s in specs])
+        batch.Advance(2)
+        for _ in range(6 // chunk):
+            batch.Advance(chunk)
+        assert batch.ReadFlags().max() == 0
+        runs.append([gpu_fields(d) for d in batch.doughs])
+    for d, spec in enumerate(specs):
+        o32, o64 = oracle_from_spec(spec, "f32"), oracle_from_spec(spec, "f64")
+        for _ in range(8):
+            o32.Step()
+            o64.Step()
+        check_fields(f"{spec.name} pipelined", runs[0][d], o32, o64)
+        check_fields(f"{spec.name} one at a time", runs[1][d], o32, o64)
+        check_same(f"{spec.name} pipelined vs one at a time", runs[0][d], runs[1][d], o32, o64)
