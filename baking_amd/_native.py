@@ -131,6 +131,9 @@ SIGNATURES = {
     "bk_substep_graph_create_pipelined": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid),
                                            P], c_void_p),
     "bk_batch_graph_create_pipelined": ([P, POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid)], c_void_p),
+    "bk_substep_graph_create_steps": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), P,
+                                       I, I], c_void_p),
+    "bk_batch_graph_create_steps": ([P, POINTER(BkScene), POINTER(BkState), POINTER(BkState), POINTER(BkGrid), I, I], c_void_p),
     "bk_substep_finish": ([POINTER(BkParams), POINTER(BkScene), POINTER(BkState), POINTER(BkState), P, P], c_int),
     "bk_batch_finish": ([P, POINTER(BkScene), POINTER(BkState), POINTER(BkState), P], c_int),
     "bk_graph_destroy": ([P], None),

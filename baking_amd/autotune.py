@@ -11,6 +11,9 @@ under candidate ``bk_tuning`` values and keeps the fastest:
 * ``deferred_wgs``   -- workgroups of the deferred return-map pass where it runs beside the next substep in a
   pipelined run of substeps (0: runs are not pipelined).  A short queue finishes within the next substep's first
   stages however it is spread; a long one slows them unless it is held to fewer wavefront slots.
+* ``graph_steps``    -- substeps per HIP-graph launch in a run of substeps (the simulator's ``graph_steps``):
+  more per launch saves the gap between launches, but the pipelined deferred passes inside one graph may then be
+  scheduled less well.
 
 Coordinate descent: each parameter in turn is swept with the others held,
 twice over.  Every trial replays the captured HIP graph for ``steps``
@@ -28,7 +31,19 @@ CANDIDATES = {
     "particle_block": (64, 128, 256),
     "grid_cap": (512, 1024, 2048, 4096, 8192),
     "deferred_wgs": (0, 128, 256, 512, 1024),
+    "graph_steps": (1, 2, 4, 8),
 }
+
+
+def _tuning(sim):
+    """The simulator's bk_tuning values and its substeps per graph launch."""
+    return dict(sim.Tuning(), graph_steps=sim.graph_steps)
+
+
+def _set_tuning(sim, tuning):
+    tuning = dict(tuning)
+    sim.graph_steps = tuning.pop("graph_steps")
+    sim.SetTuning(**tuning)
 
 
 def time_substeps(sim, steps, warmup):
@@ -51,8 +66,8 @@ def autotune(sim, steps=10, warmup=3, rounds=2, candidates=None, log=None):
     candidates = candidates or CANDIDATES
     saved = sim._save_for_tuning()
     try:
-        best = sim.Tuning()
-        sim.SetTuning(**best)
+        best = _tuning(sim)
+        _set_tuning(sim, best)
         best_ms = time_substeps(sim, steps, warmup)
         trials = [(dict(best), best_ms)]
         start = (dict(best), best_ms)
@@ -63,7 +78,7 @@ def autotune(sim, steps=10, warmup=3, rounds=2, candidates=None, log=None):
                     if value == best[name]:
                         continue
                     trial = dict(best, **{name: value})
-                    sim.SetTuning(**trial)
+                    _set_tuning(sim, trial)
                     ms = time_substeps(sim, steps, warmup)
                     trials.append((trial, ms))
                     if log:
@@ -72,7 +87,7 @@ def autotune(sim, steps=10, warmup=3, rounds=2, candidates=None, log=None):
                         best, best_ms, improved = trial, ms, True
             if not improved:
                 break
-        sim.SetTuning(**best)
+        _set_tuning(sim, best)
     finally:
         sim._restore_after_tuning(saved)
     return {"best": best, "ms": best_ms, "start": start, "trials": trials}

@@ -2053,6 +2053,8 @@ static int upload_doughs(bk_workspace* ws, HostWs* hw, const bk_dough* table, in
   return 0;
 }
 
+constexpr int kMaxGraphSteps = 64;  // substeps per captured graph
+
 struct bk_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -2061,9 +2063,10 @@ struct bk_graph {
 };
 
 // pipelined: the substep with its first stages beside the previous substep's deferred pass, when the batch allows
-// it (can_pipeline); else the plain substep.
+// it (can_pipeline); else the plain substep.  steps: that many substeps in one graph, alternating in -> out -> in ...
+// (one graph launch instead of `steps`).
 static bk_graph* capture(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const bk_state* in, bk_state* out, bk_grid* g,
-                         bool pipelined = false) {
+                         bool pipelined = false, int steps = 1) {
   hipStream_t cap = nullptr;
   if (check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreateWithFlags")) return nullptr;
   Pipe pipe{nullptr, nullptr, nullptr};
@@ -2079,7 +2082,10 @@ static bk_graph* capture(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const
   gr->table = hw->doughs;
   if (!rc) rc = check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   if (!rc) {
-    int rc_sub = batch_substep(ws, hw, sc, in, out, g, cap, false, pipe_on ? &pipe : nullptr);
+    bk_state a = *in, b = *out;
+    int rc_sub = 0;
+    for (int k = 0; k < steps && !rc_sub; ++k)
+      rc_sub = batch_substep(ws, hw, sc, k % 2 ? &b : &a, k % 2 ? &a : &b, g, cap, false, pipe_on ? &pipe : nullptr);
     int rc_end = check_hip(hipStreamEndCapture(cap, &gr->graph), "hipStreamEndCapture");
     rc = rc_sub ? rc_sub : rc_end;
   }
@@ -2497,6 +2503,13 @@ bk_graph* bk_batch_graph_create_pipelined(bk_workspace* ws, const bk_scene* scen
   return capture(ws, hw, scene, in, out, grid, true);
 }
 
+bk_graph* bk_batch_graph_create_steps(bk_workspace* ws, const bk_scene* scene, const bk_state* a, bk_state* b, bk_grid* grid,
+                                      int32_t steps, int32_t pipelined) {
+  HostWs* hw = host_view(ws);
+  if (!hw || steps < 1 || steps > kMaxGraphSteps) { fail("bk_batch_graph_create_steps: bad workspace or steps"); return nullptr; }
+  return capture(ws, hw, scene, a, b, grid, pipelined != 0, steps);
+}
+
 // The deferred pass a run of pipelined substeps left (a no-op when none is pending), on the last substep's in / out.
 static int finish(bk_workspace* ws, HostWs* hw, const bk_scene* sc, const bk_state* in, bk_state* out, hipStream_t st) {
   if (!sc || !in || !out) return fail("bk_*_finish: null argument");
@@ -2543,6 +2556,15 @@ bk_graph* bk_substep_graph_create_pipelined(const bk_params* P, const bk_scene* 
   bk_dough d = single_dough(P);
   if (upload_doughs(ws, hw, &d, 1)) return nullptr;
   return capture(ws, hw, sc, in, out, g, true);
+}
+
+bk_graph* bk_substep_graph_create_steps(const bk_params* P, const bk_scene* sc, const bk_state* a, bk_state* b, bk_grid* g,
+                                        bk_workspace* ws, int32_t steps, int32_t pipelined) {
+  HostWs* hw = host_view(ws);
+  if (!hw || !P || steps < 1 || steps > kMaxGraphSteps) { fail("bk_substep_graph_create_steps: bad argument"); return nullptr; }
+  bk_dough d = single_dough(P);
+  if (upload_doughs(ws, hw, &d, 1)) return nullptr;
+  return capture(ws, hw, sc, a, b, g, pipelined != 0, steps);
 }
 
 int bk_substep_finish(const bk_params* P, const bk_scene* sc, const bk_state* in, bk_state* out, bk_workspace* ws, void* stream) {

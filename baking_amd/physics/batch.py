@@ -37,6 +37,7 @@ _FLAG_NAMES = ("J", "stencil", "porosity", "coupled", "gas_cfl", "heat", "water"
 
 class BatchSimulator:
     use_graphs = True
+    graph_steps = 4  # substeps per graph launch in runs of substeps (even; 1: one launch per substep)
 
     def __init__(self, simulators, tuning=None):
         lib = _native.load()
@@ -295,26 +296,29 @@ class BatchSimulator:
             lib.bk_graph_destroy(handle)
         self._graphs = {}
 
-    def _graph(self, cur, pipelined):
-        """The captured substep graph from state ``cur`` to ``1 - cur`` (plain or pipelined), captured on first use."""
-        key = (cur, bool(pipelined))
+    def _graph(self, cur, pipelined, steps=1):
+        """The captured graph of ``steps`` substeps from state ``cur`` (alternating buffers; plain or pipelined),
+        captured on first use."""
+        key = (cur, bool(pipelined), steps)
         graph = self._graphs.get(key)
         if graph is None:
             lib = _native.load()
             s_in, s_out = self._state_struct(cur), self._state_struct(1 - cur)
-            create = lib.bk_batch_graph_create_pipelined if pipelined else lib.bk_batch_graph_create
-            handle = create(self._workspace, ctypes.byref(self._scene), ctypes.byref(s_in), ctypes.byref(s_out),
-                            ctypes.byref(self._grid))
+            handle = lib.bk_batch_graph_create_steps(self._workspace, ctypes.byref(self._scene), ctypes.byref(s_in),
+                                                     ctypes.byref(s_out), ctypes.byref(self._grid), steps, int(pipelined))
             if not handle:
-                _native.check(-2, "bk_batch_graph_create")
+                _native.check(-2, "bk_batch_graph_create_steps")
             graph = self._graphs[key] = ctypes.c_void_p(handle)
         return graph
 
     def _ensure_graphs(self):
-        """Capture both ping-pong directions, plain and pipelined, now (capture launches nothing)."""
+        """Capture both ping-pong directions, plain and pipelined, one substep and graph_steps, now (capture
+        launches nothing)."""
         for cur in (0, 1):
             for pipelined in (False, True):
                 self._graph(cur, pipelined)
+            if self.graph_steps > 1:
+                self._graph(cur, True, self.graph_steps)
 
     def _launch(self, stream=None, use_graph=None, pipelined=False):
         """One substep.  ``pipelined`` (graphs only): beside the previous substep's deferred return maps, leaving
@@ -442,9 +446,14 @@ class BatchSimulator:
             for _ in range(count):
                 self._launch(stream)
             return
+        lib = _native.load()
+        k = self.graph_steps
+        while k > 1 and count >= k:  # k substeps per graph launch
+            _native.check(lib.bk_graph_launch(self._graph(self._cur, True, k), _native.stream_ptr(stream)), "bk_graph_launch")
+            self._cur ^= k & 1
+            count -= k
         for _ in range(count):
             self._launch(stream, pipelined=True)
-        lib = _native.load()
         s_in, s_out = self._state_struct(1 - self._cur), self._state_struct(self._cur)  # the last substep's
         _native.check(lib.bk_batch_finish(self._workspace, ctypes.byref(self._scene), ctypes.byref(s_in), ctypes.byref(s_out),
                                           _native.stream_ptr(stream)), "bk_batch_finish")

@@ -397,6 +397,7 @@ class MpmSimulator:
 
     # ------------------------------------------------------------------ substep
     use_graphs = True
+    graph_steps = 4  # substeps per graph launch in runs of substeps (even; 1: one launch per substep)
     # Coupled-porosity solve of the fused substep: "bracketed" (default) solves the
     # reference's end-state porosity equation to a few ulps in ~10-20 residual
     # evaluations; "bisection" runs the reference's 64 fixed bisection steps.
@@ -454,6 +455,10 @@ class MpmSimulator:
                 self._launch_substep(stream)
                 self._accept()
             return
+        k = self.graph_steps
+        while k > 1 and count >= k:  # k substeps per graph launch (k even: each launch ends on its first buffers)
+            self._launch_graph_steps(k, stream)
+            count -= k
         for _ in range(count):
             self._launch_substep(stream, pipelined=True)
             self._accept()
@@ -511,6 +516,25 @@ class MpmSimulator:
         rc = lib.bk_substep(ctypes.byref(self._params), ctypes.byref(self._scene), ctypes.byref(s_in), ctypes.byref(s_out),
                             ctypes.byref(self._grid), self._workspace, _native.stream_ptr(stream))
         _native.check(rc, "bk_substep")
+
+    def _launch_graph_steps(self, k, stream=None):
+        """Enqueue and accept ``k`` pipelined substeps captured in one HIP graph (alternating ping-pong buffers)."""
+        lib = _native.load()
+        s_in = self._state_struct(self.current_state)
+        s_out = self._state_struct(self.next_state)
+        key = bytes(s_in) + bytes(s_out) + bytes(self._params) + bytes([2, k])
+        graph = self._graphs.get(key)
+        if graph is None:
+            if len(self._graphs) >= 8:
+                self._destroy_graphs()
+            handle = lib.bk_substep_graph_create_steps(ctypes.byref(self._params), ctypes.byref(self._scene), ctypes.byref(s_in),
+                                                       ctypes.byref(s_out), ctypes.byref(self._grid), self._workspace, k, 1)
+            if not handle:
+                _native.check(-2, "bk_substep_graph_create_steps")
+            graph = self._graphs[key] = ctypes.c_void_p(handle)
+        _native.check(lib.bk_graph_launch(graph, _native.stream_ptr(stream)), "bk_graph_launch")
+        for _ in range(k):
+            self._accept()
 
     def _accept(self):
         self.current_state, self.next_state = self.next_state, self.current_state

@@ -91,6 +91,7 @@ def parse():
     ap.add_argument("--cpu-baseline-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch kernels one by one instead of replaying a HIP graph")
+    ap.add_argument("--graph-steps", type=int, default=None, help="substeps per graph launch in timed runs (default: the simulator's; the autotuner starts there)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a 1 GiB device-to-device copy 5 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
@@ -342,6 +343,8 @@ def run_batched(args):
     plain_graph_rate = n * args.plain_steps / (plain_graph_ms * 1e-3)
 
     batch = BatchSimulator(sims)
+    if args.graph_steps is not None:
+        batch.graph_steps = args.graph_steps
     model = None
     if not args.no_surrogate:
         model = surrogate.ReturnMapSurrogate.load(args.model or surrogate.default_model("batched_48"))
@@ -414,7 +417,7 @@ def run_batched(args):
                    "grids": sorted(set(s.simulation_config.grid_resolution for s in sims)),
                    "dt": sims[0].simulation_config.time_step,
                    "active_stencil_nodes": active_nodes, "yield_fraction": yield_fraction, "deferred_return_maps": wcounts[2],
-                   "tiles": wcounts[0], "active_node_blocks": wcounts[1], "tuning": batch.Tuning(),
+                   "tiles": wcounts[0], "active_node_blocks": wcounts[1], "tuning": dict(batch.Tuning(), graph_steps=batch.graph_steps),
                    "autotune": None if tuned is None else {"start": tuned["start"], "best_ms": tuned["ms"],
                                                            "trials": len(tuned["trials"])},
                    "return_map_stats": {"exact": exact_stats, "used": stats_used,
@@ -668,6 +671,8 @@ def main():
     steps = args.steps or (spec.steps if spec.name == "loaf_256" else 1000)
     sim = build_simulator(spec)
     sim.use_graphs = not args.no_graph
+    if args.graph_steps is not None:
+        sim.graph_steps = args.graph_steps
     n = sim.particle_data.particle_count
     stream = torch.cuda.current_stream()
     model = None
@@ -691,8 +696,8 @@ def main():
     start_aggr = measure(sim)
 
     def timed():
-        if sim.use_graphs:  # capture both ping-pong graphs outside the timed region
-            sim._launch_substeps(2, stream)
+        if sim.use_graphs:  # capture the graphs the timed run replays outside the timed region
+            sim._launch_substeps(max(2, sim.graph_steps), stream)
         sim._restore_after_tuning(snapshot)
         start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
@@ -744,7 +749,7 @@ def main():
         "config": {"workload": spec.name, "method": method, "hip_graph": sim.use_graphs, "particles": n,
                    "grid": spec.simulation.grid_resolution, "dt": spec.simulation.time_step, "active_stencil_nodes": active_nodes,
                    "yield_fraction": yield_fraction, "deferred_return_maps": deferred,
-                   "tiles": int(counts[0]), "active_node_blocks": int(counts[1]), "tuning": sim.Tuning(),
+                   "tiles": int(counts[0]), "active_node_blocks": int(counts[1]), "tuning": dict(sim.Tuning(), graph_steps=sim.graph_steps),
                    "autotune": None if tuned is None else {"start": tuned["start"], "best_ms": tuned["ms"],
                                                            "trials": len(tuned["trials"])},
                    "return_map_stats": {"exact": exact_stats, "used": stats_used,

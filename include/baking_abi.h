@@ -331,6 +331,13 @@ bk_graph* bk_substep_graph_create_pipelined(const bk_params* params, const bk_sc
                                             bk_grid* grid, bk_workspace* ws);
 bk_graph* bk_batch_graph_create_pipelined(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out,
                                           bk_grid* grid);
+/* `steps` substeps (1-64) captured into one graph, alternating a -> b -> a ... (plain or, with pipelined != 0,
+ * pipelined): one launch replays them all, saving the per-launch gap between substeps.  An even count ends on
+ * `a`.  After a pipelined one, bk_*_finish takes the last substep's in / out as above. */
+bk_graph* bk_substep_graph_create_steps(const bk_params* params, const bk_scene* scene, const bk_state* a, bk_state* b,
+                                        bk_grid* grid, bk_workspace* ws, int32_t steps, int32_t pipelined);
+bk_graph* bk_batch_graph_create_steps(bk_workspace* ws, const bk_scene* scene, const bk_state* a, bk_state* b, bk_grid* grid,
+                                      int32_t steps, int32_t pipelined);
 int bk_substep_finish(const bk_params* params, const bk_scene* scene, const bk_state* in, bk_state* out, bk_workspace* ws,
                       void* stream);
 int bk_batch_finish(bk_workspace* ws, const bk_scene* scene, const bk_state* in, bk_state* out, void* stream);
