@@ -142,7 +142,7 @@ def field_errors(a, b, names=None, floors=None, weights=None):
 ROUNDING_CAP = 1e-2
 
 
-def judge_fields(e_gpu, e_ref, rtol=1e-5, factor=4.0, cap=ROUNDING_CAP):
+def rate_fields(e_gpu, e_ref, rtol=1e-5, factor=4.0, cap=ROUNDING_CAP):
     """Per field: 'rtol' when err(gpu, o32) <= rtol; 'fp32' when it is within factor x err(o32, o64) (the
     reference's own FP32 rounding) and that allowance is at most ``cap``; otherwise 'FAIL'.  So a field passes
     either at 1e-5 of its physical scale or at a few FP32 roundings, and never at 1e-2 relative error or more.
@@ -161,18 +161,18 @@ def judge_fields(e_gpu, e_ref, rtol=1e-5, factor=4.0, cap=ROUNDING_CAP):
 
 
 def check_fields(tag, got, o32, o64, log=print, rtol=1e-5, factor=4.0, cap=ROUNDING_CAP):
-    """Fields ``got`` against the FP32 oracle simulator ``o32`` under ``judge_fields``, with the FP64 oracle
+    """Fields ``got`` against the FP32 oracle simulator ``o32`` under ``rate_fields``, with the FP64 oracle
     ``o64`` at the same substep giving the FP32 rounding allowance.  Logs each field's error, the oracle's own
     FP32/FP64 gap and the branch it passed by; returns (errors, gaps)."""
     r32, r64 = oracle_fields(o32), oracle_fields(o64)
     floors = field_scales(r32, o32)
     e_gpu = field_errors(got, r32, floors=floors)
     e_ref = field_errors(r32, r64, floors=floors)
-    verdict = judge_fields(e_gpu, e_ref, rtol, factor, cap)
+    outcome = rate_fields(e_gpu, e_ref, rtol, factor, cap)
     if log:
         log(f"{tag}: worst {max(e_gpu.values()):.2e}; "
-            + ", ".join(f"{k} {e_gpu[k]:.1e}/{e_ref[k]:.1e} {verdict[k]}" for k in sorted(e_gpu) if e_gpu[k] > 1e-7))
-    bad = {k: (e_gpu[k], e_ref[k]) for k, v in verdict.items() if v == "FAIL"}
+            + ", ".join(f"{k} {e_gpu[k]:.1e}/{e_ref[k]:.1e} {outcome[k]}" for k in sorted(e_gpu) if e_gpu[k] > 1e-7))
+    bad = {k: (e_gpu[k], e_ref[k]) for k, v in outcome.items() if v == "FAIL"}
     assert not bad, f"{tag}: fields outside tolerance (gpu/o32, o32/o64): {bad}"
     return e_gpu, e_ref
 
@@ -184,8 +184,8 @@ def check_same(tag, a, b, o32, o64, rtol=1e-5, factor=4.0, cap=ROUNDING_CAP):
     floors = field_scales(r32, o32)
     e = field_errors(a, b, floors=floors)
     e_ref = field_errors(r32, r64, floors=floors)
-    verdict = judge_fields(e, e_ref, rtol, factor, cap)
-    bad = {k: (e[k], e_ref[k]) for k, v in verdict.items() if v == "FAIL"}
+    outcome = rate_fields(e, e_ref, rtol, factor, cap)
+    bad = {k: (e[k], e_ref[k]) for k, v in outcome.items() if v == "FAIL"}
     assert not bad, f"{tag}: runs differ beyond tolerance: {bad}"
 
 
@@ -237,7 +237,7 @@ class OraclePoint:
 def three_way(spec, steps, rtol=1e-5, factor=4.0, check_every=1, log=print, solver=None, cap=ROUNDING_CAP):
     """Run the GPU path, the FP32 oracle and the FP64 oracle side by side.
 
-    After each checked substep every field must satisfy (``judge_fields``)
+    After each checked substep every field must satisfy (``rate_fields``)
         err(gpu, oracle32) <= rtol                                   ("rtol"), or
         err(gpu, oracle32) <= min(factor * err(oracle32, oracle64), cap)   ("fp32"),
     with errors scale-relative to max(max|field|, physical floor) (``field_scales``): the GPU agrees with
