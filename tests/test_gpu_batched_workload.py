@@ -124,22 +124,26 @@ def test_flagged_dough_stops_at_its_last_accepted_state(run_on, kind):
 
 def test_pipelined_substeps_match_single_ones():
     """A chunk of substeps runs pipelined (each substep's transport pre-pass and P2G set-up beside the previous
-    substep's deferred return maps, the last one's finished at the end); substeps one at a time run plain.  From
-    the same start the two agree field by field under the parity rule, and both agree with the oracle."""
+    substep's deferred return maps, the last one's finished at the end), several substeps per graph launch (4, and
+    3: an odd count ends on the other buffer); substeps one at a time run plain.  From the same start the runs
+    agree field by field under the parity rule, and each agrees with the oracle."""
     specs = _small_specs()
     runs = []
-    for chunk in (6, 1):
+    for chunk, graph_steps in ((7, 4), (7, 3), (1, 1)):
         batch = BatchSimulator([scenes.build_simulator(s) for s in specs])
+        batch.graph_steps = graph_steps
         batch.Advance(2)
-        for _ in range(6 // chunk):
+        for _ in range(7 // chunk):
             batch.Advance(chunk)
         assert batch.ReadFlags().max() == 0
+        assert all(d.step_count == 9 for d in batch.doughs)
         runs.append([gpu_fields(d) for d in batch.doughs])
     for d, spec in enumerate(specs):
         o32, o64 = oracle_from_spec(spec, "f32"), oracle_from_spec(spec, "f64")
-        for _ in range(8):
+        for _ in range(9):
             o32.Step()
             o64.Step()
-        check_fields(f"{spec.name} pipelined", runs[0][d], o32, o64)
-        check_fields(f"{spec.name} one at a time", runs[1][d], o32, o64)
-        check_same(f"{spec.name} pipelined vs one at a time", runs[0][d], runs[1][d], o32, o64)
+        for r, tag in zip(runs, ("pipelined, 4 per graph", "pipelined, 3 per graph", "one at a time")):
+            check_fields(f"{spec.name} {tag}", r[d], o32, o64)
+        check_same(f"{spec.name} pipelined vs one at a time", runs[0][d], runs[2][d], o32, o64)
+        check_same(f"{spec.name} 3 per graph vs one at a time", runs[1][d], runs[2][d], o32, o64)
