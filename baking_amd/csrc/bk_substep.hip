@@ -1342,21 +1342,41 @@ __device__ inline void material_outputs(const bk_params& P, const bk_scene& sc, 
   if (P.gas_enabled) {
     double moles = P.transport_enabled ? out.moles[p] : in.moles[p];
     if (!P.transport_enabled) out.moles[p] = moles;
-    float nn = out.porosity[p];
-    if (nn > 0.0f && nn < 1.0f) out.pressure[p] = gas_pressure(moles, out.T_mat[p], Jn, nn);
+    // a finished update's porosity and temperature are the values just stored: no reload behind the stores
+    const bool done = status == kDingDone;
+    float nn = done ? r.porosity : out.porosity[p];
+    if (nn > 0.0f && nn < 1.0f) out.pressure[p] = gas_pressure(moles, done ? r.T : out.T_mat[p], Jn, nn);
   }
   if (P.transport_enabled) out.s[p] = out.W[p] / (double(P.rho_w) * double(sc.V0[p]) * double(Jn));
+}
+
+// The per-particle inputs of pass C's Ding update.  Pass C loads them before
+// its G2P gather, whose stores the compiler cannot move loads across, so that
+// their latency overlaps the gather.
+struct DingInputs {
+  mat3<float> Fe;
+  float porosity, T;
+  int gelled;
+};
+
+__device__ __forceinline__ DingInputs ding_inputs(const bk_params& P, const bk_state& in, const bk_state& out, int p) {
+  DingInputs q;
+  q.Fe = m_load(in.Fe + 9 * p);
+  q.porosity = in.porosity[p];
+  q.gelled = in.gelled[p];
+  q.T = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
+  return q;
 }
 
 // Pass C's update of particle p.  A particle deferred to the second pass
 // (`defer`) leaves the first half's record at prep_out + kDingPrepVec * p.
 __device__ inline int material_update(const bk_params& P, float visc_param, const bk_scene& sc, const bk_state& in,
-                                      const bk_state& out, int p, const mat3<float>& F_new, const mat3<float>& Cn, bool defer,
-                                      bool& f0, bool& f1, DingAux* aux, float4* __restrict__ prep_out) {
+                                      const bk_state& out, int p, const DingInputs& q, const mat3<float>& F_new,
+                                      const mat3<float>& Cn, bool defer, bool& f0, bool& f1, DingAux* aux,
+                                      float4* __restrict__ prep_out) {
   const DingParams D = ding_params(P);
-  float T_local = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
   DingOut r;
-  int status = ding_update(D, m_load(in.Fe + 9 * p), in.porosity[p], in.gelled[p], F_new, Cn, T_local, visc_param, r, defer, aux,
+  int status = ding_update(D, q.Fe, q.porosity, q.gelled, F_new, Cn, q.T, visc_param, r, defer, aux,
                            prep_out + kDingPrepVec * int64_t(p));
   f0 = r.flag0;
   f1 = r.flag1;
@@ -1607,6 +1627,8 @@ __global__ __launch_bounds__(kParticleThreads) __attribute__((amdgpu_waves_per_e
       if (!valid_stencil(b, P.res)) {
         stencil_bad = true;
       } else {
+        DingInputs q;
+        if (P.model == 1) q = ding_inputs(P, in, out, p);
         float l[3] = {g3[0] - float(b[0]), g3[1] - float(b[1]), g3[2] - float(b[2])};
         float wx[3], wy[3], wz[3];
         axis_weights(l[0], wx);
@@ -1657,7 +1679,7 @@ __global__ __launch_bounds__(kParticleThreads) __attribute__((amdgpu_waves_per_e
         m_store(out.F + 9 * p, F_new);
         if (P.model == 1) {
           DingAux aux;
-          deferred = material_update(P, sc.visc_param[d], sc, in, out, p, F_new, Cn, true, f0, f1, P.gas_enabled ? &aux : nullptr,
+          deferred = material_update(P, sc.visc_param[d], sc, in, out, p, q, F_new, Cn, true, f0, f1, P.gas_enabled ? &aux : nullptr,
                                      ws->prep) == kDingDeferred;
           if (deferred) surrogate_features(P, in, p, aux, feat);
         }
