@@ -1357,6 +1357,7 @@ struct DingInputs {
   mat3<float> Fe;
   float porosity, T;
   int gelled;
+  float sat, fp_dev;  // the network features of a deferral: saturation, |Fp - I| (with pore gas only)
 };
 
 __device__ __forceinline__ DingInputs ding_inputs(const bk_params& P, const bk_state& in, const bk_state& out, int p) {
@@ -1365,6 +1366,19 @@ __device__ __forceinline__ DingInputs ding_inputs(const bk_params& P, const bk_s
   q.porosity = in.porosity[p];
   q.gelled = in.gelled[p];
   q.T = P.transport_enabled ? out.T_mat[p] : in.T_mat[p];
+  q.sat = 0.0f;
+  q.fp_dev = 0.0f;
+  if (P.gas_enabled) {  // only a coupled particle can be deferred to the network
+    q.sat = P.transport_enabled ? float(in.s[p]) : 0.0f;
+    const mat3<float> Fp = m_load(in.Fp + 9 * p);
+    float acc = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const float e = Fp.a[i] - ((i % 4 == 0) ? 1.0f : 0.0f);
+      acc += e * e;
+    }
+    q.fp_dev = sqrtf(acc);
+  }
   return q;
 }
 
@@ -1411,21 +1425,14 @@ __device__ inline int material_finish(const bk_params& P, const bk_scene& sc, co
 }
 
 // Surrogate inputs of particle p from its Ding extras (see bk_surrogate.h).
-__device__ __forceinline__ void surrogate_features(const bk_params& P, const bk_state& in, int p, const DingAux& aux, float f[kMlpRow]) {
+__device__ __forceinline__ void surrogate_features(const DingInputs& q, const DingAux& aux, float f[kMlpRow]) {
   f[0] = aux.eps[0];
   f[1] = aux.eps[1];
   f[2] = aux.eps[2];
   f[3] = aux.T;
-  f[4] = P.transport_enabled ? float(in.s[p]) : 0.0f;
-  f[5] = in.porosity[p];
-  const mat3<float> Fp = m_load(in.Fp + 9 * p);
-  float acc = 0.0f;
-#pragma unroll
-  for (int i = 0; i < 9; ++i) {
-    const float e = Fp.a[i] - ((i % 4 == 0) ? 1.0f : 0.0f);
-    acc += e * e;
-  }
-  f[6] = sqrtf(acc);
+  f[4] = q.sat;
+  f[5] = q.porosity;
+  f[6] = q.fp_dev;
   f[7] = 0.0f;
 }
 
@@ -1681,7 +1688,7 @@ __global__ __launch_bounds__(kParticleThreads) __attribute__((amdgpu_waves_per_e
           DingAux aux;
           deferred = material_update(P, sc.visc_param[d], sc, in, out, p, q, F_new, Cn, true, f0, f1, P.gas_enabled ? &aux : nullptr,
                                      ws->prep) == kDingDeferred;
-          if (deferred) surrogate_features(P, in, p, aux, feat);
+          if (deferred) surrogate_features(q, aux, feat);
         }
       }
       int32_t* fl = dough_flags(ws, d);
